@@ -1,0 +1,257 @@
+"""CKKS scheme on the HIP core.
+
+Conventions (frozen; the reference mixes several, see its
+scripts/exp_e33_real_ciphertext.py:138-175 for the one kept here):
+  ciphertext ct = (c0, c1) in coefficient form, [B, 2, l, N] uint64,
+  decrypt m = c0 + c1 s;  fresh encryption c1 = -a, c0 = a s + e + m;
+  key-switch key from s_from to s, one digit per prime (dnum = L) and one
+  special prime p: digit j = (b_j, a_j) over q_0..q_(L-1), p with
+  b_j = a_j s + e_j + g_j s_from, g_j = p on limb j and 0 elsewhere;
+  relinearisation key: s_from = s^2; Galois key for g: s_from = s(X^g).
+Sampling (host, numpy Generator): s ternary uniform in {-1, 0, 1}, errors
+rounded Gaussian sigma = 3.2, a uniform per limb."""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from ._lib import call
+from .params import CHAIN4, SPECIAL_P
+from .ring import RingContext, to_numpy
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@dataclass
+class CKKSParams:
+    N: int = 8192
+    primes: Sequence[int] = CHAIN4
+    special: int = SPECIAL_P
+    scale: float = 2.0 ** 40
+    sigma: float = 3.2
+
+    @property
+    def L(self) -> int:
+        return len(self.primes)
+
+
+@dataclass
+class Ciphertext:
+    data: torch.Tensor          # [B, 2, l, N] uint64, coefficient form
+    scale: float
+
+    @property
+    def level(self) -> int:
+        return self.data.shape[2]
+
+    @property
+    def batch(self) -> int:
+        return self.data.shape[0]
+
+
+@dataclass
+class SecretKey:
+    s: np.ndarray               # [N] int64 in {-1, 0, 1}
+    rns: torch.Tensor           # [K, N] uint64 over all primes (incl. special)
+
+
+@dataclass
+class SwitchKey:
+    coeff: torch.Tensor         # [L, 2, L+1, N] uint64, coefficient form (b_j, a_j)
+    prepared: torch.Tensor      # [L, 2, L+1, N, 2] uint64 (value, Shoup), evaluation order
+    galois: int = 0             # 0 for the relinearisation key
+
+
+class CKKS:
+    def __init__(self, params: CKKSParams = CKKSParams(), device: int = 0, seed: int = 0):
+        self.p = params
+        self.N = params.N
+        self.L = params.L
+        self.primes = [int(q) for q in params.primes] + [int(params.special)]
+        self.sp = self.L                      # ring index of the special prime
+        self.ring = RingContext(self.N, self.primes, device=device)
+        self.device = self.ring.device
+        self.rng = np.random.default_rng(seed)
+        self._ws = None
+
+    # -- sampling (host) ------------------------------------------------------
+    def ternary(self, *shape) -> np.ndarray:
+        return self.rng.integers(-1, 2, size=shape + (self.N,), dtype=np.int64)
+
+    def gaussian(self, *shape) -> np.ndarray:
+        return np.rint(self.rng.normal(0.0, self.p.sigma, size=shape + (self.N,))).astype(np.int64)
+
+    def uniform(self, *shape, limbs: Sequence[int]) -> torch.Tensor:
+        """[..., len(limbs), N] uniform in [0, q_i) per limb."""
+        out = np.empty(shape + (len(limbs), self.N), dtype=np.uint64)
+        for k, i in enumerate(limbs):
+            out[..., k, :] = self.rng.integers(0, self.primes[i], size=shape + (self.N,), dtype=np.uint64)
+        return self.ring.tensor(out)
+
+    def small_to_rns(self, x: np.ndarray, limbs: Sequence[int]) -> torch.Tensor:
+        """Small signed integers [..., N] -> [..., len(limbs), N] residues."""
+        x = np.asarray(x, dtype=np.int64)
+        out = np.empty(x.shape[:-1] + (len(limbs), self.N), dtype=np.uint64)
+        for k, i in enumerate(limbs):
+            q = self.primes[i]
+            out[..., k, :] = np.where(x < 0, np.uint64(q) - (-x).astype(np.uint64), x.astype(np.uint64))
+        return self.ring.tensor(out)
+
+    # -- keys -----------------------------------------------------------------
+    def keygen(self) -> SecretKey:
+        s = self.ternary()
+        return SecretKey(s=s, rns=self.small_to_rns(s, range(len(self.primes))))
+
+    def switch_key(self, sk: SecretKey, s_from: torch.Tensor, galois: int = 0,
+                   a: Optional[torch.Tensor] = None, e: Optional[np.ndarray] = None) -> SwitchKey:
+        """Hybrid key from s_from ([K, N] RNS over all primes) to sk.
+        a ([L, L+1, N]) and e ([L, N]) may be given (tests); else sampled."""
+        L, K = self.L, self.L + 1
+        if a is None:
+            a = self.uniform(L, limbs=range(K))
+        if e is None:
+            e = self.gaussian(L)
+        E = self.small_to_rns(e, range(K))
+        S = sk.rns.unsqueeze(0).expand(L, K, self.N).contiguous()
+        b = self.ring.add(self.ring.poly_mul(a, S), E)
+        gs = torch.zeros_like(b)
+        for j in range(L):
+            gs[j, j] = s_from[j]
+        g = np.zeros((L, K, self.N), dtype=np.uint64)
+        for j in range(L):
+            g[j, j, :] = self.p.special % self.primes[j]
+        b = self.ring.add(b, self.ring.mul(gs, self.ring.tensor(g)))
+        coeff = torch.stack([b, a], dim=1).contiguous()          # [L, 2, K, N]
+        prepared = torch.empty(L, 2, K, self.N, 2, dtype=torch.uint64, device=self.device)
+        work = coeff.clone()
+        call("ckks_key_prepare", self.ring.handle, _ptr(work), _ptr(prepared), L, K, self.sp, self.ring.stream())
+        return SwitchKey(coeff=coeff, prepared=prepared, galois=galois)
+
+    def relin_key(self, sk: SecretKey, **kw) -> SwitchKey:
+        s2 = self.ring.poly_mul(sk.rns, sk.rns)
+        return self.switch_key(sk, s2, 0, **kw)
+
+    def galois_key(self, sk: SecretKey, step: int, **kw) -> SwitchKey:
+        g = self.galois_elt(step)
+        sg = self.ring.automorphism(sk.rns, g)
+        return self.switch_key(sk, sg, g, **kw)
+
+    def galois_elt(self, step: int) -> int:
+        return pow(5, step % (self.N // 2), 2 * self.N)
+
+    # -- encode / encrypt ---------------------------------------------------------
+    def encode(self, z, level: Optional[int] = None, scale: Optional[float] = None) -> torch.Tensor:
+        l = self.L if level is None else level
+        return self.ring.encode(z, self.p.scale if scale is None else scale, l)
+
+    def decode(self, pt: torch.Tensor, scale: float) -> torch.Tensor:
+        return self.ring.decode(pt, scale)
+
+    def encrypt(self, pt: torch.Tensor, sk: SecretKey, scale: Optional[float] = None,
+                a: Optional[torch.Tensor] = None, e: Optional[np.ndarray] = None) -> Ciphertext:
+        """pt [B, l, N] -> ct [B, 2, l, N]:  c1 = -a, c0 = a s + e + m
+        (exp_e33_real_ciphertext.py:171-175)."""
+        B, l = pt.shape[0], pt.shape[1]
+        if a is None:
+            a = self.uniform(B, limbs=range(l))
+        if e is None:
+            e = self.gaussian(B)
+        S = sk.rns[:l].unsqueeze(0).expand(B, l, self.N).contiguous()
+        c0 = self.ring.add(self.ring.add(self.ring.poly_mul(a, S), self.small_to_rns(e, range(l))), pt)
+        c1 = self.ring.neg(a)
+        return Ciphertext(torch.stack([c0, c1], dim=1).contiguous(),
+                          self.p.scale if scale is None else scale)
+
+    def decrypt(self, ct: Ciphertext, sk: SecretKey) -> torch.Tensor:
+        d = ct.data
+        B, l = d.shape[0], d.shape[2]
+        S = sk.rns[:l].unsqueeze(0).expand(B, l, self.N).contiguous()
+        return self.ring.add(d[:, 0].contiguous(), self.ring.poly_mul(d[:, 1].contiguous(), S))
+
+    # -- evaluation -----------------------------------------------------------------
+    def workspace(self, batch: int, l: int) -> torch.Tensor:
+        nbytes = call("ckks_workspace_bytes", self.ring.handle, batch, l)
+        words = (nbytes + 7) // 8
+        if self._ws is None or self._ws.numel() < words:
+            self._ws = torch.empty(words, dtype=torch.uint64, device=self.device)
+        return self._ws
+
+    def hmult(self, a: Ciphertext, b: Ciphertext, rlk: SwitchKey) -> Ciphertext:
+        """HMult + relinearise + rescale (fused; verify_e2e_mult.py:78-102)."""
+        if a.data.shape != b.data.shape:
+            raise ValueError("ciphertext shapes differ")
+        B, l = a.batch, a.level
+        out = torch.empty(B, 2, l - 1, self.N, dtype=torch.uint64, device=self.device)
+        ws = self.workspace(B, l)
+        call("ckks_hmult", self.ring.handle, _ptr(a.data), _ptr(b.data), _ptr(rlk.prepared), self.L + 1,
+             _ptr(out), B, l, self.sp, _ptr(ws), self.ring.stream())
+        return Ciphertext(out, a.scale * b.scale / self.primes[l - 1])
+
+    def relin(self, d0: torch.Tensor, d1: torch.Tensor, d2: torch.Tensor, rlk: SwitchKey) -> torch.Tensor:
+        """(d0, d1, d2) [B, l, N] each -> [B, 2, l, N] = (d0, d1) + keyswitch(d2)
+        (verify_keyswitch.py:62-68 generalised to the hybrid key)."""
+        B, l = d2.shape[0], d2.shape[1]
+        d01 = torch.stack([d0, d1], dim=1).contiguous()
+        out = torch.empty(B, 2, l, self.N, dtype=torch.uint64, device=self.device)
+        ws = self.workspace(B, l)
+        call("ckks_relin", self.ring.handle, _ptr(d01), _ptr(d2.contiguous()), _ptr(rlk.prepared), self.L + 1,
+             _ptr(out), B, l, self.sp, _ptr(ws), self.ring.stream())
+        return out
+
+    def keyswitch(self, d2: torch.Tensor, key: SwitchKey) -> torch.Tensor:
+        B, l = d2.shape[0], d2.shape[1]
+        out = torch.empty(B, 2, l, self.N, dtype=torch.uint64, device=self.device)
+        ws = self.workspace(B, l)
+        call("ckks_relin", self.ring.handle, None, _ptr(d2.contiguous()), _ptr(key.prepared), self.L + 1,
+             _ptr(out), B, l, self.sp, _ptr(ws), self.ring.stream())
+        return out
+
+    def tensor(self, a: Ciphertext, b: Ciphertext):
+        """(d0, d1, d2) in coefficient form."""
+        A0, A1 = a.data[:, 0].contiguous(), a.data[:, 1].contiguous()
+        B0, B1 = b.data[:, 0].contiguous(), b.data[:, 1].contiguous()
+        r = self.ring
+        return (r.poly_mul(A0, B0), r.add(r.poly_mul(A0, B1), r.poly_mul(A1, B0)), r.poly_mul(A1, B1))
+
+    def rescale(self, ct: Ciphertext) -> Ciphertext:
+        l = ct.level
+        return Ciphertext(self.ring.rescale(ct.data), ct.scale / self.primes[l - 1])
+
+    def rotate(self, ct: Ciphertext, gk: SwitchKey) -> Ciphertext:
+        """Slot rotation by the step gk was made for."""
+        B, l = ct.batch, ct.level
+        out = torch.empty_like(ct.data)
+        ws = self.workspace(B, l)
+        call("ckks_rotate", self.ring.handle, _ptr(ct.data), _ptr(gk.prepared), self.L + 1, _ptr(out), B, l,
+             self.sp, gk.galois, _ptr(ws), self.ring.stream())
+        return Ciphertext(out, ct.scale)
+
+    def add(self, a: Ciphertext, b: Ciphertext) -> Ciphertext:
+        return Ciphertext(self.ring.add(a.data, b.data), a.scale)
+
+    def sub(self, a: Ciphertext, b: Ciphertext) -> Ciphertext:
+        return Ciphertext(self.ring.sub(a.data, b.data), a.scale)
+
+    def mul_plain(self, ct: Ciphertext, pt: torch.Tensor, pt_scale: float) -> Ciphertext:
+        """ct * pt (pt [B, l, N] or [1, l, N] coefficient form), no rescale."""
+        B, l = ct.batch, ct.level
+        P = pt.expand(B, l, self.N).contiguous() if pt.shape[0] != B else pt
+        c0 = self.ring.poly_mul(ct.data[:, 0].contiguous(), P)
+        c1 = self.ring.poly_mul(ct.data[:, 1].contiguous(), P)
+        return Ciphertext(torch.stack([c0, c1], 1).contiguous(), ct.scale * pt_scale)
+
+    def add_plain(self, ct: Ciphertext, pt: torch.Tensor) -> Ciphertext:
+        B, l = ct.batch, ct.level
+        P = pt.expand(B, l, self.N).contiguous() if pt.shape[0] != B else pt
+        return Ciphertext(torch.stack([self.ring.add(ct.data[:, 0].contiguous(), P), ct.data[:, 1]], 1).contiguous(),
+                          ct.scale)
+
+    def drop_levels(self, ct: Ciphertext, l: int) -> Ciphertext:
+        """Keep the first l limbs (modulus reduction without division)."""
+        return Ciphertext(ct.data[:, :, :l].contiguous(), ct.scale)

@@ -1,0 +1,252 @@
+"""Drop-in replacements for the reference's golden-model functions.
+
+Same names, argument meaning and return type (lists of ints / numpy arrays)
+as the Python functions the reference's scripts call; the work runs in the
+HIP library on the GPU.  Contexts are cached per (N, q, root, ring).
+
+Reference surface -> here:
+  tb/ref/gen_poly_mult_vectors_n12.py  bit_rev_perm, ntt, intt,
+                                       poly_mult_via_ntt, cyclic_conv_direct
+  scripts/verify_keyswitch.py          poly_add, poly_sub, poly_mult, relin
+  scripts/verify_rescale.py            rescale_coeff, rescale_poly
+  scripts/verify_e2e_mult.py           e2e_mult (exact modulus switch instead
+                                       of the toy rescale, see below)
+  scripts/exp_e24_rotation_bootstrap.py automorphism_exact, automorphism_key_exact
+  scripts/verify_multi_slot_simd.py    neg_fft, neg_ifft, encode, decode
+"""
+from __future__ import annotations
+
+from functools import lru_cache
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from .ring import RingContext, bitrev_, mod_switch, to_numpy
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+@lru_cache(maxsize=64)
+def _ring(N: int, q: int, root, cyclic: bool) -> RingContext:
+    return RingContext(N, [q], roots=None if root is None else [root], cyclic=cyclic,
+                       device=torch.cuda.current_device())
+
+
+def _t(a, ctx: RingContext):
+    return ctx.tensor(np.asarray(a, dtype=np.uint64).reshape(1, -1))
+
+
+def _list(t) -> list:
+    return [int(v) for v in to_numpy(t).reshape(-1)]
+
+
+def bit_rev_perm(a: Sequence[int], n: int) -> list:
+    """gen_poly_mult_vectors_n12.py:54-65"""
+    assert len(a) == n and n & (n - 1) == 0
+    x = torch.from_numpy(np.asarray(a, dtype=np.uint64).copy()).to(_dev())
+    return _list(bitrev_(x.view(1, n)))
+
+
+def ntt(a: Sequence[int], omega: int, N: int, q: int) -> list:
+    """Cyclic transform X[k] = sum_j a_j omega^(jk), natural order
+    (gen_poly_mult_vectors_n12.py:68-82)."""
+    ctx = _ring(N, q, omega % q, True)
+    return _list(ctx.ntt_(_t(a, ctx), natural=True))
+
+
+def intt(a: Sequence[int], omega_inv: int, N: int, q: int) -> list:
+    """gen_poly_mult_vectors_n12.py:85-100"""
+    ctx = _ring(N, q, pow(omega_inv, -1, q), True)
+    return _list(ctx.intt_(_t(a, ctx), natural=True))
+
+
+def poly_mult_via_ntt(a, b, N, q, omega, omega_inv) -> list:
+    """Cyclic product (gen_poly_mult_vectors_n12.py:103-107)."""
+    assert omega * omega_inv % q == 1
+    ctx = _ring(N, q, omega % q, True)
+    return _list(ctx.poly_mul(_t(a, ctx), _t(b, ctx)))
+
+
+def cyclic_conv_direct(a, b, N, q) -> list:
+    """Cyclic convolution (gen_poly_mult_vectors_n12.py:110-117); computed
+    through the transform with the default N-th root of q."""
+    ctx = _ring(N, q, None, True)
+    return _list(ctx.poly_mul(_t(a, ctx), _t(b, ctx)))
+
+
+def poly_add(a, b, q) -> list:
+    """verify_keyswitch.py:40-41"""
+    return _list(_ew(a, b, q, 1))
+
+
+def poly_sub(a, b, q) -> list:
+    """verify_keyswitch.py:44-45"""
+    return _list(_ew(a, b, q, 2))
+
+
+def _ntt_ok(n, q):
+    return n >= 2 and n & (n - 1) == 0 and (q - 1) % (2 * n) == 0
+
+
+def _ew(a, b, q, op):
+    n = len(a)
+    ctx = _ring(n, q, None, not _ntt_ok(n, q))
+    return ctx._pw(_t(a, ctx), _t(b, ctx) if b is not None else None, op, 0, None)
+
+
+def poly_mult(a, b, q) -> list:
+    """Negacyclic product mod (X^n + 1, q) (verify_keyswitch.py:48-59).
+    Needs 2n | q - 1 (an NTT-friendly prime)."""
+    n = len(a)
+    if not _ntt_ok(n, q):
+        raise ValueError(f"poly_mult on the GPU needs a prime with 2n | q-1 (n={n}, q={q})")
+    ctx = _ring(n, q, None, False)
+    return _list(ctx.poly_mul(_t(a, ctx), _t(b, ctx)))
+
+
+def relin(e0, e1, e2, a, b, q):
+    """f0 = e0 + e2*b, f1 = e1 - e2*a (verify_keyswitch.py:62-68)."""
+    n = len(e0)
+    ctx = _ring(n, q, None, False)
+    E2 = _t(e2, ctx)
+    f0 = ctx.add(_t(e0, ctx), ctx.poly_mul(E2, _t(b, ctx)))
+    f1 = ctx.sub(_t(e1, ctx), ctx.poly_mul(E2, _t(a, ctx)))
+    return _list(f0), _list(f1)
+
+
+def rescale_coeff(c: int, q_in: int, q_out: int) -> int:
+    """round-half-up(c q_out / q_in) mod q_out (verify_rescale.py:38-53)."""
+    return rescale_poly([c], q_in, q_out)[0]
+
+
+def rescale_poly(c, q_in, q_out) -> list:
+    """verify_rescale.py:56-57"""
+    x = torch.from_numpy(np.asarray(c, dtype=np.uint64)).to(_dev())
+    return _list(mod_switch(x, q_in, q_out))
+
+
+def e2e_mult(c0_1, c1_1, c0_2, c1_2, a, b, q_in, q_out, scale_bits=None):
+    """Tensor product -> relinearise -> rescale on single-prime ciphertexts
+    (verify_e2e_mult.py:78-102).  The rescale is the exact modulus switch
+    q_in -> q_out of verify_rescale.py; the toy division by 2^scale_bits
+    (verify_e2e_mult.py:70-75) is not reproduced, so scale_bits must be None."""
+    if scale_bits is not None:
+        raise ValueError("e2e_mult uses the exact modulus switch; pass scale_bits=None")
+    n = len(c0_1)
+    ctx = _ring(n, q_in, None, False)
+    A0, A1, B0, B1 = (_t(v, ctx) for v in (c0_1, c1_1, c0_2, c1_2))
+    m0 = ctx.poly_mul(A0, B0)
+    m1 = ctx.add(ctx.poly_mul(A0, B1), ctx.poly_mul(A1, B0))
+    m2 = ctx.poly_mul(A1, B1)
+    f0 = ctx.add(m0, ctx.poly_mul(m2, _t(b, ctx)))
+    f1 = ctx.sub(m1, ctx.poly_mul(m2, _t(a, ctx)))
+    return _list(mod_switch(f0, q_in, q_out)), _list(mod_switch(f1, q_in, q_out))
+
+
+def automorphism_exact(ct, nu, n, q):
+    """(c0, c1) -> (c0(X^nu), c1(X^nu)) mod X^n + 1 for odd nu
+    (exp_e24_rotation_bootstrap.py:143-162)."""
+    if nu % 2 == 0:
+        raise ValueError("automorphism needs an odd Galois element")
+    ctx = _ring(n, q, None, False)
+    x = ctx.tensor(np.asarray([np.asarray(ct[0]) % q, np.asarray(ct[1]) % q], dtype=np.uint64).reshape(2, 1, n))
+    y = to_numpy(ctx.automorphism(x, nu % (2 * n)))
+    return y[0, 0].astype(np.int64), y[1, 0].astype(np.int64)
+
+
+def automorphism_key_exact(s, nu, n, q):
+    """exp_e24_rotation_bootstrap.py:165-176"""
+    return automorphism_exact((s, np.zeros(n, dtype=np.int64)), nu, n, q)[0]
+
+
+# -- natural-order canonical embedding (verify_multi_slot_simd.py) -----------
+
+@lru_cache(maxsize=16)
+def _emb_ring(n: int) -> RingContext:
+    # any NTT-friendly prime works: only the floating-point tables are used
+    return RingContext(n, [_friendly_prime(n)], device=torch.cuda.current_device())
+
+
+def _friendly_prime(n):
+    q = (1 << 44) // (2 * n) * (2 * n) + 1
+    while not _isprime(q):
+        q += 2 * n
+    return q
+
+
+def _isprime(q):
+    if q < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if q % p == 0:
+            return q == p
+    d, s = q - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, q)
+        if x in (1, q - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % q
+            if x == q - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def _natural_gather(n: int):
+    """Slot k of the natural order sits at Phi^(2k+1).  With e = 2k+1:
+    e = 1 mod 4 -> FFT-order slot (e-1)/4; e = 3 mod 4 -> conjugate of
+    FFT-order slot (2n - e - 1)/4 (m is real)."""
+    e = 2 * np.arange(n) + 1
+    direct = (e % 4) == 1
+    idx = np.where(direct, (e - 1) // 4, (2 * n - e - 1) // 4)
+    return idx, ~direct
+
+
+def neg_fft(m):
+    """Evaluation at Phi^(2k+1), k < n, natural order, for real m
+    (verify_multi_slot_simd.py:32-39)."""
+    m = np.asarray(m, dtype=np.float64)
+    n = m.shape[-1]
+    ctx = _emb_ring(n)
+    v = ctx.embed(torch.from_numpy(m.reshape(-1, n).copy()), order=1)
+    idx, conj = _natural_gather(n)
+    out = v[:, torch.from_numpy(idx).to(v.device)]
+    cj = torch.from_numpy(conj).to(v.device)
+    out = torch.where(cj, out.conj(), out)
+    return to_numpy(out).reshape(m.shape[:-1] + (n,))
+
+
+def neg_ifft(V):
+    """Inverse of ``neg_fft`` for Hermitian V (V[n-1-k] = conj V[k]), i.e.
+    real polynomials (verify_multi_slot_simd.py:42-52)."""
+    V = np.asarray(V, dtype=np.complex128)
+    n = V.shape[-1]
+    ctx = _emb_ring(n)
+    fft_order = V.reshape(-1, n)[:, 0::2]   # k = 2k' has e = 4k' + 1
+    m = ctx.embed_inv(torch.from_numpy(np.ascontiguousarray(fft_order)), order=1)
+    return to_numpy(m).reshape(V.shape[:-1] + (n,)).astype(np.complex128)
+
+
+def encode(v, scale, n):
+    """verify_multi_slot_simd.py:55-84"""
+    nslot = len(v)
+    V = np.zeros(n, dtype=np.complex128)
+    for k in range(nslot):
+        V[k] = scale * v[k]
+        sk = (n - 1 - k) % n
+        if sk != k and sk >= nslot:
+            V[sk] = np.conj(V[k])
+    return neg_ifft(V).real
+
+
+def decode(m, scale, n, nslot):
+    """verify_multi_slot_simd.py:87-91"""
+    return neg_fft(np.asarray(m, dtype=np.float64)).real[:nslot] / scale

@@ -1,0 +1,47 @@
+"""Parameter sets and the reference's on-disk parameter formats.
+
+CHAIN4: the four 45-bit primes of rtl/m2/modulus_chain_chain4.json (all
+1 mod 2^15).  SPECIAL_P: the next prime of the same form
+2^44 + 32768 j + 1 (j = 22) above the chain, used as the hybrid key-switch
+special prime.  CONFIG1_Q / CONFIG1_PSI: the largest 45-bit prime that is
+1 mod 8192 and its smallest primitive 8192-th root (N = 4096 product)."""
+from __future__ import annotations
+
+import json
+from typing import List
+
+CHAIN4 = (17592186634241, 17592186372097, 17592186273793, 17592186175489)
+SPECIAL_P = 17592186765313
+CONFIG1_Q = 35184371884033
+CONFIG1_PSI = 7388705119
+
+
+def load_chain_json(path: str) -> List[dict]:
+    """Modulus chain in the reference's JSON schema
+    (rtl/m2/modulus_chain_chain4.json: list of {q, primitive_root, omega,
+    omega_inv, N, N_INV, ...})."""
+    with open(path, encoding="utf-8") as f:
+        chain = json.load(f)
+    for e in chain:
+        q, w, N = int(e["q"]), int(e["omega"]), int(e["N"])
+        if pow(w, N, q) != 1 or w * int(e["omega_inv"]) % q != 1 or N * int(e["N_INV"]) % q != 1:
+            raise ValueError(f"inconsistent chain entry for q={q}")
+    return chain
+
+
+def save_chain_json(path: str, entries: List[dict]) -> None:
+    with open(path, "w", encoding="utf-8") as f:
+        json.dump(entries, f, indent=2)
+
+
+def read_hex_table(path: str) -> List[int]:
+    """$readmemh twiddle ROM: one hex word per line
+    (scripts/gen_omega_tables.py:244-247)."""
+    with open(path) as f:
+        return [int(l.strip(), 16) for l in f if l.strip() and not l.lstrip().startswith("//")]
+
+
+def write_hex_table(path: str, values, width: int = 12) -> None:
+    with open(path, "w") as f:
+        for v in values:
+            f.write(f"{int(v):0{width}x}\n")

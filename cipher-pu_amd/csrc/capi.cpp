@@ -1,0 +1,581 @@
+// C ABI of the CKKS evaluation core: contexts, argument checking and the
+// composition of kernels into the reference's operations.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/ckks_core.h"
+#include "launch.h"
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((u128)a * b) % q); }
+
+uint64_t powmod(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) r = mulmod(r, b, q);
+        b = mulmod(b, b, q);
+        e >>= 1;
+    }
+    return r;
+}
+
+uint64_t invmod(uint64_t a, uint64_t q) { return powmod(a % q, q - 2, q); }
+
+bool is_prime(uint64_t n) {
+    if (n < 2) return false;
+    static const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (uint64_t p : bases)
+        if (n % p == 0) return n == p;
+    uint64_t d = n - 1;
+    int s = 0;
+    while (!(d & 1)) { d >>= 1; ++s; }
+    for (uint64_t a : bases) {
+        uint64_t x = powmod(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int r = 1; r < s; ++r) {
+            x = mulmod(x, x, n);
+            if (x == n - 1) { comp = false; break; }
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+
+// smallest generator of Z_q^* (tb/ref/gen_ntt_reference.py:38-61)
+uint64_t primitive_root(uint64_t q) {
+    std::vector<uint64_t> fs;
+    uint64_t n = q - 1;
+    for (uint64_t d = 2; d * d <= n; d += (d == 2 ? 1 : 2)) {
+        if (n % d == 0) {
+            fs.push_back(d);
+            while (n % d == 0) n /= d;
+        }
+    }
+    if (n > 1) fs.push_back(n);
+    for (uint64_t g = 2;; ++g) {
+        bool ok = true;
+        for (uint64_t f : fs)
+            if (powmod(g, (q - 1) / f, q) == 1) { ok = false; break; }
+        if (ok) return g;
+    }
+}
+
+uint64_t shoup(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+
+uint32_t bitrev(uint32_t i, int bits) {
+    uint32_t r = 0;
+    for (int b = 0; b < bits; ++b) { r = (r << 1) | (i & 1); i >>= 1; }
+    return r;
+}
+
+int bitlen(uint64_t x) { int l = 0; while (x) { ++l; x >>= 1; } return l; }
+
+}  // namespace
+
+struct ckks_ctx {
+    int device = 0, logN = 0, K = 0, flags = 0;
+    uint64_t N = 0;
+    uint64_t primes[CKKS_MAX_PRIMES] = {}, roots[CKKS_MAX_PRIMES] = {};
+    PrimeConst hpc[CKKS_MAX_PRIMES] = {};
+    PrimeConst* dpc = nullptr;
+    ulonglong2* dtw = nullptr;
+    double2* fft_tw = nullptr;    // exp(2 pi i k / n), k < n/2, n = N/2
+    double2* fft_zeta = nullptr;  // exp(i pi j / N), j < n
+    int* slot_k = nullptr;        // (5^j mod 2N - 1)/4
+    int ntt_threads = 0, ks_threads = 0;
+    DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN}; }
+};
+
+namespace {
+
+#define HIPCHK(x)                                  \
+    do {                                           \
+        if ((x) != hipSuccess) return CKKS_ERR_HIP; \
+    } while (0)
+
+inline bool misaligned(const void* p) { return ((uintptr_t)p & 7) != 0; }
+
+// CKKS_OK, an error, or NOTHING_TO_DO for a valid call with zero rows
+const int NOTHING_TO_DO = 1;
+
+int check_rows(const ckks_ctx* ctx, int n_polys, int n_limbs, int prime0) {
+    if (!ctx) return CKKS_ERR_NULL;
+    if (n_polys < 0 || n_limbs < 1 || prime0 < 0 || prime0 + n_limbs > ctx->K) return CKKS_ERR_BAD_SHAPE;
+    return n_polys == 0 ? NOTHING_TO_DO : CKKS_OK;
+}
+
+LimbMap limb_map(int prime0, int n) {
+    LimbMap m;
+    for (int i = 0; i < CKKS_MAX_PRIMES; ++i) m.idx[i] = i < n ? prime0 + i : 0;
+    return m;
+}
+
+hipStream_t S(void* s) { return (hipStream_t)s; }
+
+int set_device(const ckks_ctx* ctx) { return hipSetDevice(ctx->device) == hipSuccess ? CKKS_OK : CKKS_ERR_HIP; }
+
+// constants for ModDown by the special prime sp and rescale by limb l-1
+ModDownConst moddown_const(const ckks_ctx* c, int l, int sp) {
+    ModDownConst m;
+    memset(&m, 0, sizeof(m));
+    const uint64_t p = sp >= 0 ? c->primes[sp] : 1;
+    const uint64_t qL = c->primes[l - 1];
+    for (int i = 0; i < l && i < CKKS_MAX_PRIMES; ++i) {
+        const uint64_t qi = c->primes[i];
+        if (sp >= 0) {
+            m.hp[i] = (p >> 1) % qi;
+            m.pinv[i] = invmod(p, qi);
+            m.pinv_s[i] = shoup(m.pinv[i], qi);
+            m.p_mod[i] = p % qi;
+            m.p_mod_s[i] = shoup(m.p_mod[i], qi);
+        }
+        if (i < l - 1) {
+            m.hq[i] = (qL >> 1) % qi;
+            m.qinv[i] = invmod(qL, qi);
+            m.qinv_s[i] = shoup(m.qinv[i], qi);
+        }
+    }
+    return m;
+}
+
+// little-endian multiword helpers for the CRT constants
+void mw_mul_small(uint64_t* a, int w, uint64_t m) {
+    u128 carry = 0;
+    for (int k = 0; k < w; ++k) {
+        u128 t = (u128)a[k] * m + carry;
+        a[k] = (uint64_t)t;
+        carry = t >> 64;
+    }
+}
+
+uint64_t mw_mod_small(const uint64_t* a, int w, uint64_t m) {
+    u128 r = 0;
+    for (int k = w - 1; k >= 0; --k) r = ((r << 64) | a[k]) % m;
+    return (uint64_t)r;
+}
+
+int crt_const(const ckks_ctx* c, int prime0, int l, CrtConst* cc) {
+    if (l > CRT_MAX_LIMBS) return CKKS_ERR_BAD_SHAPE;
+    memset(cc, 0, sizeof(*cc));
+    cc->l = l;
+    int bits = 2;
+    for (int i = 0; i < l; ++i) bits += bitlen(c->primes[prime0 + i]);
+    cc->words = (bits + 63) / 64;
+    if (cc->words > CRT_MAX_WORDS) return CKKS_ERR_BAD_SHAPE;
+    cc->Q[0] = 1;
+    for (int i = 0; i < l; ++i) mw_mul_small(cc->Q, cc->words, c->primes[prime0 + i]);
+    for (int i = 0; i < l; ++i) {
+        const uint64_t qi = c->primes[prime0 + i];
+        cc->qhat[i][0] = 1;
+        for (int j = 0; j < l; ++j)
+            if (j != i) mw_mul_small(cc->qhat[i], cc->words, c->primes[prime0 + j]);
+        cc->qhat_inv[i] = invmod(mw_mod_small(cc->qhat[i], cc->words, qi), qi);
+        cc->qhat_inv_s[i] = shoup(cc->qhat_inv[i], qi);
+    }
+    for (int k = 0; k < cc->words; ++k)
+        cc->half_Q[k] = (cc->Q[k] >> 1) | (k + 1 < cc->words ? cc->Q[k + 1] << 63 : 0);
+    return CKKS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ckks_version(void) { return "cipherpu-ckks 0.1 gfx950"; }
+
+int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* primes, const uint64_t* roots,
+                    int flags, int device) {
+    if (!out || !primes) return CKKS_ERR_NULL;
+    *out = nullptr;
+    if (N < 2 || N > 16384 || (N & (N - 1))) return CKKS_ERR_BAD_N;
+    if (n_primes < 1 || n_primes > CKKS_MAX_PRIMES) return CKKS_ERR_BAD_ARG;
+    const bool cyclic = flags & CKKS_CYCLIC;
+    const uint64_t order = cyclic ? N : 2 * N;
+    for (int i = 0; i < n_primes; ++i) {
+        const uint64_t q = primes[i];
+        if (q < 3 || q >= (1ull << 45) || !is_prime(q) || (q - 1) % order) return CKKS_ERR_BAD_MODULUS;
+        if (roots) {
+            const uint64_t r = roots[i] % q;
+            if (powmod(r, order, q) != 1 || powmod(r, order / 2, q) != q - 1) return CKKS_ERR_BAD_MODULUS;
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return CKKS_ERR_NO_DEVICE;
+    ckks_ctx* c = new ckks_ctx();
+    c->device = device;
+    c->N = N;
+    c->logN = bitlen(N) - 1;
+    c->K = n_primes;
+    c->flags = flags;
+    const int logN = c->logN;
+    std::vector<ulonglong2> tw((size_t)n_primes * 2 * N);
+    for (int p = 0; p < n_primes; ++p) {
+        const uint64_t q = primes[p];
+        const uint64_t r = roots ? roots[p] % q : powmod(primitive_root(q), (q - 1) / order, q);
+        c->primes[p] = q;
+        c->roots[p] = r;
+        ulonglong2* f = &tw[(size_t)p * 2 * N];
+        ulonglong2* iv = f + N;
+        f[0] = make_ulonglong2(1, shoup(1, q));
+        iv[0] = f[0];
+        for (uint64_t k = 1; k < N; ++k) {
+            int s = bitlen(k) - 1;
+            uint64_t i = k - (1ull << s);
+            uint64_t e;
+            if (cyclic)
+                e = (uint64_t)bitrev((uint32_t)i, s) * (N >> (s + 1));   // omega^(br_s(i) N / 2^(s+1))
+            else
+                e = bitrev((uint32_t)k, logN);                          // psi^(br_logN(k))
+            const uint64_t w = powmod(r, e, q);
+            const uint64_t wi = invmod(w, q);
+            f[k] = make_ulonglong2(w, shoup(w, q));
+            iv[k] = make_ulonglong2(wi, shoup(wi, q));
+        }
+        PrimeConst& pc = c->hpc[p];
+        pc.q = q;
+        pc.two_q = 2 * q;
+        pc.ell = bitlen(q);
+        pc.mu = (uint64_t)(((u128)1 << (2 * pc.ell)) / q);
+        pc.mu64 = (uint64_t)(~0ull / q);
+        pc.ninv = invmod(N % q, q);
+        pc.ninv_s = shoup(pc.ninv, q);
+        pc.ninv_w1 = mulmod(pc.ninv, iv[1].x, q);
+        pc.ninv_w1_s = shoup(pc.ninv_w1, q);
+    }
+    bool ok = hipSetDevice(device) == hipSuccess;
+    ok = ok && hipMalloc(&c->dpc, sizeof(PrimeConst) * n_primes) == hipSuccess;
+    ok = ok && hipMalloc(&c->dtw, tw.size() * sizeof(ulonglong2)) == hipSuccess;
+    ok = ok && hipMemcpy(c->dpc, c->hpc, sizeof(PrimeConst) * n_primes, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->dtw, tw.data(), tw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !cyclic && N >= 4) {
+        const uint64_t n = N / 2;
+        std::vector<double2> ft(n / 2 > 0 ? n / 2 : 1), zt(n);
+        std::vector<int> sk(n);
+        const long double PI = 3.141592653589793238462643383279502884L;
+        for (uint64_t k = 0; k < n / 2; ++k) {
+            const long double a = 2.0L * PI * (long double)k / (long double)n;
+            ft[k] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        for (uint64_t j = 0; j < n; ++j) {
+            const long double a = PI * (long double)j / (long double)N;
+            zt[j] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        uint64_t e = 1;
+        for (uint64_t j = 0; j < n; ++j) {
+            sk[j] = (int)((e - 1) / 4);
+            e = e * 5 % (2 * N);
+        }
+        ok = ok && hipMalloc(&c->fft_tw, ft.size() * sizeof(double2)) == hipSuccess;
+        ok = ok && hipMalloc(&c->fft_zeta, zt.size() * sizeof(double2)) == hipSuccess;
+        ok = ok && hipMalloc(&c->slot_k, sk.size() * sizeof(int)) == hipSuccess;
+        ok = ok && hipMemcpy(c->fft_tw, ft.data(), ft.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(c->fft_zeta, zt.data(), zt.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(c->slot_k, sk.data(), sk.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) {
+        ckks_ctx_destroy(c);
+        return CKKS_ERR_HIP;
+    }
+    *out = c;
+    return CKKS_OK;
+}
+
+int ckks_ctx_destroy(ckks_ctx* c) {
+    if (!c) return CKKS_ERR_NULL;
+    hipSetDevice(c->device);
+    if (c->dpc) hipFree(c->dpc);
+    if (c->dtw) hipFree(c->dtw);
+    if (c->fft_tw) hipFree(c->fft_tw);
+    if (c->fft_zeta) hipFree(c->fft_zeta);
+    if (c->slot_k) hipFree(c->slot_k);
+    delete c;
+    return CKKS_OK;
+}
+
+int ckks_ctx_root(const ckks_ctx* c, int prime, uint64_t* root) {
+    if (!c || !root) return CKKS_ERR_NULL;
+    if (prime < 0 || prime >= c->K) return CKKS_ERR_BAD_SHAPE;
+    *root = c->roots[prime];
+    return CKKS_OK;
+}
+
+int ckks_ctx_set_threads(ckks_ctx* c, int ntt_threads, int ks_threads) {
+    if (!c) return CKKS_ERR_NULL;
+    c->ntt_threads = ntt_threads;
+    c->ks_threads = ks_threads;
+    return CKKS_OK;
+}
+
+static int do_ntt(const ckks_ctx* c, uint64_t* data, int n_polys, int n_limbs, int prime0, int flags, void* stream,
+                  bool inv) {
+    int st = check_rows(c, n_polys, n_limbs, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!data) return CKKS_ERR_NULL;
+    if (misaligned(data)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const int rows = n_polys * n_limbs;
+    if (inv && (flags & CKKS_NATURAL_ORDER)) HIPCHK(launch_bitrev(data, rows, c->logN, S(stream)));
+    HIPCHK(launch_ntt(t, data, data, n_polys, n_limbs, limb_map(prime0, n_limbs), inv, false, c->ntt_threads,
+                      S(stream)));
+    if (!inv && (flags & CKKS_NATURAL_ORDER)) HIPCHK(launch_bitrev(data, rows, c->logN, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_ntt_fwd(const ckks_ctx* c, uint64_t* data, int n_polys, int n_limbs, int prime0, int flags, void* stream) {
+    return do_ntt(c, data, n_polys, n_limbs, prime0, flags, stream, false);
+}
+
+int ckks_ntt_inv(const ckks_ctx* c, uint64_t* data, int n_polys, int n_limbs, int prime0, int flags, void* stream) {
+    return do_ntt(c, data, n_polys, n_limbs, prime0, flags, stream, true);
+}
+
+int ckks_poly_mul(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, uint64_t* out, int n_polys, int n_limbs,
+                  int prime0, void* stream) {
+    int st = check_rows(c, n_polys, n_limbs, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!a || !b || !out) return CKKS_ERR_NULL;
+    if (misaligned(a) || misaligned(b) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_polymul(c->tables(), a, b, out, n_polys, n_limbs, limb_map(prime0, n_limbs), c->ntt_threads,
+                          S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_pointwise(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, uint64_t* out, int n_polys, int n_limbs,
+                   int prime0, int op, void* stream) {
+    int st = check_rows(c, n_polys, n_limbs, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (op < 0 || op > 3) return CKKS_ERR_BAD_ARG;
+    if (!a || !out || (op != 3 && !b)) return CKKS_ERR_NULL;
+    if (misaligned(a) || misaligned(out) || (b && misaligned(b))) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_pointwise(c->tables(), a, b, out, n_polys, n_limbs, limb_map(prime0, n_limbs), op, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_rescale(const ckks_ctx* c, const uint64_t* in, uint64_t* out, int n_polys, int l, int prime0, void* stream) {
+    int st = check_rows(c, n_polys, l, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (l < 2) return CKKS_ERR_BAD_SHAPE;
+    if (!in || !out) return CKKS_ERR_NULL;
+    if (misaligned(in) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    // constants are indexed by limb; shift the prime window to prime0
+    ckks_ctx shifted = *c;
+    for (int i = 0; i < l; ++i) shifted.primes[i] = c->primes[prime0 + i];
+    const ModDownConst mc = moddown_const(&shifted, l, -1);
+    HIPCHK(launch_rescale(c->tables(), in, out, n_polys, l, limb_map(prime0, l), mc, S(stream)));
+    return CKKS_OK;
+}
+
+uint64_t ckks_workspace_bytes(const ckks_ctx* c, int batch, int l) {
+    if (!c || batch < 0 || l < 1) return 0;
+    return (uint64_t)batch * c->N * 8 * (uint64_t)(10 * l + 2);
+}
+
+int ckks_key_prepare(const ckks_ctx* c, const uint64_t* key_coeff, uint64_t* key_out, int dnum, int key_limbs,
+                     int special, void* stream) {
+    if (!c || !key_coeff || !key_out) return CKKS_ERR_NULL;
+    if (dnum < 1 || key_limbs < 2 || key_limbs - 1 > c->K || special < 0 || special >= c->K)
+        return CKKS_ERR_BAD_SHAPE;
+    if (misaligned(key_coeff) || misaligned(key_out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    LimbMap m = limb_map(0, key_limbs);
+    m.idx[key_limbs - 1] = special;
+    const DevTables t = c->tables();
+    uint64_t* tmp = const_cast<uint64_t*>(key_coeff);   // transformed in place (documented)
+    HIPCHK(launch_ntt(t, tmp, tmp, dnum * 2, key_limbs, m, false, true, c->ntt_threads, S(stream)));
+    HIPCHK(launch_shoup_pairs(t, tmp, (ulonglong2*)key_out, dnum * 2, key_limbs, m, S(stream)));
+    return CKKS_OK;
+}
+
+static int ks_common_check(const ckks_ctx* c, const void* key, int key_limbs, const void* out, int batch, int l,
+                           int special, const void* ws) {
+    if (!c) return CKKS_ERR_NULL;
+    if (batch < 0 || l < 1 || l + 1 > CKKS_MAX_PRIMES || l > key_limbs - 1 || key_limbs - 1 > c->K)
+        return CKKS_ERR_BAD_SHAPE;
+    if (batch == 0) return NOTHING_TO_DO;
+    if (!key || !out || !ws) return CKKS_ERR_NULL;
+    if (special < 0 || special >= c->K || special < l) return CKKS_ERR_BAD_SHAPE;
+    if (c->flags & CKKS_CYCLIC) return CKKS_ERR_BAD_ARG;
+    if (c->logN < 12) return CKKS_ERR_BAD_N;
+    if (misaligned(key) || misaligned(out) || misaligned(ws)) return CKKS_ERR_BAD_ALIGNMENT;
+    return CKKS_OK;
+}
+
+int ckks_relin(const ckks_ctx* c, const uint64_t* d01, const uint64_t* d2, const uint64_t* key, int key_limbs,
+               uint64_t* out, int batch, int l, int special, void* workspace, void* stream) {
+    int st = ks_common_check(c, key, key_limbs, out, batch, l, special, workspace);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!d2) return CKKS_ERR_NULL;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const LimbMap m = limb_map(0, l);
+    const ModDownConst mc = moddown_const(c, l, special);
+    uint64_t* acc = (uint64_t*)workspace;
+    const size_t LN = (size_t)l * c->N;
+    HIPCHK(launch_keyswitch(t, d2, LN, nullptr, nullptr, (const ulonglong2*)key, key_limbs, acc, batch, l, m, special,
+                            c->ks_threads, S(stream)));
+    HIPCHK(launch_moddown_rescale(t, acc, d01, 3, out, batch, l, m, special, mc, false, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const uint64_t* key, int key_limbs,
+               uint64_t* out, int batch, int l, int special, void* workspace, void* stream) {
+    int st = ks_common_check(c, key, key_limbs, out, batch, l, special, workspace);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (l < 2) return CKKS_ERR_BAD_SHAPE;
+    if (!a || !b) return CKKS_ERR_NULL;
+    if (misaligned(a) || misaligned(b)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const LimbMap m = limb_map(0, l);
+    const ModDownConst mc = moddown_const(c, l, special);
+    const size_t LN = (size_t)l * c->N, B = (size_t)batch;
+    uint64_t* ea = (uint64_t*)workspace;
+    uint64_t* eb = ea + B * 2 * LN;
+    uint64_t* d01 = eb + B * 2 * LN;
+    uint64_t* d2e = d01 + B * 2 * LN;
+    uint64_t* d2c = d2e + B * LN;
+    uint64_t* acc = d2c + B * LN;
+    HIPCHK(launch_ntt(t, ea, a, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+    HIPCHK(launch_ntt(t, eb, b, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+    HIPCHK(launch_tensor_intt(t, ea, eb, d01, d2e, d2c, batch, l, m, mc, c->ntt_threads, S(stream)));
+    HIPCHK(launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc, batch, l, m, special,
+                            c->ks_threads, S(stream)));
+    HIPCHK(launch_moddown_rescale(t, acc, nullptr, 0, out, batch, l, m, special, mc, true, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_automorphism(const ckks_ctx* c, const uint64_t* in, uint64_t* out, int n_polys, int n_limbs, int prime0,
+                      uint64_t galois, void* stream) {
+    int st = check_rows(c, n_polys, n_limbs, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!in || !out) return CKKS_ERR_NULL;
+    if (in == out) return CKKS_ERR_BAD_ARG;
+    if (!(galois & 1) || galois >= 2 * c->N) return CKKS_ERR_BAD_ARG;
+    if (c->flags & CKKS_CYCLIC) return CKKS_ERR_BAD_ARG;
+    if (misaligned(in) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_automorphism(c->tables(), in, out, n_polys, n_limbs, limb_map(prime0, n_limbs), galois, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int key_limbs, uint64_t* out, int batch,
+                int l, int special, uint64_t galois, void* workspace, void* stream) {
+    int st = ks_common_check(c, gkey, key_limbs, out, batch, l, special, workspace);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!ct) return CKKS_ERR_NULL;
+    if (ct == out) return CKKS_ERR_BAD_ARG;
+    if (!(galois & 1) || galois >= 2 * c->N) return CKKS_ERR_BAD_ARG;
+    if (misaligned(ct)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const LimbMap m = limb_map(0, l);
+    const ModDownConst mc = moddown_const(c, l, special);
+    const size_t LN = (size_t)l * c->N, B = (size_t)batch;
+    uint64_t* sct = (uint64_t*)workspace;   // sigma(c0), sigma(c1)
+    uint64_t* acc = sct + B * 2 * LN;
+    HIPCHK(launch_automorphism(t, ct, sct, batch * 2, l, m, galois, S(stream)));
+    HIPCHK(launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, (const ulonglong2*)gkey, key_limbs, acc, batch, l,
+                            m, special, c->ks_threads, S(stream)));
+    HIPCHK(launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_encode(const ckks_ctx* c, const double* slots, uint64_t* pt, int n_vec, int l, int prime0, double scale,
+                double* work, void* stream) {
+    int st = check_rows(c, n_vec, l, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!slots || !pt || !work) return CKKS_ERR_NULL;
+    if (!c->fft_tw) return CKKS_ERR_BAD_ARG;
+    if (!(scale > 0) || !(scale < 9.0e18)) return CKKS_ERR_BAD_ARG;
+    if (misaligned(slots) || misaligned(pt) || misaligned(work)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_encode_fft(c->fft_tw, c->fft_zeta, c->slot_k, (const double2*)slots, work, n_vec, c->logN,
+                             S(stream)));
+    HIPCHK(launch_encode_round(c->tables(), work, pt, n_vec, l, limb_map(prime0, l), scale, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_decode(const ckks_ctx* c, const uint64_t* pt, double* slots, int n_vec, int l, int prime0, double scale,
+                double* work, void* stream) {
+    int st = check_rows(c, n_vec, l, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!slots || !pt || !work) return CKKS_ERR_NULL;
+    if (!c->fft_tw) return CKKS_ERR_BAD_ARG;
+    if (!(scale > 0)) return CKKS_ERR_BAD_ARG;
+    if (misaligned(slots) || misaligned(pt) || misaligned(work)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    CrtConst cc;
+    st = crt_const(c, prime0, l, &cc);
+    if (st) return st;
+    HIPCHK(launch_decode_crt(c->tables(), pt, work, n_vec, limb_map(prime0, l), cc, 1.0 / scale, S(stream)));
+    HIPCHK(launch_decode_fft(c->fft_tw, c->fft_zeta, c->slot_k, work, (double2*)slots, n_vec, c->logN, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_bitrev(uint64_t* data, int n_rows, uint64_t N, void* stream) {
+    if (N < 1 || N > (1u << 20) || (N & (N - 1))) return CKKS_ERR_BAD_N;
+    if (n_rows < 0) return CKKS_ERR_BAD_SHAPE;
+    if (n_rows == 0) return CKKS_OK;
+    if (!data) return CKKS_ERR_NULL;
+    if (misaligned(data)) return CKKS_ERR_BAD_ALIGNMENT;
+    HIPCHK(launch_bitrev(data, n_rows, bitlen(N) - 1, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_mod_switch(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t q_in, uint64_t q_out, void* stream) {
+    if (n == 0) return CKKS_OK;
+    if (!in || !out) return CKKS_ERR_NULL;
+    if (misaligned(in) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (q_in < 3 || q_out < 2 || q_in >= (1ull << 45) || bitlen(q_out) > bitlen(q_in)) return CKKS_ERR_BAD_MODULUS;
+    const int ell = bitlen(q_in);
+    const uint64_t mu = (uint64_t)(((u128)1 << (2 * ell)) / q_in);
+    HIPCHK(launch_mod_switch(in, out, n, q_in, q_out, mu, ell, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_embed(const ckks_ctx* c, const double* coeffs, double* slots, int n_vec, int order, void* stream) {
+    if (!c || !coeffs || !slots) return CKKS_ERR_NULL;
+    if (!c->fft_tw) return CKKS_ERR_BAD_ARG;
+    if (n_vec < 0 || order < 0 || order > 1) return CKKS_ERR_BAD_ARG;
+    if (misaligned(coeffs) || misaligned(slots)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_decode_fft(c->fft_tw, c->fft_zeta, order ? nullptr : c->slot_k, coeffs, (double2*)slots, n_vec,
+                             c->logN, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_embed_inv(const ckks_ctx* c, const double* slots, double* coeffs, int n_vec, int order, void* stream) {
+    if (!c || !coeffs || !slots) return CKKS_ERR_NULL;
+    if (!c->fft_tw) return CKKS_ERR_BAD_ARG;
+    if (n_vec < 0 || order < 0 || order > 1) return CKKS_ERR_BAD_ARG;
+    if (misaligned(coeffs) || misaligned(slots)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_encode_fft(c->fft_tw, c->fft_zeta, order ? nullptr : c->slot_k, (const double2*)slots, coeffs,
+                             n_vec, c->logN, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_noise_stats(const ckks_ctx* c, const uint64_t* data, uint64_t* stats, int n_polys, int n_limbs, int prime0,
+                     void* stream) {
+    int st = check_rows(c, n_polys, n_limbs, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!data || !stats) return CKKS_ERR_NULL;
+    if (misaligned(data) || misaligned(stats)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_noise_stats(c->tables(), data, stats, n_polys, n_limbs, limb_map(prime0, n_limbs), S(stream)));
+    return CKKS_OK;
+}
+
+}  // extern "C"

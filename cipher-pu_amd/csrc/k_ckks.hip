@@ -1,0 +1,409 @@
+// CKKS evaluation kernels: tensor product, hybrid key switch (ModUp fused
+// with the digit NTTs and the key inner product, then the inverse
+// transforms), ModDown + rescale, standalone rescale, Galois automorphism
+// and the plain noise monitor.
+//
+// Layouts (uint64, row-major, limb i reduced mod prime map.idx[i]):
+//   coefficient / evaluation tensors  [batch][poly][limb][N]
+//   evaluation order = bit-reversed NTT output order (as written by the
+//   forward transform)
+//   key  [digit][2][key_limbs][N] of (value, Shoup companion) pairs,
+//        evaluation order; digit j = (b_j, a_j), last limb = special prime
+#include "launch.h"
+#include "ntt_core.h"
+
+namespace {
+
+using ntt::Cfg;
+
+__device__ __forceinline__ uint64_t reduce4q(uint64_t x, const PrimeConst& pc) {
+    return csub(csub(x, pc.two_q), pc.q);
+}
+
+// Tensor product in the evaluation domain (verify_e2e_mult.py:78-92 /
+// rtl/m4/e2e_ckks_mult_ctrl_m4.sv S_MUL_*):
+//   d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1.
+// Writes p*d0, p*d1 (evaluation form, ready to be folded into the key
+// switch accumulators), d2 in evaluation form, and d2 back in coefficient
+// form (the key-switch digits).
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_tensor_intt(const uint64_t* __restrict__ ea, const uint64_t* __restrict__ eb,
+                                                   uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
+                                                   uint64_t* __restrict__ d2c, const ulonglong2* __restrict__ tw_all,
+                                                   const PrimeConst* __restrict__ pcs, LimbMap map, int l,
+                                                   ModDownConst mc) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x / l, i = blockIdx.x % l;
+    const int p = map.idx[i];
+    const PrimeConst pc = pcs[p];
+    const size_t LN = (size_t)l * C::N;
+    const size_t ct0 = (size_t)b * 2 * LN + (size_t)i * C::N;  // poly 0, limb i
+    const uint64_t pm = mc.p_mod[i], pms = mc.p_mod_s[i];
+    uint64_t x[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+        const size_t j = (size_t)e * C::T + tid;
+        const uint64_t a0 = ea[ct0 + j], a1 = ea[ct0 + LN + j];
+        const uint64_t b0 = eb[ct0 + j], b1 = eb[ct0 + LN + j];
+        const uint64_t d0 = mul_barrett(a0, b0, pc);
+        const uint64_t d1 = add_mod(mul_barrett(a0, b1, pc), mul_barrett(a1, b0, pc), pc.q);
+        const uint64_t d2 = mul_barrett(a1, b1, pc);
+        d01[ct0 + j] = csub(mul_shoup(d0, pm, pms, pc.q), pc.q);
+        d01[ct0 + LN + j] = csub(mul_shoup(d1, pm, pms, pc.q), pc.q);
+        d2e[(size_t)b * LN + (size_t)i * C::N + j] = d2;
+        x[e] = d2;
+    }
+    ntt::coalesced_to_last<C>(x, lds, tid);
+    ntt::inv_ntt<C>(x, lds, tw_all + ((size_t)p * 2 + 1) * C::N, pc, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = csub(x[e], pc.q);
+    ntt::store_coalesced<C>(x, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
+}
+
+// Hybrid key switch for one (ciphertext, target limb t) pair, t = 0..l where
+// t = l is the special prime p.  Generalises the single-digit relinearisation
+// f0 = e0 + e2 b, f1 = e1 - e2 a (verify_keyswitch.py:62-68,
+// rtl/m3/keyswitch_pipe.sv:90-100) to one digit per RNS prime:
+//   acc0_t = [p d0]_t + sum_j NTT_t([d2]_(q_j)) * b_j,t
+//   acc1_t = [p d1]_t - sum_j NTT_t([d2]_(q_j)) * a_j,t
+// then both accumulators go back to coefficient form.  The digit lift
+// ("ModUp") is the reduction of the digit's coefficients mod the target
+// prime on load; the digit NTTs, the key products and the inverse
+// transforms never leave the workgroup.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2c, size_t d2c_bstride,
+                                                 const uint64_t* __restrict__ d2e,
+                                                 const uint64_t* __restrict__ d01, const ulonglong2* __restrict__ key,
+                                                 int key_limbs, uint64_t* __restrict__ acc,
+                                                 const ulonglong2* __restrict__ tw_all,
+                                                 const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x / (l + 1), t = blockIdx.x % (l + 1);
+    const int pt = t < l ? map.idx[t] : pidx_p;
+    const int kt = t < l ? t : key_limbs - 1;
+    const PrimeConst pc = pcs[pt];
+    const ulonglong2* tw = tw_all + (size_t)pt * 2 * C::N;
+    const size_t LN = (size_t)l * C::N;
+    uint64_t a0[C::E], a1[C::E], x[C::E];
+    if (t < l && d01) {
+        const uint64_t* s0 = d01 + (size_t)b * 2 * LN + (size_t)t * C::N;
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const int j = ntt::eval_index<C>(k, tid);
+            a0[k] = s0[j];
+            const uint64_t v = s0[LN + j];
+            a1[k] = v ? pc.q - v : 0;   // acc1 holds -(p d1) + sum d a, negated at the end
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0;
+    }
+    for (int j = 0; j < l; ++j) {
+        if (j == t && d2e) {
+            const uint64_t* s = d2e + (size_t)b * LN + (size_t)j * C::N;
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) x[k] = s[ntt::eval_index<C>(k, tid)];
+        } else {
+            ntt::load_coalesced<C>(x, d2c + (size_t)b * d2c_bstride + (size_t)j * C::N, tid);
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
+            ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
+        }
+        const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+        const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const int idx = ntt::eval_index<C>(k, tid);
+            const ulonglong2 wb = kb[idx], wa = ka[idx];
+            a0[k] += mul_shoup(x[k], wb.x, wb.y, pc.q);
+            a1[k] += mul_shoup(x[k], wa.x, wa.y, pc.q);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        a0[k] = reduce64(a0[k], pc);
+        const uint64_t v = reduce64(a1[k], pc);
+        a1[k] = v ? pc.q - v : 0;
+    }
+    uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
+    const ulonglong2* itw = tw + C::N;
+    ntt::inv_ntt<C>(a0, lds, itw, pc, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a0[k] = csub(a0[k], pc.q);
+    ntt::store_coalesced<C>(a0, out, tid);
+    __syncthreads();
+    ntt::inv_ntt<C>(a1, lds, itw, pc, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a1[k] = csub(a1[k], pc.q);
+    ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
+}
+
+// ModDown by p (round-half-up of X/p), optional addition of coefficient-form
+// (d0, d1), then optional rescale by the last prime (verify_rescale.py:38-53
+// on the CRT value, in the per-limb form
+//   ((y_i + h) - [(y_L + h) mod q_L]) q_L^-1 mod q_i,  h = (q_L - 1)/2).
+__global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64_t* __restrict__ dcoef, int dmask,
+                                  uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map,
+                                  int l, int pidx_p, ModDownConst mc, int rescale, int logN, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const size_t N = (size_t)1 << logN;
+    const size_t n = g & (N - 1);
+    const size_t bp = g >> logN;              // (batch, poly) row
+    const uint64_t* in = acc + bp * (l + 1) * N + n;
+    const PrimeConst& pp = pcs[pidx_p];
+    const uint64_t tp = csub(in[(size_t)l * N] + (pp.q >> 1), pp.q);
+    const int lo = rescale ? l - 1 : l;
+    uint64_t* o = out + bp * lo * N + n;
+    const uint64_t* dc = (dcoef && ((dmask >> (bp & 1)) & 1)) ? dcoef + bp * l * N + n : nullptr;
+    auto moddown = [&](int i) -> uint64_t {
+        const PrimeConst& pc = pcs[map.idx[i]];
+        uint64_t v = sub_mod(add_mod(in[(size_t)i * N], mc.hp[i], pc.q), reduce64(tp, pc), pc.q);
+        v = csub(mul_shoup(v, mc.pinv[i], mc.pinv_s[i], pc.q), pc.q);
+        if (dc) v = add_mod(v, dc[(size_t)i * N], pc.q);
+        return v;
+    };
+    if (!rescale) {
+        for (int i = 0; i < l; ++i) o[(size_t)i * N] = moddown(i);
+        return;
+    }
+    const PrimeConst& pl = pcs[map.idx[l - 1]];
+    const uint64_t tl = csub(moddown(l - 1) + (pl.q >> 1), pl.q);
+    for (int i = 0; i < l - 1; ++i) {
+        const PrimeConst& pc = pcs[map.idx[i]];
+        uint64_t v = sub_mod(add_mod(moddown(i), mc.hq[i], pc.q), reduce64(tl, pc), pc.q);
+        o[(size_t)i * N] = csub(mul_shoup(v, mc.qinv[i], mc.qinv_s[i], pc.q), pc.q);
+    }
+}
+
+// Rescale by the last limb (verify_rescale.py:38-57, RNS per-limb form).
+__global__ void k_rescale(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                          const PrimeConst* __restrict__ pcs, LimbMap map, int l, ModDownConst mc, int logN,
+                          size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const size_t N = (size_t)1 << logN;
+    const size_t n = g & (N - 1), row = g >> logN;
+    const uint64_t* src = in + row * l * N + n;
+    uint64_t* dst = out + row * (l - 1) * N + n;
+    const PrimeConst& pl = pcs[map.idx[l - 1]];
+    const uint64_t tl = csub(src[(size_t)(l - 1) * N] + (pl.q >> 1), pl.q);
+    for (int i = 0; i < l - 1; ++i) {
+        const PrimeConst& pc = pcs[map.idx[i]];
+        uint64_t v = sub_mod(add_mod(src[(size_t)i * N], mc.hq[i], pc.q), reduce64(tl, pc), pc.q);
+        dst[(size_t)i * N] = csub(mul_shoup(v, mc.qinv[i], mc.qinv_s[i], pc.q), pc.q);
+    }
+}
+
+// f(X) -> f(X^g) mod X^N + 1 for odd g, as a gather: output coefficient d
+// takes input k = g^-1 d mod 2N, negated when k >= N
+// (exp_e24_rotation_bootstrap.py:143-162 accumulates the same map).
+__global__ void k_automorphism(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                               const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, uint64_t ginv,
+                               int logN, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint64_t N = 1ull << logN;
+    const uint64_t d = g & (N - 1);
+    const size_t row = g >> logN;
+    const uint64_t k = (ginv * d) & (2 * N - 1);
+    const uint64_t* src = in + row * N;
+    uint64_t v;
+    if (k < N) {
+        v = src[k];
+    } else {
+        const uint64_t q = pcs[map.idx[row % n_limbs]].q;
+        const uint64_t c = src[k - N];
+        v = c ? q - c : 0;
+    }
+    out[g] = v;
+}
+
+// Plain noise monitor per (poly, limb) row: with c' = min(c, q - c)
+// (rtl/m4/noise_monitor.sv:44-78, verify_m4_h_multilayer.py:162-174) the row
+// reports sum c', sum c'^2 (128-bit), max c' and the sum of the centered
+// values, all exact.  stats: [row][5] = {sum_abs, sumsq_lo, sumsq_hi,
+// max_abs, sum_centered (two's complement)}.
+__global__ void __launch_bounds__(256) k_noise_stats(const uint64_t* __restrict__ in, uint64_t* __restrict__ stats,
+                                                     const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs,
+                                                     int logN) {
+    __shared__ uint64_t s_abs[256], s_lo[256], s_hi[256], s_max[256], s_sgn[256];
+    const int N = 1 << logN;
+    const size_t row = blockIdx.x;
+    const uint64_t q = pcs[map.idx[row % n_limbs]].q;
+    const uint64_t* src = in + row * N;
+    uint64_t sa = 0, lo = 0, hi = 0, mx = 0, sg = 0;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        const uint64_t c = src[k];
+        const bool neg = c > q - c;
+        const uint64_t a = neg ? q - c : c;
+        sa += a;
+        mx = a > mx ? a : mx;
+        sg += neg ? (uint64_t)0 - a : a;
+        const uint64_t plo = a * a, phi = __umul64hi(a, a);
+        lo += plo;
+        hi += phi + (lo < plo ? 1 : 0);
+    }
+    const int t = threadIdx.x;
+    s_abs[t] = sa; s_lo[t] = lo; s_hi[t] = hi; s_max[t] = mx; s_sgn[t] = sg;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s_abs[t] += s_abs[t + w];
+            const uint64_t l2 = s_lo[t] + s_lo[t + w];
+            s_hi[t] += s_hi[t + w] + (l2 < s_lo[t] ? 1 : 0);
+            s_lo[t] = l2;
+            s_max[t] = s_max[t] > s_max[t + w] ? s_max[t] : s_max[t + w];
+            s_sgn[t] += s_sgn[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        uint64_t* o = stats + row * 5;
+        o[0] = s_abs[0]; o[1] = s_lo[0]; o[2] = s_hi[0]; o[3] = s_max[0]; o[4] = s_sgn[0];
+    }
+}
+
+// Single-prime modulus switch round-half-up(c q_out / q_in) mod q_out
+// (scripts/verify_rescale.py:38-53).  c q_out < 2^(2l), l = bit length of
+// q_in, so Barrett (HAC 14.42) with mu = floor(2^(2l) / q_in) gives the
+// quotient to within 2; the remainder decides the rounding.
+__global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, size_t n, uint64_t q_in,
+                             uint64_t q_out, uint64_t mu, int ell) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    uint64_t c = in[g];
+    // c = c1 q_in + c0 gives the same result mod q_out as c0 alone
+    if (c >= q_in) c %= q_in;
+    const uint64_t lo = c * q_out, hi = __umul64hi(c, q_out);
+    const uint64_t q1 = (hi << (65 - ell)) | (lo >> (ell - 1));
+    uint64_t quo = __umul64hi(q1 << (63 - ell), mu);
+    uint64_t r = lo - quo * q_in;   // in [0, 3 q_in)
+    for (int it = 0; it < 2 && r >= q_in; ++it) {
+        r -= q_in;
+        ++quo;
+    }
+    if (2 * r >= q_in) ++quo;
+    out[g] = quo >= q_out ? quo - q_out : quo;
+}
+
+template <typename K>
+hipError_t set_lds(K kernel, int bytes) {
+    return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+template <int LOGN, int T>
+hipError_t run_tensor(const DevTables& t, const uint64_t* ea, const uint64_t* eb, uint64_t* d01, uint64_t* d2e,
+                      uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_tensor_intt<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_tensor_intt<LOGN, T>), dim3(batch * l), dim3(T), C::LDS_BYTES, s, ea, eb, d01, d2e, d2c,
+                       t.tw, t.pc, map, l, mc);
+    return hipGetLastError();
+}
+
+template <int LOGN, int T>
+hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uint64_t* d2e, const uint64_t* d01,
+                  const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
+                  int pidx_p, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_keyswitch<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_keyswitch<LOGN, T>), dim3(batch * (l + 1)), dim3(T), C::LDS_BYTES, s, d2c, d2s, d2e, d01, key,
+                       key_limbs, acc, t.tw, t.pc, map, l, pidx_p);
+    return hipGetLastError();
+}
+
+int default_threads(int logN) { return logN >= 14 ? 512 : (logN == 13 ? 512 : 256); }
+
+}  // namespace
+
+hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint64_t* eb, uint64_t* d01,
+                              uint64_t* d2e, uint64_t* d2c, int batch, int l, const LimbMap& map,
+                              const ModDownConst& mc, int threads, hipStream_t s) {
+    if (batch * l == 0) return hipSuccess;
+    if (!threads) threads = default_threads(t.logN);
+    switch (t.logN * 10000 + threads) {
+        case 120256: return run_tensor<12, 256>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        case 120512: return run_tensor<12, 512>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        case 130256: return run_tensor<13, 256>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        case 130512: return run_tensor<13, 512>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        case 131024: return run_tensor<13, 1024>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        case 140512: return run_tensor<14, 512>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        case 141024: return run_tensor<14, 1024>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
+                            const uint64_t* d01,
+                            const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l,
+                            const LimbMap& map, int pidx_p, int threads, hipStream_t s) {
+    if (batch * l == 0) return hipSuccess;
+    if (!threads) threads = default_threads(t.logN);
+    switch (t.logN * 10000 + threads) {
+        case 120256: return run_ks<12, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        case 120512: return run_ks<12, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        case 130256: return run_ks<13, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        case 130512: return run_ks<13, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        case 131024: return run_ks<13, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        case 140512: return run_ks<14, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        case 141024: return run_ks<14, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
+                                  uint64_t* out,
+                                  int batch, int l, const LimbMap& map, int pidx_special, const ModDownConst& mc,
+                                  bool rescale, hipStream_t s) {
+    const size_t total = ((size_t)batch * 2) << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_moddown_rescale, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, acc, dcoef, dmask, out,
+                       t.pc, map, l, pidx_special, mc, rescale ? 1 : 0, t.logN, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys, int l,
+                          const LimbMap& map, const ModDownConst& mc, hipStream_t s) {
+    const size_t total = (size_t)n_polys << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_rescale, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, map, l, mc,
+                       t.logN, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys, int n_limbs,
+                               const LimbMap& map, uint64_t galois, hipStream_t s) {
+    const size_t total = ((size_t)n_polys * n_limbs) << t.logN;
+    if (!total) return hipSuccess;
+    const uint64_t M = 2ull << t.logN;
+    // inverse of the odd Galois element modulo 2N (a power of two)
+    uint64_t inv = 1;
+    for (int k = 0; k < 7; ++k) inv = inv * (2 - galois * inv);
+    inv &= M - 1;
+    hipLaunchKernelGGL(k_automorphism, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, map,
+                       n_limbs, inv, t.logN, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_stats(const DevTables& t, const uint64_t* in, uint64_t* stats, int n_polys, int n_limbs,
+                              const LimbMap& map, hipStream_t s) {
+    const int rows = n_polys * n_limbs;
+    if (!rows) return hipSuccess;
+    hipLaunchKernelGGL(k_noise_stats, dim3(rows), dim3(256), 0, s, in, stats, t.pc, map, n_limbs, t.logN);
+    return hipGetLastError();
+}
+
+hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
+                             int ell, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_mod_switch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n, q_in, q_out, mu,
+                       ell);
+    return hipGetLastError();
+}

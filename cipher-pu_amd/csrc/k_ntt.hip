@@ -1,0 +1,328 @@
+// Batched NTT / INTT, fused negacyclic polynomial product, bit-reversal
+// permutation and element-wise RNS arithmetic.
+//
+// Tensors are uint64 [poly][limb][N], limb i reduced modulo the prime
+// map.idx[i].  One workgroup transforms one (poly, limb) row.
+#include "launch.h"
+#include "ntt_core.h"
+
+namespace {
+
+using ntt::Cfg;
+
+__device__ __forceinline__ uint64_t reduce4q(uint64_t x, const PrimeConst& pc) {
+    return csub(csub(x, pc.two_q), pc.q);
+}
+
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_ntt_fwd(uint64_t* dst, const uint64_t* src, const ulonglong2* __restrict__ tw_all,
+                                               const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs,
+                                               int reduce_in) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int limb = blockIdx.x % n_limbs;
+    const int p = map.idx[limb];
+    const PrimeConst pc = pcs[p];
+    const size_t off = (size_t)blockIdx.x * C::N;
+    const ulonglong2* tw = tw_all + (size_t)p * 2 * C::N;
+    uint64_t x[C::E];
+    ntt::load_coalesced<C>(x, src + off, tid);
+    if (reduce_in) {
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) x[e] = reduce64(x[e], pc);
+    }
+    ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = reduce4q(x[e], pc);
+    __syncthreads();
+    ntt::store_eval<C>(x, lds, dst + off, tid);
+}
+
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_ntt_inv(uint64_t* dst, const uint64_t* src, const ulonglong2* __restrict__ tw_all,
+                                               const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs,
+                                               int reduce_in) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int limb = blockIdx.x % n_limbs;
+    const int p = map.idx[limb];
+    const PrimeConst pc = pcs[p];
+    const size_t off = (size_t)blockIdx.x * C::N;
+    const ulonglong2* itw = tw_all + ((size_t)p * 2 + 1) * C::N;
+    uint64_t x[C::E];
+    ntt::load_eval<C>(x, lds, src + off, tid);
+    if (reduce_in) {
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) x[e] = reduce64(x[e], pc);
+    }
+    ntt::inv_ntt<C>(x, lds, itw, pc, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = csub(x[e], pc.q);
+    ntt::store_coalesced<C>(x, dst + off, tid);
+}
+
+// out = a * b mod (X^N + 1, q): forward transforms of both operands, the
+// point-wise product and the inverse transform in one workgroup
+// (rtl/m2/poly_mult.sv:1-43 runs the same three phases on one NTT unit).
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
+                                               uint64_t* __restrict__ out, const ulonglong2* __restrict__ tw_all,
+                                               const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int limb = blockIdx.x % n_limbs;
+    const int p = map.idx[limb];
+    const PrimeConst pc = pcs[p];
+    const size_t off = (size_t)blockIdx.x * C::N;
+    const ulonglong2* tw = tw_all + (size_t)p * 2 * C::N;
+    uint64_t x[C::E], y[C::E];
+    ntt::load_coalesced<C>(x, A + off, tid);
+    ntt::load_coalesced<C>(y, B + off, tid);
+    ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
+    ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = mul_barrett(reduce4q(x[e], pc), reduce4q(y[e], pc), pc);
+    ntt::inv_ntt<C>(x, lds, tw + C::N, pc, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = csub(x[e], pc.q);
+    ntt::store_coalesced<C>(x, out + off, tid);
+}
+
+// Generic transform for small rings (N <= 2048): one stage per barrier.
+__device__ void small_fwd(uint64_t* s, const ulonglong2* tw, const PrimeConst& pc, int logN) {
+    const int H = 1 << (logN - 1);
+    for (int b = logN - 1; b >= 0; --b) {
+        for (int t = threadIdx.x; t < H; t += blockDim.x) {
+            const int j = ((t >> b) << (b + 1)) | (t & ((1 << b) - 1));
+            const ulonglong2 w = tw[(1 << (logN - 1 - b)) + (j >> (b + 1))];
+            ct_bfly(s[j], s[j + (1 << b)], w.x, w.y, pc.q, pc.two_q);
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < (1 << logN); k += blockDim.x) s[k] = reduce4q(s[k], pc);
+    __syncthreads();
+}
+
+__device__ void small_inv(uint64_t* s, const ulonglong2* itw, const PrimeConst& pc, int logN) {
+    const int H = 1 << (logN - 1);
+    for (int b = 0; b < logN; ++b) {
+        for (int t = threadIdx.x; t < H; t += blockDim.x) {
+            const int j = ((t >> b) << (b + 1)) | (t & ((1 << b) - 1));
+            uint64_t& X = s[j];
+            uint64_t& Y = s[j + (1 << b)];
+            if (b == logN - 1) {
+                const uint64_t u = X, v = Y;
+                X = mul_shoup(u + v, pc.ninv, pc.ninv_s, pc.q);
+                Y = mul_shoup(u - v + pc.two_q, pc.ninv_w1, pc.ninv_w1_s, pc.q);
+            } else {
+                const ulonglong2 w = itw[(1 << (logN - 1 - b)) + (j >> (b + 1))];
+                gs_bfly(X, Y, w.x, w.y, pc.q, pc.two_q);
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < (1 << logN); k += blockDim.x) s[k] = csub(s[k], pc.q);
+    __syncthreads();
+}
+
+__global__ void k_ntt_small(uint64_t* dst, const uint64_t* src, const ulonglong2* __restrict__ tw_all,
+                            const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, int logN, int inverse,
+                            int reduce_in) {
+    extern __shared__ uint64_t s[];
+    const int N = 1 << logN;
+    const int p = map.idx[blockIdx.x % n_limbs];
+    const PrimeConst pc = pcs[p];
+    const size_t off = (size_t)blockIdx.x * N;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        uint64_t v = src[off + k];
+        s[k] = reduce_in ? reduce64(v, pc) : v;
+    }
+    __syncthreads();
+    if (inverse)
+        small_inv(s, tw_all + ((size_t)p * 2 + 1) * N, pc, logN);
+    else
+        small_fwd(s, tw_all + (size_t)p * 2 * N, pc, logN);
+    for (int k = threadIdx.x; k < N; k += blockDim.x) dst[off + k] = s[k];
+}
+
+__global__ void k_polymul_small(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
+                                uint64_t* __restrict__ out, const ulonglong2* __restrict__ tw_all,
+                                const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, int logN) {
+    extern __shared__ uint64_t s[];
+    const int N = 1 << logN;
+    const int p = map.idx[blockIdx.x % n_limbs];
+    const PrimeConst pc = pcs[p];
+    const size_t off = (size_t)blockIdx.x * N;
+    const ulonglong2* tw = tw_all + (size_t)p * 2 * N;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        s[k] = A[off + k];
+        s[N + k] = B[off + k];
+    }
+    __syncthreads();
+    small_fwd(s, tw, pc, logN);
+    small_fwd(s + N, tw, pc, logN);
+    for (int k = threadIdx.x; k < N; k += blockDim.x) s[k] = mul_barrett(s[k], s[N + k], pc);
+    __syncthreads();
+    small_inv(s, tw + N, pc, logN);
+    for (int k = threadIdx.x; k < N; k += blockDim.x) out[off + k] = s[k];
+}
+
+__global__ void k_bitrev(uint64_t* __restrict__ data, int logN, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t N = 1u << logN;
+    const uint32_t i = (uint32_t)(g & (N - 1));
+    const uint32_t j = logN ? (__brev(i) >> (32 - logN)) : 0;
+    if (i < j) {
+        uint64_t* row = data + (g - i);
+        const uint64_t t = row[i];
+        row[i] = row[j];
+        row[j] = t;
+    }
+}
+
+// op 0: a*b, 1: a+b, 2: a-b, 3: -a  (all mod the limb's prime)
+__global__ void k_pointwise(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
+                            uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map,
+                            int n_limbs, int logN, size_t total, int op) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int limb = (int)((g >> logN) % n_limbs);
+    const PrimeConst& pc = pcs[map.idx[limb]];
+    const uint64_t x = a[g];
+    uint64_t r;
+    switch (op) {
+        case 0: r = mul_barrett(x, b[g], pc); break;
+        case 1: r = add_mod(x, b[g], pc.q); break;
+        case 2: r = sub_mod(x, b[g], pc.q); break;
+        default: r = x ? pc.q - x : 0; break;
+    }
+    out[g] = r;
+}
+
+// (w, floor(w 2^64 / q)) pairs for key material; w < q < 2^45, the quotient
+// by restoring division.
+__global__ void k_shoup_pairs(const uint64_t* __restrict__ in, ulonglong2* __restrict__ out,
+                              const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, int logN, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint64_t q = pcs[map.idx[(g >> logN) % n_limbs]].q;
+    const uint64_t w = in[g];
+    uint64_t r = w, quo = 0;
+    for (int k = 0; k < 64; ++k) {
+        r <<= 1;
+        quo <<= 1;
+        if (r >= q) {
+            r -= q;
+            quo |= 1;
+        }
+    }
+    out[g] = make_ulonglong2(w, quo);
+}
+
+template <typename K>
+hipError_t set_lds(K kernel, int bytes) {
+    return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+template <int LOGN, int T>
+hipError_t run_ntt(const DevTables& t, uint64_t* dst, const uint64_t* src, int rows, int n_limbs, const LimbMap& map, bool inv,
+                   bool red, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    auto k = inv ? k_ntt_inv<LOGN, T> : k_ntt_fwd<LOGN, T>;
+    hipError_t e = set_lds(k, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(rows), dim3(T), C::LDS_BYTES, s, dst, src, t.tw, t.pc, map, n_limbs, red ? 1 : 0);
+    return hipGetLastError();
+}
+
+template <int LOGN, int T>
+hipError_t run_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out, int rows,
+                       int n_limbs, const LimbMap& map, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_polymul<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_polymul<LOGN, T>), dim3(rows), dim3(T), C::LDS_BYTES, s, a, b, out, t.tw, t.pc, map,
+                       n_limbs);
+    return hipGetLastError();
+}
+
+int default_threads(int logN) { return logN >= 14 ? 512 : 256; }
+
+}  // namespace
+
+hipError_t launch_ntt(const DevTables& t, uint64_t* dst, const uint64_t* src, int n_polys, int n_limbs, const LimbMap& map,
+                      bool inverse, bool reduce_input, int threads, hipStream_t s) {
+    const int rows = n_polys * n_limbs;
+    if (rows == 0) return hipSuccess;
+    if (!threads) threads = default_threads(t.logN);
+    switch (t.logN * 10000 + threads) {
+        case 120128: return run_ntt<12, 128>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        case 120256: return run_ntt<12, 256>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        case 120512: return run_ntt<12, 512>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        case 130256: return run_ntt<13, 256>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        case 130512: return run_ntt<13, 512>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        case 131024: return run_ntt<13, 1024>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        case 140512: return run_ntt<14, 512>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        case 141024: return run_ntt<14, 1024>(t, dst, src, rows, n_limbs, map, inverse, reduce_input, s);
+        default: break;
+    }
+    if (t.logN > 11) return hipErrorInvalidValue;
+    const int N = 1 << t.logN;
+    const int thr = N / 2 < 64 ? 64 : (N / 2 > 1024 ? 1024 : N / 2);
+    hipLaunchKernelGGL(k_ntt_small, dim3(rows), dim3(thr), N * 8, s, dst, src, t.tw, t.pc, map, n_limbs, t.logN,
+                       inverse ? 1 : 0, reduce_input ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out, int n_polys,
+                          int n_limbs, const LimbMap& map, int threads, hipStream_t s) {
+    const int rows = n_polys * n_limbs;
+    if (rows == 0) return hipSuccess;
+    if (!threads) threads = default_threads(t.logN);
+    switch (t.logN * 10000 + threads) {
+        case 120128: return run_polymul<12, 128>(t, a, b, out, rows, n_limbs, map, s);
+        case 120256: return run_polymul<12, 256>(t, a, b, out, rows, n_limbs, map, s);
+        case 120512: return run_polymul<12, 512>(t, a, b, out, rows, n_limbs, map, s);
+        case 130256: return run_polymul<13, 256>(t, a, b, out, rows, n_limbs, map, s);
+        case 130512: return run_polymul<13, 512>(t, a, b, out, rows, n_limbs, map, s);
+        case 131024: return run_polymul<13, 1024>(t, a, b, out, rows, n_limbs, map, s);
+        case 140512: return run_polymul<14, 512>(t, a, b, out, rows, n_limbs, map, s);
+        case 141024: return run_polymul<14, 1024>(t, a, b, out, rows, n_limbs, map, s);
+        default: break;
+    }
+    if (t.logN > 11) return hipErrorInvalidValue;
+    const int N = 1 << t.logN;
+    const int thr = N / 2 < 64 ? 64 : (N / 2 > 1024 ? 1024 : N / 2);
+    hipLaunchKernelGGL(k_polymul_small, dim3(rows), dim3(thr), 2 * N * 8, s, a, b, out, t.tw, t.pc, map, n_limbs,
+                       t.logN);
+    return hipGetLastError();
+}
+
+hipError_t launch_bitrev(uint64_t* data, int n_rows, int logN, hipStream_t s) {
+    const size_t total = (size_t)n_rows << logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_bitrev, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, data, logN, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_pointwise(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out, int n_polys,
+                            int n_limbs, const LimbMap& map, int op, hipStream_t s) {
+    const size_t total = ((size_t)n_polys * n_limbs) << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_pointwise, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, b, out, t.pc, map,
+                       n_limbs, t.logN, total, op);
+    return hipGetLastError();
+}
+
+hipError_t launch_shoup_pairs(const DevTables& t, const uint64_t* in, ulonglong2* out, int n_polys, int n_limbs,
+                              const LimbMap& map, hipStream_t s) {
+    const size_t total = ((size_t)n_polys * n_limbs) << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_shoup_pairs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, map,
+                       n_limbs, t.logN, total);
+    return hipGetLastError();
+}

@@ -1,0 +1,84 @@
+// Internal launcher interface shared by the kernel translation units and
+// the C ABI (capi.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "modarith.h"
+
+struct DevTables {
+    const PrimeConst* pc;   // [K]
+    const ulonglong2* tw;   // [K][2][N]: forward table, inverse table
+    int N;
+    int logN;
+};
+
+// Thread-count choice for the LDS-resident kernels, per log2 N.
+struct NttPlan {
+    int threads;   // 0 = default
+};
+
+// ntt kernels (k_ntt.cu)
+hipError_t launch_ntt(const DevTables& t, uint64_t* dst, const uint64_t* src, int n_polys, int n_limbs, const LimbMap& map,
+                      bool inverse, bool reduce_input, int threads, hipStream_t s);
+hipError_t launch_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out,
+                          int n_polys, int n_limbs, const LimbMap& map, int threads, hipStream_t s);
+hipError_t launch_shoup_pairs(const DevTables& t, const uint64_t* in, ulonglong2* out, int n_polys, int n_limbs,
+                              const LimbMap& map, hipStream_t s);
+hipError_t launch_bitrev(uint64_t* data, int n_rows, int logN, hipStream_t s);
+hipError_t launch_pointwise(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out,
+                            int n_polys, int n_limbs, const LimbMap& map, int op, hipStream_t s);
+
+// ckks kernels (k_ckks.cu)
+struct ModDownConst {
+    // ModDown by the special prime (index pidx_p): for limb i < l
+    uint64_t hp[CKKS_MAX_PRIMES];      // (p-1)/2 mod q_i
+    uint64_t pinv[CKKS_MAX_PRIMES];    // p^-1 mod q_i
+    uint64_t pinv_s[CKKS_MAX_PRIMES];
+    // rescale by the last limb q_(l-1): for limb i < l-1
+    uint64_t hq[CKKS_MAX_PRIMES];
+    uint64_t qinv[CKKS_MAX_PRIMES];
+    uint64_t qinv_s[CKKS_MAX_PRIMES];
+    uint64_t p_mod[CKKS_MAX_PRIMES];   // p mod q_i, and Shoup companion
+    uint64_t p_mod_s[CKKS_MAX_PRIMES];
+};
+
+hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint64_t* eb, uint64_t* d01,
+                              uint64_t* d2e, uint64_t* d2c, int batch, int l, const LimbMap& map,
+                              const ModDownConst& mc, int threads, hipStream_t s);
+hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
+                            const uint64_t* d01, const ulonglong2* key, int key_limbs, uint64_t* acc,
+                            int batch, int l, const LimbMap& map, int pidx_special, int threads,
+                            hipStream_t s);
+hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
+                                  uint64_t* out, int batch, int l, const LimbMap& map, int pidx_special,
+                                  const ModDownConst& mc, bool rescale, hipStream_t s);
+hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys, int l,
+                          const LimbMap& map, const ModDownConst& mc, hipStream_t s);
+hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys,
+                               int n_limbs, const LimbMap& map, uint64_t galois, hipStream_t s);
+hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
+                             int ell, hipStream_t s);
+hipError_t launch_noise_stats(const DevTables& t, const uint64_t* in, uint64_t* stats, int n_polys,
+                              int n_limbs, const LimbMap& map, hipStream_t s);
+
+// fft kernels (k_fft.hip)
+#define CRT_MAX_WORDS 8
+#define CRT_MAX_LIMBS 10
+struct CrtConst {
+    int l, words;
+    uint64_t qhat[CRT_MAX_LIMBS][CRT_MAX_WORDS];   // Q / q_i, little-endian words
+    uint64_t qhat_inv[CRT_MAX_LIMBS];              // (Q/q_i)^-1 mod q_i
+    uint64_t qhat_inv_s[CRT_MAX_LIMBS];
+    uint64_t Q[CRT_MAX_WORDS];
+    uint64_t half_Q[CRT_MAX_WORDS];                // floor(Q / 2)
+};
+
+hipError_t launch_decode_fft(const double2* tw, const double2* zeta, const int* slot_k, const double* coeffs,
+                             double2* slots, int n_vec, int logN, hipStream_t s);
+hipError_t launch_encode_fft(const double2* tw, const double2* zeta, const int* slot_k, const double2* slots,
+                             double* coeffs, int n_vec, int logN, hipStream_t s);
+hipError_t launch_encode_round(const DevTables& t, const double* coeffs, uint64_t* pt, int n_vec, int l,
+                               const LimbMap& map, double scale, hipStream_t s);
+hipError_t launch_decode_crt(const DevTables& t, const uint64_t* pt, double* coeffs, int n_vec, const LimbMap& map,
+                             const CrtConst& cc, double inv_scale, hipStream_t s);

@@ -1,0 +1,86 @@
+// Device-side modular arithmetic for primes q < 2^45 (gfx950).
+//
+// Replaces the RTL multiplier/reducer pair of the reference
+// (rtl/m1/mult_43x43_horner4.sv:31-121, rtl/m2/mod45_fastred.sv:61-122,
+// rtl/m2/mod45_barrett.sv:49-75).  The RTL reduction needs the special form
+// q = 2^44 + c; these routines work for any prime below 2^45.
+//
+//  * mul_shoup: a * w mod q for a fixed operand w with w' = floor(w 2^64 / q)
+//    precomputed (twiddles, key material, constants).  Result in [0, 2q).
+//  * mul_barrett: a * b mod q for two data operands (a, b < q), result in
+//    [0, q).  Barrett with mu = floor(2^(2l) / q), l = bit length of q.
+//  * the forward (Cooley-Tukey) and inverse (Gentleman-Sande) butterflies
+//    keep values lazily in [0, 4q) / [0, 2q) (Harvey's bounds).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CKKS_MAX_PRIMES 16
+
+// Per-prime constants, resident in device memory (one entry per prime).
+struct PrimeConst {
+    uint64_t q;
+    uint64_t two_q;
+    uint64_t mu;        // floor(2^(2l) / q)
+    uint64_t mu64;      // floor(2^64 / q) for reducing arbitrary 64-bit values
+    uint32_t ell;       // bit length of q
+    uint32_t pad;
+    uint64_t ninv, ninv_s;        // N^-1 and its Shoup companion
+    uint64_t ninv_w1, ninv_w1_s;  // N^-1 * (inverse root of the last GS stage)
+};
+
+// Up to CKKS_MAX_PRIMES prime indices, passed by value to kernels: limb i of
+// a tensor uses prime idx[i].
+struct LimbMap {
+    int idx[CKKS_MAX_PRIMES];
+};
+
+
+__device__ __forceinline__ uint64_t mul_shoup(uint64_t a, uint64_t w, uint64_t ws, uint64_t q) {
+    const uint64_t hi = __umul64hi(a, ws);
+    return a * w - hi * q;  // exact modulo 2^64, lies in [0, 2q)
+}
+
+__device__ __forceinline__ uint64_t csub(uint64_t a, uint64_t m) { return a >= m ? a - m : a; }
+
+__device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) { return csub(a + b, q); }
+
+__device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t q) { return a >= b ? a - b : a + q - b; }
+
+// a * b mod q for a, b < q < 2^45 (HAC 14.42 with base 2).
+__device__ __forceinline__ uint64_t mul_barrett(uint64_t a, uint64_t b, const PrimeConst& P) {
+    const uint64_t lo = a * b;
+    const uint64_t hi = __umul64hi(a, b);
+    const uint32_t l = P.ell;
+    const uint64_t q1 = (hi << (65 - l)) | (lo >> (l - 1));   // floor(ab / 2^(l-1)) < 2^(l+1)
+    const uint64_t q3 = __umul64hi(q1 << (63 - l), P.mu);    // floor(q1 mu / 2^(l+1))
+    uint64_t r = lo - q3 * P.q;                               // in [0, 3q)
+    r = csub(r, P.two_q);
+    return csub(r, P.q);
+}
+
+// x mod q for any 64-bit x: floor(x/q) is estimated from mu64 = floor(2^64/q)
+// to within 1, so one conditional subtraction finishes.
+__device__ __forceinline__ uint64_t reduce64(uint64_t x, const PrimeConst& P) {
+    const uint64_t t = __umul64hi(x, P.mu64);
+    return csub(x - t * P.q, P.q);
+}
+
+// Forward CT butterfly (natural -> bit-reversed order), inputs/outputs in
+// [0, 4q):  X' = X + W Y,  Y' = X - W Y.
+__device__ __forceinline__ void ct_bfly(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t ws,
+                                        uint64_t q, uint64_t two_q) {
+    const uint64_t x = csub(X, two_q);
+    const uint64_t t = mul_shoup(Y, w, ws, q);
+    X = x + t;
+    Y = x - t + two_q;
+}
+
+// Inverse GS butterfly, inputs/outputs in [0, 2q):  X' = X + Y,
+// Y' = (X - Y) W.
+__device__ __forceinline__ void gs_bfly(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t ws,
+                                        uint64_t q, uint64_t two_q) {
+    const uint64_t x = X, y = Y;
+    X = csub(x + y, two_q);
+    Y = mul_shoup(x - y + two_q, w, ws, q);
+}

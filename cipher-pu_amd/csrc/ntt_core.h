@@ -1,0 +1,243 @@
+// LDS-resident negacyclic (or cyclic) NTT building blocks for one transform
+// per workgroup.
+//
+// Behavioural spec: the reference's radix-2 transform
+// (tb/ref/gen_poly_mult_vectors_n12.py:54-100; rtl/m1/ntt_unit.sv:154-382),
+// re-laid out for a 64-wide wavefront machine:
+//
+//  * T threads, each holding E = N/T coefficients in registers.
+//  * The log2(N) radix-2 stages are grouped into passes of at most log2(E)
+//    consecutive bits.  Inside a pass every butterfly is register-to-register;
+//    between passes the coefficients are exchanged once through LDS.
+//  * Pass 0 covers the top log2(E) index bits, so its register layout
+//    x[e] = a[e*T + tid] is also the coalesced HBM layout.  The inverse
+//    transform runs the same passes in reverse order and ends in that layout.
+//  * Forward: Cooley-Tukey, natural order in, bit-reversed order out, twiddle
+//    for stage s / group i at tw[2^s + i] (psi^bitrev(2^s+i) for the
+//    negacyclic ring).  Inverse: Gentleman-Sande with the inverse table and
+//    N^-1 folded into the last stage.  Twiddles are (w, floor(w 2^64/q))
+//    pairs, one 16-byte load each.
+//  * LDS index pad(j) = j + (j >> PS) keeps the small-stride pass
+//    conflict-free (PS = size of the last forward pass).
+#pragma once
+#include "modarith.h"
+
+namespace ntt {
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+
+template <int LOGN_, int T_>
+struct Cfg {
+    static constexpr int LOGN = LOGN_;
+    static constexpr int N = 1 << LOGN;
+    static constexpr int T = T_;
+    static constexpr int E = N / T;
+    static constexpr int R = ilog2c(E);
+    static constexpr int REST = LOGN - R;
+    static constexpr int NREST = (REST + R - 1) / R;
+    static constexpr int NP = 1 + NREST;
+    static_assert(E >= 2 && (E & (E - 1)) == 0, "E must be a power of two");
+    static_assert(REST > 0, "need at least two passes");
+    static constexpr int rsize(int p) {
+        return p == 0 ? R : (REST / NREST + ((p - 1) < (REST % NREST) ? 1 : 0));
+    }
+    static constexpr int hi(int p) {
+        int h = LOGN - 1;
+        for (int k = 0; k < p; ++k) h -= rsize(k);
+        return h;
+    }
+    static constexpr int lo(int p) { return hi(p) - rsize(p) + 1; }
+    static constexpr int PS = rsize(NP - 1);
+    static constexpr int LDS_WORDS = N + (N >> PS);
+    static constexpr int LDS_BYTES = LDS_WORDS * 8;
+};
+
+template <class C>
+__device__ __forceinline__ int pad(int j) { return j + (j >> C::PS); }
+
+// Global index of register (g, e) in the layout of pass P.
+template <class C, int P>
+__device__ __forceinline__ int index_of(int g, int e, int tid) {
+    constexpr int LO = C::lo(P), HI = C::hi(P);
+    const int f = g * C::T + tid;
+    return ((f >> LO) << (HI + 1)) | (e << LO) | (f & ((1 << LO) - 1));
+}
+
+template <class C, int P>
+__device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ tw,
+                                         uint64_t q, uint64_t two_q, int tid) {
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = (g * C::T + tid) >> LO;
+#pragma unroll
+        for (int b = HI; b >= LO; --b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                const ulonglong2 w = tw[base + i];
+                ct_bfly(x[g * M + e], x[g * M + e + h], w.x, w.y, q, two_q);
+            }
+        }
+    }
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ itw,
+                                         const PrimeConst& pc, int tid) {
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+    const uint64_t q = pc.q, two_q = pc.two_q;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = (g * C::T + tid) >> LO;
+#pragma unroll
+        for (int b = LO; b <= HI; ++b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                uint64_t& X = x[g * M + e];
+                uint64_t& Y = x[g * M + e + h];
+                if (b == C::LOGN - 1) {
+                    // last stage: fold N^-1 into both outputs
+                    const uint64_t u = X, v = Y;
+                    X = mul_shoup(u + v, pc.ninv, pc.ninv_s, q);
+                    Y = mul_shoup(u - v + two_q, pc.ninv_w1, pc.ninv_w1_s, q);
+                } else {
+                    const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                    const ulonglong2 w = itw[base + i];
+                    gs_bfly(X, Y, w.x, w.y, q, two_q);
+                }
+            }
+        }
+    }
+}
+
+template <class C, int P>
+__device__ __forceinline__ void to_lds(const uint64_t (&x)[C::E], uint64_t* lds, int tid) {
+    constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int e = 0; e < M; ++e) lds[pad<C>(index_of<C, P>(g, e, tid))] = x[g * M + e];
+}
+
+template <class C, int P>
+__device__ __forceinline__ void from_lds(uint64_t (&x)[C::E], const uint64_t* lds, int tid) {
+    constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int e = 0; e < M; ++e) x[g * M + e] = lds[pad<C>(index_of<C, P>(g, e, tid))];
+}
+
+// Forward passes 1..NP-1 (pass 0 already applied), exchanging through LDS.
+template <class C, int P>
+__device__ __forceinline__ void fwd_rest(uint64_t (&x)[C::E], uint64_t* lds,
+                                         const ulonglong2* __restrict__ tw, uint64_t q,
+                                         uint64_t two_q, int tid) {
+    if constexpr (P < C::NP) {
+        to_lds<C, P - 1>(x, lds, tid);
+        __syncthreads();
+        from_lds<C, P>(x, lds, tid);
+        __syncthreads();
+        fwd_pass<C, P>(x, tw, q, two_q, tid);
+        fwd_rest<C, P + 1>(x, lds, tw, q, two_q, tid);
+    }
+}
+
+// Full forward transform: x holds pass-0 layout on entry (x[e] = a[e*T+tid]),
+// the last pass's layout on exit, values in [0, 4q).
+template <class C>
+__device__ __forceinline__ void fwd_ntt(uint64_t (&x)[C::E], uint64_t* lds,
+                                        const ulonglong2* __restrict__ tw, uint64_t q,
+                                        uint64_t two_q, int tid) {
+    fwd_pass<C, 0>(x, tw, q, two_q, tid);
+    fwd_rest<C, 1>(x, lds, tw, q, two_q, tid);
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_rest(uint64_t (&x)[C::E], uint64_t* lds,
+                                         const ulonglong2* __restrict__ itw, const PrimeConst& pc,
+                                         int tid) {
+    if constexpr (P >= 0) {
+        to_lds<C, P + 1>(x, lds, tid);
+        __syncthreads();
+        from_lds<C, P>(x, lds, tid);
+        __syncthreads();
+        inv_pass<C, P>(x, itw, pc, tid);
+        inv_rest<C, P - 1>(x, lds, itw, pc, tid);
+    }
+}
+
+// Full inverse transform: x holds the last forward pass's layout on entry
+// (values in [0, 2q)), pass-0 (coalesced) layout on exit, values in [0, 2q).
+template <class C>
+__device__ __forceinline__ void inv_ntt(uint64_t (&x)[C::E], uint64_t* lds,
+                                        const ulonglong2* __restrict__ itw, const PrimeConst& pc,
+                                        int tid) {
+    inv_pass<C, C::NP - 1>(x, itw, pc, tid);
+    inv_rest<C, C::NP - 2>(x, lds, itw, pc, tid);
+}
+
+// HBM <-> registers in pass-0 (coalesced) layout.
+template <class C>
+__device__ __forceinline__ void load_coalesced(uint64_t (&x)[C::E], const uint64_t* __restrict__ src,
+                                               int tid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = src[e * C::T + tid];
+}
+
+template <class C>
+__device__ __forceinline__ void store_coalesced(const uint64_t (&x)[C::E], uint64_t* __restrict__ dst,
+                                                int tid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) dst[e * C::T + tid] = x[e];
+}
+
+// Registers in the last forward pass's layout <-> HBM in bit-reversed
+// evaluation order, through LDS so that the HBM side is coalesced.
+template <class C>
+__device__ __forceinline__ void store_eval(const uint64_t (&x)[C::E], uint64_t* lds,
+                                           uint64_t* __restrict__ dst, int tid) {
+    to_lds<C, C::NP - 1>(x, lds, tid);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) dst[e * C::T + tid] = lds[pad<C>(e * C::T + tid)];
+    __syncthreads();
+}
+
+template <class C>
+__device__ __forceinline__ void load_eval(uint64_t (&x)[C::E], uint64_t* lds,
+                                          const uint64_t* __restrict__ src, int tid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) lds[pad<C>(e * C::T + tid)] = src[e * C::T + tid];
+    __syncthreads();
+    from_lds<C, C::NP - 1>(x, lds, tid);
+    __syncthreads();
+}
+
+// Registers in pass-0 (coalesced) layout -> last forward pass's layout.
+template <class C>
+__device__ __forceinline__ void coalesced_to_last(uint64_t (&x)[C::E], uint64_t* lds, int tid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) lds[pad<C>(e * C::T + tid)] = x[e];
+    __syncthreads();
+    from_lds<C, C::NP - 1>(x, lds, tid);
+    __syncthreads();
+}
+
+// Evaluation-order index of register k in the last forward pass's layout.
+template <class C>
+__device__ __forceinline__ int eval_index(int k, int tid) {
+    constexpr int M = 1 << C::rsize(C::NP - 1);
+    return index_of<C, C::NP - 1>(k / M, k % M, tid);
+}
+
+}  // namespace ntt

@@ -1,0 +1,184 @@
+/* ckks_core.h -- C ABI of the MI355X (gfx950) CKKS evaluation core.
+ *
+ * The reference (YI-HAIBO/cipher-pu) has no FFI: its drop-in surface is the
+ * set of Python golden-model functions on lists of integers, driven by
+ * scripts/gen_ntt_vectors.py, verify_keyswitch.py, verify_rescale.py and
+ * verify_e2e_mult.py.  Each entry point below names the function it
+ * replaces.  The Python package cipherpu binds this ABI with ctypes and
+ * keeps the reference's names.
+ *
+ * Conventions
+ *  - All data pointers are caller-owned device pointers (hipMalloc or a
+ *    torch tensor's data_ptr()), 8-byte aligned.  Nothing is allocated
+ *    behind the caller's back except inside ckks_ctx_create.
+ *  - Polynomials are uint64 rows of N coefficients; RNS tensors are
+ *    [poly][limb][N] with limb i reduced modulo prime (prime0 + i) of the
+ *    context, or [batch][2][limb][N] for ciphertexts (c0, c1), decrypt =
+ *    c0 + c1*s (scripts/exp_e33_real_ciphertext.py:138-141).
+ *  - "eval" data is in bit-reversed NTT order as produced by
+ *    ckks_ntt_fwd; CKKS_NATURAL_ORDER selects the reference's natural order.
+ *  - Every call is asynchronous on the given stream (NULL = default stream)
+ *    and returns 0 or a negative ckks_status.  Contexts are read-only after
+ *    creation; calls on one stream run in order.
+ *  - Primes must be odd primes below 2^45 (the reference's 45-bit datapath,
+ *    rtl/m1/ntt_unit.sv:38-51); 2N must divide q-1 for the negacyclic ring
+ *    (N for the cyclic one).
+ */
+#ifndef CKKS_CORE_H
+#define CKKS_CORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CKKS_OK = 0,
+    CKKS_ERR_BAD_N = -1,          /* N not a power of two in [2, 16384] */
+    CKKS_ERR_BAD_MODULUS = -2,    /* not an odd prime < 2^45, or no root of the needed order */
+    CKKS_ERR_BAD_ALIGNMENT = -3,  /* device pointer not 8-byte aligned */
+    CKKS_ERR_BAD_SHAPE = -4,      /* limb range outside the context, batch < 0, ... */
+    CKKS_ERR_HIP = -5,            /* a HIP runtime call failed */
+    CKKS_ERR_NULL = -6,           /* required pointer is NULL */
+    CKKS_ERR_BAD_ARG = -7,        /* other invalid argument */
+    CKKS_ERR_NO_DEVICE = -8       /* no usable gfx950 device */
+} ckks_status;
+
+/* flags */
+#define CKKS_CYCLIC 1          /* ring Z_q[X]/(X^N - 1) (reference ntt/intt); default negacyclic */
+#define CKKS_NATURAL_ORDER 2   /* transforms read/write natural evaluation order */
+
+typedef struct ckks_ctx ckks_ctx;
+
+/* Create a context for ring degree N over the given primes (all limbs, then
+ * any special primes).  roots[i] is the root of unity to use for prime i:
+ * of order 2N (negacyclic) or N (cyclic); NULL derives g^((q-1)/order) from
+ * the smallest generator g, the chain's construction
+ * (scripts/gen_modulus_chain.py:128-130).  Precomputes the twiddle tables
+ * (scripts/gen_omega_tables.py:96-106) with their Shoup companions.
+ * Replaces the module constants of tb/ref/gen_poly_mult_vectors_n12.py:36-44
+ * and the chain loader verify_rescale.py:34-35. */
+int ckks_ctx_create(ckks_ctx **out, uint64_t N, int n_primes, const uint64_t *primes,
+                    const uint64_t *roots, int flags, int device);
+int ckks_ctx_destroy(ckks_ctx *ctx);
+/* Root of unity used for prime i (host value). */
+int ckks_ctx_root(const ckks_ctx *ctx, int prime, uint64_t *root);
+/* Library version string and the device architecture it was built for. */
+const char *ckks_version(void);
+
+/* In-place forward transform of n_polys x n_limbs rows (limb i uses prime
+ * prime0 + i).  Replaces ntt(a, omega, N, q)
+ * (tb/ref/gen_poly_mult_vectors_n12.py:68-82): with CKKS_NATURAL_ORDER and
+ * a CKKS_CYCLIC context the result equals the reference's output. */
+int ckks_ntt_fwd(const ckks_ctx *ctx, uint64_t *data, int n_polys, int n_limbs, int prime0, int flags,
+                 void *stream);
+/* In-place inverse transform; replaces intt(a, omega_inv, N, q)
+ * (gen_poly_mult_vectors_n12.py:85-100). */
+int ckks_ntt_inv(const ckks_ctx *ctx, uint64_t *data, int n_polys, int n_limbs, int prime0, int flags,
+                 void *stream);
+
+/* out = a * b in the ring, coefficient form, fused NTT -> point-wise ->
+ * INTT per row.  Replaces poly_mult (scripts/verify_keyswitch.py:48-59)
+ * and poly_mult_via_ntt (gen_poly_mult_vectors_n12.py:103-107). */
+int ckks_poly_mul(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, int n_polys,
+                  int n_limbs, int prime0, void *stream);
+
+/* Element-wise RNS arithmetic, any domain: op 0 = a*b, 1 = a+b, 2 = a-b,
+ * 3 = -a.  Replaces poly_add / poly_sub (verify_keyswitch.py:40-45) and the
+ * point-wise product of poly_mult_via_ntt. */
+int ckks_pointwise(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, int n_polys,
+                   int n_limbs, int prime0, int op, void *stream);
+
+/* Rescale by the last limb: in [n_polys][l][N] -> out [n_polys][l-1][N],
+ * coefficient form, out_i = [round-half-up(CRT(c) / q_(l-1))]_(q_i).
+ * Replaces rescale_poly / rescale_coeff (scripts/verify_rescale.py:38-57)
+ * applied to the CRT value. */
+int ckks_rescale(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, int n_polys, int l, int prime0,
+                 void *stream);
+
+/* Bytes of device workspace needed by ckks_keyswitch / ckks_relin /
+ * ckks_hmult / ckks_rotate for a batch at level l. */
+uint64_t ckks_workspace_bytes(const ckks_ctx *ctx, int batch, int l);
+
+/* Key material for the hybrid key switch, in the layout the kernels read:
+ * [dnum][2][key_limbs][N] pairs of (value, Shoup companion), evaluation
+ * order.  Builds it from coefficient-form b_j, a_j ([dnum][2][key_limbs][N]
+ * uint64, limbs = primes 0..key_limbs-2 then the special prime).  Device
+ * pointers; key_out holds 2x the input's words.  key_coeff is transformed
+ * in place (it holds the evaluation form afterwards). */
+int ckks_key_prepare(const ckks_ctx *ctx, const uint64_t *key_coeff, uint64_t *key_out, int dnum,
+                     int key_limbs, int special, void *stream);
+
+/* Hybrid key switch (one digit per prime, one special prime) of d2
+ * ([batch][l][N], coefficient form).  out: [batch][2][l][N] =
+ * (ModDown(sum_j D_j b_j), ModDown(-sum_j D_j a_j)), then plus (d0, d1) if
+ * given ([batch][2][l][N], coefficient form).  Generalises relin(e0, e1, e2,
+ * a, b, q) (scripts/verify_keyswitch.py:62-68; key convention
+ * exp_e33_real_ciphertext.py:153-175) to RNS; with d0/d1 it IS relin. */
+int ckks_relin(const ckks_ctx *ctx, const uint64_t *d01, const uint64_t *d2, const uint64_t *key,
+               int key_limbs, uint64_t *out, int batch, int l, int special, void *workspace, void *stream);
+
+/* Fused HMult + relinearise + rescale of ciphertext pairs, coefficient form:
+ * a, b [batch][2][l][N] -> out [batch][2][l-1][N].  Replaces e2e_mult
+ * (scripts/verify_e2e_mult.py:78-102; RTL rtl/m4/e2e_ckks_mult_ctrl_m4.sv
+ * S_MUL_0..S_RESCALE_1) with the exact RNS rescale. */
+int ckks_hmult(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, const uint64_t *key, int key_limbs,
+               uint64_t *out, int batch, int l, int special, void *workspace, void *stream);
+
+/* Galois automorphism f(X) -> f(X^g), g odd, coefficient form, out of
+ * place.  Replaces automorphism_exact (exp_e24_rotation_bootstrap.py:143-162). */
+int ckks_automorphism(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, int n_polys, int n_limbs,
+                      int prime0, uint64_t galois, void *stream);
+
+/* Slot rotation: automorphism by g = 5^r mod 2N then key switch from
+ * sigma_g(s) back to s with the Galois key.  ct, out [batch][2][l][N].
+ * Completes the EvalRotate that verify_multi_slot_simd.py:146-161 leaves
+ * open (exp_e24:232-238 only multiplies by X^r). */
+int ckks_rotate(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *gkey, int key_limbs, uint64_t *out,
+                int batch, int l, int special, uint64_t galois, void *workspace, void *stream);
+
+/* Canonical-embedding encode: slots [n_vec][N/2] complex (re, im doubles),
+ * scaled by `scale`, rounded, reduced into l limbs -> pt [n_vec][l][N].
+ * work: n_vec*N doubles of device scratch.  Replaces encode
+ * (scripts/verify_multi_slot_simd.py:75-84) in the 5^j slot order. */
+int ckks_encode(const ckks_ctx *ctx, const double *slots, uint64_t *pt, int n_vec, int l, int prime0,
+                double scale, double *work, void *stream);
+/* Decode: exact CRT of l limbs, centered, / scale, evaluated at the slots.
+ * slots out [n_vec][N/2] complex.  Replaces decode
+ * (verify_multi_slot_simd.py:87-91). */
+int ckks_decode(const ckks_ctx *ctx, const uint64_t *pt, double *slots, int n_vec, int l, int prime0,
+                double scale, double *work, void *stream);
+
+/* In-place bit-reversal permutation of n_rows rows of N words
+ * (bit_rev_perm, tb/ref/gen_poly_mult_vectors_n12.py:54-65). */
+int ckks_bitrev(uint64_t *data, int n_rows, uint64_t N, void *stream);
+
+/* Single-prime modulus switch out = round-half-up(in * q_out / q_in) mod
+ * q_out for n values in [0, q_in).  Replaces rescale_coeff / rescale_poly
+ * exactly (scripts/verify_rescale.py:38-57).  Needs bitlen(q_out) <=
+ * bitlen(q_in) < 46. */
+int ckks_mod_switch(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t q_in, uint64_t q_out, void *stream);
+
+/* Raw canonical embedding of real coefficients (doubles [n_vec][N]) to
+ * complex slots [n_vec][N/2]: order 0 = slot j at zeta^(5^j), order 1 =
+ * slot k at zeta^(4k+1).  ckks_embed_inv is its inverse (no rounding).
+ * The float halves of encode/decode (verify_multi_slot_simd.py:32-52). */
+int ckks_embed(const ckks_ctx *ctx, const double *coeffs, double *slots, int n_vec, int order, void *stream);
+int ckks_embed_inv(const ckks_ctx *ctx, const double *slots, double *coeffs, int n_vec, int order, void *stream);
+
+/* Plain noise monitor (rtl/m4/noise_monitor.sv:44-78,
+ * verify_m4_h_multilayer.py:162-174): per row, with c' = min(c, q-c),
+ * stats[row][5] = {sum c', sum c'^2 low word, high word, max c',
+ * sum of centered values}. */
+int ckks_noise_stats(const ckks_ctx *ctx, const uint64_t *data, uint64_t *stats, int n_polys, int n_limbs,
+                     int prime0, void *stream);
+
+/* Threads per workgroup for the LDS-resident kernels of this context
+ * (0 = default; see the tuner in the Python package). */
+int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

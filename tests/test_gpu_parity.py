@@ -1,0 +1,356 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle.
+
+Integer work is compared bit-exactly; encode/decode (the only floating
+point) within stated tolerances.  Sizes are chosen so the oracle finishes
+in seconds; full-size batches are checked through size-independent
+properties (decrypt error bounds) in test_gpu_ckks.py."""
+import random
+
+import numpy as np
+import pytest
+
+import ckks_oracle as O
+import oracle_c as OC
+
+pytestmark = pytest.mark.gpu
+
+CHAIN4 = [17592186634241, 17592186372097, 17592186273793, 17592186175489]
+SPECIAL_P = 17592186765313
+Q1, PSI1 = 35184371884033, 7388705119
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from cipherpu import _lib
+    _lib.load()
+    return torch
+
+
+def test_kats_through_drop_in_api(gpu, fixtures):
+    from cipherpu import golden_api as G
+    k = fixtures["kats"]["ntt_n8"]
+    assert G.bit_rev_perm(k["input"], 8) == k["bitrev_input"]
+    assert G.ntt(k["input"], 18, 8, 113) == k["expected"]
+    assert G.intt(k["expected"], pow(18, -1, 113), 8, 113) == k["input"]
+    k8 = fixtures["kats"]["pm_n8"]
+    assert G.poly_mult_via_ntt(k8["a"], k8["b"], 8, 113, 18, pow(18, -1, 113)) == k8["expected"]
+    k16 = fixtures["kats"]["pm_n16"]
+    assert G.poly_mult_via_ntt(k16["a"], k16["b"], 16, 113, 40, pow(40, -1, 113)) == k16["expected"]
+
+
+@pytest.mark.parametrize("N", [16, 256, 2048, 4096, 8192, 16384])
+def test_cyclic_ntt_matches_reference_ntt(gpu, N):
+    from cipherpu import golden_api as G
+    q = CHAIN4[3]
+    w = O.root_of_unity(q, N)
+    rng = np.random.default_rng(N)
+    a = rng.integers(0, q, N, dtype=np.uint64)
+    X = G.ntt(list(a), w, N, q)
+    assert X == list(OC.ntt_cyclic(a, N, q, w))
+    assert G.intt(X, pow(w, -1, q), N, q) == list(a)
+
+
+@pytest.mark.parametrize("N,threads", [(8, 0), (64, 0), (1024, 0), (4096, 128), (4096, 256), (4096, 512),
+                                       (8192, 256), (8192, 512), (8192, 1024), (16384, 512), (16384, 1024)])
+def test_negacyclic_ntt_batched(gpu, N, threads):
+    torch = gpu
+    from cipherpu.ring import RingContext, bitrev_, to_numpy
+    primes = CHAIN4 + [SPECIAL_P]
+    ring = RingContext(N, primes)
+    ring.set_threads(threads, 0)
+    rng = np.random.default_rng(N + threads)
+    x = rng.integers(0, 1 << 44, (3, 5, N), dtype=np.uint64) % np.array(primes, dtype=np.uint64)[None, :, None]
+    X = ring.ntt(x)
+    Xn = to_numpy(bitrev_(X.clone()))
+    for p in range(3):
+        for i in range(5):
+            want = OC.ntt_negacyclic(x[p, i], N, primes[i], ring.roots[i])
+            assert np.array_equal(Xn[p, i], want), (p, i)
+    back = to_numpy(ring.intt(X))
+    assert np.array_equal(back, x)
+    # natural-order flag gives the same as the explicit permutation
+    assert np.array_equal(to_numpy(ring.ntt(x, natural=True)), Xn)
+
+
+def test_ntt_edge_values(gpu):
+    from cipherpu.ring import RingContext, to_numpy
+    N = 8192
+    ring = RingContext(N, CHAIN4[:1])
+    q = CHAIN4[0]
+    for fill in (0, 1, q - 1):
+        x = np.full((1, 1, N), fill, dtype=np.uint64)
+        X = ring.ntt(x)
+        assert np.array_equal(to_numpy(ring.intt(X)), x)
+    x = np.zeros((1, 1, N), dtype=np.uint64)
+    x[0, 0, 0] = 1   # delta -> all ones
+    assert np.all(to_numpy(ring.ntt(x)) == 1)
+    empty = ring.tensor(np.zeros((0, 1, N), dtype=np.uint64))
+    ring.ntt_(empty)
+
+
+@pytest.mark.parametrize("threads", [0, 128, 512])
+def test_config1_product_bit_exact(gpu, threads):
+    """Config 1: N=4096, q = largest 45-bit prime = 1 mod 8192, psi =
+    smallest primitive 8192-th root; forward NTT -> point-wise -> INTT."""
+    from cipherpu.ring import RingContext, to_numpy
+    ring = RingContext(4096, [Q1], roots=[PSI1])
+    ring.set_threads(threads, 0)
+    assert ring.roots == [PSI1]
+    rng = np.random.default_rng(123)
+    a = rng.integers(0, Q1, (6, 1, 4096), dtype=np.uint64)
+    b = rng.integers(0, Q1, (6, 1, 4096), dtype=np.uint64)
+    a[4, 0] = 0
+    a[4, 0, 0] = 1   # delta: product = b
+    b[5] = Q1 - 1
+    c = to_numpy(ring.poly_mul(ring.tensor(a), ring.tensor(b)))
+    for i in range(6):
+        assert np.array_equal(c[i, 0], OC.polymul_ntt(a[i, 0], b[i, 0], Q1, PSI1)), i
+    assert np.array_equal(c[4, 0], b[4, 0])
+    assert np.array_equal(c[0, 0], OC.polymul_schoolbook(a[0, 0], b[0, 0], Q1))
+
+
+def test_poly_mult_drop_in_small(gpu):
+    from cipherpu import golden_api as G
+    rng = random.Random(5)
+    q = CHAIN4[1]
+    for n in (16, 32):
+        a = [rng.randrange(q) for _ in range(n)]
+        b = [rng.randrange(q) for _ in range(n)]
+        assert G.poly_mult(a, b, q) == O.poly_mult(a, b, q)
+        assert G.poly_add(a, b, q) == O.poly_add(a, b, q)
+        assert G.poly_sub(a, b, q) == O.poly_sub(a, b, q)
+
+
+def test_relin_drop_in(gpu):
+    """verify_keyswitch.relin semantics at N=16 against the schoolbook oracle."""
+    from cipherpu import golden_api as G
+    rng = random.Random(42)
+    q, n = CHAIN4[0], 16
+    s = [rng.randrange(q) for _ in range(n)]
+    for _ in range(5):
+        e0, e1, e2, a = ([rng.randrange(q) for _ in range(n)] for _ in range(4))
+        b = O.poly_add(O.poly_mult(a, s, q), [rng.randrange(100) for _ in range(n)], q)
+        assert G.relin(e0, e1, e2, a, b, q) == O.relin(e0, e1, e2, a, b, q)
+
+
+def test_rescale_coeff_drop_in(gpu):
+    from cipherpu import golden_api as G
+    rng = random.Random(1)
+    for i in range(3):
+        q_in, q_out = CHAIN4[i], CHAIN4[i + 1]
+        pts = [0, 1, 2, q_in - 3, q_in - 2, q_in - 1, q_out - 1, q_out, q_out + 1]
+        pts += [rng.randrange(q_in) for _ in range(4000)]
+        pts = [p for p in pts if 0 <= p < q_in]
+        assert G.rescale_poly(pts, q_in, q_out) == O.rescale_poly(pts, q_in, q_out)
+        assert G.rescale_coeff(q_in - 1, q_in, q_out) == O.rescale_coeff(q_in - 1, q_in, q_out)
+    # the other direction and a non-chain pair
+    assert G.rescale_poly([5, 10 ** 12], CHAIN4[1], CHAIN4[0]) == O.rescale_poly([5, 10 ** 12], CHAIN4[1], CHAIN4[0])
+
+
+@pytest.mark.parametrize("N", [4096, 8192])
+def test_rns_rescale(gpu, N):
+    from cipherpu.ring import RingContext, to_numpy
+    ring = RingContext(N, CHAIN4)
+    rng = np.random.default_rng(N)
+    for l in (2, 3, 4):
+        x = np.stack([rng.integers(0, q, (3, N), dtype=np.uint64) for q in CHAIN4[:l]], axis=1)
+        x[0, :, :4] = 0
+        x[0, :, 4:8] = np.array(CHAIN4[:l], dtype=np.uint64)[:, None] - 1
+        got = to_numpy(ring.rescale(ring.tensor(x)))
+        for p in range(3):
+            assert np.array_equal(got[p], OC.rescale(x[p], CHAIN4[:l]))
+        small = [list(map(int, x[1, i, :32])) for i in range(l)]
+        assert got[1, :, :32].tolist() == O.rns_rescale(small, CHAIN4[:l])
+
+
+def test_automorphism(gpu):
+    from cipherpu.ring import RingContext, to_numpy
+    from cipherpu import golden_api as G
+    N = 8192
+    ring = RingContext(N, CHAIN4)
+    rng = np.random.default_rng(3)
+    x = np.stack([rng.integers(0, q, (2, N), dtype=np.uint64) for q in CHAIN4], axis=1)
+    for g in (5, 25, 5 ** 7 % (2 * N), 2 * N - 1, 3):
+        got = to_numpy(ring.automorphism(ring.tensor(x), g))
+        for p in range(2):
+            for i in range(4):
+                assert np.array_equal(got[p, i], OC.automorphism(x[p, i], CHAIN4[i], g)), (g, p, i)
+    n, q = 16, 97
+    ct = ([rng.integers(0, q) for _ in range(n)], [rng.integers(0, q) for _ in range(n)])
+    a0, a1 = G.automorphism_exact(ct, 5, n, q)
+    assert list(a0) == O.automorphism(ct[0], 5, n, q) and list(a1) == O.automorphism(ct[1], 5, n, q)
+
+
+def test_noise_stats(gpu):
+    from cipherpu.ring import RingContext
+    N = 4096
+    ring = RingContext(N, CHAIN4[:2])
+    rng = np.random.default_rng(9)
+    x = np.stack([rng.integers(0, q, (2, N), dtype=np.uint64) for q in CHAIN4[:2]], axis=1)
+    st = ring.noise_stats(ring.tensor(x))
+    for p in range(2):
+        for i in range(2):
+            q = CHAIN4[i]
+            c = [int(v) for v in x[p, i]]
+            a = [min(v, q - v) for v in c]
+            row = st[p * 2 + i]
+            assert row[0] == sum(a) and row[1] == sum(v * v for v in a) and row[2] == max(a)
+            assert row[3] == sum(v if v <= q - v else v - q for v in c)
+            E, r = O.noise_monitor_energy(c, q)
+            assert row[1] / N == E
+
+
+# ---------------------------------------------------------------- CKKS ops
+
+@pytest.fixture(scope="module")
+def ck(gpu):
+    from cipherpu.ckks import CKKS, CKKSParams
+    c = CKKS(CKKSParams(), seed=11)
+    sk = c.keygen()
+    return c, sk
+
+
+def test_keygen_bit_exact(ck):
+    """b_j = a_j s + e_j + g_j s^2 per limb, against the oracle's products."""
+    from cipherpu.ring import to_numpy
+    c, sk = ck
+    rng = np.random.default_rng(4)
+    L, K, N = 4, 5, c.N
+    a = np.stack([rng.integers(0, q, (L, N), dtype=np.uint64) for q in c.primes], axis=1)
+    e = rng.integers(-6, 7, (L, N)).astype(np.int64)
+    rlk = c.relin_key(sk, a=c.ring.tensor(a), e=e)
+    got = to_numpy(rlk.coeff)
+    s = sk.s
+    for j in (0, 3):
+        for i in range(K):
+            q = c.primes[i]
+            srns = np.where(s < 0, q + s, s).astype(np.uint64)
+            want = OC.polymul_ntt(a[j, i], srns, q, c.ring.roots[i])
+            want = (want + np.where(e[j] < 0, q + e[j], e[j]).astype(np.uint64)) % np.uint64(q)
+            if i == j:
+                s2 = OC.polymul_ntt(srns, srns, q, c.ring.roots[i])
+                want = np.array([(int(w) + (SPECIAL_P % q) * int(v)) % q for w, v in zip(want, s2)], dtype=np.uint64)
+            assert np.array_equal(got[j, 0, i], want), (j, i)
+            assert np.array_equal(got[j, 1, i], a[j, i])
+
+
+def _oracle_args(c):
+    return c.primes[:4], c.primes[4], c.ring.roots[:4], c.ring.roots[4]
+
+
+def test_relin_keyswitch_bit_exact(ck):
+    from cipherpu.ring import to_numpy
+    c, sk = ck
+    rlk = c.relin_key(sk)
+    rng = np.random.default_rng(6)
+    N = c.N
+    for l in (4, 2):
+        d = [np.stack([rng.integers(0, q, (2, N), dtype=np.uint64) for q in c.primes[:l]], axis=1) for _ in range(3)]
+        out = to_numpy(c.relin(*(c.ring.tensor(v) for v in d), rlk))
+        key = to_numpy(rlk.coeff)
+        pr, p, ps, pp = _oracle_args(c)
+        for b in range(2):
+            k0, k1 = OC.keyswitch(d[2][b], key, pr[:l], p, ps[:l], pp)
+            want0 = (k0 + d[0][b]) % np.array(pr[:l], dtype=np.uint64)[:, None]
+            want1 = (k1 + d[1][b]) % np.array(pr[:l], dtype=np.uint64)[:, None]
+            assert np.array_equal(out[b, 0], want0) and np.array_equal(out[b, 1], want1), (l, b)
+
+
+@pytest.mark.parametrize("l", [4, 3, 2])
+def test_hmult_bit_exact(ck, l):
+    from cipherpu.ring import to_numpy
+    c, sk = ck
+    rlk = c.relin_key(sk)
+    rng = np.random.default_rng(l)
+    z = rng.uniform(-1, 1, (3, c.N // 2))
+    cta = c.encrypt(c.encode(z, level=l), sk)
+    ctb = c.encrypt(c.encode(z[::-1].copy(), level=l), sk)
+    out = c.hmult(cta, ctb, rlk)
+    got = to_numpy(out.data)
+    key = to_numpy(rlk.coeff)
+    pr, p, ps, pp = _oracle_args(c)
+    for b in range(3):
+        want = OC.hmult(to_numpy(cta.data[b]), to_numpy(ctb.data[b]), key, pr[:l], p, ps[:l], pp)
+        assert np.array_equal(got[b], want), b
+
+
+def test_hmult_uniform_inputs_and_small_n(gpu):
+    """Uniformly random (non-RLWE) inputs at N=4096: pure datapath parity,
+    as the reference's 'uniform' regime (exp_e33_real_ciphertext.py:160-170)."""
+    from cipherpu.ckks import CKKS, CKKSParams, Ciphertext
+    from cipherpu.ring import to_numpy
+    c = CKKS(CKKSParams(N=4096), seed=2)
+    sk = c.keygen()
+    rlk = c.relin_key(sk)
+    rng = np.random.default_rng(8)
+    mk = lambda: np.stack([rng.integers(0, q, (2, 2, 4096), dtype=np.uint64) for q in c.primes[:4]], axis=2)
+    a, b = mk(), mk()
+    out = to_numpy(c.hmult(Ciphertext(c.ring.tensor(a), 1.0), Ciphertext(c.ring.tensor(b), 1.0), rlk).data)
+    key = to_numpy(rlk.coeff)
+    pr, p, ps, pp = _oracle_args(c)
+    for i in range(2):
+        assert np.array_equal(out[i], OC.hmult(a[i], b[i], key, pr, p, ps, pp))
+
+
+def test_rotate_bit_exact_and_slots_move(ck):
+    from cipherpu.ring import to_numpy
+    c, sk = ck
+    rng = np.random.default_rng(10)
+    z = rng.uniform(-1, 1, (2, c.N // 2))
+    ct = c.encrypt(c.encode(z), sk)
+    pr, p, ps, pp = _oracle_args(c)
+    for step in (1, 16):
+        gk = c.galois_key(sk, step)
+        rot = c.rotate(ct, gk)
+        got = to_numpy(rot.data)
+        key = to_numpy(gk.coeff)
+        for b in range(2):
+            want = OC.rotate(to_numpy(ct.data[b]), key, pr, p, ps, pp, gk.galois)
+            assert np.array_equal(got[b], want), (step, b)
+        dec = to_numpy(c.decode(c.decrypt(rot, sk), rot.scale))
+        assert np.max(np.abs(dec.real - np.roll(z, -step, axis=1))) < 1e-6
+
+
+def test_encode_decode_vs_fp64_oracle(ck):
+    from cipherpu.ring import to_numpy
+    c, sk = ck
+    rng = np.random.default_rng(12)
+    z = rng.uniform(-1, 1, (2, c.N // 2)) + 1j * rng.uniform(-1, 1, (2, c.N // 2))
+    pt = to_numpy(c.encode(z))
+    for b in range(2):
+        m = np.rint(O.encode_fft(z[b] * 2.0 ** 40, c.N)).astype(np.int64)
+        want = np.where(m < 0, np.uint64(c.primes[0]) - (-m).astype(np.uint64), m.astype(np.uint64))
+        # FP64 rounding ties may differ by one unit in rare coefficients
+        diff = (pt[b, 0].astype(np.int64) - want.astype(np.int64)) % c.primes[0]
+        diff = np.minimum(diff, c.primes[0] - diff)
+        assert diff.max() <= 1 and (diff != 0).mean() < 1e-3
+    dec = to_numpy(c.decode(c.ring.tensor(pt), 2.0 ** 40))
+    assert np.max(np.abs(dec - z)) < 1e-9
+    # decode of a multi-limb plaintext equals the oracle's exact-CRT decode
+    want = O.decode_rns([list(map(int, pt[0, i])) for i in range(4)], 2.0 ** 40, c.N, c.primes[:4])
+    assert np.max(np.abs(dec[0] - want)) < 1e-9
+
+
+def test_natural_order_embedding_drop_in(gpu):
+    from cipherpu import golden_api as G
+    v = np.array([1.0, 2.0, 3.0, 4.0])
+    m = G.encode(v, 2.0, 8)
+    np.testing.assert_allclose(m, O.encode_natural(v, 2.0, 8), atol=1e-12)
+    np.testing.assert_allclose(G.decode(m, 2.0, 8, 4), v, atol=1e-12)
+    rng = np.random.default_rng(0)
+    mm = rng.normal(size=64)
+    np.testing.assert_allclose(G.neg_fft(mm), O.neg_fft(mm), atol=1e-10)
+
+
+def test_error_codes(gpu):
+    from cipherpu import _lib
+    from cipherpu.ring import RingContext
+    with pytest.raises(_lib.CkksError) as e:
+        RingContext(8192, [Q1])     # 2N = 16384 does not divide q-1
+    assert e.value.code == -2
+    ring = RingContext(4096, CHAIN4[:2])
+    x = ring.tensor(np.zeros((1, 3, 4096), dtype=np.uint64))
+    with pytest.raises(ValueError):
+        ring.ntt_(x)                # 3 limbs on a 2-prime context
