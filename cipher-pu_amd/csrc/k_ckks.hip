@@ -111,6 +111,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
             ntt::load_coalesced<C>(x, d2c + (size_t)b * d2c_bstride + (size_t)j * C::N, tid);
 #pragma unroll
             for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
+            // left unreduced (below 2^50): the key product is exact for any 64-bit input
             ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
         }
         const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;

@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(T) k_ntt_fwd(uint64_t* dst, const uint64_t* sr
     }
     ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) x[e] = reduce4q(x[e], pc);
+    for (int e = 0; e < C::E; ++e) x[e] = reduce64(x[e], pc);
     __syncthreads();
     ntt::store_eval<C>(x, lds, dst + off, tid);
 }
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, c
     ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
     ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) x[e] = mul_barrett(reduce4q(x[e], pc), reduce4q(y[e], pc), pc);
+    for (int e = 0; e < C::E; ++e) x[e] = mul_barrett(reduce64(x[e], pc), reduce64(y[e], pc), pc);
     ntt::inv_ntt<C>(x, lds, tw + C::N, pc, tid);
 #pragma unroll
     for (int e = 0; e < C::E; ++e) x[e] = csub(x[e], pc.q);

@@ -66,8 +66,8 @@ __device__ __forceinline__ uint64_t reduce64(uint64_t x, const PrimeConst& P) {
     return csub(x - t * P.q, P.q);
 }
 
-// Forward CT butterfly (natural -> bit-reversed order), inputs/outputs in
-// [0, 4q):  X' = X + W Y,  Y' = X - W Y.
+// Forward CT butterfly (natural -> bit-reversed order), Harvey's bounds:
+// inputs/outputs in [0, 4q).  X' = X + W Y,  Y' = X - W Y.
 __device__ __forceinline__ void ct_bfly(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t ws,
                                         uint64_t q, uint64_t two_q) {
     const uint64_t x = csub(X, two_q);
@@ -83,4 +83,34 @@ __device__ __forceinline__ void gs_bfly(uint64_t& X, uint64_t& Y, uint64_t w, ui
     const uint64_t x = X, y = Y;
     X = csub(x + y, two_q);
     Y = mul_shoup(x - y + two_q, w, ws, q);
+}
+
+// Shoup product with the subtraction folded into one multiply-add:
+// a*w + qt*(2^64 - q) = a*w - qt*q (mod 2^64).  nq = 2^64 - q.
+__device__ __forceinline__ uint64_t mul_shoup_nq(uint64_t a, uint64_t w, uint64_t ws, uint64_t nq) {
+    const uint64_t hi = __umul64hi(a, ws);
+    return a * w + hi * nq;
+}
+
+// Unreduced butterflies for the LDS-resident transforms.  Shoup's product is
+// exact in [0, 2q) for any 64-bit input, so no per-stage reduction is needed:
+//  * forward:  X, Y < B  ->  X' = X + WY < B + 2q,  Y' = X + 2q - WY < B + 2q.
+//    From inputs below 2q, after s stages every value is below (2 + 2s) q
+//    (< 2^50 for log2 N <= 14 and q < 2^45).
+//  * inverse:  X, Y < B (B a multiple of q)  ->  X' = X + Y < 2B,
+//    Y' = (X - Y + B) W < 2q.  From inputs below 2q the bound at stage s is
+//    2^(s+1) q (< 2^60 for log2 N <= 14).
+__device__ __forceinline__ void ct_bfly_lazy(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t ws, uint64_t nq,
+                                             uint64_t two_q) {
+    const uint64_t t = mul_shoup_nq(Y, w, ws, nq);
+    const uint64_t x = X;
+    X = x + t;
+    Y = x + two_q - t;
+}
+
+__device__ __forceinline__ void gs_bfly_lazy(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t ws, uint64_t nq,
+                                             uint64_t bound) {
+    const uint64_t x = X, y = Y;
+    X = x + y;
+    Y = mul_shoup_nq(x + bound - y, w, ws, nq);
 }

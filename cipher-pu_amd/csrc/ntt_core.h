@@ -66,6 +66,7 @@ __device__ __forceinline__ int index_of(int g, int e, int tid) {
 template <class C, int P>
 __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ tw,
                                          uint64_t q, uint64_t two_q, int tid) {
+    const uint64_t nq = (uint64_t)0 - q;
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
@@ -80,7 +81,7 @@ __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* 
                 if (e & h) continue;
                 const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
                 const ulonglong2 w = tw[base + i];
-                ct_bfly(x[g * M + e], x[g * M + e + h], w.x, w.y, q, two_q);
+                ct_bfly_lazy(x[g * M + e], x[g * M + e + h], w.x, w.y, nq, two_q);
             }
         }
     }
@@ -91,7 +92,7 @@ __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* 
                                          const PrimeConst& pc, int tid) {
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
-    const uint64_t q = pc.q, two_q = pc.two_q;
+    const uint64_t q = pc.q, nq = (uint64_t)0 - q;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int fh = (g * C::T + tid) >> LO;
@@ -99,6 +100,7 @@ __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* 
         for (int b = LO; b <= HI; ++b) {
             const int h = 1 << (b - LO);
             const int base = 1 << (C::LOGN - 1 - b);
+            const uint64_t bound = q << (b + 1);   // every input of stage b is below 2^(b+1) q
 #pragma unroll
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
@@ -107,12 +109,12 @@ __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* 
                 if (b == C::LOGN - 1) {
                     // last stage: fold N^-1 into both outputs
                     const uint64_t u = X, v = Y;
-                    X = mul_shoup(u + v, pc.ninv, pc.ninv_s, q);
-                    Y = mul_shoup(u - v + two_q, pc.ninv_w1, pc.ninv_w1_s, q);
+                    X = mul_shoup_nq(u + v, pc.ninv, pc.ninv_s, nq);
+                    Y = mul_shoup_nq(u + bound - v, pc.ninv_w1, pc.ninv_w1_s, nq);
                 } else {
                     const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
                     const ulonglong2 w = itw[base + i];
-                    gs_bfly(X, Y, w.x, w.y, q, two_q);
+                    gs_bfly_lazy(X, Y, w.x, w.y, nq, bound);
                 }
             }
         }
@@ -152,8 +154,9 @@ __device__ __forceinline__ void fwd_rest(uint64_t (&x)[C::E], uint64_t* lds,
     }
 }
 
-// Full forward transform: x holds pass-0 layout on entry (x[e] = a[e*T+tid]),
-// the last pass's layout on exit, values in [0, 4q).
+// Full forward transform: x holds pass-0 layout on entry (x[e] = a[e*T+tid],
+// values below 2q), the last pass's layout on exit, values below
+// (2 + 2 log2 N) q -- reduce with reduce64.
 template <class C>
 __device__ __forceinline__ void fwd_ntt(uint64_t (&x)[C::E], uint64_t* lds,
                                         const ulonglong2* __restrict__ tw, uint64_t q,
@@ -177,7 +180,7 @@ __device__ __forceinline__ void inv_rest(uint64_t (&x)[C::E], uint64_t* lds,
 }
 
 // Full inverse transform: x holds the last forward pass's layout on entry
-// (values in [0, 2q)), pass-0 (coalesced) layout on exit, values in [0, 2q).
+// (values below 2q), pass-0 (coalesced) layout on exit, values in [0, 2q).
 template <class C>
 __device__ __forceinline__ void inv_ntt(uint64_t (&x)[C::E], uint64_t* lds,
                                         const ulonglong2* __restrict__ itw, const PrimeConst& pc,
