@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ks-threads", type=int, default=0)
     ap.add_argument("--ntt-threads", type=int, default=0)
+    ap.add_argument("--tensor-threads", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape tuner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -117,7 +119,6 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     B = args.batch
     ck = CKKS(CKKSParams(), seed=2024)
-    ck.ring.set_threads(args.ntt_threads, args.ks_threads)
     sk = ck.keygen()
     rlk = ck.relin_key(sk)
     rng = np.random.default_rng(7)
@@ -125,6 +126,12 @@ def main():
     zb = rng.uniform(-1, 1, (B, N // 2))
     cta = ck.encrypt(ck.encode(za), sk)
     ctb = ck.encrypt(ck.encode(zb), sk)
+    tuned = None
+    if args.ntt_threads or args.ks_threads or args.tensor_threads or args.no_tune:
+        ck.ring.set_threads(args.ntt_threads, args.ks_threads, args.tensor_threads)
+    else:
+        from cipherpu.tune import tune_hmult
+        tuned = tune_hmult(ck, cta, ctb, rlk, reps=3)
     out = ck.hmult(cta, ctb, rlk)
     torch.cuda.synchronize()
     # decrypted product vs the FP64 slot-wise product (all pairs)
@@ -169,6 +176,8 @@ def main():
                      "kernel": "ckks_hmult step (5 launches: 2x ntt_fwd, tensor_intt, keyswitch, moddown_rescale)",
                      "algorithmic_bytes_per_step": nbytes},
         "max_abs_err_vs_fp64": max_err,
+        "launch_shape": ({k: tuned[k] for k in ("ntt", "tensor", "ks")} if tuned else
+                         {"ntt": args.ntt_threads, "tensor": args.tensor_threads, "ks": args.ks_threads}),
         "wall_s_timed": round(wall, 4),
         "device": torch.cuda.get_device_name(0),
     }

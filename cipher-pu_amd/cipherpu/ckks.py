@@ -133,6 +133,18 @@ class CKKS:
         call("ckks_key_prepare", self.ring.handle, _ptr(work), _ptr(prepared), L, K, self.sp, self.ring.stream())
         return SwitchKey(coeff=coeff, prepared=prepared, galois=galois)
 
+    def load_switch_key(self, coeff, galois: int = 0) -> SwitchKey:
+        """Key from its coefficient form [L, 2, L+1, N] (b_j, a_j) -- e.g. read
+        back from disk -- prepared for the kernels."""
+        coeff = self.ring.tensor(coeff)
+        L, K = self.L, self.L + 1
+        if tuple(coeff.shape) != (L, 2, K, self.N):
+            raise ValueError(f"key must be [{L}, 2, {K}, {self.N}]")
+        prepared = torch.empty(L, 2, K, self.N, 2, dtype=torch.uint64, device=self.device)
+        work = coeff.clone()
+        call("ckks_key_prepare", self.ring.handle, _ptr(work), _ptr(prepared), L, K, self.sp, self.ring.stream())
+        return SwitchKey(coeff=coeff, prepared=prepared, galois=galois)
+
     def relin_key(self, sk: SecretKey, **kw) -> SwitchKey:
         s2 = self.ring.poly_mul(sk.rns, sk.rns)
         return self.switch_key(sk, s2, 0, **kw)
@@ -255,3 +267,38 @@ class CKKS:
     def drop_levels(self, ct: Ciphertext, l: int) -> Ciphertext:
         """Keep the first l limbs (modulus reduction without division)."""
         return Ciphertext(ct.data[:, :, :l].contiguous(), ct.scale)
+
+
+# -- serialisation (keys, ciphertexts) ------------------------------------------
+
+def save(path: str, **objs) -> None:
+    """Write ciphertexts / switch keys / secret keys to one .npz (raw uint64)."""
+    out = {}
+    for name, o in objs.items():
+        if isinstance(o, Ciphertext):
+            out[f"{name}.ct"] = to_numpy(o.data)
+            out[f"{name}.scale"] = np.array([o.scale])
+        elif isinstance(o, SwitchKey):
+            out[f"{name}.ksk"] = to_numpy(o.coeff)
+            out[f"{name}.galois"] = np.array([o.galois], dtype=np.uint64)
+        elif isinstance(o, SecretKey):
+            out[f"{name}.sk"] = o.s
+        else:
+            raise TypeError(f"cannot save {type(o).__name__}")
+    np.savez(path, **out)
+
+
+def load(path: str, ck: "CKKS") -> dict:
+    """Inverse of ``save``; keys are prepared for ck's kernels."""
+    d = np.load(path, allow_pickle=False)
+    res = {}
+    for k in d.files:
+        name, kind = k.rsplit(".", 1)
+        if kind == "ct":
+            res[name] = Ciphertext(ck.ring.tensor(d[k]), float(d[name + ".scale"][0]))
+        elif kind == "ksk":
+            res[name] = ck.load_switch_key(d[k], int(d[name + ".galois"][0]))
+        elif kind == "sk":
+            s = d[k].astype(np.int64)
+            res[name] = SecretKey(s=s, rns=ck.small_to_rns(s, range(len(ck.primes))))
+    return res

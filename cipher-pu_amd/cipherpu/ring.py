@@ -73,8 +73,11 @@ class RingContext:
     def handle(self):
         return self._h
 
-    def set_threads(self, ntt_threads: int = 0, ks_threads: int = 0):
-        call("ckks_ctx_set_threads", self._h, ntt_threads, ks_threads)
+    def set_threads(self, ntt_threads: int = 0, ks_threads: int = 0, tensor_threads: int = 0):
+        call("ckks_ctx_set_threads", self._h, ntt_threads, ks_threads, tensor_threads)
+
+    def set_front(self, fused: bool = True):
+        call("ckks_ctx_set_front", self._h, 1 if fused else 0)
 
     # -- helpers ---------------------------------------------------------
     def stream(self):

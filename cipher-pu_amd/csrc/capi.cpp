@@ -68,7 +68,8 @@ uint64_t primitive_root(uint64_t q) {
     }
 }
 
-uint64_t shoup(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+// Shoup companion floor(w 2^63 / q) (see mul_shoup in modarith.h)
+uint64_t shoup(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 63) / q); }
 
 uint32_t bitrev(uint32_t i, int bits) {
     uint32_t r = 0;
@@ -90,7 +91,8 @@ struct ckks_ctx {
     double2* fft_tw = nullptr;    // exp(2 pi i k / n), k < n/2, n = N/2
     double2* fft_zeta = nullptr;  // exp(i pi j / N), j < n
     int* slot_k = nullptr;        // (5^j mod 2N - 1)/4
-    int ntt_threads = 0, ks_threads = 0;
+    int ntt_threads = 0, ks_threads = 0, tensor_threads = 0;
+    int front_fused = 0;   // HMult front: one fused kernel (1) or 2 transforms + tensor (0)
     DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN}; }
 };
 
@@ -307,10 +309,19 @@ int ckks_ctx_root(const ckks_ctx* c, int prime, uint64_t* root) {
     return CKKS_OK;
 }
 
-int ckks_ctx_set_threads(ckks_ctx* c, int ntt_threads, int ks_threads) {
+int ckks_ctx_set_front(ckks_ctx* c, int fused) {
     if (!c) return CKKS_ERR_NULL;
+    c->front_fused = fused ? 1 : 0;
+    return CKKS_OK;
+}
+
+int ckks_ctx_set_threads(ckks_ctx* c, int ntt_threads, int ks_threads, int tensor_threads) {
+    if (!c) return CKKS_ERR_NULL;
+    for (int t : {ntt_threads, ks_threads, tensor_threads})
+        if (t != 0 && t != 128 && t != 256 && t != 512 && t != 1024) return CKKS_ERR_BAD_ARG;
     c->ntt_threads = ntt_threads;
     c->ks_threads = ks_threads;
+    c->tensor_threads = tensor_threads;
     return CKKS_OK;
 }
 
@@ -447,9 +458,13 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
     uint64_t* d2e = d01 + B * 2 * LN;
     uint64_t* d2c = d2e + B * LN;
     uint64_t* acc = d2c + B * LN;
-    HIPCHK(launch_ntt(t, ea, a, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
-    HIPCHK(launch_ntt(t, eb, b, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
-    HIPCHK(launch_tensor_intt(t, ea, eb, d01, d2e, d2c, batch, l, m, mc, c->ntt_threads, S(stream)));
+    if (c->front_fused) {
+        HIPCHK(launch_front(t, a, b, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
+    } else {
+        HIPCHK(launch_ntt(t, ea, a, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+        HIPCHK(launch_ntt(t, eb, b, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+        HIPCHK(launch_tensor_intt(t, ea, eb, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
+    }
     HIPCHK(launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc, batch, l, m, special,
                             c->ks_threads, S(stream)));
     HIPCHK(launch_moddown_rescale(t, acc, nullptr, 0, out, batch, l, m, special, mc, true, S(stream)));

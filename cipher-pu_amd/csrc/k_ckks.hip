@@ -62,6 +62,61 @@ __global__ void __launch_bounds__(T) k_tensor_intt(const uint64_t* __restrict__ 
     ntt::store_coalesced<C>(x, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
 }
 
+// The whole front of HMult for one (pair, limb i), coefficient form in:
+// forward transforms of a0, a1, b0, b1, the tensor product, p*d0 and p*d1
+// and d2 written in evaluation form, d2 transformed back to coefficient form.
+// Limb-major grid: workgroups in flight share one prime's twiddles.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
+                                             uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
+                                             uint64_t* __restrict__ d2c, const ulonglong2* __restrict__ tw_all,
+                                             const PrimeConst* __restrict__ pcs, LimbMap map, int l, int batch,
+                                             ModDownConst mc) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x / batch, b = blockIdx.x % batch;
+    const int p = map.idx[i];
+    const PrimeConst pc = pcs[p];
+    const ulonglong2* tw = tw_all + (size_t)p * 2 * C::N;
+    const size_t LN = (size_t)l * C::N;
+    const size_t c0 = (size_t)b * 2 * LN + (size_t)i * C::N;   // poly 0 of pair b, limb i
+    const uint64_t pm = mc.p_mod[i], pms = mc.p_mod_s[i];
+    uint64_t x0[C::E], x1[C::E], y[C::E], z[C::E];
+    ntt::load_coalesced<C>(x0, A + c0, tid);
+    ntt::fwd_ntt<C>(x0, lds, tw, pc.q, pc.two_q, tid);
+    ntt::load_coalesced<C>(x1, A + c0 + LN, tid);
+    ntt::fwd_ntt<C>(x1, lds, tw, pc.q, pc.two_q, tid);
+    ntt::load_coalesced<C>(y, B + c0, tid);
+    ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
+    uint64_t* o0 = d01 + c0;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const uint64_t a0 = reduce64(x0[k], pc), a1 = reduce64(x1[k], pc), b0 = reduce64(y[k], pc);
+        o0[ntt::eval_index<C>(k, tid)] = csub(mul_shoup(mul_barrett(a0, b0, pc), pm, pms, pc.q), pc.q);
+        z[k] = mul_barrett(a1, b0, pc);
+        x0[k] = a0;
+        x1[k] = a1;
+    }
+    ntt::load_coalesced<C>(y, B + c0 + LN, tid);
+    ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
+    uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const uint64_t b1 = reduce64(y[k], pc);
+        const int idx = ntt::eval_index<C>(k, tid);
+        const uint64_t d1 = add_mod(z[k], mul_barrett(x0[k], b1, pc), pc.q);
+        o0[LN + idx] = csub(mul_shoup(d1, pm, pms, pc.q), pc.q);
+        const uint64_t d2 = mul_barrett(x1[k], b1, pc);
+        o2[idx] = d2;
+        x0[k] = d2;
+    }
+    ntt::inv_ntt<C>(x0, lds, tw + C::N, pc, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) x0[k] = csub(x0[k], pc.q);
+    ntt::store_coalesced<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
+}
+
 // Hybrid key switch for one (ciphertext, target limb t) pair, t = 0..l where
 // t = l is the special prime p.  Generalises the single-digit relinearisation
 // f0 = e0 + e2 b, f1 = e1 - e2 a (verify_keyswitch.py:62-68,
@@ -78,11 +133,15 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
                                                  const uint64_t* __restrict__ d01, const ulonglong2* __restrict__ key,
                                                  int key_limbs, uint64_t* __restrict__ acc,
                                                  const ulonglong2* __restrict__ tw_all,
-                                                 const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p) {
+                                                 const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
+                                                 int batch) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
-    const int b = blockIdx.x / (l + 1), t = blockIdx.x % (l + 1);
+    // Target-major order: workgroups in flight together share one target
+    // limb, so each XCD's L2 holds one limb's key slice (2 dnum N 16 B) and
+    // one prime's twiddles rather than the whole key.
+    const int t = blockIdx.x / batch, b = blockIdx.x % batch;
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeConst pc = pcs[pt];
@@ -309,6 +368,17 @@ hipError_t run_tensor(const DevTables& t, const uint64_t* ea, const uint64_t* eb
 }
 
 template <int LOGN, int T>
+hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
+                     uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_front<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(batch * l), dim3(T), C::LDS_BYTES, s, a, b, d01, d2e, d2c, t.tw,
+                       t.pc, map, l, batch, mc);
+    return hipGetLastError();
+}
+
+template <int LOGN, int T>
 hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uint64_t* d2e, const uint64_t* d01,
                   const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
                   int pidx_p, hipStream_t s) {
@@ -316,7 +386,7 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
     hipError_t e = set_lds(k_keyswitch<LOGN, T>, C::LDS_BYTES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_keyswitch<LOGN, T>), dim3(batch * (l + 1)), dim3(T), C::LDS_BYTES, s, d2c, d2s, d2e, d01, key,
-                       key_limbs, acc, t.tw, t.pc, map, l, pidx_p);
+                       key_limbs, acc, t.tw, t.pc, map, l, pidx_p, batch);
     return hipGetLastError();
 }
 
@@ -337,6 +407,23 @@ hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint
         case 131024: return run_tensor<13, 1024>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
         case 140512: return run_tensor<14, 512>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
         case 141024: return run_tensor<14, 1024>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
+                        uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, int threads,
+                        hipStream_t s) {
+    if (batch * l == 0) return hipSuccess;
+    if (!threads) threads = default_threads(t.logN);
+    switch (t.logN * 10000 + threads) {
+        case 120256: return run_front<12, 256>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
+        case 120512: return run_front<12, 512>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
+        case 130256: return run_front<13, 256>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
+        case 130512: return run_front<13, 512>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
+        case 131024: return run_front<13, 1024>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
+        case 140512: return run_front<14, 512>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
+        case 141024: return run_front<14, 1024>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -203,7 +203,7 @@ __global__ void k_pointwise(const uint64_t* __restrict__ a, const uint64_t* __re
     out[g] = r;
 }
 
-// (w, floor(w 2^64 / q)) pairs for key material; w < q < 2^45, the quotient
+// (w, floor(w 2^63 / q)) pairs for key material; w < q < 2^45, the quotient
 // by restoring division.
 __global__ void k_shoup_pairs(const uint64_t* __restrict__ in, ulonglong2* __restrict__ out,
                               const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, int logN, size_t total) {
@@ -212,7 +212,7 @@ __global__ void k_shoup_pairs(const uint64_t* __restrict__ in, ulonglong2* __res
     const uint64_t q = pcs[map.idx[(g >> logN) % n_limbs]].q;
     const uint64_t w = in[g];
     uint64_t r = w, quo = 0;
-    for (int k = 0; k < 64; ++k) {
+    for (int k = 0; k < 63; ++k) {
         r <<= 1;
         quo <<= 1;
         if (r >= q) {

@@ -46,6 +46,9 @@ struct ModDownConst {
 hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint64_t* eb, uint64_t* d01,
                               uint64_t* d2e, uint64_t* d2c, int batch, int l, const LimbMap& map,
                               const ModDownConst& mc, int threads, hipStream_t s);
+hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
+                        uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, int threads,
+                        hipStream_t s);
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01, const ulonglong2* key, int key_limbs, uint64_t* acc,
                             int batch, int l, const LimbMap& map, int pidx_special, int threads,

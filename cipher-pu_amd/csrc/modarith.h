@@ -5,8 +5,9 @@
 // rtl/m2/mod45_barrett.sv:49-75).  The RTL reduction needs the special form
 // q = 2^44 + c; these routines work for any prime below 2^45.
 //
-//  * mul_shoup: a * w mod q for a fixed operand w with w' = floor(w 2^64 / q)
-//    precomputed (twiddles, key material, constants).  Result in [0, 2q).
+//  * mul_shoup: a * w mod q for a fixed operand w with the companion
+//    w' = floor(w 2^63 / q) precomputed (twiddles, key material, constants).
+//    Result in [0, 2q) for any a < 2^62.
 //  * mul_barrett: a * b mod q for two data operands (a, b < q), result in
 //    [0, q).  Barrett with mu = floor(2^(2l) / q), l = bit length of q.
 //  * the forward (Cooley-Tukey) and inverse (Gentleman-Sande) butterflies
@@ -36,9 +37,23 @@ struct LimbMap {
 };
 
 
+// floor(a * ws / 2^63) for a < 2^62 and ws < 2^63.  With ws below 2^63 the
+// middle partial sum a0*ws1 + (a1*ws0 + hi(a0*ws0)) stays below 2^64, so
+// the 64-bit multiply-adds chain without a carry word.
+__device__ __forceinline__ uint64_t mulhi63(uint64_t a, uint64_t ws) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t b0 = (uint32_t)ws, b1 = (uint32_t)(ws >> 32);
+    const uint64_t f = (uint64_t)a0 * b0;
+    const uint64_t m = (uint64_t)a1 * b0 + (f >> 32);
+    const uint64_t n = (uint64_t)a0 * b1 + m;
+    const uint64_t h = (uint64_t)a1 * b1 + (n >> 32);
+    return (h << 1) | ((uint32_t)n >> 31);
+}
+
+// Shoup: a w - floor(a ws / 2^63) q.  The quotient estimate is low by at most
+// one (a w/q - a ws/2^63 < a / 2^63 < 1), so the result lies in [0, 2q).
 __device__ __forceinline__ uint64_t mul_shoup(uint64_t a, uint64_t w, uint64_t ws, uint64_t q) {
-    const uint64_t hi = __umul64hi(a, ws);
-    return a * w - hi * q;  // exact modulo 2^64, lies in [0, 2q)
+    return a * w - mulhi63(a, ws) * q;
 }
 
 __device__ __forceinline__ uint64_t csub(uint64_t a, uint64_t m) { return a >= m ? a - m : a; }
@@ -85,11 +100,10 @@ __device__ __forceinline__ void gs_bfly(uint64_t& X, uint64_t& Y, uint64_t w, ui
     Y = mul_shoup(x - y + two_q, w, ws, q);
 }
 
-// Shoup product with the subtraction folded into one multiply-add:
+// Shoup product with the subtraction written as a multiply-add:
 // a*w + qt*(2^64 - q) = a*w - qt*q (mod 2^64).  nq = 2^64 - q.
 __device__ __forceinline__ uint64_t mul_shoup_nq(uint64_t a, uint64_t w, uint64_t ws, uint64_t nq) {
-    const uint64_t hi = __umul64hi(a, ws);
-    return a * w + hi * nq;
+    return a * w + mulhi63(a, ws) * nq;
 }
 
 // Unreduced butterflies for the LDS-resident transforms.  Shoup's product is

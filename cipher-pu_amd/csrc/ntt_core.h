@@ -15,7 +15,7 @@
 //  * Forward: Cooley-Tukey, natural order in, bit-reversed order out, twiddle
 //    for stage s / group i at tw[2^s + i] (psi^bitrev(2^s+i) for the
 //    negacyclic ring).  Inverse: Gentleman-Sande with the inverse table and
-//    N^-1 folded into the last stage.  Twiddles are (w, floor(w 2^64/q))
+//    N^-1 folded into the last stage.  Twiddles are (w, floor(w 2^63/q))
 //    pairs, one 16-byte load each.
 //  * LDS index pad(j) = j + (j >> PS) keeps the small-stride pass
 //    conflict-free (PS = size of the last forward pass).

@@ -174,9 +174,15 @@ int ckks_embed_inv(const ckks_ctx *ctx, const double *slots, double *coeffs, int
 int ckks_noise_stats(const ckks_ctx *ctx, const uint64_t *data, uint64_t *stats, int n_polys, int n_limbs,
                      int prime0, void *stream);
 
-/* Threads per workgroup for the LDS-resident kernels of this context
- * (0 = default; see the tuner in the Python package). */
-int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads);
+/* Threads per workgroup (0 = default, else 128/256/512/1024 where built)
+ * for this context's LDS-resident kernels: forward/inverse transforms and
+ * products, the key switch, and the tensor-product kernel.  Set by the
+ * tuner in cipherpu.tune. */
+int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads, int tensor_threads);
+/* HMult front end: 1 = one fused kernel per (pair, limb) doing the four
+ * forward transforms, the tensor product and INTT(d2); 0 (default) =
+ * separate transform launches then the tensor kernel.  Same results. */
+int ckks_ctx_set_front(ckks_ctx *ctx, int fused);
 
 #ifdef __cplusplus
 }

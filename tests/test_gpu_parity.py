@@ -354,3 +354,48 @@ def test_error_codes(gpu):
     x = ring.tensor(np.zeros((1, 3, 4096), dtype=np.uint64))
     with pytest.raises(ValueError):
         ring.ntt_(x)                # 3 limbs on a 2-prime context
+
+
+def test_committed_golden_vectors(gpu):
+    """The HIP path against tests/golden/golden_vectors.npz (made by
+    tests/golden/make_golden.py from the oracle)."""
+    import hashlib
+    import os
+    import sys
+    from cipherpu.ckks import CKKS, CKKSParams, Ciphertext
+    from cipherpu.ring import RingContext, to_numpy
+    here = os.path.dirname(os.path.abspath(__file__))
+    g = np.load(os.path.join(here, "golden", "golden_vectors.npz"))
+    ring1 = RingContext(4096, [Q1], roots=[PSI1])
+    c = to_numpy(ring1.poly_mul(ring1.tensor(g["cfg1_a"][None, None]), ring1.tensor(g["cfg1_b"][None, None])))
+    assert np.array_equal(c[0, 0], g["cfg1_c"])
+    ring = RingContext(8192, CHAIN4)
+    assert np.array_equal(to_numpy(ring.ntt(g["ntt8192_q0_in"][None, None], natural=True))[0, 0],
+                          g["ntt8192_q0_out"])
+    r64 = RingContext(64, CHAIN4)
+    assert np.array_equal(to_numpy(r64.rescale(r64.tensor(g["rescale_in"][None]))[0]), g["rescale_out"])
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_golden
+    s, key, cta, ctb, _ = make_golden.hmult_case(8192, 4, int(g["hmult8192_seed"][0]))
+    ck = CKKS(CKKSParams())
+    rlk = ck.load_switch_key(key)
+    out = ck.hmult(Ciphertext(ck.ring.tensor(cta[None]), 1.0), Ciphertext(ck.ring.tensor(ctb[None]), 1.0), rlk)
+    d = hashlib.sha256(np.ascontiguousarray(to_numpy(out.data)[0], dtype="<u8").tobytes()).digest()
+    assert d == bytes(g["hmult8192_out_sha256"])
+
+
+def test_serialisation_roundtrip(ck, tmp_path):
+    from cipherpu import ckks as CK
+    from cipherpu.ring import to_numpy
+    c, sk = ck
+    rlk = c.relin_key(sk)
+    z = np.random.default_rng(1).uniform(-1, 1, (1, c.N // 2))
+    ct = c.encrypt(c.encode(z), sk)
+    p = str(tmp_path / "state.npz")
+    CK.save(p, ct=ct, rlk=rlk, sk=sk)
+    back = CK.load(p, c)
+    assert np.array_equal(to_numpy(back["ct"].data), to_numpy(ct.data)) and back["ct"].scale == ct.scale
+    assert np.array_equal(to_numpy(back["rlk"].prepared), to_numpy(rlk.prepared))
+    out1 = to_numpy(c.hmult(ct, ct, rlk).data)
+    out2 = to_numpy(c.hmult(back["ct"], back["ct"], back["rlk"]).data)
+    assert np.array_equal(out1, out2)
