@@ -61,6 +61,14 @@ _SIGS = {
     "ckks_embed": (_I, [_P, _P, _P, _I, _I, _P]),
     "ckks_embed_inv": (_I, [_P, _P, _P, _I, _I, _P]),
     "ckks_noise_stats": (_I, [_P, _P, _P, _I, _I, _I, _P]),
+    "ckks_pt_prepare": (_I, [_P, _P, _P, _I, _I, _I, _P]),
+    "ckks_lincomb": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    "ckks_lift_small": (_I, [_P, _P, _P, _I, _I, _I, _P]),
+    "ckks_gear_dense_fwd": (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _P]),
+    "ckks_gear_dense_bwd": (_I, [_P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    "ckks_gear_adam": (_I, [_P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                            _I, _P]),
+    "ckks_gear_mse_grad": (_I, [_P, _P, _P, _P, _I, _I, _P]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -69,10 +77,13 @@ _lib = None
 
 
 def load(path: str = LIB_PATH) -> ctypes.CDLL:
-    """Load the HIP library (no GPU call is made)."""
+    """Load the HIP library (no GPU call is made).  torch is imported first so
+    that the library binds to the HIP runtime torch already carries (same
+    soname) instead of loading a second runtime into the process."""
     global _lib
     if _lib is not None:
         return _lib
+    import torch  # noqa: F401  (HIP runtime first)
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build the HIP library first (python -m cipherpu.build); "

@@ -14,9 +14,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(PKG_ROOT, "csrc")
+INCLUDE = os.path.join(os.path.dirname(PKG_ROOT), "include")
 BUILD = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(HERE, "libcipherpu_hip.so")
-SOURCES = ["k_ntt.hip", "k_ckks.hip", "k_fft.hip", "capi.cpp"]
+SOURCES = ["k_ntt.hip", "k_ckks.hip", "k_fft.hip", "k_gear.hip", "capi.cpp"]
 ARCH = os.environ.get("CIPHERPU_ARCH", "gfx950")
 
 
@@ -30,7 +31,7 @@ def hipcc() -> str:
 def flags():
     return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-command-line-argument", "-I", CSRC]
+            "-Wno-unused-command-line-argument", "-I", CSRC, "-I", INCLUDE]
 
 
 def build(verbose: bool = False) -> str:

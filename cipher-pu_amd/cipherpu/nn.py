@@ -1,0 +1,326 @@
+"""Encrypted MLP 784-128-10 on the CKKS core (x^2 activation, DARE refresh,
+block-sparsity gears), with the diagonal matrix-vector method.
+
+Packing.  N = 8192 gives 4096 slots; a ciphertext carries S = 4 samples in
+1024-slot blocks.  In each block the 784 pixels sit in slots [0, 784),
+zeros in [784, 896) and a copy of slots [0, 128) in [896, 1024), so that
+every rotation by k < 128 used below stays inside the sample's block.
+
+Layer 1 (W1: 128 x 784, padded to 128 x 896).  Hybrid diagonal method:
+    z = sum_(k<128) d_k * rot(x, k),   d_k[i] = W1[i mod 128][(i + k) mod 896]
+for i < 896 (0 on [896, 1024)), then y_j = sum_(t<8) z[j + 128 t] by
+rotate-and-sum over 512, 256, 128.  k = 16 g + b is evaluated baby-step /
+giant-step: z = sum_g rot(sum_b rot(d_(16g+b), -16g) * rot(x, b), 16 g), so
+only 15 + 7 rotations are needed; the plaintext products run in the
+evaluation domain (ckks_lincomb).  The reference computes the same layer
+as coefficient-encoded dot products (scripts/exp_e11_ckks_e2e_real.py:126-166).
+
+Then: mask slots [0, 128) of each block (the mask's scale is chosen so that
+the rescale returns the ciphertext to the base scale 2^40), copy the 128
+outputs to [128, 256) so that layer 2's rotations wrap, DARE refresh (decrypt, keep the scale-carrying integer message, encrypt
+afresh at the top level; exp_e24_rotation_bootstrap.py:287-304), x^2 by
+HMult + relinearise + rescale, and layer 2 (W2: 10 x 128 padded to 16 x 128)
+with 4 x 4 baby/giant steps and folds over 64, 32, 16.
+
+Gears (block sparsity, verify_m4_a_gear_ctrl.py:41-57, exp_e10:102-124):
+gear g in {0, 1, 2, 3} zeroes {0, 25, 50, 75} % of the 64-weight blocks
+(row-major runs of the flattened [out][in] weight matrix) with the smallest
+max |w| -- the quantile form of the reference's max|w| < threshold rule.
+The pruned matrix runs through the same diagonal plan; a diagonal that
+pruning empties is skipped (no plaintext product; its baby rotation is
+skipped when no giant step needs it).  Under diagonal packing the cost is
+set by the rotations, which fine-grained pruning rarely removes, so the
+gears change the output SNR far more than the cost.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from ._lib import call
+from .ckks import CKKS, Ciphertext, SecretKey, SwitchKey
+from .ring import to_numpy
+
+SLOTS_PER_SAMPLE = 1024
+GEAR_FRACTIONS = (0.0, 0.25, 0.50, 0.75)
+BLOCK = 64
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------- plaintext side
+
+def synthetic_images(n: int, seed: int = 0) -> np.ndarray:
+    """n synthetic 28x28 images, uint8 uniform in [0, 255], scaled to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 256, size=(n, 784), dtype=np.uint8).astype(np.float64) / 255.0
+
+
+def make_mlp(seed: int = 0) -> Tuple[np.ndarray, np.ndarray]:
+    """784-128-10 weights ~ N(0, 1/fan_in), FP64, no biases."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.normal(0.0, 1.0 / math.sqrt(784), (128, 784))
+    W2 = rng.normal(0.0, 1.0 / math.sqrt(128), (10, 128))
+    return W1, W2
+
+
+def plain_forward(W1, W2, X) -> np.ndarray:
+    return (W2 @ ((W1 @ X.T) ** 2)).T
+
+
+def extended_diagonals(W: np.ndarray, rows: int, cols: int) -> np.ndarray:
+    """d_k[i] = W[i mod rows][(i + k) mod cols] for k < rows, i < cols
+    (W zero-padded to rows x cols)."""
+    P = np.zeros((rows, cols))
+    P[: W.shape[0], : W.shape[1]] = W
+    i = np.arange(cols)
+    return np.stack([P[i % rows, (i + k) % cols] for k in range(rows)])
+
+
+def block_keep_mask(W: np.ndarray, frac: float, block: int = BLOCK) -> np.ndarray:
+    """Boolean keep-mask of W: the `frac` share of `block`-weight blocks
+    (row-major runs of the flattened matrix, the last one short) with the
+    smallest max |w| is dropped; ties keep the earlier block
+    (scripts/exp_e10_sparsity_sweep_v2.py:102-124, quantile form)."""
+    flat = np.abs(np.asarray(W, dtype=np.float64)).ravel()
+    nb = -(-flat.size // block)
+    padded = np.zeros(nb * block)
+    padded[: flat.size] = flat
+    bmax = padded.reshape(nb, block).max(axis=1)
+    k = int(round(frac * nb))
+    keep = np.ones(nb, dtype=bool)
+    if k:
+        keep[np.argsort(bmax, kind="stable")[:k]] = False
+    return np.repeat(keep, block)[: flat.size].reshape(np.shape(W))
+
+
+def weights_from_diagonals(D: np.ndarray, rows: int, cols: int, shape) -> np.ndarray:
+    """Inverse of extended_diagonals (to evaluate pruned networks in FP64)."""
+    P = np.zeros((rows, cols))
+    i = np.arange(cols)
+    for k in range(rows):
+        P[i % rows, (i + k) % cols] = D[k]
+    return P[: shape[0], : shape[1]]
+
+
+# ---------------------------------------------------------------- encrypted layers
+
+@dataclass
+class LayerPlan:
+    rows: int            # output dimension (padded)
+    cols: int            # input span of the diagonals (padded)
+    baby: int
+    giant: int
+    folds: Tuple[int, ...]
+    D: torch.Tensor = None          # [ng][nb][l][N][2] prepared plaintext rows
+    used_b: List[int] = field(default_factory=list)
+    used_g: List[int] = field(default_factory=list)
+    nnz_diagonals: int = 0
+
+
+class EncryptedMLP:
+    def __init__(self, ck: CKKS, sk: SecretKey, W1: np.ndarray, W2: np.ndarray, gear1: int = 0, gear2: int = 0,
+                 w_scale: float = 2.0 ** 40):
+        self.ck, self.sk = ck, sk
+        self.W1, self.W2 = W1, W2
+        self.w_scale = w_scale
+        self.S = (ck.N // 2) // SLOTS_PER_SAMPLE
+        self.rlk = ck.relin_key(sk)
+        self._gk: Dict[int, SwitchKey] = {}
+        self.ops = {"rot": 0, "ptmul_rows": 0, "hmult": 0, "dare": 0}
+        self._masks = {}
+        self._plan1: Dict[int, tuple] = {}
+        self._plan2: Dict[int, tuple] = {}
+        self.set_gears(gear1, gear2)
+
+    # -- keys -------------------------------------------------------------
+    def gk(self, step: int) -> SwitchKey:
+        step %= self.ck.N // 2
+        if step not in self._gk:
+            self._gk[step] = self.ck.galois_key(self.sk, step)
+        return self._gk[step]
+
+    def rot(self, ct: Ciphertext, step: int) -> Ciphertext:
+        if step % (self.ck.N // 2) == 0:
+            return ct
+        self.ops["rot"] += ct.batch
+        return self.ck.rotate(ct, self.gk(step))
+
+    # -- plans --------------------------------------------------------------
+    def _slots(self, v: np.ndarray) -> np.ndarray:
+        """A 1024-slot block pattern repeated for the S samples of a ciphertext."""
+        return np.tile(v, self.S)
+
+    def _plan(self, D: np.ndarray, rows, cols, baby, giant, folds, level) -> LayerPlan:
+        ck = self.ck
+        p = LayerPlan(rows, cols, baby, giant, folds)
+        nz = [bool(np.any(D[k])) for k in range(rows)]
+        p.nnz_diagonals = int(sum(nz))
+        p.used_b = [b for b in range(baby) if any(nz[g * baby + b] for g in range(giant) if g * baby + b < rows)]
+        p.used_g = [g for g in range(giant) if any(nz[g * baby + b] for b in range(baby) if g * baby + b < rows)]
+        if not p.used_g:
+            raise ValueError("every diagonal of the layer is pruned")
+        rows_pt = []
+        for g in p.used_g:
+            for b in p.used_b:
+                k = g * baby + b
+                blk = np.zeros(SLOTS_PER_SAMPLE)
+                if k < rows:
+                    blk[:cols] = D[k]
+                # pre-rotate by -baby*g so that the giant rotation aligns it
+                v = np.roll(self._slots(blk), baby * g)
+                rows_pt.append(v)
+        z = np.asarray(rows_pt, dtype=np.complex128)
+        pt = ck.encode(z, level=level, scale=self.w_scale)                  # [ng*nb, l, N]
+        prepared = torch.empty(*pt.shape, 2, dtype=torch.uint64, device=ck.device)
+        call("ckks_pt_prepare", ck.ring.handle, _ptr(pt), _ptr(prepared), pt.shape[0], level, 0, ck.ring.stream())
+        p.D = prepared.view(len(p.used_g), len(p.used_b), level, ck.N, 2)
+        return p
+
+    def set_gears(self, gear1: int, gear2: int):
+        """Select the layer gears; the pruned weights and prepared diagonal
+        plans of each gear are built once and cached."""
+        L = self.ck.L
+        if gear1 not in self._plan1:
+            P1 = self.W1 * block_keep_mask(self.W1, GEAR_FRACTIONS[gear1])
+            self._plan1[gear1] = (P1, self._plan(extended_diagonals(P1, 128, 896), 128, 896, 16, 8,
+                                                 (512, 256, 128), L))
+        if gear2 not in self._plan2:
+            P2 = self.W2 * block_keep_mask(self.W2, GEAR_FRACTIONS[gear2])
+            self._plan2[gear2] = (P2, self._plan(extended_diagonals(P2, 16, 128), 16, 128, 4, 4, (64, 32, 16),
+                                                 L - 1))
+        self.gears = (gear1, gear2)
+        (self.P1, self.p1), (self.P2, self.p2) = self._plan1[gear1], self._plan2[gear2]
+
+    def pruned_weights(self):
+        return self.P1, self.P2
+
+    def rotation_steps(self) -> List[int]:
+        """Rotation steps the current plans use (baby, giant, fold, replicate)."""
+        steps = {-128}
+        for p in (self.p1, self.p2):
+            steps.update(b for b in p.used_b if b)
+            steps.update(p.baby * g for g in p.used_g if g)
+            steps.update(p.folds)
+        return sorted(steps)
+
+    def prepare(self, gears=range(4)) -> None:
+        """Build every gear's plans and every Galois key they need up front,
+        so that gear switches inside a timed loop cost no set-up."""
+        cur = self.gears
+        for g in gears:
+            self.set_gears(g, g)
+            for s in self.rotation_steps():
+                self.gk(s)
+        self.set_gears(*cur)
+
+    # -- evaluation ---------------------------------------------------------
+    def encrypt_inputs(self, X: np.ndarray) -> Ciphertext:
+        ck = self.ck
+        n = X.shape[0]
+        nct = -(-n // self.S)
+        z = np.zeros((nct, ck.N // 2))
+        for s in range(n):
+            c, j = divmod(s, self.S)
+            blk = np.zeros(SLOTS_PER_SAMPLE)
+            blk[:784] = X[s]
+            blk[896:1024] = blk[:128]
+            z[c, j * SLOTS_PER_SAMPLE:(j + 1) * SLOTS_PER_SAMPLE] = blk
+        return ck.encrypt(ck.encode(z), self.sk)
+
+    def _linear(self, ct: Ciphertext, p: LayerPlan) -> Ciphertext:
+        ck = self.ck
+        B, l = ct.batch, ct.level
+        babies = [ct if b == 0 else self.rot(ct, b) for b in p.used_b]
+        E = torch.stack([x.data for x in babies]).contiguous()                 # [nb][B][2][l][N]
+        ck.ring.ntt_(E.view(-1, l, ck.N))
+        nb, ng = len(p.used_b), len(p.used_g)
+        acc = torch.empty(ng, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)
+        call("ckks_lincomb", ck.ring.handle, _ptr(E), _ptr(p.D), _ptr(acc), nb, ng, B, l, ck.ring.stream())
+        self.ops["ptmul_rows"] += nb * ng * B
+        ck.ring.intt_(acc.view(-1, l, ck.N))
+        z = None
+        for gi, g in enumerate(p.used_g):
+            part = ck.rescale(Ciphertext(acc[gi], ct.scale * self.w_scale))
+            part = self.rot(part, p.baby * g)
+            z = part if z is None else ck.add(z, part)
+        for s in p.folds:
+            z = ck.add(z, self.rot(z, s))
+        return z
+
+    def _mask(self, ct: Ciphertext, keep: int) -> Ciphertext:
+        """Multiply by the 0/1 slot mask [0, keep) of each block, encoded at
+        scale q_(l-1) * delta / scale(ct) so that the rescale brings the
+        ciphertext back to the base scale delta (the DARE refresh keeps the
+        scale, and the activation and layer 2 need its headroom: at a lower
+        scale each later rotation's key-switch noise, about 700 per
+        coefficient, would reach 1e-2 in the decoded slots)."""
+        ck = self.ck
+        l = ct.level
+        m_scale = float(ck.primes[l - 1]) * ck.p.scale / ct.scale
+        key = (keep, l, m_scale)
+        if key not in self._masks:
+            blk = np.zeros(SLOTS_PER_SAMPLE)
+            blk[:keep] = 1.0
+            pt = ck.encode(self._slots(blk)[None].astype(np.complex128), level=l, scale=m_scale)
+            prep = torch.empty(*pt.shape, 2, dtype=torch.uint64, device=ck.device)
+            call("ckks_pt_prepare", ck.ring.handle, _ptr(pt), _ptr(prep), 1, l, 0, ck.ring.stream())
+            self._masks[key] = prep
+        E = ct.data.clone()
+        ck.ring.ntt_(E.view(-1, l, ck.N))
+        out = torch.empty_like(E)
+        call("ckks_lincomb", ck.ring.handle, _ptr(E), _ptr(self._masks[key]), _ptr(out), 1, 1, ct.batch, l,
+             ck.ring.stream())
+        ck.ring.intt_(out.view(-1, l, ck.N))
+        return ck.rescale(Ciphertext(out, ct.scale * m_scale))
+
+    def dare(self, ct: Ciphertext) -> Ciphertext:
+        """Decrypt, keep the scale-carrying integer message, encrypt afresh
+        at the top level (exp_e24_rotation_bootstrap.py:287-304)."""
+        ck = self.ck
+        pt = ck.decrypt(ct, self.sk)                                            # [B, l, N]
+        B, l = pt.shape[0], pt.shape[1]
+        top = torch.empty(B, ck.L, ck.N, dtype=torch.uint64, device=ck.device)
+        call("ckks_lift_small", ck.ring.handle, _ptr(pt), _ptr(top), B, l, ck.L, ck.ring.stream())
+        self.ops["dare"] += B
+        return ck.encrypt(top, self.sk, scale=ct.scale)
+
+    def layer1(self, ct: Ciphertext) -> Ciphertext:
+        y = self._linear(ct, self.p1)                 # outputs in slots [0, 128) of each block
+        y = self._mask(y, 128)
+        return self.ck.add(y, self.rot(y, -128))      # copy to [128, 256) for layer 2's wrap
+
+    def activation(self, ct: Ciphertext) -> Ciphertext:
+        self.ops["hmult"] += ct.batch
+        return self.ck.hmult(ct, ct, self.rlk)
+
+    def layer2(self, ct: Ciphertext) -> Ciphertext:
+        return self._linear(ct, self.p2)
+
+    def forward(self, ct: Ciphertext) -> Ciphertext:
+        h = self.dare(self.layer1(ct))
+        return self.layer2(self.activation(h))
+
+    def decrypt_outputs(self, ct: Ciphertext, n: int) -> np.ndarray:
+        ck = self.ck
+        z = to_numpy(ck.decode(ck.decrypt(ct, self.sk), ct.scale)).real
+        out = np.zeros((n, 10))
+        for s in range(n):
+            c, j = divmod(s, self.S)
+            out[s] = z[c, j * SLOTS_PER_SAMPLE: j * SLOTS_PER_SAMPLE + 10]
+        return out
+
+
+def output_snr_db(ref: np.ndarray, got: np.ndarray) -> float:
+    """10 log10(sum ref^2 / sum (got - ref)^2)
+    (verify_m4h2_prod_calibration.py:64-69)."""
+    noise = float(np.sum((got - ref) ** 2))
+    return math.inf if noise == 0 else 10.0 * math.log10(float(np.sum(ref ** 2)) / noise)

@@ -175,6 +175,24 @@ class RingContext:
             out[r] = [sa, lo + (hi << 64), mx, sg - (1 << 64) if sg >> 63 else sg]
         return out
 
+    def noise_moments(self, x: torch.Tensor, prime0: int = 0) -> torch.Tensor:
+        """Per (poly, limb) row, on the device in float64: [mean c' / q,
+        rms c' / q, max c' / q, mean centred c / q], c' = min(c, q - c)
+        (the monitor's energy E = sum c'^2 / N gives rms = sqrt(E);
+        rtl/m4/noise_monitor.sv:44-78)."""
+        p, l = self._rows(x, prime0)
+        st = torch.empty(p * l, 5, dtype=torch.uint64, device=self.device)
+        call("ckks_noise_stats", self._h, _ptr(x), _ptr(st), p, l, prime0, self.stream())
+        v = st.view(torch.int64).to(torch.float64)
+        v = torch.where(v < 0, v + 2.0 ** 64, v)          # unsigned words
+        sg = st[:, 4].view(torch.int64).to(torch.float64)  # signed sum
+        q = torch.tensor([float(self.primes[prime0 + i]) for i in range(l)], dtype=torch.float64,
+                         device=self.device).repeat(p)
+        n = float(self.N)
+        out = torch.stack([v[:, 0] / n / q, torch.sqrt((v[:, 1] + v[:, 2] * 2.0 ** 64) / n) / q, v[:, 3] / q,
+                           sg / n / q], 1)
+        return out
+
     # -- canonical embedding -------------------------------------------------------
     def embed(self, coeffs: torch.Tensor, order: int = 0) -> torch.Tensor:
         """Real coefficients [B, N] (float64) -> complex slots [B, N/2]."""

@@ -6,7 +6,7 @@
 
 #include <vector>
 
-#include "../../include/ckks_core.h"
+#include "ckks_core.h"
 #include "launch.h"
 
 namespace {
@@ -579,6 +579,44 @@ int ckks_embed_inv(const ckks_ctx* c, const double* slots, double* coeffs, int n
     if (set_device(c)) return CKKS_ERR_HIP;
     HIPCHK(launch_encode_fft(c->fft_tw, c->fft_zeta, order ? nullptr : c->slot_k, (const double2*)slots, coeffs,
                              n_vec, c->logN, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_pt_prepare(const ckks_ctx* c, uint64_t* pt, uint64_t* out, int n_polys, int l, int prime0, void* stream) {
+    int st = check_rows(c, n_polys, l, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!pt || !out) return CKKS_ERR_NULL;
+    if (misaligned(pt) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (c->flags & CKKS_CYCLIC) return CKKS_ERR_BAD_ARG;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const LimbMap m = limb_map(prime0, l);
+    HIPCHK(launch_ntt(t, pt, pt, n_polys, l, m, false, true, c->ntt_threads, S(stream)));
+    HIPCHK(launch_shoup_pairs(t, pt, (ulonglong2*)out, n_polys, l, m, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_lincomb(const ckks_ctx* c, const uint64_t* E, const uint64_t* D, uint64_t* out, int nb, int ng, int batch,
+                 int l, void* stream) {
+    if (!c) return CKKS_ERR_NULL;
+    if (nb < 1 || nb > 32 || ng < 0 || batch < 0 || l < 1 || l > c->K) return CKKS_ERR_BAD_SHAPE;
+    if (!batch || !ng) return CKKS_OK;
+    if (!E || !D || !out) return CKKS_ERR_NULL;
+    if (misaligned(E) || misaligned(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_lift_small(const ckks_ctx* c, const uint64_t* in, uint64_t* out, int rows, int l_in, int l_out,
+                    void* stream) {
+    int st = check_rows(c, rows, l_out, 0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (l_in < 1 || l_in > c->K) return CKKS_ERR_BAD_SHAPE;
+    if (!in || !out) return CKKS_ERR_NULL;
+    if (misaligned(in) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_lift_small(c->tables(), in, out, rows, l_in, l_out, limb_map(0, l_out), S(stream)));
     return CKKS_OK;
 }
 

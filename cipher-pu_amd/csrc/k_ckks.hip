@@ -351,6 +351,55 @@ __global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restri
     out[g] = quo >= q_out ? quo - q_out : quo;
 }
 
+// Plaintext linear combination in the evaluation domain, the inner loop of
+// the diagonal (Halevi-Shoup) matrix-vector product:
+//   out[g][c][poly][i][n] = sum_b E[b][c][poly][i][n] * D[g][b][i][n]  (mod q_i)
+// E: [nb][B][2][l][N] evaluation-form ciphertext rows, D: [ng][nb][l][N]
+// (value, Shoup companion) plaintext rows shared by the whole batch.
+__global__ void k_lincomb(const uint64_t* __restrict__ E, const ulonglong2* __restrict__ D,
+                          uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map, int nb,
+                          int ng, int l, int logN, size_t per_b) {
+    const size_t g0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g0 >= per_b) return;
+    const size_t N = (size_t)1 << logN;
+    const size_t n = g0 & (N - 1);
+    const int i = (int)((g0 >> logN) % l);
+    const PrimeConst& pc = pcs[map.idx[i]];
+    const size_t dstride = (size_t)l * N;
+    for (int g = 0; g < ng; ++g) {
+        const ulonglong2* d = D + (size_t)g * nb * dstride + (size_t)i * N + n;
+        uint64_t acc = 0;
+        for (int b = 0; b < nb; ++b) {   // the E row of this thread stays in L1 across g
+            const ulonglong2 w = d[(size_t)b * dstride];
+            acc += mul_shoup(E[(size_t)b * per_b + g0], w.x, w.y, pc.q);   // < 2q each; nb <= 32 keeps acc < 2^51
+        }
+        out[(size_t)g * per_b + g0] = reduce64(acc, pc);
+    }
+}
+
+// Re-express a decrypted message in L limbs (the DARE refresh of
+// exp_e24_rotation_bootstrap.py:287-304 keeps the scale-carrying integer
+// message as is).  While |m| < q_0 / 2 the centred residue in limb 0 is the
+// integer m itself; it is reduced into every target prime.
+__global__ void k_lift_small(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                             const PrimeConst* __restrict__ pcs, LimbMap map, int l_in, int l_out, int logN,
+                             size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const size_t N = (size_t)1 << logN;
+    const size_t n = g & (N - 1), row = g >> logN;
+    const uint64_t q0 = pcs[map.idx[0]].q;
+    const uint64_t c = in[row * l_in * N + n];
+    const bool neg = c > (q0 >> 1);
+    const uint64_t a = neg ? q0 - c : c;
+    uint64_t* o = out + row * l_out * N + n;
+    for (int i = 0; i < l_out; ++i) {
+        const PrimeConst& pc = pcs[map.idx[i]];
+        const uint64_t r = reduce64(a, pc);
+        o[(size_t)i * N] = (neg && r) ? pc.q - r : r;
+    }
+}
+
 template <typename K>
 hipError_t set_lds(K kernel, int bytes) {
     return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -493,5 +542,23 @@ hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_mod_switch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n, q_in, q_out, mu,
                        ell);
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
+                          int batch, int l, const LimbMap& map, hipStream_t s) {
+    const size_t per_b = ((size_t)batch * 2 * l) << t.logN;
+    if (!per_b || !ng) return hipSuccess;
+    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((per_b + 255) / 256)), dim3(256), 0, s, E, D, out, t.pc, map, nb, ng,
+                       l, t.logN, per_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_lift_small(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, int l_in, int l_out,
+                             const LimbMap& map, hipStream_t s) {
+    const size_t total = (size_t)rows << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_lift_small, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, map, l_in,
+                       l_out, t.logN, total);
     return hipGetLastError();
 }

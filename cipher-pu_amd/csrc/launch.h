@@ -62,6 +62,10 @@ hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t*
                                int n_limbs, const LimbMap& map, uint64_t galois, hipStream_t s);
 hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
                              int ell, hipStream_t s);
+hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
+                          int batch, int l, const LimbMap& map, hipStream_t s);
+hipError_t launch_lift_small(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, int l_in, int l_out,
+                             const LimbMap& map, hipStream_t s);
 hipError_t launch_noise_stats(const DevTables& t, const uint64_t* in, uint64_t* stats, int n_polys,
                               int n_limbs, const LimbMap& map, hipStream_t s);
 

@@ -55,6 +55,15 @@ struct Cfg {
 template <class C>
 __device__ __forceinline__ int pad(int j) { return j + (j >> C::PS); }
 
+// Hide the twiddle table (a workgroup-uniform pointer) and the thread index
+// from the optimiser at the start of every transform.  Without this, a kernel
+// that runs several transforms with one table (key switching: one per digit)
+// gets every twiddle address -- or the twiddles themselves -- hoisted and
+// kept live across all of them, and spills.  The statement emits no code.
+__device__ __forceinline__ void opaque(const ulonglong2* __restrict__& tw, int& tid) {
+    asm volatile("" : "+s"(tw), "+v"(tid));
+}
+
 // Global index of register (g, e) in the layout of pass P.
 template <class C, int P>
 __device__ __forceinline__ int index_of(int g, int e, int tid) {
@@ -161,6 +170,7 @@ template <class C>
 __device__ __forceinline__ void fwd_ntt(uint64_t (&x)[C::E], uint64_t* lds,
                                         const ulonglong2* __restrict__ tw, uint64_t q,
                                         uint64_t two_q, int tid) {
+    opaque(tw, tid);
     fwd_pass<C, 0>(x, tw, q, two_q, tid);
     fwd_rest<C, 1>(x, lds, tw, q, two_q, tid);
 }
@@ -185,6 +195,7 @@ template <class C>
 __device__ __forceinline__ void inv_ntt(uint64_t (&x)[C::E], uint64_t* lds,
                                         const ulonglong2* __restrict__ itw, const PrimeConst& pc,
                                         int tid) {
+    opaque(itw, tid);
     inv_pass<C, C::NP - 1>(x, itw, pc, tid);
     inv_rest<C, C::NP - 2>(x, lds, itw, pc, tid);
 }

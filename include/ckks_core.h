@@ -167,12 +167,55 @@ int ckks_mod_switch(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t q_in
 int ckks_embed(const ckks_ctx *ctx, const double *coeffs, double *slots, int n_vec, int order, void *stream);
 int ckks_embed_inv(const ckks_ctx *ctx, const double *slots, double *coeffs, int n_vec, int order, void *stream);
 
+/* Plaintext rows for ckks_lincomb: coefficient-form pt [n_polys][l][N]
+ * (transformed in place) -> out [n_polys][l][N] (value, Shoup companion)
+ * pairs in evaluation order. */
+int ckks_pt_prepare(const ckks_ctx *ctx, uint64_t *pt, uint64_t *out, int n_polys, int l, int prime0,
+                    void *stream);
+
+/* Batched plaintext linear combination in the evaluation domain, the inner
+ * product of the diagonal matrix-vector method (encrypted_linear_layer,
+ * scripts/exp_e11_ckks_e2e_real.py:142-166, re-packed as diagonals):
+ *   out[g] = sum_(b<nb) E[b] * D[g][b]
+ * E: [nb][batch][2][l][N] evaluation-form ciphertexts (limbs = primes
+ * 0..l-1); D: [ng][nb][l][N] rows from ckks_pt_prepare; out:
+ * [ng][batch][2][l][N].  nb <= 32. */
+int ckks_lincomb(const ckks_ctx *ctx, const uint64_t *E, const uint64_t *D, uint64_t *out, int nb, int ng,
+                 int batch, int l, void *stream);
+
+/* DARE lift (the message-keeping refresh of
+ * exp_e24_rotation_bootstrap.py:287-304): rows of a decrypted plaintext
+ * in [rows][l_in][N] -> [rows][l_out][N], taking the centred residue of
+ * limb 0 as the integer message (exact while |m| < q_0 / 2). */
+int ckks_lift_small(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, int rows, int l_in, int l_out,
+                    void *stream);
+
 /* Plain noise monitor (rtl/m4/noise_monitor.sv:44-78,
  * verify_m4_h_multilayer.py:162-174): per row, with c' = min(c, q-c),
  * stats[row][5] = {sum c', sum c'^2 low word, high word, max c',
  * sum of centered values}. */
 int ckks_noise_stats(const ckks_ctx *ctx, const uint64_t *data, uint64_t *stats, int n_polys, int n_limbs,
                      int prime0, void *stream);
+
+/* ---- learned gear controller (small dense network, FP32 weights,
+ * activations optionally stored as BF16) ----
+ * It predicts the output SNR of each sparsity gear from ciphertext
+ * statistics; the plain noise monitor stays the fallback and the judge.  It
+ * has no counterpart in the reference (whose gear choice is the threshold
+ * policy of rtl/m4/adaptive_gear_ctrl.sv:42-100).  Activations [B][I] /
+ * [B][J] row-major, weights [J][I]; *_bf16 selects BF16 storage. */
+int ckks_gear_dense_fwd(const void *in, int in_bf16, const float *W, const float *bias, void *out, int out_bf16,
+                        int B, int I, int J, int relu, void *stream);
+/* Backward of one dense layer: dpre = dout * relu'(out); dW = dpre^T in;
+ * db = column sums of dpre; din = dpre W (skipped if din is NULL). */
+int ckks_gear_dense_bwd(const float *dout, const void *out, int out_bf16, const void *in, int in_bf16,
+                        const float *W, float *dpre, float *dW, float *db, float *din, int B, int I, int J, int relu,
+                        void *stream);
+/* One Adam step on n parameters (step >= 1 for the bias corrections). */
+int ckks_gear_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps,
+                   int step, void *stream);
+/* dy = 2 (y - target) / B and per-row squared error, for the MSE loss. */
+int ckks_gear_mse_grad(const float *y, const float *target, float *dy, float *row_loss, int B, int J, void *stream);
 
 /* Threads per workgroup (0 = default, else 128/256/512/1024 where built)
  * for this context's LDS-resident kernels: forward/inverse transforms and
