@@ -1,17 +1,28 @@
-"""Throughput of CKKS HMult + relinearise + rescale on one MI355X.
+"""Throughput of CKKS HMult + relinearise + rescale on one MI355X, and of
+the encrypted 784-128-10 MLP at a fixed output SNR.
 
 Workload (one step): the fused ckks_hmult call over a batch of ciphertext
 pairs at N = 2^13, L = 4 limbs (the four chain primes), one special prime,
 dnum = 4, scale 2^40, inputs already resident in HBM; 256 pairs encrypting
 4096 uniform real slots in [-1, 1] under a ternary secret, sigma = 3.2.
 
-Prints one JSON line.  ``value`` = HMult/s over the timed steps; the
-roofline entry prices the step at its algorithmic HBM bytes (inputs
-1,048,576 B + outputs 393,216 B + the relinearisation key 2,621,440 B read
-once per batch, i.e. 1,452,032 B per HMult at batch 256) against 8 TB/s;
-``cpu_baseline`` times the CPU oracle on a bounded sample.
+Prints one JSON line.  ``value`` = HMult/s over the timed steps.  The
+roofline entry is for the dominant kernel of the step (the key switch),
+timed per launch with HIP events on the call's stream over the timed
+region, priced at its algorithmic bytes: per ciphertext the coefficient
+digits d2 (l limbs), d2 of the own limb in evaluation form (l limbs),
+d0/d1 in evaluation form (2 l limbs) and the accumulators written
+(2 (l + 1) limbs); per launch the relinearisation key once (l digits x
+2 x (l + 1) limbs, 8 B a coefficient).  The whole step is reported
+beside it at 1,452,032 B per HMult (inputs, outputs and the key once per
+batch).  ``cpu_baseline`` times the CPU oracle on a bounded sample.
+``mlp`` runs the encrypted MLP on 4096 synthetic images under the plain
+decrypt-based monitor gear loop and under the learned gear controller;
+``batch_graph`` runs 2048 mixed operations (N = 4096 / 8192, levels 1..4,
+HMult / rotation / plaintext product) as one captured HIP graph.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256] [--no-mlp] [--no-graph]
+    python bench.py --copy-probe      # plain copy of known size (counter scaling)
 """
 from __future__ import annotations
 
@@ -31,11 +42,29 @@ N, L = 8192, 4
 
 
 def algorithmic_bytes(batch: int, l: int = L) -> int:
+    """Whole HMult step: inputs, outputs, the relinearisation key once."""
     limb = N * 8
     inputs = 2 * 2 * l * limb                 # two ciphertexts, 2 polys, l limbs
     outputs = 2 * (l - 1) * limb
     key = l * 2 * (l + 1) * limb               # dnum digits x (b, a) x (l + special) limbs
     return batch * (inputs + outputs) + key
+
+
+def keyswitch_bytes(batch: int, l: int = L) -> int:
+    """One key-switch launch inside HMult: per ciphertext d2 in coefficient
+    form (l limbs) and its own-limb evaluation form (l limbs), d0/d1 in
+    evaluation form (2 l), accumulators out (2 (l + 1)); the key once."""
+    limb = N * 8
+    per_ct = (l + l + 2 * l + 2 * (l + 1)) * limb
+    key = l * 2 * (l + 1) * limb
+    return batch * per_ct + key
+
+
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# this bench, scaled by the same counters on the --copy-probe copy (the
+# summaries and commands are under profiles/).  Keyed by (kernel kind,
+# batch, threads); None where not measured for the running shape.
+MEASURED_TRAFFIC = {("keyswitch", 256, 1024): 692557162}
 
 
 def cpu_model() -> str:
@@ -93,6 +122,88 @@ def cpu_baseline(sample_s: float, cta, ctb, key_coeff, primes, roots):
     return py, c
 
 
+def copy_probe(nbytes: int = 1 << 28, reps: int = 5) -> None:
+    """A plain device copy of known size, for scaling HBM counters."""
+    import torch
+    src = torch.empty(nbytes // 8, dtype=torch.int64, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": reps}))
+
+
+def mlp_leg(samples: int, target_db: float, chunk_cts: int):
+    """Encrypted MLP samples/s at a fixed output SNR: the plain decrypt-based
+    monitor gear loop against the learned gear controller, plus static
+    gear 0 without any monitor."""
+    import numpy as np
+    import torch
+    from cipherpu import gear, nn
+    from cipherpu.ckks import CKKS, CKKSParams
+    ck = CKKS(CKKSParams(), seed=99)
+    sk = ck.keygen()
+    W1, W2 = nn.make_mlp(0)
+    mlp = nn.EncryptedMLP(ck, sk, W1, W2)
+    net = gear.GearNet(seed=0)
+    trained = gear.load_trained(net)
+    if not trained:
+        F, T = gear.sweep(mlp, 512, seed=1)
+        net.fit(F, T, epochs=60, batch=256)
+    X = nn.synthetic_images(samples, seed=4096)
+    ref = nn.plain_forward(W1, W2, X)
+    out = {"metric": "enc-MLP samples/s at fixed output SNR", "network": "784-128-10 x^2, DARE between layers",
+           "samples": samples, "target_snr_db": target_db, "chunk_ciphertexts": chunk_cts,
+           "samples_per_ciphertext": mlp.S,
+           "controller_weights": trained or "trained in this run on a 512-ciphertext sweep"}
+    for name, fn in (("plain_monitor", lambda: gear.plain_monitor_loop(mlp, X, target_db, chunk_cts)),
+                     ("learned", lambda: gear.learned_loop(mlp, net, X, target_db, chunk_cts)),
+                     ("static_gear0", lambda: gear.static_loop(mlp, X, 0, chunk_cts))):
+        r = fn()
+        out[name] = {"samples_per_s": round(r.samples_per_s, 1), "snr_db": round(gear.snr_db(ref, r.outputs), 2),
+                     "evaluations": r.evaluations, "monitor_calls": r.monitor_calls, "fallbacks": r.fallbacks,
+                     "gears": {str(g): r.gears.count(g) for g in sorted(set(r.gears))}}
+    out["learned_over_plain"] = round(out["learned"]["samples_per_s"] / out["plain_monitor"]["samples_per_s"], 3)
+    return out
+
+
+def graph_leg(n_ops: int, reps: int = 5):
+    """Uneven batch: n_ops operations at N = 4096 / 8192, levels 1..4, mix
+    60% HMult+rescale / 30% rotation / 10% plaintext product, bucketed,
+    tuned per bucket and replayed as one captured HIP graph."""
+    import torch
+    from cipherpu.sched import BatchScheduler, make_workload
+    ops = make_workload(n_ops, seed=0)
+    s = BatchScheduler(ops, seed=1)
+    s.fill_random(seed=2)
+    t0 = time.perf_counter()
+    s.tune(reps=2)
+    t_tune = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.run_eager()
+    e0.record()
+    for _ in range(reps):
+        s.run_eager()
+    e1.record()
+    torch.cuda.synchronize()
+    eager_ms = e0.elapsed_time(e1) / reps
+    s.capture()
+    s.replay()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        s.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    graph_ms = e0.elapsed_time(e1) / reps
+    kinds = [o.kind for o in ops]
+    return {"workload": "mixed N 4096/8192, levels 1..4, 60/30/10 hmult/rotate/ptmul", "ops": n_ops,
+            "ops_per_s": round(n_ops / (graph_ms / 1e3), 1), "ms_per_batch": round(graph_ms, 3),
+            "eager_ms_per_batch": round(eager_ms, 3), "buckets": len(s.buckets),
+            "kinds": {k: kinds.count(k) for k in ("hmult", "rotate", "ptmul")}, "tune_s": round(t_tune, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,12 +215,22 @@ def main():
     ap.add_argument("--ks-threads", type=int, default=0)
     ap.add_argument("--ntt-threads", type=int, default=0)
     ap.add_argument("--tensor-threads", type=int, default=0)
+    ap.add_argument("--front", action="store_true", help="fused HMult front kernel (with fixed shapes)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape tuner")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the encrypted-MLP leg")
+    ap.add_argument("--mlp-samples", type=int, default=4096)
+    ap.add_argument("--target-snr", type=float, default=3.0)
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the uneven-batch graph leg")
+    ap.add_argument("--graph-ops", type=int, default=2048)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
               "multi-GPU path (the batch shards over ciphertexts with no exchange).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import numpy as np
     import torch
@@ -128,6 +249,7 @@ def main():
     ctb = ck.encrypt(ck.encode(zb), sk)
     tuned = None
     if args.ntt_threads or args.ks_threads or args.tensor_threads or args.no_tune:
+        ck.ring.set_front(args.front)
         ck.ring.set_threads(args.ntt_threads, args.ks_threads, args.tensor_threads)
     else:
         from cipherpu.tune import tune_hmult
@@ -143,6 +265,7 @@ def main():
         ck.hmult(cta, ctb, rlk)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ck.ring.timing_start(8 * args.steps)
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
@@ -150,10 +273,19 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kern = ck.ring.timing_stop()
     ms = ev0.elapsed_time(ev1) / args.steps
     value = B / (ms / 1e3)
     nbytes = algorithmic_bytes(B)
-    achieved = nbytes / (ms / 1e3) / 1e9
+    shape = ({k: tuned[k] for k in ("front", "ntt", "tensor", "ks")} if tuned else
+             {"front": int(args.front), "ntt": args.ntt_threads, "tensor": args.tensor_threads,
+              "ks": args.ks_threads})
+    dom = max(kern, key=lambda k: kern[k][0])
+    dom_ms = kern[dom][0] / kern[dom][1]
+    assert dom == "keyswitch", f"dominant kernel is {dom}; the byte model covers the key switch"
+    kbytes = keyswitch_bytes(B)
+    k_achieved = kbytes / (dom_ms / 1e3) / 1e9
+    traffic = MEASURED_TRAFFIC.get((dom, B, shape.get("ks")))
 
     res = {
         "metric": "HMult+relin+rescale/s at N=2^13 L=4",
@@ -171,16 +303,23 @@ def main():
         "config": {"workload": "ckks_hmult_relin_rescale", "N": N, "L": L, "dnum": L, "special_primes": 1,
                    "batch": B, "scale_bits": 40, "prime_bits": 45, "domain": "coefficient in/out",
                    "parallelism": "single_gpu"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "ckks_hmult step (5 launches: 2x ntt_fwd, tensor_intt, keyswitch, moddown_rescale)",
-                     "algorithmic_bytes_per_step": nbytes},
+        "roofline": {"bound": "hbm", "achieved": round(k_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(k_achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"k_keyswitch<13,{shape.get('ks') or 'default'}>",
+                     "avg_launch_ms": round(dom_ms, 4), "algorithmic_bytes_per_launch": kbytes},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kern.items()},
+        "step_roofline": {"achieved": round(nbytes / (ms / 1e3) / 1e9, 1), "unit": "GB/s",
+                          "frac": round(nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "algorithmic_bytes_per_step": nbytes},
         "max_abs_err_vs_fp64": max_err,
-        "launch_shape": ({k: tuned[k] for k in ("ntt", "tensor", "ks")} if tuned else
-                         {"ntt": args.ntt_threads, "tensor": args.tensor_threads, "ks": args.ks_threads}),
+        "launch_shape": shape,
         "wall_s_timed": round(wall, 4),
         "device": torch.cuda.get_device_name(0),
     }
+    if not args.no_mlp:
+        res["mlp"] = mlp_leg(args.mlp_samples, args.target_snr, 64)
+    if not args.no_graph:
+        res["batch_graph"] = graph_leg(args.graph_ops)
     if not args.no_cpu_baseline:
         py, c = cpu_baseline(args.cpu_sample_s, to_numpy(cta.data[:8]), to_numpy(ctb.data[:8]),
                              to_numpy(rlk.coeff), ck.primes, ck.ring.roots)

@@ -69,7 +69,12 @@ _SIGS = {
     "ckks_gear_adam": (_I, [_P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                             _I, _P]),
     "ckks_gear_mse_grad": (_I, [_P, _P, _P, _P, _I, _I, _P]),
+    "ckks_timing_start": (_I, [_P, _I]),
+    "ckks_timing_stop": (_I, [_P, _P, _P, _I]),
 }
+
+# Kernel kinds of ckks_timing_stop (include/ckks_core.h CKKS_KIND_*)
+KERNEL_KINDS = ("ntt", "tensor", "front", "keyswitch", "moddown", "automorphism", "polymul")
 
 EXPORTED = tuple(_SIGS)
 

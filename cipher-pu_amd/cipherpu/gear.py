@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 import time
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
@@ -211,6 +212,20 @@ class GearNet:
             self.load_state({k: z[k] for k in z.files})
 
 
+# Controller weights trained by cipherpu.gear_train on monitor sweeps of the
+# 784-128-10 network of nn.make_mlp(0) (synthetic images, every gear).
+TRAINED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "gearnet_mlp784.npz")
+
+
+def load_trained(net: GearNet, path: str = TRAINED_PATH) -> Optional[str]:
+    """Load committed controller weights into net; the file name, or None
+    when there are none."""
+    if not os.path.exists(path):
+        return None
+    net.load(path)
+    return os.path.basename(path)
+
+
 def choose_gear(pred_snr, target_db: float, margin_db: float = MARGIN_DB) -> Optional[int]:
     """Cheapest (highest) gear whose predicted SNR clears target + margin;
     None when no gear does (the plain monitor then decides)."""
@@ -351,8 +366,18 @@ def learned_loop(mlp, net: GearNet, X: np.ndarray, target_db: float, chunk_cts: 
     prev = 0
     _sync(mlp)
     t0 = time.perf_counter()
-    for Xc, ct in chunks:
-        pred = net.predict(ciphertext_features(mlp.ck, ct, prev)).mean(0).cpu().numpy()
+    # one batched prediction for every chunk and every previous-gear value
+    # (the only feature the decisions feed back into): a single device sync
+    feats = torch.cat([ciphertext_features(mlp.ck, ct, 0) for _, ct in chunks])
+    sizes = [ct.batch for _, ct in chunks]
+    table = []
+    for pg in range(N_GEARS):
+        f = feats.clone()
+        f[:, 14] = pg / 3.0
+        table.append(torch.stack([p.mean(0) for p in torch.split(net.predict(f), sizes)]))
+    table = torch.stack(table).cpu().numpy()              # [prev gear, chunk, gear]
+    for i, (Xc, ct) in enumerate(chunks):
+        pred = table[prev, i]
         g = choose_gear(pred, target_db, margin_db)
         if g is None:
             fb += 1
@@ -370,3 +395,18 @@ def learned_loop(mlp, net: GearNet, X: np.ndarray, target_db: float, chunk_cts: 
     dt = time.perf_counter() - t0
     res = np.concatenate([mlp.decrypt_outputs(o, Xc.shape[0]) for o, (Xc, _) in zip(outs, chunks)])
     return LoopResult(res, gears, evals, mon, fb, dt, X.shape[0])
+
+
+def static_loop(mlp, X: np.ndarray, g: int, chunk_cts: int = 64) -> LoopResult:
+    """Every chunk at one fixed gear, no monitor (the reference point)."""
+    chunks = _setup(mlp, X, chunk_cts)
+    mlp.set_gears(g, g)
+    outs = []
+    _sync(mlp)
+    t0 = time.perf_counter()
+    for _, ct in chunks:
+        outs.append(mlp.forward(ct))
+    _sync(mlp)
+    dt = time.perf_counter() - t0
+    res = np.concatenate([mlp.decrypt_outputs(o, Xc.shape[0]) for o, (Xc, _) in zip(outs, chunks)])
+    return LoopResult(res, [g] * len(chunks), len(chunks), 0, 0, dt, X.shape[0])

@@ -4,7 +4,9 @@ CHAIN4: the four 45-bit primes of rtl/m2/modulus_chain_chain4.json (all
 1 mod 2^15).  SPECIAL_P: the next prime of the same form
 2^44 + 32768 j + 1 (j = 22) above the chain, used as the hybrid key-switch
 special prime.  CONFIG1_Q / CONFIG1_PSI: the largest 45-bit prime that is
-1 mod 8192 and its smallest primitive 8192-th root (N = 4096 product)."""
+1 mod 8192 and its smallest primitive 8192-th root (N = 4096 product).
+CHAIN5: CHAIN4 and the next prime of the form 2^44 + 32768 j + 1 above the
+special prime (j = 48), for ciphertexts with four rescales left."""
 from __future__ import annotations
 
 import json
@@ -12,6 +14,7 @@ from typing import List
 
 CHAIN4 = (17592186634241, 17592186372097, 17592186273793, 17592186175489)
 SPECIAL_P = 17592186765313
+CHAIN5 = CHAIN4 + (17592187617281,)
 CONFIG1_Q = 35184371884033
 CONFIG1_PSI = 7388705119
 

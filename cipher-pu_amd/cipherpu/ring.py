@@ -175,6 +175,20 @@ class RingContext:
             out[r] = [sa, lo + (hi << 64), mx, sg - (1 << 64) if sg >> 63 else sg]
         return out
 
+    # -- kernel timing --------------------------------------------------------------
+    def timing_start(self, max_launches: int = 4096) -> None:
+        """Bracket every kernel the transform, product, HMult, relinearise and
+        rotate entry points launch with HIP events on their stream."""
+        call("ckks_timing_start", self._h, int(max_launches))
+
+    def timing_stop(self) -> dict:
+        """{kernel kind: (summed ms, launches)} since timing_start."""
+        n = len(_lib.KERNEL_KINDS)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        call("ckks_timing_stop", self._h, ms, cnt, n)
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.KERNEL_KINDS) if cnt[i]}
+
     def noise_moments(self, x: torch.Tensor, prime0: int = 0) -> torch.Tensor:
         """Per (poly, limb) row, on the device in float64: [mean c' / q,
         rms c' / q, max c' / q, mean centred c / q], c' = min(c, q - c)

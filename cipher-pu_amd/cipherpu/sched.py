@@ -1,0 +1,204 @@
+"""Uneven batched CKKS evaluation in one captured HIP graph.
+
+A workload is a list of operations on independent ciphertexts at mixed
+ring degrees and levels: HMult + relinearise + rescale with a partner
+ciphertext, rotation by a power-of-two slot step with its Galois key, or a
+plaintext-ciphertext product.  The scheduler
+
+  * buckets the operations by (N, level, kind, step) -- every bucket is one
+    batched call of the C ABI on contiguous device buffers that the
+    scheduler owns (inputs are written bucket-major, so no gather pass);
+  * chooses a launch shape per bucket with the tuner (threads per
+    workgroup of the transform, tensor and key-switch kernels, i.e. the
+    radix of the register passes);
+  * captures every bucket's launches, each under its own launch shape, into
+    one HIP graph, replayed as a single launch.
+
+Level semantics: a ciphertext at level l has l rescales left and l + 1 RNS
+limbs over the primes CHAIN5 (cipherpu.params), plus the special prime for
+key switching.  The reference has no scheduler; its RTL time-multiplexes
+one datapath (scripts/exp_e17_*:92-115 sketches that)."""
+from __future__ import annotations
+
+import ctypes
+import itertools
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from ._lib import call
+from .ckks import CKKS, CKKSParams, Ciphertext, SecretKey, SwitchKey
+from .params import CHAIN5
+
+KINDS = ("hmult", "rotate", "ptmul")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@dataclass(frozen=True)
+class Op:
+    kind: str          # "hmult" | "rotate" | "ptmul"
+    N: int
+    level: int         # rescales left; limbs = level + 1
+    step: int = 0      # rotation step (slots)
+
+
+def make_workload(n: int = 2048, seed: int = 0, degrees=(4096, 8192), levels=(1, 2, 3, 4),
+                  mix=(0.6, 0.3, 0.1)) -> List[Op]:
+    """n operations: half at each ring degree, level uniform in `levels`,
+    kind drawn with probabilities `mix` (HMult, rotation, plaintext
+    product), rotation steps uniform over the powers of two below N/2."""
+    rng = np.random.default_rng(seed)
+    ops = []
+    for i in range(n):
+        N = degrees[i * len(degrees) // n]
+        level = int(rng.choice(levels))
+        kind = KINDS[int(rng.choice(3, p=mix))]
+        step = 1 << int(rng.integers(0, int(np.log2(N // 2)))) if kind == "rotate" else 0
+        ops.append(Op(kind, N, level, step))
+    return ops
+
+
+@dataclass
+class Bucket:
+    N: int
+    level: int
+    kind: str
+    step: int
+    ops: List[int]                          # indices into the workload
+    a: torch.Tensor = None                  # [B, 2, l+1, N] inputs
+    b: torch.Tensor = None                  # partner ciphertexts (hmult) or plaintexts [B*2, l+1, N] (ptmul)
+    out: torch.Tensor = None
+    shape: Tuple[int, int, int] = (0, 0, 0)  # (ntt, ks, tensor) threads
+    ms: float = 0.0
+
+    @property
+    def limbs(self) -> int:
+        return self.level + 1
+
+    @property
+    def batch(self) -> int:
+        return len(self.ops)
+
+
+class BatchScheduler:
+    def __init__(self, ops: List[Op], seed: int = 0, device: int = 0):
+        self.ops = ops
+        self.ck: Dict[int, CKKS] = {}
+        self.sk: Dict[int, SecretKey] = {}
+        self.rlk: Dict[int, SwitchKey] = {}
+        self.gk: Dict[Tuple[int, int], SwitchKey] = {}
+        for N in sorted({o.N for o in ops}):
+            ck = CKKS(CKKSParams(N=N, primes=CHAIN5), device=device, seed=seed + N)
+            self.ck[N] = ck
+            self.sk[N] = ck.keygen()
+            self.rlk[N] = ck.relin_key(self.sk[N])
+        for N, step in sorted({(o.N, o.step) for o in ops if o.kind == "rotate"}):
+            self.gk[(N, step)] = self.ck[N].galois_key(self.sk[N], step)
+        groups: Dict[tuple, List[int]] = {}
+        for i, o in enumerate(ops):
+            groups.setdefault((o.N, o.level, o.kind, o.step), []).append(i)
+        self.buckets = [Bucket(N, lv, k, st, idx) for (N, lv, k, st), idx in sorted(groups.items())]
+        self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self._ws: Dict[int, torch.Tensor] = {}
+
+    # -- data -------------------------------------------------------------
+    def fill_random(self, seed: int = 1) -> None:
+        """Encrypt uniform slots in [-1, 1] into every bucket's inputs."""
+        rng = np.random.default_rng(seed)
+        for bk in self.buckets:
+            ck, sk, l, B = self.ck[bk.N], self.sk[bk.N], bk.limbs, bk.batch
+            z = rng.uniform(-1, 1, (B, bk.N // 2))
+            bk.a = ck.encrypt(ck.encode(z, level=l), sk).data
+            if bk.kind == "hmult":
+                bk.b = ck.encrypt(ck.encode(rng.uniform(-1, 1, (B, bk.N // 2)), level=l), sk).data
+                bk.out = torch.empty(B, 2, l - 1, bk.N, dtype=torch.uint64, device=ck.device)
+            elif bk.kind == "rotate":
+                bk.out = torch.empty_like(bk.a)
+            else:
+                pt = ck.encode(rng.uniform(-1, 1, (B, bk.N // 2)), level=l)
+                bk.b = pt.unsqueeze(1).expand(B, 2, l, bk.N).contiguous().view(B * 2, l, bk.N)
+                bk.out = torch.empty_like(bk.a)
+
+    def _workspace(self, N: int) -> torch.Tensor:
+        if N not in self._ws:
+            ck = self.ck[N]
+            need = max(call("ckks_workspace_bytes", ck.ring.handle, bk.batch, bk.limbs)
+                       for bk in self.buckets if bk.N == N)
+            self._ws[N] = torch.empty((need + 7) // 8, dtype=torch.uint64, device=ck.device)
+        return self._ws[N]
+
+    # -- launches ---------------------------------------------------------
+    def _launch(self, bk: Bucket) -> None:
+        ck = self.ck[bk.N]
+        r = ck.ring
+        n, k, t = bk.shape
+        r.set_threads(n, k, t)
+        l, B, K = bk.limbs, bk.batch, ck.L + 1
+        if bk.kind == "hmult":
+            call("ckks_hmult", r.handle, _ptr(bk.a), _ptr(bk.b), _ptr(self.rlk[bk.N].prepared), K, _ptr(bk.out), B,
+                 l, ck.sp, _ptr(self._workspace(bk.N)), r.stream())
+        elif bk.kind == "rotate":
+            gk = self.gk[(bk.N, bk.step)]
+            call("ckks_rotate", r.handle, _ptr(bk.a), _ptr(gk.prepared), K, _ptr(bk.out), B, l, ck.sp, gk.galois,
+                 _ptr(self._workspace(bk.N)), r.stream())
+        else:
+            call("ckks_poly_mul", r.handle, _ptr(bk.a), _ptr(bk.b), _ptr(bk.out), B * 2, l, 0, r.stream())
+
+    def run_eager(self) -> None:
+        for bk in self.buckets:
+            self._launch(bk)
+
+    # -- tuning -----------------------------------------------------------
+    def tune(self, reps: int = 3) -> None:
+        """Launch shape per bucket: the fastest built thread count of each
+        kernel family the bucket's operation uses, timed on the bucket."""
+        from .tune import BUILT, BUILT_FUSED, _time
+        for bk in self.buckets:
+            logN = bk.N.bit_length() - 1
+            if bk.kind == "hmult":
+                grid = itertools.product(BUILT[logN], BUILT_FUSED[logN], BUILT_FUSED[logN])
+                cands = [(n, k, t) for n, t, k in grid]
+            elif bk.kind == "rotate":
+                cands = [(0, k, 0) for k in BUILT_FUSED[logN]]
+            else:
+                cands = [(n, 0, 0) for n in BUILT[logN]]
+            best = None
+            for c in cands:
+                bk.shape = c
+                ms = _time(lambda: self._launch(bk), reps)
+                if best is None or ms < best[1]:
+                    best = (c, ms)
+            bk.shape, bk.ms = best
+
+    # -- graph ------------------------------------------------------------
+    def capture(self) -> torch.cuda.CUDAGraph:
+        """Record every bucket's launches into one HIP graph."""
+        for N in self.ck:
+            self._workspace(N)
+        s = torch.cuda.Stream(self.ck[next(iter(self.ck))].device)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):          # warm-up outside capture
+            self.run_eager()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run_eager()
+        self.graph = g
+        return g
+
+    def replay(self) -> None:
+        self.graph.replay()
+
+    def outputs(self) -> List[torch.Tensor]:
+        return [bk.out for bk in self.buckets]
+
+    def summary(self) -> List[dict]:
+        return [{"N": bk.N, "level": bk.level, "kind": bk.kind, "step": bk.step, "batch": bk.batch,
+                 "threads": dict(zip(("ntt", "ks", "tensor"), bk.shape)), "tuned_ms": round(bk.ms, 4)}
+                for bk in self.buckets]

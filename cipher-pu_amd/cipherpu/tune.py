@@ -34,27 +34,34 @@ def _time(fn, reps: int) -> float:
 
 
 def tune_hmult(ck, a, b, rlk, reps: int = 5, ntt: Iterable[int] = None, tensor: Iterable[int] = None,
-               ks: Iterable[int] = None) -> dict:
-    """Grid search over (ntt, tensor, keyswitch) threads for ck.hmult(a, b, rlk).
-    Leaves the best setting applied on ck.ring and returns
-    {"ntt": T, "tensor": T, "ks": T, "ms": best, "grid": {...}}."""
+               ks: Iterable[int] = None, fused: Iterable[int] = (0, 1)) -> dict:
+    """Grid search for ck.hmult(a, b, rlk) over the HMult front (two batched
+    transforms + tensor kernel, or the fused front kernel) and the thread
+    counts of the transform, tensor/front and key-switch kernels.  Leaves
+    the best setting applied on ck.ring and returns
+    {"front": 0|1, "ntt": T, "tensor": T, "ks": T, "ms": best, "grid": {...}}."""
     logN = ck.N.bit_length() - 1
     key = (torch.cuda.get_device_name(ck.device), ck.N, a.level, a.batch)
     if key in _cache:
         r = _cache[key]
+        ck.ring.set_front(bool(r["front"]))
         ck.ring.set_threads(r["ntt"], r["ks"], r["tensor"])
         return r
     ntt = tuple(ntt or BUILT[logN])
     tensor = tuple(tensor or BUILT_FUSED[logN])
     ks = tuple(ks or BUILT_FUSED[logN])
     grid = {}
-    for n, t, k in itertools.product(ntt, tensor, ks):
-        ck.ring.set_threads(n, k, t)
-        grid[f"{n}/{t}/{k}"] = _time(lambda: ck.hmult(a, b, rlk), reps)
+    for f in fused:
+        ck.ring.set_front(bool(f))
+        # the fused front does not use the transform kernel's thread count
+        for n, t, k in itertools.product(ntt if not f else ntt[:1], tensor, ks):
+            ck.ring.set_threads(n, k, t)
+            grid[f"{f}/{n}/{t}/{k}"] = _time(lambda: ck.hmult(a, b, rlk), reps)
     best = min(grid, key=grid.get)
-    n, t, k = (int(x) for x in best.split("/"))
+    f, n, t, k = (int(x) for x in best.split("/"))
+    ck.ring.set_front(bool(f))
     ck.ring.set_threads(n, k, t)
-    r = {"ntt": n, "tensor": t, "ks": k, "ms": grid[best], "grid": grid}
+    r = {"front": f, "ntt": n, "tensor": t, "ks": k, "ms": grid[best], "grid": grid}
     _cache[key] = r
     return r
 

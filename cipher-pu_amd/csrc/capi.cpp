@@ -93,7 +93,17 @@ struct ckks_ctx {
     int* slot_k = nullptr;        // (5^j mod 2N - 1)/4
     int ntt_threads = 0, ks_threads = 0, tensor_threads = 0;
     int front_fused = 0;   // HMult front: one fused kernel (1) or 2 transforms + tensor (0)
-    DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN}; }
+    // Optional per-launch timing (ckks_timing_start/stop): an event pair
+    // around every kernel the entry points launch, tagged by kernel kind.
+    struct Timer {
+        bool on = false;
+        int used = 0;
+        std::vector<hipEvent_t> ev;   // 2 per slot
+        std::vector<int> kind;
+    };
+    mutable Timer timer;
+    int narrow = 0;        // every prime below twice every other one
+    DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN, narrow}; }
 };
 
 namespace {
@@ -121,6 +131,29 @@ LimbMap limb_map(int prime0, int n) {
 }
 
 hipStream_t S(void* s) { return (hipStream_t)s; }
+
+// Timing slots: -1 when timing is off or every slot is taken.
+int timer_begin(const ckks_ctx* c, int kind, hipStream_t s) {
+    auto& t = c->timer;
+    if (!t.on || 2 * (t.used + 1) > (int)t.ev.size()) return -1;
+    const int slot = t.used++;
+    t.kind[slot] = kind;
+    hipEventRecord(t.ev[2 * slot], s);
+    return slot;
+}
+
+void timer_end(const ckks_ctx* c, int slot, hipStream_t s) {
+    if (slot >= 0) hipEventRecord(c->timer.ev[2 * slot + 1], s);
+}
+
+// Launch a kernel through its launcher, timed when timing is on.
+#define TIMED(KIND, CALL)                                 \
+    do {                                                  \
+        const int slot_ = timer_begin(c, KIND, S(stream)); \
+        const hipError_t e_ = (CALL);                     \
+        timer_end(c, slot_, S(stream));                   \
+        if (e_ != hipSuccess) return CKKS_ERR_HIP;        \
+    } while (0)
 
 int set_device(const ckks_ctx* ctx) { return hipSetDevice(ctx->device) == hipSuccess ? CKKS_OK : CKKS_ERR_HIP; }
 
@@ -217,6 +250,14 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
     c->logN = bitlen(N) - 1;
     c->K = n_primes;
     c->flags = flags;
+    {
+        uint64_t lo = primes[0], hi = primes[0];
+        for (int i = 1; i < n_primes; ++i) {
+            lo = primes[i] < lo ? primes[i] : lo;
+            hi = primes[i] > hi ? primes[i] : hi;
+        }
+        c->narrow = hi < 2 * lo;
+    }
     const int logN = c->logN;
     std::vector<ulonglong2> tw((size_t)n_primes * 2 * N);
     for (int p = 0; p < n_primes; ++p) {
@@ -293,6 +334,7 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
 int ckks_ctx_destroy(ckks_ctx* c) {
     if (!c) return CKKS_ERR_NULL;
     hipSetDevice(c->device);
+    for (hipEvent_t e : c->timer.ev) hipEventDestroy(e);
     if (c->dpc) hipFree(c->dpc);
     if (c->dtw) hipFree(c->dtw);
     if (c->fft_tw) hipFree(c->fft_tw);
@@ -315,6 +357,45 @@ int ckks_ctx_set_front(ckks_ctx* c, int fused) {
     return CKKS_OK;
 }
 
+int ckks_timing_start(ckks_ctx* c, int max_launches) {
+    if (!c) return CKKS_ERR_NULL;
+    if (max_launches < 1 || max_launches > (1 << 20)) return CKKS_ERR_BAD_SHAPE;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    auto& t = c->timer;
+    while ((int)t.ev.size() < 2 * max_launches) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        t.ev.push_back(e);
+    }
+    t.kind.assign(t.ev.size() / 2, 0);
+    t.used = 0;
+    t.on = true;
+    return CKKS_OK;
+}
+
+int ckks_timing_stop(ckks_ctx* c, double* ms, int* launches, int n_kinds) {
+    if (!c || !ms || !launches) return CKKS_ERR_NULL;
+    if (n_kinds < 1) return CKKS_ERR_BAD_SHAPE;
+    auto& t = c->timer;
+    t.on = false;
+    for (int k = 0; k < n_kinds; ++k) {
+        ms[k] = 0.0;
+        launches[k] = 0;
+    }
+    if (t.used && hipEventSynchronize(t.ev[2 * t.used - 1]) != hipSuccess) return CKKS_ERR_HIP;
+    for (int i = 0; i < t.used; ++i) {
+        float e = 0.f;
+        HIPCHK(hipEventElapsedTime(&e, t.ev[2 * i], t.ev[2 * i + 1]));
+        if (t.kind[i] < n_kinds) {
+            ms[t.kind[i]] += e;
+            ++launches[t.kind[i]];
+        }
+    }
+    const int n = t.used;
+    t.used = 0;
+    return n;
+}
+
 int ckks_ctx_set_threads(ckks_ctx* c, int ntt_threads, int ks_threads, int tensor_threads) {
     if (!c) return CKKS_ERR_NULL;
     for (int t : {ntt_threads, ks_threads, tensor_threads})
@@ -335,8 +416,8 @@ static int do_ntt(const ckks_ctx* c, uint64_t* data, int n_polys, int n_limbs, i
     const DevTables t = c->tables();
     const int rows = n_polys * n_limbs;
     if (inv && (flags & CKKS_NATURAL_ORDER)) HIPCHK(launch_bitrev(data, rows, c->logN, S(stream)));
-    HIPCHK(launch_ntt(t, data, data, n_polys, n_limbs, limb_map(prime0, n_limbs), inv, false, c->ntt_threads,
-                      S(stream)));
+    TIMED(CKKS_KIND_NTT, launch_ntt(t, data, data, n_polys, n_limbs, limb_map(prime0, n_limbs), inv, false,
+                                    c->ntt_threads, S(stream)));
     if (!inv && (flags & CKKS_NATURAL_ORDER)) HIPCHK(launch_bitrev(data, rows, c->logN, S(stream)));
     return CKKS_OK;
 }
@@ -356,8 +437,8 @@ int ckks_poly_mul(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, uint6
     if (!a || !b || !out) return CKKS_ERR_NULL;
     if (misaligned(a) || misaligned(b) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
-    HIPCHK(launch_polymul(c->tables(), a, b, out, n_polys, n_limbs, limb_map(prime0, n_limbs), c->ntt_threads,
-                          S(stream)));
+    TIMED(CKKS_KIND_POLYMUL, launch_polymul(c->tables(), a, b, out, n_polys, n_limbs, limb_map(prime0, n_limbs),
+                                            c->ntt_threads, S(stream)));
     return CKKS_OK;
 }
 
@@ -434,9 +515,9 @@ int ckks_relin(const ckks_ctx* c, const uint64_t* d01, const uint64_t* d2, const
     const ModDownConst mc = moddown_const(c, l, special);
     uint64_t* acc = (uint64_t*)workspace;
     const size_t LN = (size_t)l * c->N;
-    HIPCHK(launch_keyswitch(t, d2, LN, nullptr, nullptr, (const ulonglong2*)key, key_limbs, acc, batch, l, m, special,
-                            c->ks_threads, S(stream)));
-    HIPCHK(launch_moddown_rescale(t, acc, d01, 3, out, batch, l, m, special, mc, false, S(stream)));
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2, LN, nullptr, nullptr, (const ulonglong2*)key, key_limbs, acc,
+                                                batch, l, m, special, c->ks_threads, S(stream)));
+    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, d01, 3, out, batch, l, m, special, mc, false, S(stream)));
     return CKKS_OK;
 }
 
@@ -459,15 +540,17 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
     uint64_t* d2c = d2e + B * LN;
     uint64_t* acc = d2c + B * LN;
     if (c->front_fused) {
-        HIPCHK(launch_front(t, a, b, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
+        TIMED(CKKS_KIND_FRONT, launch_front(t, a, b, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
     } else {
-        HIPCHK(launch_ntt(t, ea, a, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
-        HIPCHK(launch_ntt(t, eb, b, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
-        HIPCHK(launch_tensor_intt(t, ea, eb, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
+        TIMED(CKKS_KIND_NTT, launch_ntt(t, ea, a, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+        TIMED(CKKS_KIND_NTT, launch_ntt(t, eb, b, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+        TIMED(CKKS_KIND_TENSOR,
+              launch_tensor_intt(t, ea, eb, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
     }
-    HIPCHK(launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc, batch, l, m, special,
-                            c->ks_threads, S(stream)));
-    HIPCHK(launch_moddown_rescale(t, acc, nullptr, 0, out, batch, l, m, special, mc, true, S(stream)));
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc, batch,
+                                                l, m, special, c->ks_threads, S(stream)));
+    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, nullptr, 0, out, batch, l, m, special, mc, true,
+                                                    S(stream)));
     return CKKS_OK;
 }
 
@@ -500,10 +583,10 @@ int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int
     const size_t LN = (size_t)l * c->N, B = (size_t)batch;
     uint64_t* sct = (uint64_t*)workspace;   // sigma(c0), sigma(c1)
     uint64_t* acc = sct + B * 2 * LN;
-    HIPCHK(launch_automorphism(t, ct, sct, batch * 2, l, m, galois, S(stream)));
-    HIPCHK(launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, (const ulonglong2*)gkey, key_limbs, acc, batch, l,
-                            m, special, c->ks_threads, S(stream)));
-    HIPCHK(launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream)));
+    TIMED(CKKS_KIND_AUTOMORPHISM, launch_automorphism(t, ct, sct, batch * 2, l, m, galois, S(stream)));
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, (const ulonglong2*)gkey,
+                                                key_limbs, acc, batch, l, m, special, c->ks_threads, S(stream)));
+    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream)));
     return CKKS_OK;
 }
 

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
                                                  int key_limbs, uint64_t* __restrict__ acc,
                                                  const ulonglong2* __restrict__ tw_all,
                                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
-                                                 int batch) {
+                                                 int batch, int reduce_digits) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
@@ -168,8 +168,12 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
             for (int k = 0; k < C::E; ++k) x[k] = s[ntt::eval_index<C>(k, tid)];
         } else {
             ntt::load_coalesced<C>(x, d2c + (size_t)b * d2c_bstride + (size_t)j * C::N, tid);
+            // the digit's residues mod q_j enter the transform mod q_t as they
+            // are when q_j < 2 q_t (reduce_digits = 0); else reduce them first
+            if (reduce_digits) {
 #pragma unroll
-            for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
+                for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
+            }
             // left unreduced (below 2^50): the key product is exact for any 64-bit input
             ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
         }
@@ -435,7 +439,7 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
     hipError_t e = set_lds(k_keyswitch<LOGN, T>, C::LDS_BYTES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_keyswitch<LOGN, T>), dim3(batch * (l + 1)), dim3(T), C::LDS_BYTES, s, d2c, d2s, d2e, d01, key,
-                       key_limbs, acc, t.tw, t.pc, map, l, pidx_p, batch);
+                       key_limbs, acc, t.tw, t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1);
     return hipGetLastError();
 }
 

@@ -11,6 +11,10 @@ struct DevTables {
     const ulonglong2* tw;   // [K][2][N]: forward table, inverse table
     int N;
     int logN;
+    // 1 when every prime is below twice every other one: a residue mod one
+    // prime is then below 2q for any other q, which the lazy forward
+    // transform accepts as input without a reduction.
+    int narrow = 0;
 };
 
 // Thread-count choice for the LDS-resident kernels, per log2 N.

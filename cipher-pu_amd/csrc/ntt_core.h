@@ -61,7 +61,9 @@ __device__ __forceinline__ int pad(int j) { return j + (j >> C::PS); }
 // gets every twiddle address -- or the twiddles themselves -- hoisted and
 // kept live across all of them, and spills.  The statement emits no code.
 __device__ __forceinline__ void opaque(const ulonglong2* __restrict__& tw, int& tid) {
-    asm volatile("" : "+s"(tw), "+v"(tid));
+    int zero = 0;   // an opaque offset keeps the table a global (not flat) pointer
+    asm volatile("" : "+v"(tid), "+s"(zero));
+    tw += zero;
 }
 
 // Global index of register (g, e) in the layout of pass P.

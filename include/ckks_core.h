@@ -227,6 +227,27 @@ int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads, int ten
  * separate transform launches then the tensor kernel.  Same results. */
 int ckks_ctx_set_front(ckks_ctx *ctx, int fused);
 
+/* Kernel timing (measurement hook; no counterpart in the reference, whose
+ * RTL counts cycles in tb/tb_ntt_cycles.sv:1-13).  Between start and stop
+ * every kernel that ckks_ntt_fwd/inv, ckks_poly_mul, ckks_hmult, ckks_relin
+ * and ckks_rotate launch is
+ * bracketed by a pair of HIP events on the call's stream, up to
+ * max_launches launches.  stop waits for the last event, writes per kind
+ * (CKKS_KIND_*) the summed milliseconds and the launch count, and returns
+ * the number of timed launches (negative status on error). */
+enum {
+    CKKS_KIND_NTT = 0,
+    CKKS_KIND_TENSOR = 1,
+    CKKS_KIND_FRONT = 2,
+    CKKS_KIND_KEYSWITCH = 3,
+    CKKS_KIND_MODDOWN = 4,
+    CKKS_KIND_AUTOMORPHISM = 5,
+    CKKS_KIND_POLYMUL = 6,
+    CKKS_KIND_COUNT = 7
+};
+int ckks_timing_start(ckks_ctx *ctx, int max_launches);
+int ckks_timing_stop(ckks_ctx *ctx, double *ms, int *launches, int n_kinds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,7 +201,7 @@ void oracle_hmult(const uint64_t *cta, const uint64_t *ctb, const uint64_t *key,
                   const uint64_t *primes, uint64_t p, const uint64_t *psis,
                   const uint64_t *psi_p) {
     uint64_t LN = (uint64_t)l * N;
-    uint64_t *d0 = malloc(LN * 8), *d1 = malloc(LN * 8), *d2 = malloc(LN * 8);
+    uint64_t *d0 = calloc(LN, 8), *d1 = calloc(LN, 8), *d2 = calloc(LN, 8);
     uint64_t *t = malloc(N * 8);
     for (int i = 0; i < l; i++) {
         uint64_t q = primes[i], ps = psis[i], o = (uint64_t)i * N;
@@ -243,7 +243,7 @@ void oracle_rotate(const uint64_t *ct, const uint64_t *key, uint64_t *out, uint6
                    int l, int key_limbs, const uint64_t *primes, uint64_t p,
                    const uint64_t *psis, const uint64_t *psi_p, uint64_t g) {
     uint64_t LN = (uint64_t)l * N;
-    uint64_t *s0 = malloc(LN * 8), *s1 = malloc(LN * 8), *k1 = malloc(LN * 8);
+    uint64_t *s0 = calloc(LN, 8), *s1 = calloc(LN, 8), *k1 = calloc(LN, 8);
     for (int i = 0; i < l; i++) {
         oracle_automorphism(ct + (uint64_t)i * N, s0 + (uint64_t)i * N, N, primes[i], g);
         oracle_automorphism(ct + LN + (uint64_t)i * N, s1 + (uint64_t)i * N, N, primes[i], g);
