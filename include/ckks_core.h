@@ -115,7 +115,9 @@ int ckks_key_prepare(const ckks_ctx *ctx, const uint64_t *key_coeff, uint64_t *k
  * (ModDown(sum_j D_j b_j), ModDown(-sum_j D_j a_j)), then plus (d0, d1) if
  * given ([batch][2][l][N], coefficient form).  Generalises relin(e0, e1, e2,
  * a, b, q) (scripts/verify_keyswitch.py:62-68; key convention
- * exp_e33_real_ciphertext.py:153-175) to RNS; with d0/d1 it IS relin. */
+ * exp_e33_real_ciphertext.py:153-175) to RNS; with d0/d1 it IS relin.
+ * Residues must be reduced (limb i in [0, q_i)), as every entry point
+ * writes them. */
 int ckks_relin(const ckks_ctx *ctx, const uint64_t *d01, const uint64_t *d2, const uint64_t *key,
                int key_limbs, uint64_t *out, int batch, int l, int special, void *workspace, void *stream);
 

@@ -110,6 +110,21 @@ def test_special_prime():
     assert O.primes_2k_plus(45, 32768, 1, 22)[0] == SPECIAL_P
 
 
+def test_chain5_extra_prime():
+    """The fifth ciphertext prime is the next prime 2^44 + 32768 j + 1 above
+    the special prime (j = 48), brute force over every j in between."""
+    from cipherpu.params import CHAIN5
+    extra = CHAIN5[4]
+    assert CHAIN5[:4] == tuple(CHAIN4)
+    assert O.is_prime(extra) and (extra - 1) % (1 << 15) == 0 and extra < 1 << 45
+    js = [j for j in range(23, 49) if O.is_prime(2 ** 44 + 32768 * j + 1)]
+    assert js == [48] and extra == 2 ** 44 + 32768 * 48 + 1
+    # every prime of the set is below twice every other one (the key switch
+    # then skips reducing a digit's residues into another prime)
+    allp = list(CHAIN5) + [SPECIAL_P]
+    assert max(allp) < 2 * min(allp)
+
+
 # ---------------------------------------------------------------- transforms
 
 def test_ntt_closed_form_and_roundtrip():
