@@ -18,10 +18,11 @@ beside it at 1,452,032 B per HMult (inputs, outputs and the key once per
 batch).  ``cpu_baseline`` times the CPU oracle on a bounded sample.
 ``mlp`` runs the encrypted MLP on 4096 synthetic images under the plain
 decrypt-based monitor gear loop and under the learned gear controller;
-``batch_graph`` runs 2048 mixed operations (N = 4096 / 8192, levels 1..4,
+``cnn`` does the same for the encrypted CNN (conv 5x5x8 stride 2, x^2,
+average pool, FC 288-10); ``batch_graph`` runs 2048 mixed operations (N = 4096 / 8192, levels 1..4,
 HMult / rotation / plaintext product) as one captured HIP graph.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256] [--no-mlp] [--no-graph]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256] [--no-mlp] [--no-cnn] [--no-graph]
     python bench.py --copy-probe      # plain copy of known size (counter scaling)
 """
 from __future__ import annotations
@@ -133,26 +134,31 @@ def copy_probe(nbytes: int = 1 << 28, reps: int = 5) -> None:
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": reps}))
 
 
-def mlp_leg(samples: int, target_db: float, chunk_cts: int):
-    """Encrypted MLP samples/s at a fixed output SNR: the plain decrypt-based
-    monitor gear loop against the learned gear controller, plus static
-    gear 0 without any monitor."""
+def mlp_leg(samples: int, target_db: float, chunk_cts: int, kind: str = "mlp"):
+    """Encrypted MLP (or CNN) samples/s at a fixed output SNR: the plain
+    decrypt-based monitor gear loop against the learned gear controller,
+    plus static gear 0 without any monitor."""
     import numpy as np
     import torch
     from cipherpu import gear, nn
     from cipherpu.ckks import CKKS, CKKSParams
     ck = CKKS(CKKSParams(), seed=99)
     sk = ck.keygen()
-    W1, W2 = nn.make_mlp(0)
-    mlp = nn.EncryptedMLP(ck, sk, W1, W2)
+    if kind == "mlp":
+        mlp = nn.EncryptedMLP(ck, sk, *nn.make_mlp(0))
+        desc = "784-128-10 x^2, DARE between layers"
+    else:
+        from cipherpu import cnn
+        mlp = cnn.EncryptedCNN(ck, sk, *cnn.make_cnn(0))
+        desc = "conv 5x5x8 s2 -> x^2 -> avgpool 2x2 -> FC 288-10, DARE after the convolution"
     net = gear.GearNet(seed=0)
-    trained = gear.load_trained(net)
+    trained = gear.load_trained(net, gear.TRAINED_PATHS[kind])
     if not trained:
         F, T = gear.sweep(mlp, 512, seed=1)
         net.fit(F, T, epochs=60, batch=256)
     X = nn.synthetic_images(samples, seed=4096)
-    ref = nn.plain_forward(W1, W2, X)
-    out = {"metric": "enc-MLP samples/s at fixed output SNR", "network": "784-128-10 x^2, DARE between layers",
+    ref = mlp.reference_outputs(X)
+    out = {"metric": f"enc-{kind.upper()} samples/s at fixed output SNR", "network": desc,
            "samples": samples, "target_snr_db": target_db, "chunk_ciphertexts": chunk_cts,
            "samples_per_ciphertext": mlp.S,
            "controller_weights": trained or "trained in this run on a 512-ciphertext sweep"}
@@ -218,6 +224,7 @@ def main():
     ap.add_argument("--front", action="store_true", help="fused HMult front kernel (with fixed shapes)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape tuner")
     ap.add_argument("--no-mlp", action="store_true", help="skip the encrypted-MLP leg")
+    ap.add_argument("--no-cnn", action="store_true", help="skip the encrypted-CNN leg")
     ap.add_argument("--mlp-samples", type=int, default=4096)
     ap.add_argument("--target-snr", type=float, default=3.0)
     ap.add_argument("--copy-probe", action="store_true")
@@ -318,6 +325,8 @@ def main():
     }
     if not args.no_mlp:
         res["mlp"] = mlp_leg(args.mlp_samples, args.target_snr, 64)
+    if not args.no_cnn:
+        res["cnn"] = mlp_leg(args.mlp_samples, args.target_snr, 64, kind="cnn")
     if not args.no_graph:
         res["batch_graph"] = graph_leg(args.graph_ops)
     if not args.no_cpu_baseline:

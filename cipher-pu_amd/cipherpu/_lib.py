@@ -63,12 +63,16 @@ _SIGS = {
     "ckks_noise_stats": (_I, [_P, _P, _P, _I, _I, _I, _P]),
     "ckks_pt_prepare": (_I, [_P, _P, _P, _I, _I, _I, _P]),
     "ckks_lincomb": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    "ckks_lincomb_scalar": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),
     "ckks_lift_small": (_I, [_P, _P, _P, _I, _I, _I, _P]),
     "ckks_gear_dense_fwd": (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _P]),
     "ckks_gear_dense_bwd": (_I, [_P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),
     "ckks_gear_adam": (_I, [_P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                             _I, _P]),
     "ckks_gear_mse_grad": (_I, [_P, _P, _P, _P, _I, _I, _P]),
+    "ckks_sample_uniform": (_I, [_P, _P, _I, _I, _I, _U, _P]),
+    "ckks_sample_gaussian": (_I, [_P, _P, _I, _I, _I, ctypes.c_double, _U, _P]),
+    "ckks_gaussian_cdf": (_I, [ctypes.c_double, _P, _I]),
     "ckks_timing_start": (_I, [_P, _I]),
     "ckks_timing_stop": (_I, [_P, _P, _P, _I]),
 }

@@ -69,7 +69,13 @@ class SwitchKey:
 
 
 class CKKS:
-    def __init__(self, params: CKKSParams = CKKSParams(), device: int = 0, seed: int = 0):
+    def __init__(self, params: CKKSParams = CKKSParams(), device: int = 0, seed: int = 0,
+                 sampler: str = "device"):
+        """sampler: "device" draws the encryption mask and errors on the GPU
+        (ckks_sample_uniform / ckks_sample_gaussian); "host" draws them with
+        numpy.  Both are seeded from `seed`."""
+        if sampler not in ("device", "host"):
+            raise ValueError("sampler must be 'device' or 'host'")
         self.p = params
         self.N = params.N
         self.L = params.L
@@ -78,6 +84,9 @@ class CKKS:
         self.ring = RingContext(self.N, self.primes, device=device)
         self.device = self.ring.device
         self.rng = np.random.default_rng(seed)
+        self.sampler = sampler
+        self._seed = int(seed) & ((1 << 64) - 1)
+        self._draws = 0
         self._ws = None
 
     # -- sampling (host) ------------------------------------------------------
@@ -93,6 +102,28 @@ class CKKS:
         for k, i in enumerate(limbs):
             out[..., k, :] = self.rng.integers(0, self.primes[i], size=shape + (self.N,), dtype=np.uint64)
         return self.ring.tensor(out)
+
+    def _next_seed(self) -> int:
+        """A fresh 64-bit seed per device draw (splitmix64 of seed and count)."""
+        self._draws += 1
+        z = (self._seed + self._draws * 0x9E3779B97F4A7C15) & ((1 << 64) - 1)
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & ((1 << 64) - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & ((1 << 64) - 1)
+        return z ^ (z >> 31)
+
+    def uniform_device(self, rows: int, l: int) -> torch.Tensor:
+        """[rows, l, N] uniform residues over primes 0..l-1, drawn on the GPU."""
+        out = torch.empty(rows, l, self.N, dtype=torch.uint64, device=self.device)
+        call("ckks_sample_uniform", self.ring.handle, _ptr(out), rows, l, 0, self._next_seed(), self.ring.stream())
+        return out
+
+    def gaussian_device(self, rows: int, l: int) -> torch.Tensor:
+        """[rows, l, N] residues of rounded N(0, sigma^2) errors, one integer
+        per (row, coefficient) across the limbs, drawn on the GPU."""
+        out = torch.empty(rows, l, self.N, dtype=torch.uint64, device=self.device)
+        call("ckks_sample_gaussian", self.ring.handle, _ptr(out), rows, l, 0, float(self.p.sigma),
+             self._next_seed(), self.ring.stream())
+        return out
 
     def small_to_rns(self, x: np.ndarray, limbs: Sequence[int]) -> torch.Tensor:
         """Small signed integers [..., N] -> [..., len(limbs), N] residues."""
@@ -170,12 +201,15 @@ class CKKS:
         """pt [B, l, N] -> ct [B, 2, l, N]:  c1 = -a, c0 = a s + e + m
         (exp_e33_real_ciphertext.py:171-175)."""
         B, l = pt.shape[0], pt.shape[1]
+        dev = self.sampler == "device"
         if a is None:
-            a = self.uniform(B, limbs=range(l))
+            a = self.uniform_device(B, l) if dev else self.uniform(B, limbs=range(l))
         if e is None:
-            e = self.gaussian(B)
+            E = self.gaussian_device(B, l) if dev else self.small_to_rns(self.gaussian(B), range(l))
+        else:
+            E = self.small_to_rns(e, range(l))
         S = sk.rns[:l].unsqueeze(0).expand(B, l, self.N).contiguous()
-        c0 = self.ring.add(self.ring.add(self.ring.poly_mul(a, S), self.small_to_rns(e, range(l))), pt)
+        c0 = self.ring.add(self.ring.add(self.ring.poly_mul(a, S), E), pt)
         c1 = self.ring.neg(a)
         return Ciphertext(torch.stack([c0, c1], dim=1).contiguous(),
                           self.p.scale if scale is None else scale)

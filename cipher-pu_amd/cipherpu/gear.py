@@ -213,8 +213,11 @@ class GearNet:
 
 
 # Controller weights trained by cipherpu.gear_train on monitor sweeps of the
-# 784-128-10 network of nn.make_mlp(0) (synthetic images, every gear).
-TRAINED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "gearnet_mlp784.npz")
+# 784-128-10 network of nn.make_mlp(0) and the CNN of cnn.make_cnn(0)
+# (synthetic images, every gear).
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+TRAINED_PATH = os.path.join(_DATA, "gearnet_mlp784.npz")
+TRAINED_PATHS = {"mlp": TRAINED_PATH, "cnn": os.path.join(_DATA, "gearnet_cnn.npz")}
 
 
 def load_trained(net: GearNet, path: str = TRAINED_PATH) -> Optional[str]:
@@ -249,12 +252,11 @@ def snr_db(ref: np.ndarray, got: np.ndarray) -> float:
 
 def monitor(mlp, ct_out, X: np.ndarray) -> Tuple[float, np.ndarray]:
     """The plain decrypt-based monitor: decrypt and decode the network output,
-    compare it with the FP64 plaintext network on the same inputs.  Returns
-    (SNR over all samples, per-ciphertext SNR) in dB."""
-    from .nn import plain_forward
+    compare it with the dense FP64 plaintext network on the same inputs.
+    Returns (SNR over all samples, per-ciphertext SNR) in dB."""
     n = X.shape[0]
     got = mlp.decrypt_outputs(ct_out, n)
-    ref = plain_forward(mlp.W1, mlp.W2, X)
+    ref = mlp.reference_outputs(X)
     S = mlp.S
     per = np.array([snr_db(ref[i:i + S], got[i:i + S]) for i in range(0, n, S)])
     return snr_db(ref, got), per
@@ -362,6 +364,7 @@ def learned_loop(mlp, net: GearNet, X: np.ndarray, target_db: float, chunk_cts: 
     of every gear -> the cheapest gear that clears target + margin; when no
     gear clears it, the plain monitor search decides the chunk."""
     chunks = _setup(mlp, X, chunk_cts)
+    net.predict(ciphertext_features(mlp.ck, chunks[0][1], 0))   # warm the feature and network kernels
     outs, gears, evals, mon, fb = [], [], 0, 0, 0
     prev = 0
     _sync(mlp)

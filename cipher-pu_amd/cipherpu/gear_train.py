@@ -1,8 +1,9 @@
 """Train the learned gear controller on plain-monitor sweeps, on the GPU.
 
-    python -m cipherpu.gear_train [--samples 200000] [--epochs 40] [--out PATH]
+    python -m cipherpu.gear_train [--net mlp|cnn] [--samples 200000] [--epochs 40] [--out PATH]
 
-A sample is one (ciphertext of 4 synthetic images, gear) pair: its
+A sample is one (ciphertext of 4 synthetic images, gear) pair for the
+encrypted MLP (nn.make_mlp(0)) or CNN (cnn.make_cnn(0)): its
 features (per-limb coefficient moments of the input ciphertext, level,
 scale, previous gear) and the output SNR that the plain decrypt-based
 monitor measures for that gear against the FP64 network.  A held-out
@@ -24,6 +25,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--chunk", type=int, default=256, help="ciphertexts per sweep evaluation")
+    ap.add_argument("--net", choices=("mlp", "cnn"), default="mlp")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
 
@@ -31,8 +33,11 @@ def main(argv=None):
     from .ckks import CKKS, CKKSParams
     ck = CKKS(CKKSParams(), seed=5)
     sk = ck.keygen()
-    W1, W2 = nn.make_mlp(0)
-    mlp = nn.EncryptedMLP(ck, sk, W1, W2)
+    if args.net == "mlp":
+        mlp = nn.EncryptedMLP(ck, sk, *nn.make_mlp(0))
+    else:
+        from . import cnn
+        mlp = cnn.EncryptedCNN(ck, sk, *cnn.make_cnn(0))
     mlp.prepare()
     n_ct = max(1, args.samples // gear.N_GEARS)
     t0 = time.perf_counter()
@@ -52,7 +57,7 @@ def main(argv=None):
     t_fit = time.perf_counter() - t0
     pred = net.predict(Fh).cpu().numpy()
     mae = np.abs(pred - Th).mean(axis=0)
-    out = args.out or gear.TRAINED_PATH
+    out = args.out or gear.TRAINED_PATHS[args.net]
     import os
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     net.save(out)

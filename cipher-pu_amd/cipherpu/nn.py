@@ -126,20 +126,20 @@ class LayerPlan:
     nnz_diagonals: int = 0
 
 
-class EncryptedMLP:
-    def __init__(self, ck: CKKS, sk: SecretKey, W1: np.ndarray, W2: np.ndarray, gear1: int = 0, gear2: int = 0,
-                 w_scale: float = 2.0 ** 40):
+class EncryptedNet:
+    """What the encrypted networks share: keys, rotations, the DARE refresh,
+    the slot pattern of the S samples a ciphertext carries, and the
+    interface the gear loops drive (set_gears, prepare, encrypt_inputs,
+    forward, decrypt_outputs, reference_outputs)."""
+
+    def __init__(self, ck: CKKS, sk: SecretKey, w_scale: float = 2.0 ** 40):
         self.ck, self.sk = ck, sk
-        self.W1, self.W2 = W1, W2
         self.w_scale = w_scale
         self.S = (ck.N // 2) // SLOTS_PER_SAMPLE
         self.rlk = ck.relin_key(sk)
         self._gk: Dict[int, SwitchKey] = {}
         self.ops = {"rot": 0, "ptmul_rows": 0, "hmult": 0, "dare": 0}
-        self._masks = {}
-        self._plan1: Dict[int, tuple] = {}
-        self._plan2: Dict[int, tuple] = {}
-        self.set_gears(gear1, gear2)
+        self.gears = (0, 0)
 
     # -- keys -------------------------------------------------------------
     def gk(self, step: int) -> SwitchKey:
@@ -154,11 +154,62 @@ class EncryptedMLP:
         self.ops["rot"] += ct.batch
         return self.ck.rotate(ct, self.gk(step))
 
-    # -- plans --------------------------------------------------------------
     def _slots(self, v: np.ndarray) -> np.ndarray:
         """A 1024-slot block pattern repeated for the S samples of a ciphertext."""
         return np.tile(v, self.S)
 
+    def rotation_steps(self) -> List[int]:
+        raise NotImplementedError
+
+    def set_gears(self, gear1: int, gear2: int):
+        raise NotImplementedError
+
+    def prepare(self, gears=range(4)) -> None:
+        """Build every gear's plans and every Galois key they need up front,
+        so that gear switches inside a timed loop cost no set-up."""
+        cur = self.gears
+        for g in gears:
+            self.set_gears(g, g)
+            for s in self.rotation_steps():
+                self.gk(s)
+        self.set_gears(*cur)
+
+    def dare(self, ct: Ciphertext) -> Ciphertext:
+        """Decrypt, keep the scale-carrying integer message, encrypt afresh
+        at the top level (exp_e24_rotation_bootstrap.py:287-304)."""
+        ck = self.ck
+        pt = ck.decrypt(ct, self.sk)                                            # [B, l, N]
+        B, l = pt.shape[0], pt.shape[1]
+        top = torch.empty(B, ck.L, ck.N, dtype=torch.uint64, device=ck.device)
+        call("ckks_lift_small", ck.ring.handle, _ptr(pt), _ptr(top), B, l, ck.L, ck.ring.stream())
+        self.ops["dare"] += B
+        return ck.encrypt(top, self.sk, scale=ct.scale)
+
+    def prepared_pt(self, z: np.ndarray, level: int, scale: float) -> torch.Tensor:
+        """Slot vectors [rows, N/2] -> evaluation-form (value, companion) rows
+        [rows, level, N, 2] for ckks_lincomb."""
+        ck = self.ck
+        pt = ck.encode(np.asarray(z, dtype=np.complex128), level=level, scale=scale)
+        prep = torch.empty(*pt.shape, 2, dtype=torch.uint64, device=ck.device)
+        call("ckks_pt_prepare", ck.ring.handle, _ptr(pt), _ptr(prep), pt.shape[0], level, 0, ck.ring.stream())
+        return prep
+
+
+class EncryptedMLP(EncryptedNet):
+    def __init__(self, ck: CKKS, sk: SecretKey, W1: np.ndarray, W2: np.ndarray, gear1: int = 0, gear2: int = 0,
+                 w_scale: float = 2.0 ** 40):
+        super().__init__(ck, sk, w_scale)
+        self.W1, self.W2 = W1, W2
+        self._masks = {}
+        self._plan1: Dict[int, tuple] = {}
+        self._plan2: Dict[int, tuple] = {}
+        self.set_gears(gear1, gear2)
+
+    def reference_outputs(self, X: np.ndarray) -> np.ndarray:
+        """The dense FP64 network: the monitor's reference."""
+        return plain_forward(self.W1, self.W2, X)
+
+    # -- plans --------------------------------------------------------------
     def _plan(self, D: np.ndarray, rows, cols, baby, giant, folds, level) -> LayerPlan:
         ck = self.ck
         p = LayerPlan(rows, cols, baby, giant, folds)
@@ -211,16 +262,6 @@ class EncryptedMLP:
             steps.update(p.baby * g for g in p.used_g if g)
             steps.update(p.folds)
         return sorted(steps)
-
-    def prepare(self, gears=range(4)) -> None:
-        """Build every gear's plans and every Galois key they need up front,
-        so that gear switches inside a timed loop cost no set-up."""
-        cur = self.gears
-        for g in gears:
-            self.set_gears(g, g)
-            for s in self.rotation_steps():
-                self.gk(s)
-        self.set_gears(*cur)
 
     # -- evaluation ---------------------------------------------------------
     def encrypt_inputs(self, X: np.ndarray) -> Ciphertext:
@@ -281,17 +322,6 @@ class EncryptedMLP:
              ck.ring.stream())
         ck.ring.intt_(out.view(-1, l, ck.N))
         return ck.rescale(Ciphertext(out, ct.scale * m_scale))
-
-    def dare(self, ct: Ciphertext) -> Ciphertext:
-        """Decrypt, keep the scale-carrying integer message, encrypt afresh
-        at the top level (exp_e24_rotation_bootstrap.py:287-304)."""
-        ck = self.ck
-        pt = ck.decrypt(ct, self.sk)                                            # [B, l, N]
-        B, l = pt.shape[0], pt.shape[1]
-        top = torch.empty(B, ck.L, ck.N, dtype=torch.uint64, device=ck.device)
-        call("ckks_lift_small", ck.ring.handle, _ptr(pt), _ptr(top), B, l, ck.L, ck.ring.stream())
-        self.ops["dare"] += B
-        return ck.encrypt(top, self.sk, scale=ct.scale)
 
     def layer1(self, ct: Ciphertext) -> Ciphertext:
         y = self._linear(ct, self.p1)                 # outputs in slots [0, 128) of each block

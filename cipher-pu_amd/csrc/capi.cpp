@@ -357,6 +357,51 @@ int ckks_ctx_set_front(ckks_ctx* c, int fused) {
     return CKKS_OK;
 }
 
+// cdf[k] = floor(2^64 P(|round(Z)| <= k)), Z ~ N(0, sigma^2), from the tail
+// P(|round(Z)| > k) = erfc((k + 1/2) / (sigma sqrt 2)) in 80-bit arithmetic.
+static void gauss_table(double sigma, GaussTable& t) {
+    const long double two64 = 18446744073709551616.0L;
+    for (int k = 0; k < GAUSS_TABLE; ++k) {
+        const long double tail = erfcl((k + 0.5L) / ((long double)sigma * 1.41421356237309504880L)) * two64;
+        t.cdf[k] = tail < 1.0L ? UINT64_MAX : (uint64_t)(two64 - tail);
+    }
+    t.cdf[GAUSS_TABLE - 1] = UINT64_MAX;
+}
+
+int ckks_gaussian_cdf(double sigma, uint64_t* cdf, int n) {
+    if (!cdf) return CKKS_ERR_NULL;
+    if (!(sigma > 0.0) || sigma > 10.0 || n < 1 || n > GAUSS_TABLE) return CKKS_ERR_BAD_ARG;
+    GaussTable t;
+    gauss_table(sigma, t);
+    for (int k = 0; k < n; ++k) cdf[k] = t.cdf[k];
+    return CKKS_OK;
+}
+
+int ckks_sample_uniform(const ckks_ctx* c, uint64_t* out, int rows, int l, int prime0, uint64_t seed,
+                        void* stream) {
+    int st = check_rows(c, rows, l, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!out) return CKKS_ERR_NULL;
+    if (misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_sample_uniform(c->tables(), out, rows, l, limb_map(prime0, l), seed, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_sample_gaussian(const ckks_ctx* c, uint64_t* out, int rows, int l, int prime0, double sigma, uint64_t seed,
+                         void* stream) {
+    int st = check_rows(c, rows, l, prime0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!out) return CKKS_ERR_NULL;
+    if (!(sigma > 0.0) || sigma > 10.0) return CKKS_ERR_BAD_ARG;
+    if (misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    GaussTable t;
+    gauss_table(sigma, t);
+    HIPCHK(launch_sample_gauss(c->tables(), out, rows, l, limb_map(prime0, l), seed, t, S(stream)));
+    return CKKS_OK;
+}
+
 int ckks_timing_start(ckks_ctx* c, int max_launches) {
     if (!c) return CKKS_ERR_NULL;
     if (max_launches < 1 || max_launches > (1 << 20)) return CKKS_ERR_BAD_SHAPE;
@@ -687,7 +732,19 @@ int ckks_lincomb(const ckks_ctx* c, const uint64_t* E, const uint64_t* D, uint64
     if (!E || !D || !out) return CKKS_ERR_NULL;
     if (misaligned(E) || misaligned(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
-    HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), S(stream)));
+    HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), 0, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_lincomb_scalar(const ckks_ctx* c, const uint64_t* E, const uint64_t* D, uint64_t* out, int nb, int ng,
+                        int batch, int l, void* stream) {
+    if (!c) return CKKS_ERR_NULL;
+    if (nb < 1 || nb > 32 || ng < 0 || batch < 0 || l < 1 || l > c->K) return CKKS_ERR_BAD_SHAPE;
+    if (!batch || !ng) return CKKS_OK;
+    if (!E || !D || !out) return CKKS_ERR_NULL;
+    if (misaligned(E) || misaligned(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), 1, S(stream)));
     return CKKS_OK;
 }
 

@@ -145,6 +145,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeConst pc = pcs[pt];
+    const uint64_t nq = (uint64_t)0 - pc.q;
     const ulonglong2* tw = tw_all + (size_t)pt * 2 * C::N;
     const size_t LN = (size_t)l * C::N;
     uint64_t a0[C::E], a1[C::E], x[C::E];
@@ -174,7 +175,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
 #pragma unroll
                 for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
             }
-            // left unreduced (below 2^50): the key product is exact for any 64-bit input
+            // left unreduced (below 2^51): the key product is exact for any such input
             ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
         }
         const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
@@ -183,8 +184,10 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
         for (int k = 0; k < C::E; ++k) {
             const int idx = ntt::eval_index<C>(k, tid);
             const ulonglong2 wb = kb[idx], wa = ka[idx];
-            a0[k] += mul_shoup(x[k], wb.x, wb.y, pc.q);
-            a1[k] += mul_shoup(x[k], wa.x, wa.y, pc.q);
+            // each product < 4q: with at most 15 digits and a start below q
+            // the sums stay below 2^51
+            a0[k] += mul_shoup4_nq(x[k], wb.x, wb.y, nq);
+            a1[k] += mul_shoup4_nq(x[k], wa.x, wa.y, nq);
         }
     }
 #pragma unroll
@@ -360,22 +363,27 @@ __global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restri
 //   out[g][c][poly][i][n] = sum_b E[b][c][poly][i][n] * D[g][b][i][n]  (mod q_i)
 // E: [nb][B][2][l][N] evaluation-form ciphertext rows, D: [ng][nb][l][N]
 // (value, Shoup companion) plaintext rows shared by the whole batch.
+// With scalar = 1, D is [ng][nb][l]: one integer per (g, b, limb), the
+// same product in either domain (a constant plaintext), so E may then be in
+// coefficient form.
 __global__ void k_lincomb(const uint64_t* __restrict__ E, const ulonglong2* __restrict__ D,
                           uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map, int nb,
-                          int ng, int l, int logN, size_t per_b) {
+                          int ng, int l, int logN, size_t per_b, int scalar) {
     const size_t g0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g0 >= per_b) return;
     const size_t N = (size_t)1 << logN;
     const size_t n = g0 & (N - 1);
     const int i = (int)((g0 >> logN) % l);
     const PrimeConst& pc = pcs[map.idx[i]];
-    const size_t dstride = (size_t)l * N;
+    const size_t dn = scalar ? 1 : N;
+    const size_t dstride = (size_t)l * dn;
+    const uint64_t nq = (uint64_t)0 - pc.q;
     for (int g = 0; g < ng; ++g) {
-        const ulonglong2* d = D + (size_t)g * nb * dstride + (size_t)i * N + n;
+        const ulonglong2* d = D + (size_t)g * nb * dstride + (size_t)i * dn + (scalar ? 0 : n);
         uint64_t acc = 0;
         for (int b = 0; b < nb; ++b) {   // the E row of this thread stays in L1 across g
             const ulonglong2 w = d[(size_t)b * dstride];
-            acc += mul_shoup(E[(size_t)b * per_b + g0], w.x, w.y, pc.q);   // < 2q each; nb <= 32 keeps acc < 2^51
+            acc += mul_shoup4_nq(E[(size_t)b * per_b + g0], w.x, w.y, nq);   // < 4q each; nb <= 32: acc < 2^52
         }
         out[(size_t)g * per_b + g0] = reduce64(acc, pc);
     }
@@ -550,11 +558,11 @@ hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64
 }
 
 hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
-                          int batch, int l, const LimbMap& map, hipStream_t s) {
+                          int batch, int l, const LimbMap& map, int scalar, hipStream_t s) {
     const size_t per_b = ((size_t)batch * 2 * l) << t.logN;
     if (!per_b || !ng) return hipSuccess;
     hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((per_b + 255) / 256)), dim3(256), 0, s, E, D, out, t.pc, map, nb, ng,
-                       l, t.logN, per_b);
+                       l, t.logN, per_b, scalar);
     return hipGetLastError();
 }
 

@@ -17,6 +17,13 @@ struct DevTables {
     int narrow = 0;
 };
 
+// Cumulative table of |e| for the rounded Gaussian: cdf[k] = floor(2^64
+// P(|round(Z)| <= k)), saturated at 2^64 - 1.
+#define GAUSS_TABLE 48
+struct GaussTable {
+    uint64_t cdf[GAUSS_TABLE];
+};
+
 // Thread-count choice for the LDS-resident kernels, per log2 N.
 struct NttPlan {
     int threads;   // 0 = default
@@ -67,7 +74,7 @@ hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t*
 hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
                              int ell, hipStream_t s);
 hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
-                          int batch, int l, const LimbMap& map, hipStream_t s);
+                          int batch, int l, const LimbMap& map, int scalar, hipStream_t s);
 hipError_t launch_lift_small(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, int l_in, int l_out,
                              const LimbMap& map, hipStream_t s);
 hipError_t launch_noise_stats(const DevTables& t, const uint64_t* in, uint64_t* stats, int n_polys,
@@ -93,3 +100,8 @@ hipError_t launch_encode_round(const DevTables& t, const double* coeffs, uint64_
                                const LimbMap& map, double scale, hipStream_t s);
 hipError_t launch_decode_crt(const DevTables& t, const uint64_t* pt, double* coeffs, int n_vec, const LimbMap& map,
                              const CrtConst& cc, double inv_scale, hipStream_t s);
+
+hipError_t launch_sample_uniform(const DevTables& t, uint64_t* out, int rows, int l, const LimbMap& map,
+                                 uint64_t seed, hipStream_t s);
+hipError_t launch_sample_gauss(const DevTables& t, uint64_t* out, int rows, int l, const LimbMap& map,
+                               uint64_t seed, const GaussTable& tab, hipStream_t s);

@@ -106,25 +106,42 @@ __device__ __forceinline__ uint64_t mul_shoup_nq(uint64_t a, uint64_t w, uint64_
     return a * w + mulhi63(a, ws) * nq;
 }
 
-// Unreduced butterflies for the LDS-resident transforms.  Shoup's product is
-// exact in [0, 2q) for any 64-bit input, so no per-stage reduction is needed:
-//  * forward:  X, Y < B  ->  X' = X + WY < B + 2q,  Y' = X + 2q - WY < B + 2q.
-//    From inputs below 2q, after s stages every value is below (2 + 2s) q
-//    (< 2^50 for log2 N <= 14 and q < 2^45).
+// floor(a ws / 2^63) minus 0, 1 or 2, for a, ws < 2^63: the low-word product
+// a0 w0 (< 2^64, i.e. < 2 at the 2^63 scale) and the fraction of the middle
+// sum are dropped.  m = a1 w0 + a0 w1 < 2^63 + 2^63 fits 64 bits.
+__device__ __forceinline__ uint64_t mulhi63_lazy(uint64_t a, uint64_t ws) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t w0 = (uint32_t)ws, w1 = (uint32_t)(ws >> 32);
+    const uint64_t m = (uint64_t)a1 * w0 + (uint64_t)a0 * w1;
+    return (((uint64_t)a1 * w1) << 1) + (m >> 31);
+}
+
+// a w mod q up to 3q, result in [0, 4q), for a < 2^63: Shoup's quotient is
+// at most 1 below floor(a w / q) and the lazy high half at most 2 below that.
+__device__ __forceinline__ uint64_t mul_shoup4_nq(uint64_t a, uint64_t w, uint64_t ws, uint64_t nq) {
+    return a * w + mulhi63_lazy(a, ws) * nq;
+}
+
+// Unreduced butterflies for the LDS-resident transforms; no per-stage
+// reduction is needed:
+//  * forward (product in [0, 4q)):  X, Y < B  ->  X' = X + WY < B + 4q,
+//    Y' = X + 4q - WY < B + 4q.  From inputs below 2q, after s stages every
+//    value is below (2 + 4s) q (< 2^51 for log2 N <= 14 and q < 2^45).
 //  * inverse:  X, Y < B (B a multiple of q)  ->  X' = X + Y < 2B,
-//    Y' = (X - Y + B) W < 2q.  From inputs below 2q the bound at stage s is
-//    2^(s+1) q (< 2^60 for log2 N <= 14).
+//    Y' = (X - Y + B) W < 4q <= 2B.  From inputs below 2q the bound at stage
+//    s is 2^(s+1) q (< 2^60 for log2 N <= 14); the last stage, which folds
+//    N^-1, uses the exact product so the transform ends in [0, 2q).
 __device__ __forceinline__ void ct_bfly_lazy(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t ws, uint64_t nq,
-                                             uint64_t two_q) {
-    const uint64_t t = mul_shoup_nq(Y, w, ws, nq);
+                                             uint64_t four_q) {
+    const uint64_t t = mul_shoup4_nq(Y, w, ws, nq);
     const uint64_t x = X;
     X = x + t;
-    Y = x + two_q - t;
+    Y = x + four_q - t;
 }
 
 __device__ __forceinline__ void gs_bfly_lazy(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t ws, uint64_t nq,
                                              uint64_t bound) {
     const uint64_t x = X, y = Y;
     X = x + y;
-    Y = mul_shoup_nq(x + bound - y, w, ws, nq);
+    Y = mul_shoup4_nq(x + bound - y, w, ws, nq);
 }

@@ -77,7 +77,7 @@ __device__ __forceinline__ int index_of(int g, int e, int tid) {
 template <class C, int P>
 __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ tw,
                                          uint64_t q, uint64_t two_q, int tid) {
-    const uint64_t nq = (uint64_t)0 - q;
+    const uint64_t nq = (uint64_t)0 - q, four_q = two_q << 1;
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
@@ -92,7 +92,7 @@ __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* 
                 if (e & h) continue;
                 const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
                 const ulonglong2 w = tw[base + i];
-                ct_bfly_lazy(x[g * M + e], x[g * M + e + h], w.x, w.y, nq, two_q);
+                ct_bfly_lazy(x[g * M + e], x[g * M + e + h], w.x, w.y, nq, four_q);
             }
         }
     }
@@ -132,22 +132,39 @@ __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* 
     }
 }
 
+// Padded LDS slot of register (g, e) in pass P's layout, as base(tid) +
+// g SG + e SE.  The e bits sit at LO..HI of the index and the g bits above
+// HI; both are multiples of 2^PS whenever LO >= PS, so the padding adds
+// e 2^(LO-PS) and g-part >> PS exactly; in the last pass (LO = 0, e < 2^PS)
+// the padding does not see e at all.  (T is a multiple of 2^LO in every
+// built configuration.)  Affine slots let the LDS instructions carry the
+// per-register part as an immediate offset.
+template <class C, int P>
+struct Lay {
+    static constexpr int LO = C::lo(P), HI = C::hi(P);
+    static_assert(C::T % (1 << LO) == 0, "threads must cover the pass's low bits");
+    static constexpr int SE = (1 << LO) + (LO >= C::PS ? (1 << (LO - C::PS)) : 0);
+    static constexpr int SG = ((C::T >> LO) << (HI + 1)) + (((C::T >> LO) << (HI + 1)) >> C::PS);
+};
+
 template <class C, int P>
 __device__ __forceinline__ void to_lds(const uint64_t (&x)[C::E], uint64_t* lds, int tid) {
     constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
+    uint64_t* base = lds + pad<C>(index_of<C, P>(0, 0, tid));
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int e = 0; e < M; ++e) lds[pad<C>(index_of<C, P>(g, e, tid))] = x[g * M + e];
+        for (int e = 0; e < M; ++e) base[g * Lay<C, P>::SG + e * Lay<C, P>::SE] = x[g * M + e];
 }
 
 template <class C, int P>
 __device__ __forceinline__ void from_lds(uint64_t (&x)[C::E], const uint64_t* lds, int tid) {
     constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
+    const uint64_t* base = lds + pad<C>(index_of<C, P>(0, 0, tid));
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int e = 0; e < M; ++e) x[g * M + e] = lds[pad<C>(index_of<C, P>(g, e, tid))];
+        for (int e = 0; e < M; ++e) x[g * M + e] = base[g * Lay<C, P>::SG + e * Lay<C, P>::SE];
 }
 
 // Forward passes 1..NP-1 (pass 0 already applied), exchanging through LDS.
@@ -167,7 +184,7 @@ __device__ __forceinline__ void fwd_rest(uint64_t (&x)[C::E], uint64_t* lds,
 
 // Full forward transform: x holds pass-0 layout on entry (x[e] = a[e*T+tid],
 // values below 2q), the last pass's layout on exit, values below
-// (2 + 2 log2 N) q -- reduce with reduce64.
+// (2 + 4 log2 N) q -- reduce with reduce64.
 template <class C>
 __device__ __forceinline__ void fwd_ntt(uint64_t (&x)[C::E], uint64_t* lds,
                                         const ulonglong2* __restrict__ tw, uint64_t q,

@@ -184,6 +184,13 @@ int ckks_pt_prepare(const ckks_ctx *ctx, uint64_t *pt, uint64_t *out, int n_poly
  * [ng][batch][2][l][N].  nb <= 32. */
 int ckks_lincomb(const ckks_ctx *ctx, const uint64_t *E, const uint64_t *D, uint64_t *out, int nb, int ng,
                  int batch, int l, void *stream);
+/* The same with one constant per (g, b, limb): D is [ng][nb][l] (value,
+ * Shoup companion) pairs, the residues of an integer weight.  A constant
+ * multiplies every coefficient alike, so E may be in either domain.  Used by
+ * the encrypted convolution (the reference's encrypted_conv,
+ * scripts/exp_e14_cnn_e2e.py:90-127, as rotations and scalar weights). */
+int ckks_lincomb_scalar(const ckks_ctx *ctx, const uint64_t *E, const uint64_t *D, uint64_t *out, int nb, int ng,
+                        int batch, int l, void *stream);
 
 /* DARE lift (the message-keeping refresh of
  * exp_e24_rotation_bootstrap.py:287-304): rows of a decrypted plaintext
@@ -228,6 +235,21 @@ int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads, int ten
  * forward transforms, the tensor product and INTT(d2); 0 (default) =
  * separate transform launches then the tensor kernel.  Same results. */
 int ckks_ctx_set_front(ckks_ctx *ctx, int fused);
+
+/* Device sampling for encryption (the reference samples on the host:
+ * uniform a and the rounded Box-Muller dgauss of
+ * scripts/exp_e33_real_ciphertext.py:75-85, sigma = 3.2).  Counter-based
+ * splitmix64 draws: the output depends only on seed and the element index.
+ * Not a cryptographic generator.  uniform: out [rows][l][N] uniform in
+ * [0, q_(prime0+i)).  gaussian: one rounded N(0, sigma^2) integer per (row,
+ * n), written as residues into all l limbs; 0 < sigma <= 10. */
+int ckks_sample_uniform(const ckks_ctx *ctx, uint64_t *out, int rows, int l, int prime0, uint64_t seed,
+                        void *stream);
+int ckks_sample_gaussian(const ckks_ctx *ctx, uint64_t *out, int rows, int l, int prime0, double sigma,
+                         uint64_t seed, void *stream);
+/* Host only: the Gaussian sampler's table, cdf[k] = floor(2^64
+ * P(|round(Z)| <= k)) for k < n <= 48. */
+int ckks_gaussian_cdf(double sigma, uint64_t *cdf, int n);
 
 /* Kernel timing (measurement hook; no counterpart in the reference, whose
  * RTL counts cycles in tb/tb_ntt_cycles.sv:1-13).  Between start and stop

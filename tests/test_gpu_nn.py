@@ -54,3 +54,94 @@ def test_gears_prune_and_match_pruned_network(mlp_env, gear):
     # pruning costs accuracy against the dense network, more for higher gears
     dense = nn.plain_forward(W1, W2, X)
     assert nn.output_snr_db(dense, out) < 60
+
+
+def test_lincomb_scalar_bit_exact(mlp_env):
+    """out[g] = sum_b D[g][b] E[b] mod q_i with one integer per (g, b, limb),
+    against Python integers."""
+    import torch
+    from cipherpu._lib import call
+    from cipherpu.ring import to_numpy
+    import ctypes
+    ck = mlp_env[0]
+    rng = np.random.default_rng(9)
+    nb, ng, B, l = 3, 2, 2, 2
+    E = np.stack([np.stack([np.stack([rng.integers(0, ck.primes[i], ck.N, dtype=np.uint64) for i in range(l)])
+                            for _ in range(2)]) for _ in range(nb * B)]).reshape(nb, B, 2, l, ck.N)
+    W = rng.integers(-2 ** 44, 2 ** 44, (ng, nb))
+    D = np.zeros((ng, nb, l, 2), dtype=np.uint64)
+    for g in range(ng):
+        for b in range(nb):
+            for i in range(l):
+                q = ck.primes[i]
+                r = int(W[g, b]) % q
+                D[g, b, i] = (r, (r << 63) // q)
+    Et, Dt = ck.ring.tensor(E), ck.ring.tensor(D)
+    out = torch.empty(ng, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    call("ckks_lincomb_scalar", ck.ring.handle, p(Et), p(Dt), p(out), nb, ng, B, l, ck.ring.stream())
+    got = to_numpy(out)
+    for g in range(ng):
+        for i in range(l):
+            q = ck.primes[i]
+            want = sum(E[b, :, :, i].astype(object) * (int(W[g, b]) % q) for b in range(nb)) % q
+            assert np.array_equal(got[g, :, :, i].astype(object), want), (g, i)
+
+
+def test_cnn_matches_plaintext(mlp_env):
+    from cipherpu import cnn, nn
+    ck, sk = mlp_env[0], mlp_env[1]
+    K, F = cnn.make_cnn(0)
+    net = cnn.EncryptedCNN(ck, sk, K, F)
+    X = nn.synthetic_images(8, seed=5)
+    out = net.decrypt_outputs(net.forward(net.encrypt_inputs(X)), 8)
+    ref = cnn.cnn_forward(K, F, X)
+    assert np.max(np.abs(out - ref)) < 1e-3 * np.max(np.abs(ref))
+    assert nn.output_snr_db(ref, out) > 60
+    net.set_gears(2, 2)
+    PK, PF = net.pruned_weights()
+    out2 = net.decrypt_outputs(net.forward(net.encrypt_inputs(X)), 8)
+    ref2 = cnn.cnn_forward(PK, PF, X)
+    assert np.max(np.abs(out2 - ref2)) < 1e-3 * np.max(np.abs(ref2))
+    assert np.mean(PF == 0) > 0.45
+
+
+def test_device_samplers(mlp_env):
+    """Uniform residues in range with the right mean; rounded-Gaussian errors
+    the same integer in every limb, with the table's distribution; both
+    reproducible from the seed; a device-sampled encryption decrypts."""
+    import ctypes
+    import torch
+    from cipherpu._lib import call, load
+    from cipherpu.ring import to_numpy
+    ck, sk = mlp_env[0], mlp_env[1]
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    rows, l = 64, 3
+    u = torch.empty(rows, l, ck.N, dtype=torch.uint64, device=ck.device)
+    call("ckks_sample_uniform", ck.ring.handle, p(u), rows, l, 0, 12345, ck.ring.stream())
+    U = to_numpy(u)
+    for i in range(l):
+        q = ck.primes[i]
+        assert U[:, i].max() < q
+        assert abs(U[:, i].astype(np.float64).mean() / q - 0.5) < 0.005
+    u2 = torch.empty_like(u)
+    call("ckks_sample_uniform", ck.ring.handle, p(u2), rows, l, 0, 12345, ck.ring.stream())
+    assert torch.equal(u, u2)
+    g = torch.empty(rows, l, ck.N, dtype=torch.uint64, device=ck.device)
+    call("ckks_sample_gaussian", ck.ring.handle, p(g), rows, l, 0, ctypes.c_double(3.2), 777, ck.ring.stream())
+    G = to_numpy(g).astype(np.int64)
+    e = np.where(G[:, 0] > ck.primes[0] // 2, G[:, 0] - ck.primes[0], G[:, 0])
+    for i in range(1, l):
+        q = ck.primes[i]
+        assert np.array_equal(np.where(G[:, i] > q // 2, G[:, i] - q, G[:, i]), e)
+    assert abs(e.mean()) < 0.05 and abs(e.std() - 3.2) < 0.05
+    cdf = (ctypes.c_uint64 * 48)()
+    load().ckks_gaussian_cdf(ctypes.c_double(3.2), cdf, 48)
+    probs = np.diff(np.concatenate([[0.0], np.array([float(c) for c in cdf]) / 2.0 ** 64]))
+    for k in range(6):
+        assert abs(np.mean(np.abs(e) == k) - probs[k]) < 0.01, k
+    z = np.random.default_rng(3).uniform(-1, 1, (2, ck.N // 2))
+    ct = ck.encrypt(ck.encode(z), sk)
+    assert ck.sampler == "device"
+    dec = to_numpy(ck.decode(ck.decrypt(ct, sk), ct.scale)).real
+    assert np.max(np.abs(dec - z)) < 1e-7

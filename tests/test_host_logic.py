@@ -1,0 +1,51 @@
+"""Host-side logic that needs no GPU: the uneven-batch workload generator,
+the learned controller's gear choice, the monitor's SNR and the trained
+controller file."""
+import math
+
+import numpy as np
+
+
+def test_workload_generator_mix_and_levels():
+    from cipherpu.sched import make_workload
+    ops = make_workload(2048, seed=0)
+    kinds = [o.kind for o in ops]
+    assert len(ops) == 2048
+    assert abs(kinds.count("hmult") / 2048 - 0.6) < 0.05
+    assert abs(kinds.count("rotate") / 2048 - 0.3) < 0.05
+    assert abs(kinds.count("ptmul") / 2048 - 0.1) < 0.03
+    assert sum(o.N == 4096 for o in ops) == 1024 and sum(o.N == 8192 for o in ops) == 1024
+    assert {o.level for o in ops} == {1, 2, 3, 4}
+    for o in ops:
+        if o.kind == "rotate":
+            assert 0 < o.step < o.N // 2 and o.step & (o.step - 1) == 0
+        else:
+            assert o.step == 0
+    assert make_workload(64, seed=3) == make_workload(64, seed=3)
+
+
+def test_choose_gear_margin_and_fallback():
+    from cipherpu.gear import choose_gear
+    # cheapest gear first: gear 3 clears 10 + 3 dB
+    assert choose_gear([60, 20, 15, 13.5], 10.0) == 3
+    assert choose_gear([60, 20, 15, 12.9], 10.0) == 2
+    assert choose_gear([60, 12, 11, 10.5], 10.0) == 0
+    # no gear clears target + margin: the plain monitor decides
+    assert choose_gear([12, 11, 10, 9], 10.0) is None
+    assert choose_gear([12, 11, 10, 9], 10.0, margin_db=0.0) == 2
+
+
+def test_snr_closed_form():
+    from cipherpu.gear import snr_db
+    ref = np.array([[3.0, 4.0]])
+    assert math.isinf(snr_db(ref, ref))
+    # signal energy 25, noise energy 0.25 -> 20 dB
+    assert abs(snr_db(ref, ref + np.array([[0.3, 0.4]])) - 20.0) < 1e-12
+
+
+def test_trained_controller_file_loads_safely():
+    from cipherpu import gear
+    with np.load(gear.TRAINED_PATH, allow_pickle=False) as z:
+        shapes = {k: z[k].shape for k in z.files}
+    assert shapes["W0"] == (64, 16) and shapes["W1"] == (64, 64) and shapes["W2"] == (4, 64)
+    assert shapes["mu"] == (16,) and shapes["y_mu"] == (4,)

@@ -70,3 +70,24 @@ def test_argument_validation_without_gpu():
     too_big = (ctypes.c_uint64 * 1)((1 << 45) + 32769)
     assert L.ckks_ctx_create(ctypes.byref(h), 8192, 1, too_big, None, 0, 0) == -2
     assert L.ckks_hmult(None, None, None, None, 5, None, 1, 4, 4, None, None) == -6
+
+
+def test_gaussian_table_against_closed_form():
+    """The sampler's table: cdf[k] = 2^64 P(|round(Z)| <= k), Z ~ N(0, 3.2^2),
+    against scipy's normal CDF (host-only entry point, no GPU)."""
+    import ctypes
+    from scipy.special import erfc
+    from cipherpu import _lib
+    L = _lib.load()
+    n = 48
+    buf = (ctypes.c_uint64 * n)()
+    assert L.ckks_gaussian_cdf(ctypes.c_double(3.2), buf, n) == 0
+    cdf = [int(v) for v in buf]
+    for k in range(n):
+        want = 2.0 ** 64 * (1.0 - erfc((k + 0.5) / (3.2 * 2 ** 0.5)))
+        assert abs(cdf[k] - want) <= 2.0 ** 64 * 1e-13 + 1, k
+    assert cdf == sorted(cdf) and cdf[-1] == 2 ** 64 - 1
+    # P(e = 0) for sigma = 3.2 is erf(0.5 / (3.2 sqrt 2)) = 0.124164...
+    assert abs(cdf[0] / 2.0 ** 64 - 0.124164) < 1e-5
+    assert L.ckks_gaussian_cdf(ctypes.c_double(0.0), buf, n) == -7
+    assert L.ckks_gaussian_cdf(ctypes.c_double(3.2), buf, 49) == -7

@@ -1,7 +1,7 @@
 """CPU checks of the gear layer: the gear-network oracle (gradients by
 finite differences, BF16 rounding against torch's conversion), the
 block-pruning rule of the encrypted MLP against the oracle's restatement of
-the reference rule, and the diagonal packing plan of the encrypted MLP
+the reference rule, and the slot plans of the encrypted MLP and CNN
 simulated in plaintext (rotations as cyclic shifts)."""
 import math
 
@@ -143,3 +143,55 @@ def test_diagonal_plan_in_plaintext():
     # pruned weights survive the diagonal round trip
     P1 = W1 * nn.block_keep_mask(W1, 0.5)
     assert np.array_equal(nn.weights_from_diagonals(nn.extended_diagonals(P1, 128, 896), 128, 896, P1.shape), P1)
+
+
+def test_cnn_packing_in_plaintext():
+    """The encrypted CNN's slot plan, run on plaintext slot vectors with
+    rotations as cyclic shifts, equals the FP64 network."""
+    from cipherpu import cnn, nn
+    K, F = cnn.make_cnn(1)
+    X = nn.synthetic_images(4, seed=2)
+    v = cnn.pack_images(X, 4)[0]
+    rot = lambda z, k: np.roll(z, -k)
+    chans = []
+    for c in range(cnn.N_CH):
+        y = sum(K[c, u, w] * rot(v, cnn.tap_step(u, w)) for u in range(5) for w in range(5))
+        a = y ** 2
+        t = a + rot(a, 1)
+        chans.append(t + rot(t, cnn.ROW))
+    outs = []
+    for k in range(cnn.N_OUT):
+        z = np.zeros_like(v)
+        for c in range(cnn.N_CH):
+            blk = np.zeros(1024)
+            for i in range(6):
+                for j in range(6):
+                    blk[32 * i + 2 * j] = F[k, 36 * c + 6 * i + j] / 4
+            z += np.tile(blk, 4) * chans[c]
+        outs.append(z)
+    got = np.zeros((4, 10))
+    for p in range(0, 10, 4):
+        z = sum(rot(outs[p + m], -256 * m) for m in range(min(4, 10 - p)))
+        for k in range(1, 9):
+            z = z + rot(z, 256 >> k)
+        for s in range(4):
+            for m in range(min(4, 10 - p)):
+                got[s, p + m] = z[1024 * s + 256 * m]
+    assert np.allclose(got, cnn.cnn_forward(K, F, X), atol=1e-10)
+
+
+def test_cnn_forward_against_direct_convolution():
+    """cnn_forward's strided slicing against a direct loop over outputs."""
+    from cipherpu import cnn, nn
+    K, F = cnn.make_cnn(3)
+    X = nn.synthetic_images(2, seed=4)
+    img = X.reshape(2, 28, 28)
+    y = np.zeros((2, 8, 12, 12))
+    for n in range(2):
+        for c in range(8):
+            for i in range(12):
+                for j in range(12):
+                    y[n, c, i, j] = np.sum(K[c] * img[n, 2 * i:2 * i + 5, 2 * j:2 * j + 5])
+    a = y ** 2
+    p = a.reshape(2, 8, 6, 2, 6, 2).mean(axis=(3, 5))
+    assert np.allclose(cnn.cnn_forward(K, F, X), p.reshape(2, 288) @ F.T, atol=1e-12)
