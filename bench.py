@@ -24,6 +24,7 @@ HMult / rotation / plaintext product) as one captured HIP graph.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256] [--no-mlp] [--no-cnn] [--no-graph]
     python bench.py --copy-probe      # plain copy of known size (counter scaling)
+    python bench.py --modmul-probe    # Barrett vs special-form point-wise product
 """
 from __future__ import annotations
 
@@ -134,6 +135,35 @@ def copy_probe(nbytes: int = 1 << 28, reps: int = 5) -> None:
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": reps}))
 
 
+def modmul_probe(rows: int = 1024, reps: int = 10) -> None:
+    """Point-wise products over the four chain primes: Barrett (op 0)
+    against the special-form reduction for q = 2^44 + c (op 4)."""
+    import numpy as np
+    import torch
+    from cipherpu.ckks import CKKSParams
+    from cipherpu.ring import RingContext
+    p = CKKSParams()
+    ring = RingContext(p.N, list(p.primes))
+    rng = np.random.default_rng(0)
+    a = ring.tensor(np.stack([rng.integers(0, q, (rows, p.N), dtype=np.uint64) for q in p.primes], axis=1))
+    b = ring.tensor(np.stack([rng.integers(0, q, (rows, p.N), dtype=np.uint64) for q in p.primes], axis=1))
+    out = torch.empty_like(a)
+    res = {"elements": a.numel()}
+    for name, op in (("barrett", 0), ("special_form", 4)):
+        ring._pw(a, b, op, 0, out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ring._pw(a, b, op, 0, out)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": round(ms, 4), "G_products_per_s": round(a.numel() / ms / 1e6, 1),
+                     "GB_per_s": round(3 * 8 * a.numel() / ms / 1e6, 1)}
+    print(json.dumps(res))
+
+
 def mlp_leg(samples: int, target_db: float, chunk_cts: int, kind: str = "mlp"):
     """Encrypted MLP (or CNN) samples/s at a fixed output SNR: the plain
     decrypt-based monitor gear loop against the learned gear controller,
@@ -228,6 +258,7 @@ def main():
     ap.add_argument("--mlp-samples", type=int, default=4096)
     ap.add_argument("--target-snr", type=float, default=3.0)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--modmul-probe", action="store_true", help="Barrett vs special-form point-wise product")
     ap.add_argument("--no-graph", action="store_true", help="skip the uneven-batch graph leg")
     ap.add_argument("--graph-ops", type=int, default=2048)
     args = ap.parse_args()
@@ -237,6 +268,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.modmul_probe:
+        modmul_probe()
         return
 
     import numpy as np

@@ -491,7 +491,12 @@ int ckks_pointwise(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, uint
                    int prime0, int op, void* stream) {
     int st = check_rows(c, n_polys, n_limbs, prime0);
     if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
-    if (op < 0 || op > 3) return CKKS_ERR_BAD_ARG;
+    if (op < 0 || op > 4) return CKKS_ERR_BAD_ARG;
+    if (op == 4)   // special form: every prime of the window 2^44 + c, 0 < c < 2^21
+        for (int i = 0; i < n_limbs; ++i) {
+            const uint64_t q = c->primes[prime0 + i];
+            if (q <= (1ull << 44) || q - (1ull << 44) >= (1ull << 21)) return CKKS_ERR_BAD_MODULUS;
+        }
     if (!a || !out || (op != 3 && !b)) return CKKS_ERR_NULL;
     if (misaligned(a) || misaligned(out) || (b && misaligned(b))) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;

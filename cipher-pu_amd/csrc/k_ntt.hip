@@ -198,6 +198,7 @@ __global__ void k_pointwise(const uint64_t* __restrict__ a, const uint64_t* __re
         case 0: r = mul_barrett(x, b[g], pc); break;
         case 1: r = add_mod(x, b[g], pc.q); break;
         case 2: r = sub_mod(x, b[g], pc.q); break;
+        case 4: r = mul_special(x, b[g], pc.q); break;
         default: r = x ? pc.q - x : 0; break;
     }
     out[g] = r;

@@ -58,6 +58,28 @@ __device__ __forceinline__ uint64_t mul_shoup(uint64_t a, uint64_t w, uint64_t w
 
 __device__ __forceinline__ uint64_t csub(uint64_t a, uint64_t m) { return a >= m ? a - m : a; }
 
+// a * b mod q for a special-form prime q = 2^44 + c, 0 < c < 2^21, a, b < q,
+// using 2^44 = -c (mod q) -- the reference's fast reduction
+// (scripts/_verify_fastred.py:55-99, rtl mod45_fastred.sv) for c < 2^18,
+// regrouped so that every partial product fits 64 bits for c < 2^21:
+//   x = a b = xh 2^44 + xl,  xh = xh1 2^23 + xh0 (xh0, xh1 < 2^23)
+//   x = xl - c xh0 - c xh1 2^23,  and with u = c xh1 < 2^44:
+//   u 2^23 = (u >> 21) 2^44 + (u mod 2^21) 2^23 = (u mod 2^21) 2^23 - c (u >> 21)
+// so x = [xl + c (u >> 21)] - [c xh0 + (u mod 2^21) 2^23], both brackets
+// below 2^45 < 2q; adding 2q leaves a value in (0, 4q).
+__device__ __forceinline__ uint64_t mul_special(uint64_t a, uint64_t b, uint64_t q) {
+    const uint64_t c = q - (1ull << 44);
+    const uint64_t lo = a * b, hi = __umul64hi(a, b);
+    const uint64_t xl = lo & ((1ull << 44) - 1);
+    const uint64_t xh = (hi << 20) | (lo >> 44);
+    const uint64_t xh0 = xh & ((1ull << 23) - 1), xh1 = xh >> 23;
+    const uint64_t u = c * xh1;
+    const uint64_t pos = xl + c * (u >> 21);
+    const uint64_t neg = c * xh0 + ((u & ((1ull << 21) - 1)) << 23);
+    const uint64_t r = pos + 2 * q - neg;
+    return csub(csub(r, 2 * q), q);
+}
+
 __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) { return csub(a + b, q); }
 
 __device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t q) { return a >= b ? a - b : a + q - b; }

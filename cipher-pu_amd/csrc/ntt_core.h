@@ -167,16 +167,55 @@ __device__ __forceinline__ void from_lds(uint64_t (&x)[C::E], const uint64_t* ld
         for (int e = 0; e < M; ++e) x[g * M + e] = base[g * Lay<C, P>::SG + e * Lay<C, P>::SE];
 }
 
+// True when the exchange between the layouts of passes P - 1 and P stays
+// inside every 64-lane wavefront: each slot a wave reads in pass P's layout
+// was written by that same wave in pass P - 1's layout.  (Each wave writes
+// and reads 64 E slots, so the inclusion makes the two sets equal, and the
+// relation is the same in the inverse direction.)
+template <class C, int P>
+constexpr bool wave_local() {
+    constexpr int LOa = C::lo(P - 1), HIa = C::hi(P - 1);
+    constexpr int LOb = C::lo(P), HIb = C::hi(P), RPb = C::rsize(P);
+    constexpr int Mb = 1 << RPb, Gb = C::E >> RPb;
+    for (int tid = 0; tid < C::T; ++tid)
+        for (int g = 0; g < Gb; ++g)
+            for (int e = 0; e < Mb; ++e) {
+                const int f = g * C::T + tid;
+                const int j = ((f >> LOb) << (HIb + 1)) | (e << LOb) | (f & ((1 << LOb) - 1));
+                const int fa = ((j >> (HIa + 1)) << LOa) | (j & ((1 << LOa) - 1));
+                if ((fa % C::T) / 64 != tid / 64) return false;
+            }
+    return true;
+}
+
+// LDS ordering for an exchange: a wave-local one needs this wave's writes
+// complete and visible to itself only; any other one needs the workgroup.
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Forward passes 1..NP-1 (pass 0 already applied), exchanging through LDS.
+// The first exchange starts with a workgroup barrier: a preceding transform
+// in the same kernel may end with a wave-local exchange, and pass 0's slots
+// of one wave are other waves' slots there.
 template <class C, int P>
 __device__ __forceinline__ void fwd_rest(uint64_t (&x)[C::E], uint64_t* lds,
                                          const ulonglong2* __restrict__ tw, uint64_t q,
                                          uint64_t two_q, int tid) {
     if constexpr (P < C::NP) {
+        constexpr bool W = wave_local<C, P>();
+        if constexpr (P == 1) __syncthreads();
         to_lds<C, P - 1>(x, lds, tid);
-        __syncthreads();
+        exchange_sync<W>();
         from_lds<C, P>(x, lds, tid);
-        __syncthreads();
+        exchange_sync<W>();
         fwd_pass<C, P>(x, tw, q, two_q, tid);
         fwd_rest<C, P + 1>(x, lds, tw, q, two_q, tid);
     }
@@ -199,10 +238,11 @@ __device__ __forceinline__ void inv_rest(uint64_t (&x)[C::E], uint64_t* lds,
                                          const ulonglong2* __restrict__ itw, const PrimeConst& pc,
                                          int tid) {
     if constexpr (P >= 0) {
+        constexpr bool W = wave_local<C, P + 1>();
         to_lds<C, P + 1>(x, lds, tid);
-        __syncthreads();
+        exchange_sync<W>();
         from_lds<C, P>(x, lds, tid);
-        __syncthreads();
+        exchange_sync<W>();
         inv_pass<C, P>(x, itw, pc, tid);
         inv_rest<C, P - 1>(x, lds, itw, pc, tid);
     }
@@ -272,5 +312,14 @@ __device__ __forceinline__ int eval_index(int k, int tid) {
     constexpr int M = 1 << C::rsize(C::NP - 1);
     return index_of<C, C::NP - 1>(k / M, k % M, tid);
 }
+
+// The wave-local exchanges of the built configurations (an exhaustive host
+// enumeration gives the same table).
+static_assert(!wave_local<Cfg<13, 512>, 1>() && wave_local<Cfg<13, 512>, 2>() && wave_local<Cfg<13, 512>, 3>());
+static_assert(!wave_local<Cfg<13, 1024>, 3>() && wave_local<Cfg<13, 1024>, 4>());
+static_assert(!wave_local<Cfg<13, 256>, 1>() && wave_local<Cfg<13, 256>, 2>());
+static_assert(!wave_local<Cfg<14, 1024>, 2>() && wave_local<Cfg<14, 1024>, 3>());
+static_assert(!wave_local<Cfg<14, 512>, 1>() && !wave_local<Cfg<14, 512>, 2>());
+static_assert(!wave_local<Cfg<12, 128>, 2>() && wave_local<Cfg<12, 512>, 3>());
 
 }  // namespace ntt

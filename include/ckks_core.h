@@ -85,8 +85,11 @@ int ckks_poly_mul(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, uin
                   int n_limbs, int prime0, void *stream);
 
 /* Element-wise RNS arithmetic, any domain: op 0 = a*b, 1 = a+b, 2 = a-b,
- * 3 = -a.  Replaces poly_add / poly_sub (verify_keyswitch.py:40-45) and the
- * point-wise product of poly_mult_via_ntt. */
+ * 3 = -a, 4 = a*b by special-form reduction (primes 2^44 + c, c < 2^21:
+ * the fast_red of scripts/_verify_fastred.py:55-99 generalised; else
+ * CKKS_ERR_BAD_MODULUS).  Replaces poly_add / poly_sub
+ * (verify_keyswitch.py:40-45) and the point-wise product of
+ * poly_mult_via_ntt. */
 int ckks_pointwise(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, int n_polys,
                    int n_limbs, int prime0, int op, void *stream);
 

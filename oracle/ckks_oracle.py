@@ -653,6 +653,32 @@ class RtlGearCtrl:
         return self.gear
 
 
+def fast_red(x: int, q: int, k: int = 44) -> int:
+    """x mod q for q = 2^k + c with small c, by the reference's three rounds
+    of 2^k = -c folding (scripts/_verify_fastred.py:55-99):
+    r1 = xl - c xh; r1 = h1 2^k + l1 (Euclidean); r2 = l1 - c h1;
+    r2 = h2 2^k + l2; r3 = l2 - c h2; one conditional +q / -q.
+    The reference fixes q = 2^44 + 131073 with 18-bit c; the same three
+    rounds are exact for every c with c^2 < 2^k here (asserted)."""
+    c = q - (1 << k)
+    assert 0 < c and c * c < (1 << k) and 0 <= x < (1 << (2 * k + 2))
+    mask = (1 << k) - 1
+    r1 = (x & mask) - (x >> k) * c
+    h1 = r1 >> k                       # floor, may be negative
+    l1 = r1 - (h1 << k)
+    r2 = l1 - h1 * c                   # -h1 >= 0
+    assert r2 >= 0
+    h2 = r2 >> k
+    l2 = r2 - (h2 << k)
+    r = l2 - h2 * c
+    if r < 0:
+        r += q
+    if r >= q:
+        r -= q
+    assert 0 <= r < q
+    return r
+
+
 def block_prune_mask(w: np.ndarray, frac: float, block: int = 64) -> np.ndarray:
     """Zero the ``frac`` share of size-``block`` blocks (row-major runs of
     the flattened matrix) with the smallest max-abs -- the quantile form of

@@ -399,3 +399,30 @@ def test_serialisation_roundtrip(ck, tmp_path):
     out1 = to_numpy(c.hmult(ct, ct, rlk).data)
     out2 = to_numpy(c.hmult(back["ct"], back["ct"], back["rlk"]).data)
     assert np.array_equal(out1, out2)
+
+
+def test_special_form_product_matches_barrett(ck):
+    """Pointwise op 4 (special-form reduction) equals op 0 (Barrett) and
+    Python's a*b % q on every chain prime; a non-special prime is rejected."""
+    import ctypes
+    import torch
+    from cipherpu._lib import CkksError
+    from cipherpu.ring import RingContext, to_numpy
+    c, _ = ck
+    rng = np.random.default_rng(31)
+    a = np.stack([rng.integers(0, q, (3, c.N), dtype=np.uint64) for q in c.primes], axis=1)
+    b = np.stack([rng.integers(0, q, (3, c.N), dtype=np.uint64) for q in c.primes], axis=1)
+    for i, q in enumerate(c.primes):          # extreme residues
+        a[0, i, 0], b[0, i, 0] = q - 1, q - 1
+        a[0, i, 1], b[0, i, 1] = 0, q - 1
+    A, B = c.ring.tensor(a), c.ring.tensor(b)
+    got4 = to_numpy(c.ring._pw(A, B, 4, 0, None))
+    got0 = to_numpy(c.ring.mul(A, B))
+    assert np.array_equal(got4, got0)
+    for i, q in enumerate(c.primes):
+        want = (a[1, i].astype(object) * b[1, i].astype(object)) % q
+        assert np.array_equal(got4[1, i].astype(object), want)
+    r1 = RingContext(4096, [Q1], roots=[PSI1])
+    x = r1.tensor(np.ones((1, 1, 4096), dtype=np.uint64))
+    with pytest.raises(CkksError):
+        r1._pw(x, x, 4, 0, None)

@@ -49,3 +49,34 @@ def test_trained_controller_file_loads_safely():
         shapes = {k: z[k].shape for k in z.files}
     assert shapes["W0"] == (64, 16) and shapes["W1"] == (64, 64) and shapes["W2"] == (4, 64)
     assert shapes["mu"] == (16,) and shapes["y_mu"] == (4,)
+
+
+def _ntt_layout(LOGN, T):
+    """The LDS-resident transform's pass plan (csrc/ntt_core.h Cfg)."""
+    E = (1 << LOGN) // T
+    R = E.bit_length() - 1
+    REST = LOGN - R
+    NREST = (REST + R - 1) // R
+    NP = 1 + NREST
+    rsize = lambda p: R if p == 0 else REST // NREST + (1 if (p - 1) < REST % NREST else 0)
+
+    def hi(p):
+        return LOGN - 1 - sum(rsize(k) for k in range(p))
+    return E, NP, rsize, hi, lambda p: hi(p) - rsize(p) + 1
+
+
+def test_wave_local_exchanges_table():
+    """Exhaustive enumeration of which transform exchanges stay inside each
+    64-lane wave (the table the kernels' static_asserts pin)."""
+    want = {(13, 512): [False, True, True], (13, 1024): [False, False, False, True], (13, 256): [False, True],
+            (14, 1024): [False, False, True], (14, 512): [False, False], (12, 128): [False, False],
+            (12, 256): [False, True], (12, 512): [False, True, True]}
+    for (LOGN, T), table in want.items():
+        E, NP, rsize, hi, lo = _ntt_layout(LOGN, T)
+
+        def slots(P, w):
+            L, H, M, G = lo(P), hi(P), 1 << rsize(P), E >> rsize(P)
+            return {((f >> L) << (H + 1)) | (e << L) | (f & ((1 << L) - 1))
+                    for tid in range(64 * w, 64 * w + 64) for g in range(G) for e in range(M) for f in [g * T + tid]}
+        got = [all(slots(P - 1, w) == slots(P, w) for w in range(T // 64)) for P in range(1, NP)]
+        assert got == table, (LOGN, T, got)

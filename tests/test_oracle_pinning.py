@@ -428,3 +428,25 @@ def test_block_prune_mask():
     w = np.arange(256, dtype=float).reshape(2, 128)
     m = O.block_prune_mask(w, 0.5, 64)
     assert m.sum() == 128 and not m.ravel()[:128].any() and m.ravel()[128:].all()
+
+
+def test_fast_red_reference_cases():
+    """The special-form reduction against x % q on the reference's own edge
+    set for q = 2^44 + 131073 (scripts/_verify_fastred.py:108-118) and its
+    hand checks (:48-51), then on every chain prime with random products."""
+    import random
+    Q, C = 17592186175489, 131073
+    assert Q == CHAIN4[3] and Q - (1 << 44) == C
+    assert O.fast_red(Q, Q) == 0 and O.fast_red(Q - 1, Q) == Q - 1
+    assert O.fast_red(0, Q) == 0 and O.fast_red(1, Q) == 1
+    edges = [0, 1, 2, Q - 1, Q, Q + 1, 2 * Q, 3 * Q, 2 * Q - 1, 1 << 44, (1 << 44) - 1, (1 << 44) + 1,
+             C, 2 * C, 3 * C, (1 << 90) - 1, (1 << 90) - Q, Q * ((1 << 44) - 1), (1 << 45) * C, (1 << 46) * C,
+             (1 << 46) * C - 1, (1 << 64) + (1 << 44), ((1 << 44) - 1) * ((1 << 18) - 1)]
+    for e in edges:
+        assert O.fast_red(e, Q) == e % Q, e
+    rng = random.Random(20261003)
+    from cipherpu.params import CHAIN5
+    for q in list(CHAIN5) + [SPECIAL_P]:
+        for _ in range(3000):
+            a, b = rng.randrange(q), rng.randrange(q)
+            assert O.fast_red(a * b, q) == a * b % q
