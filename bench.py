@@ -164,6 +164,31 @@ def modmul_probe(rows: int = 1024, reps: int = 10) -> None:
     print(json.dumps(res))
 
 
+def rotation_probe(ck, sk, batch: int = 64, steps=tuple(range(1, 16)), reps: int = 5) -> dict:
+    """15 rotations of the same ciphertexts (a baby-step set): one at a time
+    against hoisted (one shared digit decomposition)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(5)
+    ct = ck.encrypt(ck.encode(rng.uniform(-1, 1, (batch, ck.N // 2))), sk)
+    gks = [ck.galois_key(sk, s) for s in steps]
+    keys = torch.stack([ck.hoist_key(g) for g in gks]).contiguous()
+    res = {"ciphertexts": batch, "rotations": len(steps), "level": ct.level}
+    for name, fn in (("one_at_a_time", lambda: [ck.rotate(ct, g) for g in gks]),
+                     ("hoisted", lambda: ck.rotate_hoisted(ct, gks, keys))):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name + "_ms"] = round(e0.elapsed_time(e1) / reps, 3)
+    res["speedup"] = round(res["one_at_a_time_ms"] / res["hoisted_ms"], 2)
+    return res
+
+
 def mlp_leg(samples: int, target_db: float, chunk_cts: int, kind: str = "mlp"):
     """Encrypted MLP (or CNN) samples/s at a fixed output SNR: the plain
     decrypt-based monitor gear loop against the learned gear controller,
@@ -357,6 +382,7 @@ def main():
         "wall_s_timed": round(wall, 4),
         "device": torch.cuda.get_device_name(0),
     }
+    res["rotation_probe"] = rotation_probe(ck, sk)
     if not args.no_mlp:
         res["mlp"] = mlp_leg(args.mlp_samples, args.target_snr, 64)
     if not args.no_cnn:

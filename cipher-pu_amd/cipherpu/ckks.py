@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -66,6 +66,7 @@ class SwitchKey:
     coeff: torch.Tensor         # [L, 2, L+1, N] uint64, coefficient form (b_j, a_j)
     prepared: torch.Tensor      # [L, 2, L+1, N, 2] uint64 (value, Shoup), evaluation order
     galois: int = 0             # 0 for the relinearisation key
+    hoisted: Optional[torch.Tensor] = None   # prepared, permuted for ckks_rotate_hoisted
 
 
 class CKKS:
@@ -277,6 +278,40 @@ class CKKS:
         call("ckks_rotate", self.ring.handle, _ptr(ct.data), _ptr(gk.prepared), self.L + 1, _ptr(out), B, l,
              self.sp, gk.galois, _ptr(ws), self.ring.stream())
         return Ciphertext(out, ct.scale)
+
+    def hoist_key(self, gk: SwitchKey) -> torch.Tensor:
+        """The Galois key permuted by its inverse evaluation-domain
+        automorphism, as ckks_rotate_hoisted reads it (cached on the key)."""
+        if gk.hoisted is None:
+            out = torch.empty_like(gk.prepared)
+            rows = gk.prepared.numel() // (2 * self.N)
+            call("ckks_key_permute", self.ring.handle, _ptr(gk.prepared), _ptr(out), rows, gk.galois,
+                 self.ring.stream())
+            gk.hoisted = out
+        return gk.hoisted
+
+    def rotate_hoisted(self, ct: Ciphertext, gks: Sequence[SwitchKey], keys: Optional[torch.Tensor] = None,
+                       out: Optional[torch.Tensor] = None) -> List[Ciphertext]:
+        """Rotations of ct by every key of gks (at most 32) sharing one digit
+        decomposition.  keys may hold the stacked hoisted keys (reused across
+        calls); else they are stacked here.  out, if given, is a contiguous
+        [R, B, 2, l, N] tensor to write into."""
+        B, l, R = ct.batch, ct.level, len(gks)
+        if R == 0:
+            return []
+        if keys is None:
+            keys = torch.stack([self.hoist_key(g) for g in gks]).contiguous()
+        if out is None:
+            out = torch.empty(R, B, 2, l, self.N, dtype=torch.uint64, device=self.device)
+        elif tuple(out.shape) != (R, B, 2, l, self.N) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous [R, B, 2, l, N] tensor")
+        words = (call("ckks_workspace_hoisted_bytes", self.ring.handle, B, l, R) + 7) // 8
+        if self._ws is None or self._ws.numel() < words:
+            self._ws = torch.empty(words, dtype=torch.uint64, device=self.device)
+        gal = (ctypes.c_uint64 * R)(*[int(g.galois) for g in gks])
+        call("ckks_rotate_hoisted", self.ring.handle, _ptr(ct.data), _ptr(keys), self.L + 1, gal, R, _ptr(out), B, l,
+             self.sp, _ptr(self._ws), self.ring.stream())
+        return [Ciphertext(out[r], ct.scale) for r in range(R)]
 
     def add(self, a: Ciphertext, b: Ciphertext) -> Ciphertext:
         return Ciphertext(self.ring.add(a.data, b.data), a.scale)

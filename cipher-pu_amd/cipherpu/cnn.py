@@ -7,7 +7,8 @@ its four polyphase components X_(pr,pc)[r][s] = x[2r + pr][2s + pc]
 (r, s < 14), component p = 2 pr + pc at slots 256 p + 16 r + s.  Then the
 stride-2 correlation at output (i, j) needs X_(u%2, v%2)[i + u//2][j + v//2],
 which rotation by 256 p + 16 (u//2) + v//2 brings to slot 16 i + j: one
-rotation per kernel tap, shared by all 8 channels.  Channel c is
+rotation per kernel tap, shared by all 8 channels, all 24 of them one
+hoisted call (ckks_rotate_hoisted).  Channel c is
     y_c = sum_(u,v) K[c][u][v] * rot(x, s_uv)
 with integer weights, one ckks_lincomb_scalar call for all channels (in the
 coefficient domain: a constant multiplies every coefficient alike).  The
@@ -164,8 +165,7 @@ class EncryptedCNN(EncryptedNet):
         rescaled back to the base scale."""
         ck = self.ck
         B, l = ct.batch, ct.level
-        E = torch.stack([(ct if tap_step(u, v) == 0 else self.rot(ct, tap_step(u, v))).data
-                         for u, v in self.taps]).contiguous()                 # [taps][B][2][l][N]
+        E = self.rot_stack(ct, [tap_step(u, v) for u, v in self.taps])       # [taps][B][2][l][N]
         scale = float(ck.primes[l - 1]) * ck.p.scale / ct.scale
         D = self._conv_weights(l, scale)
         out = torch.empty(N_CH, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)

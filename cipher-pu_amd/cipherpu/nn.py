@@ -11,8 +11,9 @@ Layer 1 (W1: 128 x 784, padded to 128 x 896).  Hybrid diagonal method:
 for i < 896 (0 on [896, 1024)), then y_j = sum_(t<8) z[j + 128 t] by
 rotate-and-sum over 512, 256, 128.  k = 16 g + b is evaluated baby-step /
 giant-step: z = sum_g rot(sum_b rot(d_(16g+b), -16g) * rot(x, b), 16 g), so
-only 15 + 7 rotations are needed; the plaintext products run in the
-evaluation domain (ckks_lincomb).  The reference computes the same layer
+only 15 + 7 rotations are needed, the 15 baby steps as one hoisted call
+(ckks_rotate_hoisted: one shared digit decomposition); the plaintext
+products run in the evaluation domain (ckks_lincomb).  The reference computes the same layer
 as coefficient-encoded dot products (scripts/exp_e11_ckks_e2e_real.py:126-166).
 
 Then: mask slots [0, 128) of each block (the mask's scale is chosen so that
@@ -138,6 +139,8 @@ class EncryptedNet:
         self.S = (ck.N // 2) // SLOTS_PER_SAMPLE
         self.rlk = ck.relin_key(sk)
         self._gk: Dict[int, SwitchKey] = {}
+        self._hkeys: Dict[tuple, torch.Tensor] = {}
+        self.hoist = True          # baby steps / convolution taps as hoisted rotations
         self.ops = {"rot": 0, "ptmul_rows": 0, "hmult": 0, "dare": 0}
         self.gears = (0, 0)
 
@@ -157,6 +160,34 @@ class EncryptedNet:
     def _slots(self, v: np.ndarray) -> np.ndarray:
         """A 1024-slot block pattern repeated for the S samples of a ciphertext."""
         return np.tile(v, self.S)
+
+    def rot_stack(self, ct: Ciphertext, steps: List[int]) -> torch.Tensor:
+        """[len(steps), B, 2, l, N]: ct rotated by every step (0 = ct itself),
+        the nonzero steps as one hoisted call sharing a single digit
+        decomposition (up to 32 a call)."""
+        ck = self.ck
+        out = torch.empty(len(steps), ct.batch, 2, ct.level, ck.N, dtype=torch.uint64, device=ck.device)
+        nz = [i for i, s in enumerate(steps) if s % (ck.N // 2)]
+        for i, s in enumerate(steps):
+            if i not in nz:
+                out[i].copy_(ct.data)
+        if not self.hoist:
+            for i in nz:
+                out[i].copy_(self.rot(ct, steps[i]).data)
+            return out
+        for c0 in range(0, len(nz), 32):
+            idx = nz[c0:c0 + 32]
+            key = tuple(steps[i] for i in idx)
+            if key not in self._hkeys:
+                self._hkeys[key] = torch.stack([ck.hoist_key(self.gk(s)) for s in key]).contiguous()
+            gks = [self.gk(s) for s in key]
+            if idx == list(range(idx[0], idx[0] + len(idx))):      # contiguous rows: write in place
+                ck.rotate_hoisted(ct, gks, self._hkeys[key], out=out[idx[0]:idx[0] + len(idx)])
+            else:
+                for i, r in zip(idx, ck.rotate_hoisted(ct, gks, self._hkeys[key])):
+                    out[i].copy_(r.data)
+            self.ops["rot"] += ct.batch * len(idx)
+        return out
 
     def rotation_steps(self) -> List[int]:
         raise NotImplementedError
@@ -280,8 +311,7 @@ class EncryptedMLP(EncryptedNet):
     def _linear(self, ct: Ciphertext, p: LayerPlan) -> Ciphertext:
         ck = self.ck
         B, l = ct.batch, ct.level
-        babies = [ct if b == 0 else self.rot(ct, b) for b in p.used_b]
-        E = torch.stack([x.data for x in babies]).contiguous()                 # [nb][B][2][l][N]
+        E = self.rot_stack(ct, p.used_b)                                        # [nb][B][2][l][N]
         ck.ring.ntt_(E.view(-1, l, ck.N))
         nb, ng = len(p.used_b), len(p.used_g)
         acc = torch.empty(ng, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)

@@ -640,6 +640,58 @@ int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int
     return CKKS_OK;
 }
 
+uint64_t ckks_workspace_hoisted_bytes(const ckks_ctx* c, int batch, int l, int R) {
+    if (!c || batch < 0 || l < 1 || R < 1) return 0;
+    return (uint64_t)batch * c->N * 8 * (uint64_t)(l * (l + 1) + R * (4 * l + 2));
+}
+
+int ckks_key_permute(const ckks_ctx* c, const uint64_t* key_in, uint64_t* key_out, int rows, uint64_t galois,
+                     void* stream) {
+    if (!c || !key_in || !key_out) return CKKS_ERR_NULL;
+    if (rows < 0) return CKKS_ERR_BAD_SHAPE;
+    if (!(galois & 1) || galois >= 2 * c->N || (c->flags & CKKS_CYCLIC)) return CKKS_ERR_BAD_ARG;
+    if (key_in == key_out) return CKKS_ERR_BAD_ARG;
+    if (misaligned(key_in) || misaligned(key_out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_key_permute(c->tables(), (const ulonglong2*)key_in, (ulonglong2*)key_out, rows, (uint32_t)galois,
+                              S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_rotate_hoisted(const ckks_ctx* c, const uint64_t* ct, const uint64_t* keys, int key_limbs,
+                        const uint64_t* galois, int R, uint64_t* out, int batch, int l, int special, void* workspace,
+                        void* stream) {
+    int st = ks_common_check(c, keys, key_limbs, out, batch, l, special, workspace);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!ct || !galois) return CKKS_ERR_NULL;
+    if (R < 1 || R > HOIST_MAX) return CKKS_ERR_BAD_SHAPE;
+    if (misaligned(ct)) return CKKS_ERR_BAD_ALIGNMENT;
+    GaloisList gl = {};
+    for (int r = 0; r < R; ++r) {
+        if (!(galois[r] & 1) || galois[r] >= 2 * c->N) return CKKS_ERR_BAD_ARG;
+        gl.g[r] = (uint32_t)galois[r];
+    }
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const LimbMap m = limb_map(0, l);
+    const ModDownConst mc = moddown_const(c, l, special);
+    const size_t LN = (size_t)l * c->N, B = (size_t)batch;
+    uint64_t* D = (uint64_t*)workspace;                       // [B][l][l+1][N]
+    uint64_t* sct = D + B * l * (l + 1) * c->N;               // [R][B][2][l][N]: sigma_g(ct)
+    uint64_t* acc = sct + (size_t)R * B * 2 * LN;             // [R][B][2][l+1][N]
+    TIMED(CKKS_KIND_KEYSWITCH, launch_modup(t, ct, D, batch, l, m, special, c->ks_threads, S(stream)));
+    for (int r = 0; r < R; ++r)
+        TIMED(CKKS_KIND_AUTOMORPHISM,
+              launch_automorphism(t, ct, sct + (size_t)r * B * 2 * LN, batch * 2, l, m, galois[r], S(stream)));
+    // one prepared key per rotation: [dnum = key_limbs - 1][2][key_limbs][N] pairs
+    const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
+    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,
+                                                 l, m, special, R, c->ks_threads, S(stream)));
+    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch * R, l, m, special, mc, false,
+                                                    S(stream)));
+    return CKKS_OK;
+}
+
 int ckks_encode(const ckks_ctx* c, const double* slots, uint64_t* pt, int n_vec, int l, int prime0, double scale,
                 double* work, void* stream) {
     int st = check_rows(c, n_vec, l, prime0);

@@ -1,0 +1,207 @@
+// Hoisted rotations: several slot rotations of the same ciphertexts share
+// one digit decomposition ("ModUp") -- the method of Halevi and Shoup,
+// "Faster homomorphic linear transformations in HElib" (CRYPTO 2018).
+// The reference rotates one step at a time (exp_e24_rotation_bootstrap.py:
+// 143-176, automorphism then key switch); there is no hoisted counterpart.
+//
+// For ct = (c0, c1) at l limbs, digits D_j = lift of [c1]_(q_j), and the
+// evaluation-domain automorphism pi_g, NTT_t(sigma_g(D_j)) = pi_g(NTT_t(D_j))
+// where sigma_g acts over Z (a coefficient sent past X^N changes sign).  So
+//   acc_t = sum_j pi_g(NTT_t(D_j)) * K_(j,t) = pi_g( sum_j NTT_t(D_j) * pi_g^-1(K_(j,t)) ).
+// The key of each rotation is stored pre-permuted by pi_g^-1 (ckks_key_permute),
+// the sum reads every operand in place, and pi_g is applied once per
+// accumulator through LDS before the inverse transform.  The digits here
+// are sigma_g of the canonical lifts -- a different (equally small)
+// representative than the lift of sigma_g(c1) that the one-at-a-time rotation
+// decomposes, so the two agree after decryption, not bit for bit.
+//
+// Evaluation order: position i of a forward transform holds
+// a(psi^(2 brv(i) + 1)), so pi_g(v)[i] = v[j(i)] with
+// 2 brv(j(i)) + 1 = g (2 brv(i) + 1) mod 2N.
+#include "launch.h"
+#include "ntt_core.h"
+
+using ntt::Cfg;
+
+namespace {
+
+__device__ __forceinline__ int eval_perm(int i, uint32_t g, int logN) {
+    const uint32_t shift = 32 - logN;
+    const uint32_t e = 2 * (__brev((uint32_t)i) >> shift) + 1;
+    const uint32_t e2 = (g * e) & ((2u << logN) - 1);
+    return (int)(__brev((e2 - 1) >> 1) >> shift);
+}
+
+// D[b][j][t] = NTT_t(lift [c1_b]_(q_j)), evaluation order, reduced; one
+// workgroup per (ciphertext, digit j, target t), t = l is the special prime.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_modup(const uint64_t* __restrict__ ct, uint64_t* __restrict__ D,
+                                             const ulonglong2* __restrict__ tw_all,
+                                             const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
+                                             int reduce_digits) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int per_b = l * (l + 1);
+    const int b = blockIdx.x / per_b, rem = blockIdx.x % per_b;
+    const int j = rem / (l + 1), t = rem % (l + 1);
+    const int pt = t < l ? map.idx[t] : pidx_p;
+    const PrimeConst pc = pcs[pt];
+    uint64_t x[C::E];
+    ntt::load_coalesced<C>(x, ct + ((size_t)b * 2 + 1) * l * C::N + (size_t)j * C::N, tid);
+    if (reduce_digits) {
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
+    }
+    ntt::fwd_ntt<C>(x, lds, tw_all + (size_t)pt * 2 * C::N, pc.q, pc.two_q, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
+    __syncthreads();
+    ntt::store_eval<C>(x, lds, D + (((size_t)b * l + j) * (l + 1) + t) * C::N, tid);
+}
+
+// Reduce (and negate) an accumulator held at natural evaluation slots, apply
+// pi_g through LDS, inverse-transform and store it in coefficient form.
+template <class C>
+__device__ __forceinline__ void permute_inverse(uint64_t (&a)[C::E], bool negate, uint32_t g, uint64_t* lds,
+                                                const ulonglong2* __restrict__ itw, const PrimeConst& pc,
+                                                uint64_t* __restrict__ out, int tid) {
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const uint64_t v = reduce64(a[k], pc);
+        lds[ntt::pad<C>(ntt::eval_index<C>(k, tid))] = (negate && v) ? pc.q - v : v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a[k] = lds[ntt::pad<C>(eval_perm(ntt::eval_index<C>(k, tid), g, C::LOGN))];
+    __syncthreads();
+    ntt::inv_ntt<C>(a, lds, itw, pc, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a[k] = csub(a[k], pc.q);
+    ntt::store_coalesced<C>(a, out, tid);
+    __syncthreads();
+}
+
+// One workgroup per (target t, ciphertext b, rotation r), rotations of one
+// (t, b) adjacent so that the digits stay in L2.  keys: [R][dnum][2][key_limbs][N]
+// (value, companion) pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N]
+// coefficient form, the layout ckks_relin's ModDown reads.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D, const ulonglong2* __restrict__ keys,
+                                                  size_t key_stride, int key_limbs, GaloisList galois,
+                                                  uint64_t* __restrict__ acc, const ulonglong2* __restrict__ tw_all,
+                                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
+                                                  int batch, int R) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x % R;
+    const int b = (blockIdx.x / R) % batch;
+    const int t = blockIdx.x / (R * batch);
+    const int pt = t < l ? map.idx[t] : pidx_p;
+    const int kt = t < l ? t : key_limbs - 1;
+    const PrimeConst pc = pcs[pt];
+    const uint64_t nq = (uint64_t)0 - pc.q;
+    const ulonglong2* key = keys + (size_t)r * key_stride;
+    uint64_t a0[C::E], a1[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0;
+    for (int j = 0; j < l; ++j) {
+        const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
+        const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+        const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const int idx = ntt::eval_index<C>(k, tid);
+            const uint64_t x = d[idx];
+            const ulonglong2 wb = kb[idx], wa = ka[idx];
+            a0[k] += mul_shoup4_nq(x, wb.x, wb.y, nq);   // < 4q each, l <= 15 digits
+            a1[k] += mul_shoup4_nq(x, wa.x, wa.y, nq);
+        }
+    }
+    const uint32_t g = galois.g[r];
+    const ulonglong2* itw = tw_all + ((size_t)pt * 2 + 1) * C::N;
+    uint64_t* out = acc + (((size_t)r * batch + b) * 2 * (l + 1) + t) * C::N;
+    permute_inverse<C>(a0, false, g, lds, itw, pc, out, tid);
+    permute_inverse<C>(a1, true, g, lds, itw, pc, out + (size_t)(l + 1) * C::N, tid);
+}
+
+// out[row][j(i)] = in[row][i]: the key rows permuted by pi_g^-1.
+__global__ void k_key_permute(const ulonglong2* __restrict__ in, ulonglong2* __restrict__ out, uint32_t g, int logN,
+                              size_t total) {
+    const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= total) return;
+    const size_t N = (size_t)1 << logN;
+    const int i = (int)(gi & (N - 1));
+    out[(gi - i) + eval_perm(i, g, logN)] = in[gi];
+}
+
+template <class K>
+hipError_t set_lds(K kernel, int bytes) {
+    return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+template <int LOGN, int T>
+hipError_t run_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int batch, int l, const LimbMap& map,
+                     int pidx_p, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_modup<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_modup<LOGN, T>), dim3(batch * l * (l + 1)), dim3(T), C::LDS_BYTES, s, ct, D, t.tw, t.pc,
+                       map, l, pidx_p, t.narrow ? 0 : 1);
+    return hipGetLastError();
+}
+
+template <int LOGN, int T>
+hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
+                          int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l, const LimbMap& map,
+                          int pidx_p, int R, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_ks_hoisted<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ks_hoisted<LOGN, T>), dim3((l + 1) * batch * R), dim3(T), C::LDS_BYTES, s, D, keys,
+                       key_stride, key_limbs, galois, acc, t.tw, t.pc, map, l, pidx_p, batch, R);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int batch, int l, const LimbMap& map,
+                        int pidx_p, int threads, hipStream_t s) {
+    if (!batch) return hipSuccess;
+    switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : 512))) {
+        case 120256: return run_modup<12, 256>(t, ct, D, batch, l, map, pidx_p, s);
+        case 120512: return run_modup<12, 512>(t, ct, D, batch, l, map, pidx_p, s);
+        case 130256: return run_modup<13, 256>(t, ct, D, batch, l, map, pidx_p, s);
+        case 130512: return run_modup<13, 512>(t, ct, D, batch, l, map, pidx_p, s);
+        case 131024: return run_modup<13, 1024>(t, ct, D, batch, l, map, pidx_p, s);
+        case 140512: return run_modup<14, 512>(t, ct, D, batch, l, map, pidx_p, s);
+        case 141024: return run_modup<14, 1024>(t, ct, D, batch, l, map, pidx_p, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
+                             int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
+                             const LimbMap& map, int pidx_p, int R, int threads, hipStream_t s) {
+    if (!batch || !R) return hipSuccess;
+    switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : 512))) {
+        case 120256: return run_ks_hoisted<12, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        case 120512: return run_ks_hoisted<12, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        case 130256: return run_ks_hoisted<13, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        case 130512: return run_ks_hoisted<13, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        case 131024: return run_ks_hoisted<13, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        case 140512: return run_ks_hoisted<14, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        case 141024: return run_ks_hoisted<14, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_key_permute(const DevTables& t, const ulonglong2* in, ulonglong2* out, int rows, uint32_t g,
+                              hipStream_t s) {
+    const size_t total = (size_t)rows << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_key_permute, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, g, t.logN,
+                       total);
+    return hipGetLastError();
+}

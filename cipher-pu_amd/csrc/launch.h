@@ -105,3 +105,18 @@ hipError_t launch_sample_uniform(const DevTables& t, uint64_t* out, int rows, in
                                  uint64_t seed, hipStream_t s);
 hipError_t launch_sample_gauss(const DevTables& t, uint64_t* out, int rows, int l, const LimbMap& map,
                                uint64_t seed, const GaussTable& tab, hipStream_t s);
+
+// Galois elements of one hoisted-rotation call, passed by value.
+#define HOIST_MAX 32
+struct GaloisList {
+    uint32_t g[HOIST_MAX];
+};
+
+hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int batch, int l, const LimbMap& map,
+                        int pidx_p, int threads, hipStream_t s);
+hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
+                             int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
+                             const LimbMap& map, int pidx_p, int R, int threads, hipStream_t s);
+hipError_t launch_key_permute(const DevTables& t, const ulonglong2* in, ulonglong2* out, int rows, uint32_t g,
+                              hipStream_t s);
+

@@ -143,6 +143,24 @@ int ckks_automorphism(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, in
 int ckks_rotate(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *gkey, int key_limbs, uint64_t *out,
                 int batch, int l, int special, uint64_t galois, void *workspace, void *stream);
 
+/* Hoisted rotations (no counterpart in the reference, which rotates one
+ * step at a time: exp_e24_rotation_bootstrap.py:143-176; the method is
+ * Halevi-Shoup, CRYPTO 2018).  R <= 32 rotations of the same ciphertexts
+ * share one digit decomposition.  ct [batch][2][l][N]; keys: R prepared
+ * Galois keys, each permuted for its element by ckks_key_permute, contiguous;
+ * galois: host array of the R elements; out [R][batch][2][l][N].  The
+ * digits are the automorphism of the canonical lifts, so outputs decrypt to
+ * the rotated message like ckks_rotate's but are not bit-identical to them.
+ * Workspace: ckks_workspace_hoisted_bytes(ctx, batch, l, R). */
+uint64_t ckks_workspace_hoisted_bytes(const ckks_ctx *ctx, int batch, int l, int R);
+int ckks_rotate_hoisted(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *keys, int key_limbs,
+                        const uint64_t *galois, int R, uint64_t *out, int batch, int l, int special, void *workspace,
+                        void *stream);
+/* key_out = key_in (prepared pairs, rows of N) with every row permuted by
+ * the inverse evaluation-domain automorphism of galois, for ckks_rotate_hoisted. */
+int ckks_key_permute(const ckks_ctx *ctx, const uint64_t *key_in, uint64_t *key_out, int rows, uint64_t galois,
+                     void *stream);
+
 /* Canonical-embedding encode: slots [n_vec][N/2] complex (re, im doubles),
  * scaled by `scale`, rounded, reduced into l limbs -> pt [n_vec][l][N].
  * work: n_vec*N doubles of device scratch.  Replaces encode

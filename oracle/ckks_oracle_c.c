@@ -193,6 +193,40 @@ void oracle_keyswitch(const uint64_t *d2, const uint64_t *key, uint64_t *out0,
     free(acc0); free(acc1); free(dig); free(tmp);
 }
 
+/* The same key switch with the digits given as signed integers
+ * (digits[j][k] in (-q_j, q_j)), each reduced into every target prime. */
+static void keyswitch_signed(const int64_t *digits, const uint64_t *key, uint64_t *out0,
+                             uint64_t *out1, uint64_t N, int l, int key_limbs,
+                             const uint64_t *primes, uint64_t p, const uint64_t *psis,
+                             const uint64_t *psi_p) {
+    int nb = l + 1;
+    uint64_t *acc0 = calloc((size_t)nb * N, 8), *acc1 = calloc((size_t)nb * N, 8);
+    uint64_t *dig = malloc(N * 8), *tmp = malloc(N * 8);
+    for (int j = 0; j < l; j++) {
+        const uint64_t *bj = key + ((uint64_t)j * 2 + 0) * key_limbs * N;
+        const uint64_t *aj = key + ((uint64_t)j * 2 + 1) * key_limbs * N;
+        for (int i = 0; i < nb; i++) {
+            uint64_t qi = i < l ? primes[i] : p;
+            uint64_t ps = i < l ? psis[i] : *psi_p;
+            int ki = i < l ? i : key_limbs - 1;
+            for (uint64_t k = 0; k < N; k++) {
+                int64_t d = digits[(uint64_t)j * N + k];
+                uint64_t m = (uint64_t)(d < 0 ? -d : d) % qi;
+                dig[k] = (d < 0 && m) ? qi - m : m;
+            }
+            oracle_polymul_ntt(dig, bj + (uint64_t)ki * N, tmp, N, qi, ps);
+            for (uint64_t k = 0; k < N; k++)
+                acc0[(uint64_t)i * N + k] = addmod(acc0[(uint64_t)i * N + k], tmp[k], qi);
+            oracle_polymul_ntt(dig, aj + (uint64_t)ki * N, tmp, N, qi, ps);
+            for (uint64_t k = 0; k < N; k++)
+                acc1[(uint64_t)i * N + k] = submod(acc1[(uint64_t)i * N + k], tmp[k], qi);
+        }
+    }
+    oracle_mod_down(acc0, out0, N, l, primes, p);
+    oracle_mod_down(acc1, out1, N, l, primes, p);
+    free(acc0); free(acc1); free(dig); free(tmp);
+}
+
 /* HMult + relinearise + rescale on one ciphertext pair, coefficient form
  * (scripts/verify_e2e_mult.py:78-102 with the exact rescale of
  * verify_rescale.py).  out: [2][l-1][N]. */
@@ -257,3 +291,32 @@ void oracle_rotate(const uint64_t *ct, const uint64_t *key, uint64_t *out, uint6
         }
     free(s0); free(s1); free(k1);
 }
+
+/* Hoisted rotation (Halevi-Shoup): the digits are the automorphism over Z
+ * of the canonical lifts of c1's limbs -- coefficient k of [c1]_(q_j) goes
+ * to (g k) mod N, negated when (g k) mod 2N >= N -- key-switched with the
+ * Galois key and added to sigma(c0).  out: [2][l][N]. */
+void oracle_rotate_hoisted(const uint64_t *ct, const uint64_t *key, uint64_t *out, uint64_t N,
+                           int l, int key_limbs, const uint64_t *primes, uint64_t p,
+                           const uint64_t *psis, const uint64_t *psi_p, uint64_t g) {
+    uint64_t LN = (uint64_t)l * N;
+    int64_t *dig = calloc(LN, 8);
+    uint64_t *s0 = calloc(LN, 8), *k1 = calloc(LN, 8);
+    for (int j = 0; j < l; j++) {
+        const uint64_t *c1 = ct + LN + (uint64_t)j * N;
+        for (uint64_t k = 0; k < N; k++) {
+            uint64_t t = (g * k) % (2 * N);
+            dig[(uint64_t)j * N + t % N] = t < N ? (int64_t)c1[k] : -(int64_t)c1[k];
+        }
+        oracle_automorphism(ct + (uint64_t)j * N, s0 + (uint64_t)j * N, N, primes[j], g);
+    }
+    keyswitch_signed(dig, key, out, k1, N, l, key_limbs, primes, p, psis, psi_p);
+    for (int i = 0; i < l; i++)
+        for (uint64_t k = 0; k < N; k++) {
+            uint64_t o = (uint64_t)i * N + k;
+            out[o] = addmod(out[o], s0[o], primes[i]);
+            out[LN + o] = k1[o];
+        }
+    free(dig); free(s0); free(k1);
+}
+

@@ -45,6 +45,7 @@ def lib():
             "oracle_hmult": [_P, _P, _P, _P, _U, _I, _I, _P, _U, _P, _P],
             "oracle_automorphism": [_P, _P, _U, _U, _U],
             "oracle_rotate": [_P, _P, _P, _U, _I, _I, _P, _U, _P, _P, _U],
+            "oracle_rotate_hoisted": [_P, _P, _P, _U, _I, _I, _P, _U, _P, _P, _U],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -158,4 +159,17 @@ def rotate(ct, key, primes, p, psis, psi_p, g):
     out = np.empty((2, l, N), dtype=np.uint64)
     lib().oracle_rotate(_ptr(ct), _ptr(key), _ptr(out), N, l, key_limbs, _ptr(pr), p,
                         _ptr(ps), _ptr(pp), g)
+    return out
+
+
+def rotate_hoisted(ct, key, primes, p, psis, psi_p, g):
+    """The hoisted rotation's reference: digits = automorphism over Z of the
+    canonical lifts of c1's limbs."""
+    ct, key = _u64(ct), _u64(key)
+    _, l, N = ct.shape
+    key_limbs = key.shape[2]
+    pr, ps, pp = _u64(primes[:l]), _u64(psis[:l]), _u64([psi_p])
+    out = np.empty((2, l, N), dtype=np.uint64)
+    lib().oracle_rotate_hoisted(_ptr(ct), _ptr(key), _ptr(out), N, l, key_limbs, _ptr(pr), p,
+                                _ptr(ps), _ptr(pp), g)
     return out

@@ -310,7 +310,11 @@ def test_rotate_bit_exact_and_slots_move(ck):
             want = OC.rotate(to_numpy(ct.data[b]), key, pr, p, ps, pp, gk.galois)
             assert np.array_equal(got[b], want), (step, b)
         dec = to_numpy(c.decode(c.decrypt(rot, sk), rot.scale))
-        assert np.max(np.abs(dec.real - np.roll(z, -step, axis=1))) < 1e-6
+        # key-switch noise: coefficient std ~ 2 sigma q sqrt(N / 3) / p ~ 330, but the digits are
+        # the canonical lifts in [0, q), whose mean q / 2 gives a term (q / 2) sum_k zeta^k that
+        # peaks near zeta = +-1: the worst slot reaches ~2e-6 at scale 2^40 (the C oracle alone
+        # shows 1.7e-6 on one draw)
+        assert np.max(np.abs(dec.real - np.roll(z, -step, axis=1))) < 1e-5
 
 
 def test_encode_decode_vs_fp64_oracle(ck):
@@ -426,3 +430,26 @@ def test_special_form_product_matches_barrett(ck):
     x = r1.tensor(np.ones((1, 1, 4096), dtype=np.uint64))
     with pytest.raises(CkksError):
         r1._pw(x, x, 4, 0, None)
+
+
+def test_rotate_hoisted_bit_exact_and_slots_move(ck):
+    """Hoisted rotations against the oracle's hoisted restatement, bit for
+    bit, and decoded slots rolled by each step."""
+    from cipherpu.ring import to_numpy
+    c, sk = ck
+    rng = np.random.default_rng(14)
+    z = rng.uniform(-1, 1, (2, c.N // 2))
+    pr, p, ps, pp = _oracle_args(c)
+    for l in (4, 3):
+        ct = c.encrypt(c.encode(z, level=l), sk)
+        steps = [1, 5, 64, -3]
+        gks = [c.galois_key(sk, st) for st in steps]
+        outs = c.rotate_hoisted(ct, gks)
+        for st, gk, out in zip(steps, gks, outs):
+            got = to_numpy(out.data)
+            for b in range(2):
+                want = OC.rotate_hoisted(to_numpy(ct.data[b]), to_numpy(gk.coeff), pr[:l], p, ps[:l], pp,
+                                         gk.galois)
+                assert np.array_equal(got[b], want), (l, st, b)
+            dec = to_numpy(c.decode(c.decrypt(out, sk), out.scale))
+            assert np.max(np.abs(dec.real - np.roll(z, -st, axis=1))) < 1e-5, (l, st)   # as above

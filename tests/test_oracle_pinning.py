@@ -450,3 +450,78 @@ def test_fast_red_reference_cases():
         for _ in range(3000):
             a, b = rng.randrange(q), rng.randrange(q)
             assert O.fast_red(a * b, q) == a * b % q
+
+
+def test_eval_domain_automorphism_permutation():
+    """Position i of the (bit-reversed) forward transform holds
+    a(psi^(2 brv(i) + 1)); the automorphism X -> X^g then permutes positions
+    by j(i) = brv(((g (2 brv(i) + 1)) mod 2N - 1) / 2) -- the map the hoisted
+    rotation applies in the evaluation domain."""
+    import random
+    N, q = 64, 7681          # 2N | q - 1
+    psi = O.smallest_primitive_root_of_unity(q, 2 * N)
+    logn = N.bit_length() - 1
+    brv = lambda i: int(format(i, f"0{logn}b")[::-1], 2)
+    rng = random.Random(3)
+    a = [rng.randrange(q) for _ in range(N)]
+    A = O.negacyclic_ntt(a, psi, N, q)                   # natural order: A[k] = a(psi^(2k+1))
+    for k in range(N):
+        assert A[k] == sum(x * pow(psi, (2 * k + 1) * j, q) for j, x in enumerate(a)) % q
+    A_br = [A[brv(i)] for i in range(N)]
+    for g in (5, 25, 2 * N - 1, pow(5, 7, 2 * N)):
+        B = O.negacyclic_ntt(O.automorphism(a, g, N, q), psi, N, q)
+        B_br = [B[brv(i)] for i in range(N)]
+        for i in range(N):
+            e = (g * (2 * brv(i) + 1)) % (2 * N)
+            assert B_br[i] == A_br[brv((e - 1) // 2)]
+
+
+def test_rotate_hoisted_oracle_decrypts():
+    """The hoisted rotation's oracle (digits = automorphism over Z of the
+    canonical lifts) decrypts to the rotated message like the one-step
+    rotation, with noise of the same size, though not bit-identically."""
+    import oracle_c as OC
+    N, l = 1024, 2
+    pr, p = list(CHAIN4[:l]), SPECIAL_P
+    psis = [O.smallest_primitive_root_of_unity(q, 2 * N) for q in pr]
+    pp = O.smallest_primitive_root_of_unity(p, 2 * N)
+    rng = np.random.default_rng(1)
+    s = rng.integers(-1, 2, N)
+    rns = lambda x, q: (np.asarray(x, dtype=object) % q).astype(np.uint64)
+    g = pow(5, 3, 2 * N)
+    sg = np.zeros(N, dtype=np.int64)
+    for k in range(N):
+        t = (g * k) % (2 * N)
+        sg[t % N] += s[k] if t < N else -s[k]
+    allq = pr + [p]
+    key = np.zeros((l, 2, l + 1, N), dtype=np.uint64)
+    for j in range(l):
+        ej = rng.integers(-3, 4, N)          # one error polynomial per digit, in every limb
+        for i, q in enumerate(allq):
+            a = rng.integers(0, q, N, dtype=np.uint64)
+            psi = psis[i] if i < l else pp
+            b = (OC.polymul_ntt(a, rns(s, q), q, psi).astype(object) + rns(ej, q)) % q
+            if i == j:
+                b = (b + (p % q) * rns(sg, q).astype(object)) % q
+            key[j, 0, i], key[j, 1, i] = b.astype(np.uint64), a
+    m = rng.integers(-1000, 1000, N)
+    a = [rng.integers(0, q, N, dtype=np.uint64) for q in pr]
+    e = rng.integers(-3, 4, N)
+    c0 = np.stack([((OC.polymul_ntt(a[i], rns(s, q), q, psis[i]).astype(object) + rns(e + m, q)) % q)
+                   .astype(np.uint64) for i, q in enumerate(pr)])
+    c1 = np.stack([((q - a[i].astype(object)) % q).astype(np.uint64) for i, q in enumerate(pr)])
+    ct = np.stack([c0, c1])
+    q0 = pr[0]
+
+    def dec(c):
+        v = (c[0, 0].astype(object) + OC.polymul_ntt(c[1, 0], rns(s, q0), q0, psis[0]).astype(object)) % q0
+        return np.array([int(x) - q0 if x > q0 // 2 else int(x) for x in v])
+    mrot = np.zeros(N, dtype=np.int64)
+    for k in range(N):
+        t = (g * k) % (2 * N)
+        mrot[t % N] += m[k] if t < N else -m[k]
+    one = OC.rotate(ct, key, pr, p, psis, pp, g)
+    hoisted = OC.rotate_hoisted(ct, key, pr, p, psis, pp, g)
+    assert np.max(np.abs(dec(one) - mrot)) < 2000
+    assert np.max(np.abs(dec(hoisted) - mrot)) < 2000
+    assert not np.array_equal(one, hoisted)
