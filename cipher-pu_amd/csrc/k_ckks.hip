@@ -82,6 +82,35 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
     const size_t LN = (size_t)l * C::N;
     const size_t c0 = (size_t)b * 2 * LN + (size_t)i * C::N;   // poly 0 of pair b, limb i
     const uint64_t pm = mc.p_mod[i], pms = mc.p_mod_s[i];
+    if constexpr (ntt::pair_fits<C>()) {
+        // a0 with a1 and b0 with b1 as transform pairs
+        uint64_t x0[C::E], x1[C::E], y0[C::E], y1[C::E];
+        ntt::load_coalesced<C>(x0, A + c0, tid);
+        ntt::load_coalesced<C>(x1, A + c0 + LN, tid);
+        ntt::fwd_ntt2<C>(x0, x1, lds, tw, pc.q, pc.two_q, tid);
+        ntt::load_coalesced<C>(y0, B + c0, tid);
+        ntt::load_coalesced<C>(y1, B + c0 + LN, tid);
+        ntt::fwd_ntt2<C>(y0, y1, lds, tw, pc.q, pc.two_q, tid);
+        uint64_t* o0 = d01 + c0;
+        uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const uint64_t a0 = reduce64(x0[k], pc), a1 = reduce64(x1[k], pc);
+            const uint64_t b0 = reduce64(y0[k], pc), b1 = reduce64(y1[k], pc);
+            const int idx = ntt::eval_index<C>(k, tid);
+            o0[idx] = csub(mul_shoup(mul_barrett(a0, b0, pc), pm, pms, pc.q), pc.q);
+            const uint64_t d1 = add_mod(mul_barrett(a1, b0, pc), mul_barrett(a0, b1, pc), pc.q);
+            o0[LN + idx] = csub(mul_shoup(d1, pm, pms, pc.q), pc.q);
+            const uint64_t d2 = mul_barrett(a1, b1, pc);
+            o2[idx] = d2;
+            x0[k] = d2;
+        }
+        ntt::inv_ntt<C>(x0, lds, tw + C::N, pc, tid);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) x0[k] = csub(x0[k], pc.q);
+        ntt::store_coalesced<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
+        return;
+    }
     uint64_t x0[C::E], x1[C::E], y[C::E], z[C::E];
     ntt::load_coalesced<C>(x0, A + c0, tid);
     ntt::fwd_ntt<C>(x0, lds, tw, pc.q, pc.two_q, tid);
@@ -115,6 +144,24 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
 #pragma unroll
     for (int k = 0; k < C::E; ++k) x0[k] = csub(x0[k], pc.q);
     ntt::store_coalesced<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
+}
+
+// a0 += v * b_j,t and a1 += v * a_j,t for digit j in evaluation order (key
+// rows of (value, companion) pairs).  Each product is below 4q: with at most
+// 15 digits and a start below q the sums stay below 2^51.
+template <class C>
+__device__ __forceinline__ void key_mac(uint64_t (&a0)[C::E], uint64_t (&a1)[C::E], const uint64_t (&v)[C::E],
+                                        const ulonglong2* __restrict__ key, int j, int key_limbs, int kt,
+                                        uint64_t nq, int tid) {
+    const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+    const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int idx = ntt::eval_index<C>(k, tid);
+        const ulonglong2 wb = kb[idx], wa = ka[idx];
+        a0[k] += mul_shoup4_nq(v[k], wb.x, wb.y, nq);
+        a1[k] += mul_shoup4_nq(v[k], wa.x, wa.y, nq);
+    }
 }
 
 // Hybrid key switch for one (ciphertext, target limb t) pair, t = 0..l where
@@ -164,6 +211,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     }
     for (int j = 0; j < l; ++j) {
         if (j == t && d2e) {
+            // the digit of the target's own prime is already in evaluation form
             const uint64_t* s = d2e + (size_t)b * LN + (size_t)j * C::N;
 #pragma unroll
             for (int k = 0; k < C::E; ++k) x[k] = s[ntt::eval_index<C>(k, tid)];
@@ -178,17 +226,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
             // left unreduced (below 2^51): the key product is exact for any such input
             ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
         }
-        const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
-        const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
-#pragma unroll
-        for (int k = 0; k < C::E; ++k) {
-            const int idx = ntt::eval_index<C>(k, tid);
-            const ulonglong2 wb = kb[idx], wa = ka[idx];
-            // each product < 4q: with at most 15 digits and a start below q
-            // the sums stay below 2^51
-            a0[k] += mul_shoup4_nq(x[k], wb.x, wb.y, nq);
-            a1[k] += mul_shoup4_nq(x[k], wa.x, wa.y, nq);
-        }
+        key_mac<C>(a0, a1, x, key, j, key_limbs, kt, nq, tid);
     }
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
@@ -198,6 +236,18 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     }
     uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
     const ulonglong2* itw = tw + C::N;
+    if constexpr (ntt::pair_fits<C>()) {
+        // both accumulators as one transform pair
+        ntt::inv_ntt2<C>(a0, a1, lds, itw, pc, tid);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            a0[k] = csub(a0[k], pc.q);
+            a1[k] = csub(a1[k], pc.q);
+        }
+        ntt::store_coalesced<C>(a0, out, tid);
+        ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
+        return;
+    }
     ntt::inv_ntt<C>(a0, lds, itw, pc, tid);
 #pragma unroll
     for (int k = 0; k < C::E; ++k) a0[k] = csub(a0[k], pc.q);
@@ -432,9 +482,10 @@ template <int LOGN, int T>
 hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
                      uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, hipStream_t s) {
     using C = Cfg<LOGN, T>;
-    hipError_t e = set_lds(k_front<LOGN, T>, C::LDS_BYTES);
+    constexpr int lds = ntt::pair_fits<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    hipError_t e = set_lds(k_front<LOGN, T>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(batch * l), dim3(T), C::LDS_BYTES, s, a, b, d01, d2e, d2c, t.tw,
+    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(batch * l), dim3(T), lds, s, a, b, d01, d2e, d2c, t.tw,
                        t.pc, map, l, batch, mc);
     return hipGetLastError();
 }
@@ -444,9 +495,10 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
                   const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
                   int pidx_p, hipStream_t s) {
     using C = Cfg<LOGN, T>;
-    hipError_t e = set_lds(k_keyswitch<LOGN, T>, C::LDS_BYTES);
+    constexpr int lds = ntt::pair_fits<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    hipError_t e = set_lds(k_keyswitch<LOGN, T>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_keyswitch<LOGN, T>), dim3(batch * (l + 1)), dim3(T), C::LDS_BYTES, s, d2c, d2s, d2e, d01, key,
+    hipLaunchKernelGGL((k_keyswitch<LOGN, T>), dim3(batch * (l + 1)), dim3(T), lds, s, d2c, d2s, d2e, d01, key,
                        key_limbs, acc, t.tw, t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1);
     return hipGetLastError();
 }

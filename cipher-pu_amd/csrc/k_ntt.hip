@@ -81,8 +81,12 @@ __global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, c
     uint64_t x[C::E], y[C::E];
     ntt::load_coalesced<C>(x, A + off, tid);
     ntt::load_coalesced<C>(y, B + off, tid);
-    ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
-    ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
+    if constexpr (ntt::pair_fits<C>()) {
+        ntt::fwd_ntt2<C>(x, y, lds, tw, pc.q, pc.two_q, tid);
+    } else {
+        ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
+        ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
+    }
 #pragma unroll
     for (int e = 0; e < C::E; ++e) x[e] = mul_barrett(reduce64(x[e], pc), reduce64(y[e], pc), pc);
     ntt::inv_ntt<C>(x, lds, tw + C::N, pc, tid);
@@ -244,9 +248,10 @@ template <int LOGN, int T>
 hipError_t run_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out, int rows,
                        int n_limbs, const LimbMap& map, hipStream_t s) {
     using C = Cfg<LOGN, T>;
-    hipError_t e = set_lds(k_polymul<LOGN, T>, C::LDS_BYTES);
+    constexpr int lds = ntt::pair_fits<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    hipError_t e = set_lds(k_polymul<LOGN, T>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_polymul<LOGN, T>), dim3(rows), dim3(T), C::LDS_BYTES, s, a, b, out, t.tw, t.pc, map,
+    hipLaunchKernelGGL((k_polymul<LOGN, T>), dim3(rows), dim3(T), lds, s, a, b, out, t.tw, t.pc, map,
                        n_limbs);
     return hipGetLastError();
 }

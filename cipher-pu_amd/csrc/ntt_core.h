@@ -259,6 +259,134 @@ __device__ __forceinline__ void inv_ntt(uint64_t (&x)[C::E], uint64_t* lds,
     inv_rest<C, C::NP - 2>(x, lds, itw, pc, tid);
 }
 
+// ---------------------------------------------------------------------------
+// Two transforms of the same prime, interleaved.  Every twiddle load serves
+// the corresponding butterflies of both, the two register sets give the
+// scheduler independent work, and both polynomials cross each exchange
+// between one pair of barriers.  lds holds two exchange buffers of
+// LDS_WORDS each (PAIR_LDS_BYTES).  Results are those of two single
+// transforms, value for value.
+template <class C>
+constexpr bool pair_fits() { return 2 * C::LDS_BYTES <= 160 * 1024; }
+
+template <class C, int P>
+__device__ __forceinline__ void fwd_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::E],
+                                          const ulonglong2* __restrict__ tw, uint64_t q, uint64_t two_q,
+                                          int tid) {
+    const uint64_t nq = (uint64_t)0 - q, four_q = two_q << 1;
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = (g * C::T + tid) >> LO;
+#pragma unroll
+        for (int b = HI; b >= LO; --b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                const ulonglong2 w = tw[base + i];
+                ct_bfly_lazy(x[g * M + e], x[g * M + e + h], w.x, w.y, nq, four_q);
+                ct_bfly_lazy(y[g * M + e], y[g * M + e + h], w.x, w.y, nq, four_q);
+            }
+        }
+    }
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::E],
+                                          const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid) {
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+    const uint64_t q = pc.q, nq = (uint64_t)0 - q;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = (g * C::T + tid) >> LO;
+#pragma unroll
+        for (int b = LO; b <= HI; ++b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+            const uint64_t bound = q << (b + 1);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                uint64_t& X = x[g * M + e];
+                uint64_t& Y = x[g * M + e + h];
+                uint64_t& U = y[g * M + e];
+                uint64_t& V = y[g * M + e + h];
+                if (b == C::LOGN - 1) {
+                    const uint64_t x0 = X, x1 = Y, u0 = U, u1 = V;
+                    X = mul_shoup_nq(x0 + x1, pc.ninv, pc.ninv_s, nq);
+                    Y = mul_shoup_nq(x0 + bound - x1, pc.ninv_w1, pc.ninv_w1_s, nq);
+                    U = mul_shoup_nq(u0 + u1, pc.ninv, pc.ninv_s, nq);
+                    V = mul_shoup_nq(u0 + bound - u1, pc.ninv_w1, pc.ninv_w1_s, nq);
+                } else {
+                    const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                    const ulonglong2 w = itw[base + i];
+                    gs_bfly_lazy(X, Y, w.x, w.y, nq, bound);
+                    gs_bfly_lazy(U, V, w.x, w.y, nq, bound);
+                }
+            }
+        }
+    }
+}
+
+// One exchange of both register sets from pass A's layout to pass B's.
+template <class C, int A, int B, bool W>
+__device__ __forceinline__ void exchange2(uint64_t (&x)[C::E], uint64_t (&y)[C::E], uint64_t* lds, int tid) {
+    to_lds<C, A>(x, lds, tid);
+    to_lds<C, A>(y, lds + C::LDS_WORDS, tid);
+    exchange_sync<W>();
+    from_lds<C, B>(x, lds, tid);
+    from_lds<C, B>(y, lds + C::LDS_WORDS, tid);
+    exchange_sync<W>();
+}
+
+template <class C, int P>
+__device__ __forceinline__ void fwd_rest2(uint64_t (&x)[C::E], uint64_t (&y)[C::E], uint64_t* lds,
+                                          const ulonglong2* __restrict__ tw, uint64_t q, uint64_t two_q,
+                                          int tid) {
+    if constexpr (P < C::NP) {
+        if constexpr (P == 1) __syncthreads();
+        exchange2<C, P - 1, P, wave_local<C, P>()>(x, y, lds, tid);
+        fwd_pass2<C, P>(x, y, tw, q, two_q, tid);
+        fwd_rest2<C, P + 1>(x, y, lds, tw, q, two_q, tid);
+    }
+}
+
+// fwd_ntt on x and y at once (same entry and exit conditions).
+template <class C>
+__device__ __forceinline__ void fwd_ntt2(uint64_t (&x)[C::E], uint64_t (&y)[C::E], uint64_t* lds,
+                                         const ulonglong2* __restrict__ tw, uint64_t q, uint64_t two_q,
+                                         int tid) {
+    static_assert(pair_fits<C>(), "two exchange buffers exceed the LDS");
+    opaque(tw, tid);
+    fwd_pass2<C, 0>(x, y, tw, q, two_q, tid);
+    fwd_rest2<C, 1>(x, y, lds, tw, q, two_q, tid);
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_rest2(uint64_t (&x)[C::E], uint64_t (&y)[C::E], uint64_t* lds,
+                                          const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid) {
+    if constexpr (P >= 0) {
+        exchange2<C, P + 1, P, wave_local<C, P + 1>()>(x, y, lds, tid);
+        inv_pass2<C, P>(x, y, itw, pc, tid);
+        inv_rest2<C, P - 1>(x, y, lds, itw, pc, tid);
+    }
+}
+
+// inv_ntt on x and y at once (same entry and exit conditions).
+template <class C>
+__device__ __forceinline__ void inv_ntt2(uint64_t (&x)[C::E], uint64_t (&y)[C::E], uint64_t* lds,
+                                         const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid) {
+    static_assert(pair_fits<C>(), "two exchange buffers exceed the LDS");
+    opaque(itw, tid);
+    inv_pass2<C, C::NP - 1>(x, y, itw, pc, tid);
+    inv_rest2<C, C::NP - 2>(x, y, lds, itw, pc, tid);
+}
+
 // HBM <-> registers in pass-0 (coalesced) layout.
 template <class C>
 __device__ __forceinline__ void load_coalesced(uint64_t (&x)[C::E], const uint64_t* __restrict__ src,
