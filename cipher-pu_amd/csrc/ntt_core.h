@@ -74,6 +74,21 @@ __device__ __forceinline__ int index_of(int g, int e, int tid) {
     return ((f >> LO) << (HI + 1)) | (e << LO) | (f & ((1 << LO) - 1));
 }
 
+// Twiddle group of register group g in pass P's layout.  When the pass's
+// low bit LO is at least 6, all 64 lanes of a wave share it (T is a
+// multiple of 64, and lanes differ only in the low 6 bits of g T + tid):
+// taking it from the first lane makes the pass's twiddle addresses
+// wave-uniform, so they are fetched by scalar loads into SGPRs (several
+// twiddles per load) instead of one vector load per butterfly.
+template <class C, int P>
+__device__ __forceinline__ int group_of(int g, int tid) {
+    constexpr int LO = C::lo(P);
+    static_assert(C::T % 64 == 0, "workgroups are whole wavefronts");
+    const int fh = (g * C::T + tid) >> LO;
+    if constexpr (LO >= 6) return __builtin_amdgcn_readfirstlane(fh);
+    else return fh;
+}
+
 template <class C, int P>
 __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ tw,
                                          uint64_t q, uint64_t two_q, int tid) {
@@ -82,7 +97,7 @@ __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* 
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = (g * C::T + tid) >> LO;
+        const int fh = group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = HI; b >= LO; --b) {
             const int h = 1 << (b - LO);
@@ -106,7 +121,7 @@ __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* 
     const uint64_t q = pc.q, nq = (uint64_t)0 - q;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = (g * C::T + tid) >> LO;
+        const int fh = group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = LO; b <= HI; ++b) {
             const int h = 1 << (b - LO);
@@ -278,7 +293,7 @@ __device__ __forceinline__ void fwd_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = (g * C::T + tid) >> LO;
+        const int fh = group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = HI; b >= LO; --b) {
             const int h = 1 << (b - LO);
@@ -303,7 +318,7 @@ __device__ __forceinline__ void inv_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::
     const uint64_t q = pc.q, nq = (uint64_t)0 - q;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = (g * C::T + tid) >> LO;
+        const int fh = group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = LO; b <= HI; ++b) {
             const int h = 1 << (b - LO);
