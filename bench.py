@@ -66,7 +66,7 @@ def keyswitch_bytes(batch: int, l: int = L) -> int:
 # this bench, scaled by the same counters on the --copy-probe copy (the
 # summaries and commands are under profiles/).  Keyed by (kernel kind,
 # batch, threads); None where not measured for the running shape.
-MEASURED_TRAFFIC = {("keyswitch", 256, 1024): 692557162}
+MEASURED_TRAFFIC = {("keyswitch", 256, 1024): 692131469}
 
 
 def cpu_model() -> str:
@@ -279,6 +279,7 @@ def main():
     ap.add_argument("--front", action="store_true", help="fused HMult front kernel (with fixed shapes)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape tuner")
     ap.add_argument("--no-mlp", action="store_true", help="skip the encrypted-MLP leg")
+    ap.add_argument("--no-rotation-probe", action="store_true", help="skip the hoisted-rotation probe")
     ap.add_argument("--no-cnn", action="store_true", help="skip the encrypted-CNN leg")
     ap.add_argument("--mlp-samples", type=int, default=4096)
     ap.add_argument("--target-snr", type=float, default=3.0)
@@ -382,7 +383,8 @@ def main():
         "wall_s_timed": round(wall, 4),
         "device": torch.cuda.get_device_name(0),
     }
-    res["rotation_probe"] = rotation_probe(ck, sk)
+    if not args.no_rotation_probe:
+        res["rotation_probe"] = rotation_probe(ck, sk)
     if not args.no_mlp:
         res["mlp"] = mlp_leg(args.mlp_samples, args.target_snr, 64)
     if not args.no_cnn:
