@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
     const size_t LN = (size_t)l * C::N;
     const size_t c0 = (size_t)b * 2 * LN + (size_t)i * C::N;   // poly 0 of pair b, limb i
     const uint64_t pm = mc.p_mod[i], pms = mc.p_mod_s[i];
-    if constexpr (ntt::pair_fits<C>()) {
+    if constexpr (ntt::pair_pays<C>()) {
         // a0 with a1 and b0 with b1 as transform pairs
         uint64_t x0[C::E], x1[C::E], y0[C::E], y1[C::E];
         ntt::load_coalesced<C>(x0, A + c0, tid);
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     }
     uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
     const ulonglong2* itw = tw + C::N;
-    if constexpr (ntt::pair_fits<C>()) {
+    if constexpr (ntt::pair_pays<C>()) {
         // both accumulators as one transform pair
         ntt::inv_ntt2<C>(a0, a1, lds, itw, pc, tid);
 #pragma unroll
@@ -482,7 +482,7 @@ template <int LOGN, int T>
 hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
                      uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, hipStream_t s) {
     using C = Cfg<LOGN, T>;
-    constexpr int lds = ntt::pair_fits<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
     hipError_t e = set_lds(k_front<LOGN, T>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_front<LOGN, T>), dim3(batch * l), dim3(T), lds, s, a, b, d01, d2e, d2c, t.tw,
@@ -495,7 +495,7 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
                   const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
                   int pidx_p, hipStream_t s) {
     using C = Cfg<LOGN, T>;
-    constexpr int lds = ntt::pair_fits<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
     hipError_t e = set_lds(k_keyswitch<LOGN, T>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_keyswitch<LOGN, T>), dim3(batch * (l + 1)), dim3(T), lds, s, d2c, d2s, d2e, d01, key,
@@ -503,7 +503,9 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
     return hipGetLastError();
 }
 
-int default_threads(int logN) { return logN >= 14 ? 512 : (logN == 13 ? 512 : 256); }
+// Untuned launch shapes: the tuner's usual choice on MI355X at N = 8192
+// (one coefficient group of 8 per thread, paired transforms).
+int default_threads(int logN) { return logN >= 14 ? 512 : (logN == 13 ? 1024 : 256); }
 
 }  // namespace
 

@@ -82,6 +82,39 @@ __device__ __forceinline__ void permute_inverse(uint64_t (&a)[C::E], bool negate
     __syncthreads();
 }
 
+// permute_inverse for both accumulators at once (the second negated): two
+// LDS buffers, one gather each, one transform pair.
+template <class C>
+__device__ __forceinline__ void permute_inverse2(uint64_t (&a0)[C::E], uint64_t (&a1)[C::E], uint32_t g,
+                                                 uint64_t* lds, const ulonglong2* __restrict__ itw,
+                                                 const PrimeConst& pc, uint64_t* __restrict__ out0,
+                                                 uint64_t* __restrict__ out1, int tid) {
+    uint64_t* lds1 = lds + C::LDS_WORDS;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int slot = ntt::pad<C>(ntt::eval_index<C>(k, tid));
+        lds[slot] = reduce64(a0[k], pc);
+        const uint64_t v = reduce64(a1[k], pc);
+        lds1[slot] = v ? pc.q - v : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int src = ntt::pad<C>(eval_perm(ntt::eval_index<C>(k, tid), g, C::LOGN));
+        a0[k] = lds[src];
+        a1[k] = lds1[src];
+    }
+    __syncthreads();
+    ntt::inv_ntt2<C>(a0, a1, lds, itw, pc, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        a0[k] = csub(a0[k], pc.q);
+        a1[k] = csub(a1[k], pc.q);
+    }
+    ntt::store_coalesced<C>(a0, out0, tid);
+    ntt::store_coalesced<C>(a1, out1, tid);
+}
+
 // One workgroup per (target t, ciphertext b, rotation r), rotations of one
 // (t, b) adjacent so that the digits stay in L2.  keys: [R][dnum][2][key_limbs][N]
 // (value, companion) pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N]
@@ -122,8 +155,12 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
     const uint32_t g = galois.g[r];
     const ulonglong2* itw = tw_all + ((size_t)pt * 2 + 1) * C::N;
     uint64_t* out = acc + (((size_t)r * batch + b) * 2 * (l + 1) + t) * C::N;
-    permute_inverse<C>(a0, false, g, lds, itw, pc, out, tid);
-    permute_inverse<C>(a1, true, g, lds, itw, pc, out + (size_t)(l + 1) * C::N, tid);
+    if constexpr (ntt::pair_pays<C>()) {
+        permute_inverse2<C>(a0, a1, g, lds, itw, pc, out, out + (size_t)(l + 1) * C::N, tid);
+    } else {
+        permute_inverse<C>(a0, false, g, lds, itw, pc, out, tid);
+        permute_inverse<C>(a1, true, g, lds, itw, pc, out + (size_t)(l + 1) * C::N, tid);
+    }
 }
 
 // out[row][j(i)] = in[row][i]: the key rows permuted by pi_g^-1.
@@ -157,9 +194,10 @@ hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong
                           int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l, const LimbMap& map,
                           int pidx_p, int R, hipStream_t s) {
     using C = Cfg<LOGN, T>;
-    hipError_t e = set_lds(k_ks_hoisted<LOGN, T>, C::LDS_BYTES);
+    constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    hipError_t e = set_lds(k_ks_hoisted<LOGN, T>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ks_hoisted<LOGN, T>), dim3((l + 1) * batch * R), dim3(T), C::LDS_BYTES, s, D, keys,
+    hipLaunchKernelGGL((k_ks_hoisted<LOGN, T>), dim3((l + 1) * batch * R), dim3(T), lds, s, D, keys,
                        key_stride, key_limbs, galois, acc, t.tw, t.pc, map, l, pidx_p, batch, R);
     return hipGetLastError();
 }
@@ -169,7 +207,7 @@ hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong
 hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int batch, int l, const LimbMap& map,
                         int pidx_p, int threads, hipStream_t s) {
     if (!batch) return hipSuccess;
-    switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : 512))) {
+    switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : t.logN == 13 ? 1024 : 512))) {
         case 120256: return run_modup<12, 256>(t, ct, D, batch, l, map, pidx_p, s);
         case 120512: return run_modup<12, 512>(t, ct, D, batch, l, map, pidx_p, s);
         case 130256: return run_modup<13, 256>(t, ct, D, batch, l, map, pidx_p, s);
@@ -185,7 +223,7 @@ hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulongl
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
                              const LimbMap& map, int pidx_p, int R, int threads, hipStream_t s) {
     if (!batch || !R) return hipSuccess;
-    switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : 512))) {
+    switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : t.logN == 13 ? 1024 : 512))) {
         case 120256: return run_ks_hoisted<12, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
         case 120512: return run_ks_hoisted<12, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
         case 130256: return run_ks_hoisted<13, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);

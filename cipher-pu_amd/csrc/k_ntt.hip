@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, c
     uint64_t x[C::E], y[C::E];
     ntt::load_coalesced<C>(x, A + off, tid);
     ntt::load_coalesced<C>(y, B + off, tid);
-    if constexpr (ntt::pair_fits<C>()) {
+    if constexpr (ntt::pair_pays<C>()) {
         ntt::fwd_ntt2<C>(x, y, lds, tw, pc.q, pc.two_q, tid);
     } else {
         ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
@@ -248,7 +248,7 @@ template <int LOGN, int T>
 hipError_t run_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out, int rows,
                        int n_limbs, const LimbMap& map, hipStream_t s) {
     using C = Cfg<LOGN, T>;
-    constexpr int lds = ntt::pair_fits<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
     hipError_t e = set_lds(k_polymul<LOGN, T>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_polymul<LOGN, T>), dim3(rows), dim3(T), lds, s, a, b, out, t.tw, t.pc, map,
@@ -256,7 +256,9 @@ hipError_t run_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b,
     return hipGetLastError();
 }
 
-int default_threads(int logN) { return logN >= 14 ? 512 : 256; }
+// Untuned launch shape of the batched transforms (fastest standalone
+// transform on MI355X at N = 8192: 512 threads).
+int default_threads(int logN) { return logN >= 13 ? 512 : 256; }
 
 }  // namespace
 

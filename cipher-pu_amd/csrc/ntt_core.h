@@ -284,6 +284,13 @@ __device__ __forceinline__ void inv_ntt(uint64_t (&x)[C::E], uint64_t* lds,
 template <class C>
 constexpr bool pair_fits() { return 2 * C::LDS_BYTES <= 160 * 1024; }
 
+// Where kernels pair their transforms: the two exchange buffers fit the LDS
+// and each thread holds at most 8 coefficients of each polynomial.  With 16
+// or 32 the two register sets crowd out the rest of a kernel's state and
+// the compiler spills.
+template <class C>
+constexpr bool pair_pays() { return pair_fits<C>() && C::E <= 8; }
+
 template <class C, int P>
 __device__ __forceinline__ void fwd_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::E],
                                           const ulonglong2* __restrict__ tw, uint64_t q, uint64_t two_q,
