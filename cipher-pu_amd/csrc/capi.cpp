@@ -680,9 +680,10 @@ int ckks_rotate_hoisted(const ckks_ctx* c, const uint64_t* ct, const uint64_t* k
     uint64_t* sct = D + B * l * (l + 1) * c->N;               // [R][B][2][l][N]: sigma_g(ct)
     uint64_t* acc = sct + (size_t)R * B * 2 * LN;             // [R][B][2][l+1][N]
     TIMED(CKKS_KIND_KEYSWITCH, launch_modup(t, ct, D, batch, l, m, special, c->ks_threads, S(stream)));
+    // only sigma_g(c0) is read (ModDown adds poly 0 alone); sigma_g(c1)'s part of sct stays unwritten
     for (int r = 0; r < R; ++r)
-        TIMED(CKKS_KIND_AUTOMORPHISM,
-              launch_automorphism(t, ct, sct + (size_t)r * B * 2 * LN, batch * 2, l, m, galois[r], S(stream)));
+        TIMED(CKKS_KIND_AUTOMORPHISM, launch_automorphism_groups(t, ct, sct + (size_t)r * B * 2 * LN, batch, l,
+                                                                 2 * LN, m, galois[r], S(stream)));
     // one prepared key per rotation: [dnum = key_limbs - 1][2][key_limbs][N] pairs
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
     TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,

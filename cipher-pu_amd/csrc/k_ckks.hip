@@ -340,6 +340,40 @@ __global__ void k_automorphism(const uint64_t* __restrict__ in, uint64_t* __rest
     out[g] = v;
 }
 
+// The same map, one workgroup per (poly, limb) row: the row is read into LDS
+// with coalesced loads and written back with coalesced stores in the
+// permuted order, so neither side of HBM sees the gather's stride.  (It is
+// also safe in place: a row is fully staged before any of it is written.)
+// Rows are (group, limb) with group g's limb i at g group_stride + i N in
+// both tensors.
+__global__ void __launch_bounds__(1024) k_automorphism_row(const uint64_t* in, uint64_t* out,
+                                                           const PrimeConst* __restrict__ pcs, LimbMap map,
+                                                           int n_limbs, size_t group_stride, uint64_t ginv,
+                                                           int logN) {
+    extern __shared__ uint64_t lds[];
+    const int N = 1 << logN;
+    const int r = blockIdx.x;
+    const size_t off = (size_t)(r / n_limbs) * group_stride + (size_t)(r % n_limbs) * N;
+    const ulonglong2* src = reinterpret_cast<const ulonglong2*>(in + off);
+    ulonglong2* row2 = reinterpret_cast<ulonglong2*>(lds);
+    for (int k = threadIdx.x; k < N / 2; k += blockDim.x) row2[k] = src[k];
+    __syncthreads();
+    const uint64_t q = pcs[map.idx[r % n_limbs]].q;
+    const uint64_t mask = 2 * (uint64_t)N - 1;
+    uint64_t* dst = out + off;
+    for (int d = threadIdx.x; d < N; d += blockDim.x) {
+        const uint64_t k = (ginv * (uint64_t)d) & mask;
+        uint64_t v;
+        if (k < (uint64_t)N) {
+            v = lds[k];
+        } else {
+            const uint64_t c = lds[k - N];
+            v = c ? q - c : 0;
+        }
+        dst[d] = v;
+    }
+}
+
 // Plain noise monitor per (poly, limb) row: with c' = min(c, q - c)
 // (rtl/m4/noise_monitor.sv:44-78, verify_m4_h_multilayer.py:162-174) the row
 // reports sum c', sum c'^2 (128-bit), max c' and the sum of the centered
@@ -581,18 +615,39 @@ hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out,
     return hipGetLastError();
 }
 
-hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys, int n_limbs,
-                               const LimbMap& map, uint64_t galois, hipStream_t s) {
-    const size_t total = ((size_t)n_polys * n_limbs) << t.logN;
-    if (!total) return hipSuccess;
+hipError_t launch_automorphism_groups(const DevTables& t, const uint64_t* in, uint64_t* out, int n_groups,
+                                     int n_limbs, size_t group_stride, const LimbMap& map, uint64_t galois,
+                                     hipStream_t s) {
+    if (n_groups == 0 || n_limbs == 0) return hipSuccess;
+    const int N = 1 << t.logN;
     const uint64_t M = 2ull << t.logN;
     // inverse of the odd Galois element modulo 2N (a power of two)
     uint64_t inv = 1;
     for (int k = 0; k < 7; ++k) inv = inv * (2 - galois * inv);
     inv &= M - 1;
-    hipLaunchKernelGGL(k_automorphism, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, map,
-                       n_limbs, inv, t.logN, total);
-    return hipGetLastError();
+    const int lds = N * 8;
+    if (N >= 128 && lds <= 128 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_automorphism_row,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_automorphism_row, dim3((unsigned)(n_groups * n_limbs)), dim3(N >= 2048 ? 1024 : N / 2),
+                           lds, s, in, out, t.pc, map, n_limbs, group_stride, inv, t.logN);
+        return hipGetLastError();
+    }
+    // rows too long for LDS: the gather kernel, one launch per group
+    const size_t total = (size_t)n_limbs << t.logN;
+    for (int g = 0; g < n_groups; ++g) {
+        hipLaunchKernelGGL(k_automorphism, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                           in + g * group_stride, out + g * group_stride, t.pc, map, n_limbs, inv, t.logN, total);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys, int n_limbs,
+                               const LimbMap& map, uint64_t galois, hipStream_t s) {
+    return launch_automorphism_groups(t, in, out, n_polys, n_limbs, (size_t)n_limbs << t.logN, map, galois, s);
 }
 
 hipError_t launch_noise_stats(const DevTables& t, const uint64_t* in, uint64_t* stats, int n_polys, int n_limbs,

@@ -71,6 +71,10 @@ hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out,
                           const LimbMap& map, const ModDownConst& mc, hipStream_t s);
 hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys,
                                int n_limbs, const LimbMap& map, uint64_t galois, hipStream_t s);
+// The automorphism over n_groups groups of n_limbs rows, group g at g group_stride.
+hipError_t launch_automorphism_groups(const DevTables& t, const uint64_t* in, uint64_t* out, int n_groups,
+                                      int n_limbs, size_t group_stride, const LimbMap& map, uint64_t galois,
+                                      hipStream_t s);
 hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
                              int ell, hipStream_t s);
 hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
