@@ -450,26 +450,55 @@ __global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restri
 // With scalar = 1, D is [ng][nb][l]: one integer per (g, b, limb), the
 // same product in either domain (a constant plaintext), so E may then be in
 // coefficient form.
-__global__ void k_lincomb(const uint64_t* __restrict__ E, const ulonglong2* __restrict__ D,
-                          uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map, int nb,
-                          int ng, int l, int logN, size_t per_b, int scalar) {
-    const size_t g0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g0 >= per_b) return;
-    const size_t N = (size_t)1 << logN;
-    const size_t n = g0 & (N - 1);
-    const int i = (int)((g0 >> logN) % l);
+// Blocked for reuse: one thread per (block of R rows, limb i, coefficient n)
+// keeps NG x R accumulators, so each input value is
+// loaded once per call and each plaintext value once per R rows (rows are
+// the (ciphertext, poly) pairs, 2 B of them).  ng <= NG; rows past the end
+// of the last block are skipped.
+template <int NG, int R>
+__global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict__ E,
+                                                     const ulonglong2* __restrict__ D, uint64_t* __restrict__ out,
+                                                     const PrimeConst* __restrict__ pcs, LimbMap map, int nb, int ng,
+                                                     int rows, int l, int logN, size_t total, int scalar) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const size_t N = (size_t)1 << logN, LN = (size_t)l * N;
+    const size_t n = gid & (N - 1), rest = gid >> logN;
+    const int i = (int)(rest % l);
+    const int r0 = (int)(rest / l) * R;
     const PrimeConst& pc = pcs[map.idx[i]];
+    const uint64_t nq = (uint64_t)0 - pc.q;
     const size_t dn = scalar ? 1 : N;
     const size_t dstride = (size_t)l * dn;
-    const uint64_t nq = (uint64_t)0 - pc.q;
-    for (int g = 0; g < ng; ++g) {
-        const ulonglong2* d = D + (size_t)g * nb * dstride + (size_t)i * dn + (scalar ? 0 : n);
-        uint64_t acc = 0;
-        for (int b = 0; b < nb; ++b) {   // the E row of this thread stays in L1 across g
-            const ulonglong2 w = d[(size_t)b * dstride];
-            acc += mul_shoup4_nq(E[(size_t)b * per_b + g0], w.x, w.y, nq);   // < 4q each; nb <= 32: acc < 2^52
+    const ulonglong2* dbase = D + (size_t)i * dn + (scalar ? 0 : n);
+    const uint64_t* ebase = E + (size_t)r0 * LN + (size_t)i * N + n;
+    const size_t brow = (size_t)rows * LN;   // one b (or g) block of rows
+    uint64_t acc[NG][R];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[g][r] = 0;
+    for (int b = 0; b < nb; ++b) {
+        uint64_t e[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) e[r] = r0 + r < rows ? ebase[(size_t)b * brow + (size_t)r * LN] : 0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            if (g < ng) {
+                const ulonglong2 w = dbase[((size_t)g * nb + b) * dstride];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[g][r] += mul_shoup4_nq(e[r], w.x, w.y, nq);   // nb <= 32: < 2^52
+            }
         }
-        out[(size_t)g * per_b + g0] = reduce64(acc, pc);
+    }
+    uint64_t* obase = out + (size_t)r0 * LN + (size_t)i * N + n;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        if (g < ng) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (r0 + r < rows) obase[(size_t)g * brow + (size_t)r * LN] = reduce64(acc[g][r], pc);
+        }
     }
 }
 
@@ -666,13 +695,37 @@ hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64
     return hipGetLastError();
 }
 
+namespace {
+
+template <int NG, int R>
+hipError_t run_lincomb_blk(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
+                           int rows, int l, const LimbMap& map, int scalar, hipStream_t s) {
+    const size_t total = ((size_t)((rows + R - 1) / R) * l) << t.logN;
+    hipLaunchKernelGGL((k_lincomb_blk<NG, R>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, E, D, out,
+                       t.pc, map, nb, ng, rows, l, t.logN, total, scalar);
+    return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
                           int batch, int l, const LimbMap& map, int scalar, hipStream_t s) {
-    const size_t per_b = ((size_t)batch * 2 * l) << t.logN;
-    if (!per_b || !ng) return hipSuccess;
-    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((per_b + 255) / 256)), dim3(256), 0, s, E, D, out, t.pc, map, nb, ng,
-                       l, t.logN, per_b, scalar);
-    return hipGetLastError();
+    const int rows = batch * 2;
+    if (!rows || !ng || !l) return hipSuccess;
+    const size_t dn = scalar ? 1 : ((size_t)1 << t.logN);
+    const size_t LN = (size_t)l << t.logN;
+    // giant steps in chunks of at most 16
+    for (int g0 = 0; g0 < ng; g0 += 16) {
+        const int gc = ng - g0 < 16 ? ng - g0 : 16;
+        const ulonglong2* Dg = D + (size_t)g0 * nb * l * dn;
+        uint64_t* og = out + (size_t)g0 * rows * LN;
+        hipError_t e;
+        if (gc <= 4) e = run_lincomb_blk<4, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
+        else if (gc <= 8) e = run_lincomb_blk<8, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
+        else e = run_lincomb_blk<16, 2>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_lift_small(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, int l_in, int l_out,

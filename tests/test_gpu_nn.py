@@ -88,6 +88,42 @@ def test_lincomb_scalar_bit_exact(mlp_env):
             assert np.array_equal(got[g, :, :, i].astype(object), want), (g, i)
 
 
+@pytest.mark.parametrize("nb,ng,B,l,scalar", [(16, 8, 2, 4, 0), (17, 10, 3, 3, 0), (32, 5, 3, 1, 0),
+                                              (4, 20, 1, 2, 0), (24, 8, 3, 2, 1), (5, 17, 2, 1, 1)])
+def test_lincomb_shapes_bit_exact(mlp_env, nb, ng, B, l, scalar):
+    """out[g][row] = sum_b D[g][b] E[b][row] mod q_i, per coefficient, for
+    giant-step counts across the kernel's blocks (<= 4, <= 8, <= 16 and
+    chunks of 16), row counts that leave a partial row block, and up to 32
+    baby steps; against Python integers on the first 64 coefficients (the
+    combination is coefficient-wise)."""
+    import ctypes
+    import torch
+    from cipherpu._lib import call
+    from cipherpu.ring import to_numpy
+    ck = mlp_env[0]
+    rng = np.random.default_rng(nb * 100 + ng)
+    N, K = ck.N, 64
+    E = np.stack([rng.integers(0, ck.primes[i], (nb, B, 2, N), dtype=np.uint64) for i in range(l)], axis=3)
+    dshape = (ng, nb, l) if scalar else (ng, nb, l, N)
+    W = np.stack([rng.integers(0, ck.primes[i], dshape[:2] + dshape[3:], dtype=np.uint64) for i in range(l)], axis=2)
+    q = np.array([ck.primes[i] for i in range(l)], dtype=object)
+    qb = q.reshape((1, 1, l) + ((1,) if not scalar else ()))
+    comp = (W.astype(object) << 63) // qb
+    D = np.stack([W, comp.astype(np.uint64)], axis=-1)
+    Et, Dt = ck.ring.tensor(E), ck.ring.tensor(D)
+    out = torch.empty(ng, B, 2, l, N, dtype=torch.uint64, device=ck.device)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    call("ckks_lincomb_scalar" if scalar else "ckks_lincomb", ck.ring.handle, p(Et), p(Dt), p(out), nb, ng, B, l,
+         ck.ring.stream())
+    got = to_numpy(out)[..., :K].astype(object)
+    Eo = E[..., :K].astype(object)
+    for g in range(ng):
+        for i in range(l):
+            w = [W[g, b, i] if scalar else W[g, b, i, :K] for b in range(nb)]
+            want = sum(Eo[b, :, :, i] * (int(w[b]) if scalar else w[b].astype(object)) for b in range(nb))
+            assert np.array_equal(got[g, :, :, i], want % int(q[i])), (g, i)
+
+
 def test_cnn_matches_plaintext(mlp_env):
     from cipherpu import cnn, nn
     ck, sk = mlp_env[0], mlp_env[1]
