@@ -68,6 +68,12 @@ def keyswitch_bytes(batch: int, l: int = L) -> int:
 # batch, threads); None where not measured for the running shape.
 MEASURED_TRAFFIC = {("keyswitch", 256, 1024): 692131469}
 
+# Wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU of
+# the same command, profiles/), same keys.  The issue peak counts one wave64
+# VALU instruction per SIMD every 4 cycles: 1024 SIMDs x 2.4 GHz / 4.
+MEASURED_VALU_INSTR = {("keyswitch", 256, 1024): 153837568}
+VALU_ISSUE_PEAK_G = 1024 * 2.4 / 4
+
 
 def cpu_model() -> str:
     try:
@@ -353,6 +359,12 @@ def main():
     kbytes = keyswitch_bytes(B)
     k_achieved = kbytes / (dom_ms / 1e3) / 1e9
     traffic = MEASURED_TRAFFIC.get((dom, B, shape.get("ks")))
+    valu_instr = MEASURED_VALU_INSTR.get((dom, B, shape.get("ks")))
+    valu = None
+    if valu_instr:
+        rate = valu_instr / (dom_ms / 1e3) / 1e9
+        valu = {"wave_instructions": valu_instr, "achieved": round(rate, 1), "peak": VALU_ISSUE_PEAK_G,
+                "unit": "G wave-instr/s", "frac": round(rate / VALU_ISSUE_PEAK_G, 4)}
 
     res = {
         "metric": "HMult+relin+rescale/s at N=2^13 L=4",
@@ -374,6 +386,7 @@ def main():
                      "frac": round(k_achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"k_keyswitch<13,{shape.get('ks') or 'default'}>",
                      "avg_launch_ms": round(dom_ms, 4), "algorithmic_bytes_per_launch": kbytes},
+        "valu_issue": valu,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kern.items()},
         "step_roofline": {"achieved": round(nbytes / (ms / 1e3) / 1e9, 1), "unit": "GB/s",
                           "frac": round(nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
