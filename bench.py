@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--no-rotation-probe", action="store_true", help="skip the hoisted-rotation probe")
     ap.add_argument("--no-cnn", action="store_true", help="skip the encrypted-CNN leg")
     ap.add_argument("--mlp-samples", type=int, default=4096)
+    ap.add_argument("--chunk-cts", type=int, default=64,
+                    help="ciphertexts per gear decision in the encrypted-network legs")
     ap.add_argument("--target-snr", type=float, default=3.0)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--modmul-probe", action="store_true", help="Barrett vs special-form point-wise product")
@@ -399,9 +401,9 @@ def main():
     if not args.no_rotation_probe:
         res["rotation_probe"] = rotation_probe(ck, sk)
     if not args.no_mlp:
-        res["mlp"] = mlp_leg(args.mlp_samples, args.target_snr, 64)
+        res["mlp"] = mlp_leg(args.mlp_samples, args.target_snr, args.chunk_cts)
     if not args.no_cnn:
-        res["cnn"] = mlp_leg(args.mlp_samples, args.target_snr, 64, kind="cnn")
+        res["cnn"] = mlp_leg(args.mlp_samples, args.target_snr, args.chunk_cts, kind="cnn")
     if not args.no_graph:
         res["batch_graph"] = graph_leg(args.graph_ops)
     if not args.no_cpu_baseline:
