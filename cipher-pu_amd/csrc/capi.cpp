@@ -114,6 +114,9 @@ namespace {
     } while (0)
 
 inline bool misaligned(const void* p) { return ((uintptr_t)p & 7) != 0; }
+// (value, Shoup companion) pair buffers -- prepared keys and plaintexts --
+// are read as 16-byte pairs
+inline bool misaligned_pairs(const void* p) { return ((uintptr_t)p & 15) != 0; }
 
 // CKKS_OK, an error, or NOTHING_TO_DO for a valid call with zero rows
 const int NOTHING_TO_DO = 1;
@@ -529,7 +532,7 @@ int ckks_key_prepare(const ckks_ctx* c, const uint64_t* key_coeff, uint64_t* key
     if (!c || !key_coeff || !key_out) return CKKS_ERR_NULL;
     if (dnum < 1 || key_limbs < 2 || key_limbs - 1 > c->K || special < 0 || special >= c->K)
         return CKKS_ERR_BAD_SHAPE;
-    if (misaligned(key_coeff) || misaligned(key_out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (misaligned(key_coeff) || misaligned_pairs(key_out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
     LimbMap m = limb_map(0, key_limbs);
     m.idx[key_limbs - 1] = special;
@@ -550,7 +553,7 @@ static int ks_common_check(const ckks_ctx* c, const void* key, int key_limbs, co
     if (special < 0 || special >= c->K || special < l) return CKKS_ERR_BAD_SHAPE;
     if (c->flags & CKKS_CYCLIC) return CKKS_ERR_BAD_ARG;
     if (c->logN < 12) return CKKS_ERR_BAD_N;
-    if (misaligned(key) || misaligned(out) || misaligned(ws)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (misaligned_pairs(key) || misaligned(out) || misaligned(ws)) return CKKS_ERR_BAD_ALIGNMENT;
     return CKKS_OK;
 }
 
@@ -651,7 +654,7 @@ int ckks_key_permute(const ckks_ctx* c, const uint64_t* key_in, uint64_t* key_ou
     if (rows < 0) return CKKS_ERR_BAD_SHAPE;
     if (!(galois & 1) || galois >= 2 * c->N || (c->flags & CKKS_CYCLIC)) return CKKS_ERR_BAD_ARG;
     if (key_in == key_out) return CKKS_ERR_BAD_ARG;
-    if (misaligned(key_in) || misaligned(key_out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (misaligned_pairs(key_in) || misaligned_pairs(key_out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
     HIPCHK(launch_key_permute(c->tables(), (const ulonglong2*)key_in, (ulonglong2*)key_out, rows, (uint32_t)galois,
                               S(stream)));
@@ -772,7 +775,7 @@ int ckks_pt_prepare(const ckks_ctx* c, uint64_t* pt, uint64_t* out, int n_polys,
     int st = check_rows(c, n_polys, l, prime0);
     if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
     if (!pt || !out) return CKKS_ERR_NULL;
-    if (misaligned(pt) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (misaligned(pt) || misaligned_pairs(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (c->flags & CKKS_CYCLIC) return CKKS_ERR_BAD_ARG;
     if (set_device(c)) return CKKS_ERR_HIP;
     const DevTables t = c->tables();
@@ -788,7 +791,7 @@ int ckks_lincomb(const ckks_ctx* c, const uint64_t* E, const uint64_t* D, uint64
     if (nb < 1 || nb > 32 || ng < 0 || batch < 0 || l < 1 || l > c->K) return CKKS_ERR_BAD_SHAPE;
     if (!batch || !ng) return CKKS_OK;
     if (!E || !D || !out) return CKKS_ERR_NULL;
-    if (misaligned(E) || misaligned(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (misaligned(E) || misaligned_pairs(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
     HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), 0, S(stream)));
     return CKKS_OK;
@@ -800,7 +803,7 @@ int ckks_lincomb_scalar(const ckks_ctx* c, const uint64_t* E, const uint64_t* D,
     if (nb < 1 || nb > 32 || ng < 0 || batch < 0 || l < 1 || l > c->K) return CKKS_ERR_BAD_SHAPE;
     if (!batch || !ng) return CKKS_OK;
     if (!E || !D || !out) return CKKS_ERR_NULL;
-    if (misaligned(E) || misaligned(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (misaligned(E) || misaligned_pairs(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
     HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), 1, S(stream)));
     return CKKS_OK;

@@ -354,9 +354,8 @@ __global__ void __launch_bounds__(1024) k_automorphism_row(const uint64_t* in, u
     const int N = 1 << logN;
     const int r = blockIdx.x;
     const size_t off = (size_t)(r / n_limbs) * group_stride + (size_t)(r % n_limbs) * N;
-    const ulonglong2* src = reinterpret_cast<const ulonglong2*>(in + off);
-    ulonglong2* row2 = reinterpret_cast<ulonglong2*>(lds);
-    for (int k = threadIdx.x; k < N / 2; k += blockDim.x) row2[k] = src[k];
+    const uint64_t* src = in + off;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) lds[k] = src[k];
     __syncthreads();
     const uint64_t q = pcs[map.idx[r % n_limbs]].q;
     const uint64_t mask = 2 * (uint64_t)N - 1;

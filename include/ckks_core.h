@@ -9,8 +9,10 @@
  *
  * Conventions
  *  - All data pointers are caller-owned device pointers (hipMalloc or a
- *    torch tensor's data_ptr()), 8-byte aligned.  Nothing is allocated
- *    behind the caller's back except inside ckks_ctx_create.
+ *    torch tensor's data_ptr()), 8-byte aligned; buffers of (value, Shoup
+ *    companion) pairs -- prepared keys and prepared plaintexts -- 16-byte
+ *    aligned.  Nothing is allocated behind the caller's back except inside
+ *    ckks_ctx_create.
  *  - Polynomials are uint64 rows of N coefficients; RNS tensors are
  *    [poly][limb][N] with limb i reduced modulo prime (prime0 + i) of the
  *    context, or [batch][2][limb][N] for ciphertexts (c0, c1), decrypt =
@@ -37,7 +39,7 @@ typedef enum {
     CKKS_OK = 0,
     CKKS_ERR_BAD_N = -1,          /* N not a power of two in [2, 16384] */
     CKKS_ERR_BAD_MODULUS = -2,    /* not an odd prime < 2^45, or no root of the needed order */
-    CKKS_ERR_BAD_ALIGNMENT = -3,  /* device pointer not 8-byte aligned */
+    CKKS_ERR_BAD_ALIGNMENT = -3,  /* pointer not 8-byte aligned (pair buffers: 16) */
     CKKS_ERR_BAD_SHAPE = -4,      /* limb range outside the context, batch < 0, ... */
     CKKS_ERR_HIP = -5,            /* a HIP runtime call failed */
     CKKS_ERR_NULL = -6,           /* required pointer is NULL */

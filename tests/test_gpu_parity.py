@@ -360,6 +360,36 @@ def test_error_codes(gpu):
         ring.ntt_(x)                # 3 limbs on a 2-prime context
 
 
+def test_pair_buffers_need_16_byte_alignment(ck):
+    """Prepared keys and plaintext pairs are read as 16-byte pairs: a key or
+    plaintext pointer 8 bytes off is rejected before anything is launched,
+    while 8-byte alignment stays enough for ciphertext data."""
+    import ctypes
+    import torch
+    from cipherpu._lib import CkksError, call
+    c, sk = ck
+    rlk = c.relin_key(sk)
+    p = lambda t, off=0: ctypes.c_void_p(t.data_ptr() + off)
+    B, l = 1, c.L
+    d = torch.zeros(B, 3, l, c.N, dtype=torch.uint64, device=c.device)
+    out = torch.zeros(B, 2, l - 1, c.N, dtype=torch.uint64, device=c.device)
+    words = (call("ckks_workspace_bytes", c.ring.handle, B, l) + 7) // 8
+    ws = torch.zeros(words + 2, dtype=torch.uint64, device=c.device)
+    key = torch.zeros(rlk.prepared.numel() + 2, dtype=torch.uint64, device=c.device)
+    with pytest.raises(CkksError) as e:
+        call("ckks_hmult", c.ring.handle, p(d), p(d), p(key, 8), c.L + 1, p(out), B, l, c.sp, p(ws),
+             c.ring.stream())
+    assert e.value.code == -3
+    E = torch.zeros(1, B, 2, l, c.N, dtype=torch.uint64, device=c.device)
+    D = torch.zeros(1, 1, l, c.N, 2 + 1, dtype=torch.uint64, device=c.device)
+    o = torch.zeros(1, B, 2, l, c.N, dtype=torch.uint64, device=c.device)
+    with pytest.raises(CkksError) as e:
+        call("ckks_lincomb", c.ring.handle, p(E), p(D, 8), p(o), 1, 1, B, l, c.ring.stream())
+    assert e.value.code == -3
+    call("ckks_lincomb", c.ring.handle, p(E), p(D), p(o), 1, 1, B, l, c.ring.stream())   # aligned: accepted
+    torch.cuda.synchronize()
+
+
 def test_committed_golden_vectors(gpu):
     """The HIP path against tests/golden/golden_vectors.npz (made by
     tests/golden/make_golden.py from the oracle)."""
