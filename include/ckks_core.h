@@ -21,7 +21,11 @@
  *    ckks_ntt_fwd; CKKS_NATURAL_ORDER selects the reference's natural order.
  *  - Every call is asynchronous on the given stream (NULL = default stream)
  *    and returns 0 or a negative ckks_status.  Contexts are read-only after
- *    creation; calls on one stream run in order.
+ *    creation, so compute calls may share one context from several host
+ *    threads and streams; calls on one stream run in order.  The settings
+ *    calls (ckks_ctx_set_threads, ckks_ctx_set_front) and the timing hook
+ *    (ckks_timing_start/stop, which records into the context) change the
+ *    context: make them from one thread while no other call is using it.
  *  - Primes must be odd primes below 2^45 (the reference's 45-bit datapath,
  *    rtl/m1/ntt_unit.sv:38-51); 2N must divide q-1 for the negacyclic ring
  *    (N for the cyclic one).
