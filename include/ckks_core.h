@@ -24,8 +24,10 @@
  *    creation, so compute calls may share one context from several host
  *    threads and streams; calls on one stream run in order.  The settings
  *    calls (ckks_ctx_set_threads, ckks_ctx_set_front) and the timing hook
- *    (ckks_timing_start/stop, which records into the context) change the
- *    context: make them from one thread while no other call is using it.
+ *    change the context: make them from one thread while no other call is
+ *    using it.  Between ckks_timing_start and ckks_timing_stop the compute
+ *    calls record their launches into the context, so a timed region is
+ *    driven from one host thread.
  *  - Primes must be odd primes below 2^45 (the reference's 45-bit datapath,
  *    rtl/m1/ntt_unit.sv:38-51); 2N must divide q-1 for the negacyclic ring
  *    (N for the cyclic one).
