@@ -1,0 +1,30 @@
+"""The device modular arithmetic (modarith.h) compiled for the host and
+checked against exact 128-bit arithmetic: Shoup products (exact and lazy
+quotient), the lazy high half, Barrett, 64-bit reduction, the special-form
+product and both lazy butterflies on their documented bounds, for the chain
+primes, the special primes and the config-1 prime (tests/modarith_check.cpp).
+The checker itself was seen to fail on a planted error in the lazy high half."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "cipher-pu_amd", "csrc")
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    pytest.skip("hipcc not found")
+
+
+def test_modarith_against_128_bit_arithmetic(tmp_path):
+    exe = str(tmp_path / "modarith_check")
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(HERE, "modarith_check.cpp"), "-o", exe],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([exe, "200000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith(", 0 failures"), r.stdout
