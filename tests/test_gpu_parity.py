@@ -486,3 +486,38 @@ def test_rotate_hoisted_bit_exact_and_slots_move(ck):
                 assert np.array_equal(got[b], want), (l, st, b)
             dec = to_numpy(c.decode(c.decrypt(out, sk), out.scale))
             assert np.max(np.abs(dec.real - np.roll(z, -st, axis=1))) < 1e-5, (l, st)   # as above
+
+
+@pytest.mark.parametrize("N", [4096, 8192, 16384])
+def test_every_launch_shape_gives_the_same_words(gpu, N):
+    """HMult (fused and separate front), rotation and hoisted rotation under
+    every built thread count of the transform, tensor/front and key-switch
+    kernels give the same words as the default shape, whose HMult equals the
+    C oracle: the launch shape (single or paired transforms, 8 to 32
+    coefficients per thread) never changes a result."""
+    import itertools
+    from cipherpu.ckks import CKKS, CKKSParams
+    from cipherpu.ring import to_numpy
+    from cipherpu.tune import BUILT, BUILT_FUSED
+    c = CKKS(CKKSParams(N=N), seed=21)
+    sk = c.keygen()
+    rlk = c.relin_key(sk)
+    gks = [c.galois_key(sk, s) for s in (1, 3)]
+    rng = np.random.default_rng(N)
+    z = rng.uniform(-1, 1, (2, N // 2))
+    a, b = (c.encrypt(c.encode(v), sk) for v in (z, z[::-1].copy()))
+    run = lambda: (to_numpy(c.hmult(a, b, rlk).data), to_numpy(c.rotate(a, gks[0]).data),
+                   [to_numpy(r.data) for r in c.rotate_hoisted(a, gks)])
+    ref = run()
+    key = to_numpy(rlk.coeff)
+    pr, p, ps, pp = _oracle_args(c)
+    assert np.array_equal(ref[0][1], OC.hmult(to_numpy(a.data[1]), to_numpy(b.data[1]), key, pr, p, ps, pp))
+    logN = N.bit_length() - 1
+    for front, ntt, tensor, ks in itertools.product((0, 1), BUILT[logN], BUILT_FUSED[logN], BUILT_FUSED[logN]):
+        c.ring.set_front(bool(front))
+        c.ring.set_threads(ntt, ks, tensor)
+        got = run()
+        shape = (front, ntt, tensor, ks)
+        assert np.array_equal(got[0], ref[0]), ("hmult",) + shape
+        assert np.array_equal(got[1], ref[1]), ("rotate",) + shape
+        assert all(np.array_equal(g, r) for g, r in zip(got[2], ref[2])), ("hoisted",) + shape
