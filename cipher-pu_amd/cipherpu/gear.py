@@ -1,7 +1,7 @@
 """Gear selection for the encrypted network: the plain decrypt-based noise
 monitor (reference semantics) and the learned gear controller.
 
-Plain monitor (the reference's policy and judge):
+Plain monitor (the reference's policy and the arbiter of predictions):
   * noise energy E = sum min(c, q - c)^2 / n, r_hat = sqrt(E) / q
     (verify_m4_h_multilayer.py:162-174; rtl/m4/noise_monitor.sv:44-78),
     computed on the GPU by ckks_noise_stats;
@@ -333,7 +333,7 @@ def _plain_select(mlp, ct, Xc, target_db, start_gear):
 
 
 def plain_monitor_loop(mlp, X: np.ndarray, target_db: float, chunk_cts: int = 64, probe_every: int = 4) -> LoopResult:
-    """The plain decrypt-based monitor gear loop: every chunk is judged by the
+    """The plain decrypt-based monitor gear loop: every chunk is measured by the
     monitor; a chunk that misses the target is evaluated again one gear
     safer.  The search starts at the last accepted gear and, every
     `probe_every` chunks, one gear looser (the reference controller starts

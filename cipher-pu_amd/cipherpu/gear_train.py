@@ -7,7 +7,7 @@ encrypted MLP (nn.make_mlp(0)) or CNN (cnn.make_cnn(0)): its
 features (per-limb coefficient moments of the input ciphertext, level,
 scale, previous gear) and the output SNR that the plain decrypt-based
 monitor measures for that gear against the FP64 network.  A held-out
-sweep of fresh inputs gives the predictor's error, judged by the monitor.
+sweep of fresh inputs gives the predictor's error, as measured by the monitor.
 """
 from __future__ import annotations
 

@@ -3,7 +3,7 @@
 // predicts the output SNR of each sparsity gear from ciphertext statistics.
 // It never touches the modular arithmetic; the plain noise monitor
 // (rtl/m4/noise_monitor.sv, verify_m4h2_prod_calibration.py:64-82) stays the
-// fallback and the judge.
+// fallback and the arbiter of the predictions.
 //
 // Layouts: row-major, activations [B][features], weights [out][in].
 #include <hip/hip_runtime.h>

@@ -238,7 +238,7 @@ int ckks_noise_stats(const ckks_ctx *ctx, const uint64_t *data, uint64_t *stats,
 /* ---- learned gear controller (small dense network, FP32 weights,
  * activations optionally stored as BF16) ----
  * It predicts the output SNR of each sparsity gear from ciphertext
- * statistics; the plain noise monitor stays the fallback and the judge.  It
+ * statistics; the plain noise monitor stays the fallback and the arbiter.  It
  * has no counterpart in the reference (whose gear choice is the threshold
  * policy of rtl/m4/adaptive_gear_ctrl.sv:42-100).  Activations [B][I] /
  * [B][J] row-major, weights [J][I]; *_bf16 selects BF16 storage. */

@@ -212,7 +212,8 @@ def cyclic_conv_direct(a, b, N, q):
 
 
 # ---------------------------------------------------------------------------
-# negacyclic ring (scripts/verify_keyswitch.py:48-59; SURVEY R1 wrapper)
+# negacyclic ring (scripts/verify_keyswitch.py:48-59; the cyclic ntt of
+# tb/ref/gen_poly_mult_vectors_n12.py:68-82 wrapped with psi twists)
 # ---------------------------------------------------------------------------
 
 def negacyclic_ntt(a: Sequence[int], psi: int, N: int, q: int) -> List[int]:
@@ -310,7 +311,7 @@ def crt_split(values: Sequence[int], primes: Sequence[int]) -> List[List[int]]:
 def rns_rescale(limbs, primes):
     """Drop the last prime: out = [rescale_coeff(CRT(c), Q_l, Q_(l-1))]_(q_i)
     for i < l-1.  Big-int restatement of verify_rescale.py:38-53 on the CRT
-    value (SURVEY R5)."""
+    value."""
     Q_in = math.prod(primes)
     Q_out = Q_in // primes[-1]
     vals = crt_compose(limbs, primes)
@@ -342,7 +343,8 @@ def mod_down_round(limbs_qp, primes, p):
 
 
 # ---------------------------------------------------------------------------
-# hybrid key switching (generalises verify_keyswitch.relin)
+# hybrid key switching (generalises verify_keyswitch.relin,
+# scripts/verify_keyswitch.py:62-68)
 # ---------------------------------------------------------------------------
 
 def gadget_factor(primes, p, j, i):
