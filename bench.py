@@ -297,7 +297,7 @@ def main():
     ap.add_argument("--graph-ops", type=int, default=2048)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
+        print("bench.py measures one GPU: the library runs on a single MI355X and has no "
               "multi-GPU path (the batch shards over ciphertexts with no exchange).", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:

@@ -2,7 +2,7 @@
 // checked against exact 128-bit arithmetic, on each routine's stated input
 // domain, for random and extreme operands and every prime class the library
 // meets: the chain primes 2^44 + 32768 j + 1, the special primes, and the
-// config-1 prime just below 2^45.  Prints one line per failure (at most a
+// single-prime product's prime just below 2^45.  Prints one line per failure (at most a
 // few) and a summary; exit status 0 when everything holds.
 #include <cstdio>
 #include <cstdint>
@@ -141,7 +141,7 @@ int main(int argc, char** argv) {
     const uint64_t primes[] = {
         17592186634241ull, 17592186372097ull, 17592186273793ull, 17592186175489ull,   // chain4
         17592186765313ull, 17592187617281ull,                                         // special, chain5
-        35184371884033ull};                                                           // config 1, < 2^45
+        35184371884033ull};                                                           // largest 45-bit prime = 1 mod 8192
     for (uint64_t q : primes) check_prime(q, n);
     // special form q = 2^44 + c: chain primes and the largest c the routine allows
     for (uint64_t q : {17592186634241ull, 17592186175489ull, 17592187617281ull, (1ull << 44) + (1ull << 21) - 1})

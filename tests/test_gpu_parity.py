@@ -91,8 +91,8 @@ def test_ntt_edge_values(gpu):
 
 
 @pytest.mark.parametrize("threads", [0, 128, 512])
-def test_config1_product_bit_exact(gpu, threads):
-    """Config 1: N=4096, q = largest 45-bit prime = 1 mod 8192, psi =
+def test_single_prime_product_bit_exact(gpu, threads):
+    """Single-prime product: N=4096, q = largest 45-bit prime = 1 mod 8192, psi =
     smallest primitive 8192-th root; forward NTT -> point-wise -> INTT."""
     from cipherpu.ring import RingContext, to_numpy
     ring = RingContext(4096, [Q1], roots=[PSI1])

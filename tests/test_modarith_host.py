@@ -2,7 +2,8 @@
 checked against exact 128-bit arithmetic: Shoup products (exact and lazy
 quotient), the lazy high half, Barrett, 64-bit reduction, the special-form
 product and both lazy butterflies on their documented bounds, for the chain
-primes, the special primes and the config-1 prime (tests/modarith_check.cpp).
+primes, the special primes and the single-prime product's 45-bit prime
+(tests/modarith_check.cpp).
 The checker itself was seen to fail on a planted error in the lazy high half."""
 import os
 import shutil

@@ -94,7 +94,7 @@ def test_rns_twiddle_roms(fixtures, name, n):
     assert _digest(O.power_table(wi, n // 2, q)) == ti["sha256"]
 
 
-def test_config1_prime_and_psi():
+def test_single_prime_q_and_psi():
     q = O.largest_prime_1_mod(45, 8192)
     assert q == 35184371884033 and q < 1 << 45
     # v2(q-1) = 13: psi of order 8192 exists, 16384 does not
@@ -180,7 +180,7 @@ def test_c_oracle_transforms_match_python():
 
 def test_c_schoolbook_vs_ntt_4096():
     """The N=4096 dual-path check of gen_poly_mult_vectors_n12.py:123-143,
-    negacyclic form, on the config-1 prime."""
+    negacyclic form, on the single-prime product's prime."""
     q, psi = 35184371884033, 7388705119
     r = np.random.default_rng(1)
     a = r.integers(0, q, 4096, dtype=np.uint64)
