@@ -75,6 +75,7 @@ _SIGS = {
     "ckks_gaussian_cdf": (_I, [ctypes.c_double, _P, _I]),
     "ckks_workspace_hoisted_bytes": (_U, [_P, _I, _I, _I]),
     "ckks_rotate_hoisted": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, _I, _I, _P, _P]),
+    "ckks_rotate_hoisted_eval": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, _I, _I, _P, _P]),
     "ckks_key_permute": (_I, [_P, _P, _P, _I, _U, _P]),
     "ckks_timing_start": (_I, [_P, _I]),
     "ckks_timing_stop": (_I, [_P, _P, _P, _I]),

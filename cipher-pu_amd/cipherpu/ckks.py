@@ -291,11 +291,13 @@ class CKKS:
         return gk.hoisted
 
     def rotate_hoisted(self, ct: Ciphertext, gks: Sequence[SwitchKey], keys: Optional[torch.Tensor] = None,
-                       out: Optional[torch.Tensor] = None) -> List[Ciphertext]:
+                       out: Optional[torch.Tensor] = None, eval_form: bool = False) -> List[Ciphertext]:
         """Rotations of ct by every key of gks (at most 32) sharing one digit
         decomposition.  keys may hold the stacked hoisted keys (reused across
         calls); else they are stacked here.  out, if given, is a contiguous
-        [R, B, 2, l, N] tensor to write into."""
+        [R, B, 2, l, N] tensor to write into.  eval_form=True returns the
+        rotations in evaluation order (ModDown done in that domain), ready
+        for plaintext products."""
         B, l, R = ct.batch, ct.level, len(gks)
         if R == 0:
             return []
@@ -309,8 +311,8 @@ class CKKS:
         if self._ws is None or self._ws.numel() < words:
             self._ws = torch.empty(words, dtype=torch.uint64, device=self.device)
         gal = (ctypes.c_uint64 * R)(*[int(g.galois) for g in gks])
-        call("ckks_rotate_hoisted", self.ring.handle, _ptr(ct.data), _ptr(keys), self.L + 1, gal, R, _ptr(out), B, l,
-             self.sp, _ptr(self._ws), self.ring.stream())
+        call("ckks_rotate_hoisted_eval" if eval_form else "ckks_rotate_hoisted", self.ring.handle, _ptr(ct.data),
+             _ptr(keys), self.L + 1, gal, R, _ptr(out), B, l, self.sp, _ptr(self._ws), self.ring.stream())
         return [Ciphertext(out[r], ct.scale) for r in range(R)]
 
     def add(self, a: Ciphertext, b: Ciphertext) -> Ciphertext:

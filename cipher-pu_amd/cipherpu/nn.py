@@ -141,6 +141,7 @@ class EncryptedNet:
         self._gk: Dict[int, SwitchKey] = {}
         self._hkeys: Dict[tuple, torch.Tensor] = {}
         self.hoist = True          # baby steps / convolution taps as hoisted rotations
+        self.eval_rotations = True  # baby steps leave the hoisted call in evaluation form (MLP)
         self.ops = {"rot": 0, "ptmul_rows": 0, "hmult": 0, "dare": 0}
         self.gears = (0, 0)
 
@@ -161,19 +162,23 @@ class EncryptedNet:
         """A 1024-slot block pattern repeated for the S samples of a ciphertext."""
         return np.tile(v, self.S)
 
-    def rot_stack(self, ct: Ciphertext, steps: List[int]) -> torch.Tensor:
+    def rot_stack(self, ct: Ciphertext, steps: List[int], eval_form: bool = False) -> torch.Tensor:
         """[len(steps), B, 2, l, N]: ct rotated by every step (0 = ct itself),
         the nonzero steps as one hoisted call sharing a single digit
-        decomposition (up to 32 a call)."""
+        decomposition (up to 32 a call); in evaluation order when eval_form."""
         ck = self.ck
         out = torch.empty(len(steps), ct.batch, 2, ct.level, ck.N, dtype=torch.uint64, device=ck.device)
         nz = [i for i, s in enumerate(steps) if s % (ck.N // 2)]
         for i, s in enumerate(steps):
             if i not in nz:
                 out[i].copy_(ct.data)
+                if eval_form:
+                    ck.ring.ntt_(out[i].view(-1, ct.level, ck.N))
         if not self.hoist:
             for i in nz:
                 out[i].copy_(self.rot(ct, steps[i]).data)
+                if eval_form:
+                    ck.ring.ntt_(out[i].view(-1, ct.level, ck.N))
             return out
         for c0 in range(0, len(nz), 32):
             idx = nz[c0:c0 + 32]
@@ -182,9 +187,9 @@ class EncryptedNet:
                 self._hkeys[key] = torch.stack([ck.hoist_key(self.gk(s)) for s in key]).contiguous()
             gks = [self.gk(s) for s in key]
             if idx == list(range(idx[0], idx[0] + len(idx))):      # contiguous rows: write in place
-                ck.rotate_hoisted(ct, gks, self._hkeys[key], out=out[idx[0]:idx[0] + len(idx)])
+                ck.rotate_hoisted(ct, gks, self._hkeys[key], out=out[idx[0]:idx[0] + len(idx)], eval_form=eval_form)
             else:
-                for i, r in zip(idx, ck.rotate_hoisted(ct, gks, self._hkeys[key])):
+                for i, r in zip(idx, ck.rotate_hoisted(ct, gks, self._hkeys[key], eval_form=eval_form)):
                     out[i].copy_(r.data)
             self.ops["rot"] += ct.batch * len(idx)
         return out
@@ -311,8 +316,9 @@ class EncryptedMLP(EncryptedNet):
     def _linear(self, ct: Ciphertext, p: LayerPlan) -> Ciphertext:
         ck = self.ck
         B, l = ct.batch, ct.level
-        E = self.rot_stack(ct, p.used_b)                                        # [nb][B][2][l][N]
-        ck.ring.ntt_(E.view(-1, l, ck.N))
+        E = self.rot_stack(ct, p.used_b, eval_form=self.eval_rotations)         # [nb][B][2][l][N]
+        if not self.eval_rotations:
+            ck.ring.ntt_(E.view(-1, l, ck.N))
         nb, ng = len(p.used_b), len(p.used_g)
         acc = torch.empty(ng, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)
         call("ckks_lincomb", ck.ring.handle, _ptr(E), _ptr(p.D), _ptr(acc), nb, ng, B, l, ck.ring.stream())

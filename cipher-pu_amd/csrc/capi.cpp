@@ -690,9 +690,40 @@ int ckks_rotate_hoisted(const ckks_ctx* c, const uint64_t* ct, const uint64_t* k
     // one prepared key per rotation: [dnum = key_limbs - 1][2][key_limbs][N] pairs
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
     TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,
-                                                 l, m, special, R, c->ks_threads, S(stream)));
+                                                 l, m, special, R, 0, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch * R, l, m, special, mc, false,
                                                     S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_rotate_hoisted_eval(const ckks_ctx* c, const uint64_t* ct, const uint64_t* keys, int key_limbs,
+                             const uint64_t* galois, int R, uint64_t* out, int batch, int l, int special,
+                             void* workspace, void* stream) {
+    int st = ks_common_check(c, keys, key_limbs, out, batch, l, special, workspace);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!ct || !galois) return CKKS_ERR_NULL;
+    if (R < 1 || R > HOIST_MAX) return CKKS_ERR_BAD_SHAPE;
+    if (misaligned(ct)) return CKKS_ERR_BAD_ALIGNMENT;
+    GaloisList gl = {};
+    for (int r = 0; r < R; ++r) {
+        if (!(galois[r] & 1) || galois[r] >= 2 * c->N) return CKKS_ERR_BAD_ARG;
+        gl.g[r] = (uint32_t)galois[r];
+    }
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const LimbMap m = limb_map(0, l);
+    const ModDownConst mc = moddown_const(c, l, special);
+    const size_t LN = (size_t)l * c->N, B = (size_t)batch;
+    uint64_t* D = (uint64_t*)workspace;                       // [B][l][l+1][N]
+    uint64_t* cn = D + B * l * (l + 1) * c->N;                // [B][2][l][N]: ct in evaluation form
+    uint64_t* acc = cn + B * 2 * LN;                          // [R][B][2][l+1][N]
+    TIMED(CKKS_KIND_KEYSWITCH, launch_modup(t, ct, D, batch, l, m, special, c->ks_threads, S(stream)));
+    TIMED(CKKS_KIND_NTT, launch_ntt(t, cn, ct, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+    const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
+    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,
+                                                 l, m, special, R, 1, c->ks_threads, S(stream)));
+    TIMED(CKKS_KIND_MODDOWN, launch_moddown_eval(t, acc, cn, out, batch, l, m, special, mc, gl, R, c->ks_threads,
+                                                 S(stream)));
     return CKKS_OK;
 }
 

@@ -60,7 +60,8 @@ __global__ void __launch_bounds__(T) k_modup(const uint64_t* __restrict__ ct, ui
     ntt::store_eval<C>(x, lds, D + (((size_t)b * l + j) * (l + 1) + t) * C::N, tid);
 }
 
-// Reduce (and negate) an accumulator held at natural evaluation slots, apply
+// Reduce (and negate) an accumulator whose register k holds evaluation slot
+// k T + tid (the coalesced order the key products are summed in), apply
 // pi_g through LDS, inverse-transform and store it in coefficient form.
 template <class C>
 __device__ __forceinline__ void permute_inverse(uint64_t (&a)[C::E], bool negate, uint32_t g, uint64_t* lds,
@@ -69,7 +70,7 @@ __device__ __forceinline__ void permute_inverse(uint64_t (&a)[C::E], bool negate
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
         const uint64_t v = reduce64(a[k], pc);
-        lds[ntt::pad<C>(ntt::eval_index<C>(k, tid))] = (negate && v) ? pc.q - v : v;
+        lds[ntt::pad<C>(k * C::T + tid)] = (negate && v) ? pc.q - v : v;
     }
     __syncthreads();
 #pragma unroll
@@ -79,6 +80,26 @@ __device__ __forceinline__ void permute_inverse(uint64_t (&a)[C::E], bool negate
 #pragma unroll
     for (int k = 0; k < C::E; ++k) a[k] = csub(a[k], pc.q);
     ntt::store_coalesced<C>(a, out, tid);
+    __syncthreads();
+}
+
+// Reduce (and negate) an accumulator whose register k holds evaluation slot
+// k T + tid and
+// store pi_g of it in evaluation order: one LDS round trip, coalesced stores.
+template <class C>
+__device__ __forceinline__ void permute_store_eval(uint64_t (&a)[C::E], bool negate, uint32_t g, uint64_t* lds,
+                                                   const PrimeConst& pc, uint64_t* __restrict__ out, int tid) {
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const uint64_t v = reduce64(a[k], pc);
+        lds[ntt::pad<C>(k * C::T + tid)] = (negate && v) ? pc.q - v : v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+        const int j = e * C::T + tid;
+        out[j] = lds[ntt::pad<C>(eval_perm(j, g, C::LOGN))];
+    }
     __syncthreads();
 }
 
@@ -92,7 +113,7 @@ __device__ __forceinline__ void permute_inverse2(uint64_t (&a0)[C::E], uint64_t 
     uint64_t* lds1 = lds + C::LDS_WORDS;
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
-        const int slot = ntt::pad<C>(ntt::eval_index<C>(k, tid));
+        const int slot = ntt::pad<C>(k * C::T + tid);
         lds[slot] = reduce64(a0[k], pc);
         const uint64_t v = reduce64(a1[k], pc);
         lds1[slot] = v ? pc.q - v : 0;
@@ -115,11 +136,14 @@ __device__ __forceinline__ void permute_inverse2(uint64_t (&a0)[C::E], uint64_t 
     ntt::store_coalesced<C>(a1, out1, tid);
 }
 
-// One workgroup per (target t, ciphertext b, rotation r), rotations of one
-// (t, b) adjacent so that the digits stay in L2.  keys: [R][dnum][2][key_limbs][N]
+// One workgroup per (target t, ciphertext b, rotation r), the ciphertexts of
+// one (t, r) adjacent: workgroups in flight share a few key slices (1 MB
+// each at N = 8192, l = 4), which stay in L2 and are read by every
+// ciphertext, while each workgroup's digits (l rows) are read once.
+// keys: [R][dnum][2][key_limbs][N]
 // (value, companion) pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N]
 // coefficient form, the layout ckks_relin's ModDown reads.
-template <int LOGN, int T>
+template <int LOGN, int T, bool EVAL>
 __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D, const ulonglong2* __restrict__ keys,
                                                   size_t key_stride, int key_limbs, GaloisList galois,
                                                   uint64_t* __restrict__ acc, const ulonglong2* __restrict__ tw_all,
@@ -128,8 +152,8 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
-    const int r = blockIdx.x % R;
-    const int b = (blockIdx.x / R) % batch;
+    const int b = blockIdx.x % batch;
+    const int r = (blockIdx.x / batch) % R;
     const int t = blockIdx.x / (R * batch);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
@@ -145,7 +169,7 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
         const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
-            const int idx = ntt::eval_index<C>(k, tid);
+            const int idx = k * C::T + tid;          // coalesced: the sum is slot by slot
             const uint64_t x = d[idx];
             const ulonglong2 wb = kb[idx], wa = ka[idx];
             a0[k] += mul_shoup4_nq(x, wb.x, wb.y, nq);   // < 4q each, l <= 15 digits
@@ -155,11 +179,66 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
     const uint32_t g = galois.g[r];
     const ulonglong2* itw = tw_all + ((size_t)pt * 2 + 1) * C::N;
     uint64_t* out = acc + (((size_t)r * batch + b) * 2 * (l + 1) + t) * C::N;
+    if (EVAL && t < l) {
+        // chain limbs stay in evaluation form; ModDown finishes there
+        permute_store_eval<C>(a0, false, g, lds, pc, out, tid);
+        permute_store_eval<C>(a1, true, g, lds, pc, out + (size_t)(l + 1) * C::N, tid);
+        return;
+    }
     if constexpr (ntt::pair_pays<C>()) {
         permute_inverse2<C>(a0, a1, g, lds, itw, pc, out, out + (size_t)(l + 1) * C::N, tid);
     } else {
         permute_inverse<C>(a0, false, g, lds, itw, pc, out, tid);
         permute_inverse<C>(a1, true, g, lds, itw, pc, out + (size_t)(l + 1) * C::N, tid);
+    }
+}
+
+// ModDown in the evaluation domain for hoisted rotations, one workgroup per
+// (rotation r, ciphertext b, poly, limb i):
+//   out_i = (X_i - NTT_i(w)) p^-1 (+ pi_g(NTT_i(c0)) for poly 0),
+//   w = [(x_p + h) mod p] - h (mod q_i), h = (p - 1)/2,
+// the evaluation-domain image of the coefficient ModDown
+// ((x_i + h) - [(x_p + h) mod p]) p^-1 (round half up of x / p): X_i is the
+// chain-limb accumulator left in evaluation form and x_p the special limb in
+// coefficient form.  acc: [R][B][2][l+1][N]; Cn: the ciphertexts in
+// evaluation form [B][2][l][N]; out: [R][B][2][l][N] in evaluation order.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_moddown_eval(const uint64_t* __restrict__ acc, const uint64_t* __restrict__ Cn,
+                                                    uint64_t* __restrict__ out, const ulonglong2* __restrict__ tw_all,
+                                                    const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
+                                                    ModDownConst mc, GaloisList galois, int batch) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x % l;
+    const int row = blockIdx.x / l;                 // ((r batch + b) 2 + poly)
+    const int poly = row & 1, rb = row >> 1;
+    const int b = rb % batch, r = rb / batch;
+    const int p = map.idx[i];
+    const PrimeConst pc = pcs[p];
+    const uint64_t qp = pcs[pidx_p].q, hp = mc.hp[i];
+    const uint64_t* arow = acc + (size_t)row * (l + 1) * C::N;
+    uint64_t x[C::E];
+    ntt::load_coalesced<C>(x, arow + (size_t)l * C::N, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = sub_mod(reduce64(csub(x[e] + (qp >> 1), qp), pc), hp, pc.q);
+    ntt::fwd_ntt<C>(x, lds, tw_all + (size_t)p * 2 * C::N, pc.q, pc.two_q, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = reduce64(x[e], pc);
+    __syncthreads();
+    ntt::to_lds<C, C::NP - 1>(x, lds, tid);        // NTT_i(w) at its evaluation slots
+    __syncthreads();
+    const uint64_t* X = arow + (size_t)i * C::N;
+    const uint64_t* c0 = Cn + ((size_t)b * 2 * l + i) * C::N;
+    const uint32_t g = galois.g[r];
+    uint64_t* o = out + ((size_t)row * l + i) * C::N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+        const int j = e * C::T + tid;
+        uint64_t v = sub_mod(X[j], lds[ntt::pad<C>(j)], pc.q);
+        v = csub(mul_shoup(v, mc.pinv[i], mc.pinv_s[i], pc.q), pc.q);
+        if (poly == 0) v = add_mod(v, c0[eval_perm(j, g, C::LOGN)], pc.q);
+        o[j] = v;
     }
 }
 
@@ -192,13 +271,26 @@ hipError_t run_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int ba
 template <int LOGN, int T>
 hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
                           int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l, const LimbMap& map,
-                          int pidx_p, int R, hipStream_t s) {
+                          int pidx_p, int R, int eval_out, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
-    hipError_t e = set_lds(k_ks_hoisted<LOGN, T>, lds);
+    auto k = eval_out ? k_ks_hoisted<LOGN, T, true> : k_ks_hoisted<LOGN, T, false>;
+    hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ks_hoisted<LOGN, T>), dim3((l + 1) * batch * R), dim3(T), lds, s, D, keys,
-                       key_stride, key_limbs, galois, acc, t.tw, t.pc, map, l, pidx_p, batch, R);
+    hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, keys, key_stride, key_limbs, galois, acc,
+                       t.tw, t.pc, map, l, pidx_p, batch, R);
+    return hipGetLastError();
+}
+
+template <int LOGN, int T>
+hipError_t run_moddown_eval(const DevTables& t, const uint64_t* acc, const uint64_t* Cn, uint64_t* out, int batch,
+                            int l, const LimbMap& map, int pidx_p, const ModDownConst& mc, const GaloisList& galois,
+                            int R, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_moddown_eval<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_moddown_eval<LOGN, T>), dim3(R * batch * 2 * l), dim3(T), C::LDS_BYTES, s, acc, Cn, out,
+                       t.tw, t.pc, map, l, pidx_p, mc, galois, batch);
     return hipGetLastError();
 }
 
@@ -221,16 +313,32 @@ hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int
 
 hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
-                             const LimbMap& map, int pidx_p, int R, int threads, hipStream_t s) {
+                             const LimbMap& map, int pidx_p, int R, int eval_out, int threads, hipStream_t s) {
     if (!batch || !R) return hipSuccess;
     switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : t.logN == 13 ? 1024 : 512))) {
-        case 120256: return run_ks_hoisted<12, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
-        case 120512: return run_ks_hoisted<12, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
-        case 130256: return run_ks_hoisted<13, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
-        case 130512: return run_ks_hoisted<13, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
-        case 131024: return run_ks_hoisted<13, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
-        case 140512: return run_ks_hoisted<14, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
-        case 141024: return run_ks_hoisted<14, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, s);
+        case 120256: return run_ks_hoisted<12, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        case 120512: return run_ks_hoisted<12, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        case 130256: return run_ks_hoisted<13, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        case 130512: return run_ks_hoisted<13, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        case 131024: return run_ks_hoisted<13, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        case 140512: return run_ks_hoisted<14, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        case 141024: return run_ks_hoisted<14, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_moddown_eval(const DevTables& t, const uint64_t* acc, const uint64_t* Cn, uint64_t* out, int batch,
+                               int l, const LimbMap& map, int pidx_p, const ModDownConst& mc,
+                               const GaloisList& galois, int R, int threads, hipStream_t s) {
+    if (!batch || !R) return hipSuccess;
+    switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : t.logN == 13 ? 1024 : 512))) {
+        case 120256: return run_moddown_eval<12, 256>(t, acc, Cn, out, batch, l, map, pidx_p, mc, galois, R, s);
+        case 120512: return run_moddown_eval<12, 512>(t, acc, Cn, out, batch, l, map, pidx_p, mc, galois, R, s);
+        case 130256: return run_moddown_eval<13, 256>(t, acc, Cn, out, batch, l, map, pidx_p, mc, galois, R, s);
+        case 130512: return run_moddown_eval<13, 512>(t, acc, Cn, out, batch, l, map, pidx_p, mc, galois, R, s);
+        case 131024: return run_moddown_eval<13, 1024>(t, acc, Cn, out, batch, l, map, pidx_p, mc, galois, R, s);
+        case 140512: return run_moddown_eval<14, 512>(t, acc, Cn, out, batch, l, map, pidx_p, mc, galois, R, s);
+        case 141024: return run_moddown_eval<14, 1024>(t, acc, Cn, out, batch, l, map, pidx_p, mc, galois, R, s);
         default: return hipErrorInvalidValue;
     }
 }

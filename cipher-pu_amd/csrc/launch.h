@@ -118,9 +118,15 @@ struct GaloisList {
 
 hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int batch, int l, const LimbMap& map,
                         int pidx_p, int threads, hipStream_t s);
+// eval_out = 1: the chain-limb accumulators are stored in evaluation form
+// (pi_g applied, no inverse transform) for launch_moddown_eval; the special
+// limb's in coefficient form as always.
 hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
-                             const LimbMap& map, int pidx_p, int R, int threads, hipStream_t s);
+                             const LimbMap& map, int pidx_p, int R, int eval_out, int threads, hipStream_t s);
+hipError_t launch_moddown_eval(const DevTables& t, const uint64_t* acc, const uint64_t* Cn, uint64_t* out, int batch,
+                               int l, const LimbMap& map, int pidx_p, const ModDownConst& mc,
+                               const GaloisList& galois, int R, int threads, hipStream_t s);
 hipError_t launch_key_permute(const DevTables& t, const ulonglong2* in, ulonglong2* out, int rows, uint32_t g,
                               hipStream_t s);
 

@@ -164,6 +164,16 @@ uint64_t ckks_workspace_hoisted_bytes(const ckks_ctx *ctx, int batch, int l, int
 int ckks_rotate_hoisted(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *keys, int key_limbs,
                         const uint64_t *galois, int R, uint64_t *out, int batch, int l, int special, void *workspace,
                         void *stream);
+/* ckks_rotate_hoisted with the outputs in evaluation (bit-reversed NTT)
+ * order, for callers whose next step is a plaintext product (no counterpart
+ * in the reference).  ModDown runs in the evaluation domain: only the
+ * special-prime limb is transformed back, so each rotated ciphertext costs
+ * 2 inverse and 2 l forward transforms instead of 2 (l + 1) inverse ones
+ * plus the caller's 2 l forward ones.  The words equal ckks_ntt_fwd of
+ * ckks_rotate_hoisted's.  Same arguments and workspace. */
+int ckks_rotate_hoisted_eval(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *keys, int key_limbs,
+                             const uint64_t *galois, int R, uint64_t *out, int batch, int l, int special,
+                             void *workspace, void *stream);
 /* key_out = key_in (prepared pairs, rows of N) with every row permuted by
  * the inverse evaluation-domain automorphism of galois, for ckks_rotate_hoisted. */
 int ckks_key_permute(const ckks_ctx *ctx, const uint64_t *key_in, uint64_t *key_out, int rows, uint64_t galois,

@@ -507,7 +507,8 @@ def test_every_launch_shape_gives_the_same_words(gpu, N):
     z = rng.uniform(-1, 1, (2, N // 2))
     a, b = (c.encrypt(c.encode(v), sk) for v in (z, z[::-1].copy()))
     run = lambda: (to_numpy(c.hmult(a, b, rlk).data), to_numpy(c.rotate(a, gks[0]).data),
-                   [to_numpy(r.data) for r in c.rotate_hoisted(a, gks)])
+                   [to_numpy(r.data) for r in c.rotate_hoisted(a, gks)] +
+                   [to_numpy(r.data) for r in c.rotate_hoisted(a, gks, eval_form=True)])
     ref = run()
     key = to_numpy(rlk.coeff)
     pr, p, ps, pp = _oracle_args(c)
@@ -521,3 +522,24 @@ def test_every_launch_shape_gives_the_same_words(gpu, N):
         assert np.array_equal(got[0], ref[0]), ("hmult",) + shape
         assert np.array_equal(got[1], ref[1]), ("rotate",) + shape
         assert all(np.array_equal(g, r) for g, r in zip(got[2], ref[2])), ("hoisted",) + shape
+
+
+@pytest.mark.parametrize("N", [4096, 8192, 16384])
+def test_rotate_hoisted_eval_form_is_ntt_of_coefficient_form(gpu, N):
+    """Hoisted rotations with ModDown in the evaluation domain give the
+    forward transform of the coefficient-form hoisted rotations, word for
+    word (the evaluation-domain ModDown is the image of the coefficient one
+    under the linear map NTT_i), at every level and for several steps."""
+    from cipherpu.ckks import CKKS, CKKSParams
+    from cipherpu.ring import to_numpy
+    c = CKKS(CKKSParams(N=N), seed=31)
+    sk = c.keygen()
+    gks = [c.galois_key(sk, s) for s in (1, 5, N // 4 - 1)]
+    rng = np.random.default_rng(N + 7)
+    for l in (4, 3, 1):
+        ct = c.encrypt(c.encode(rng.uniform(-1, 1, (3, N // 2)), level=l), sk)
+        coef = c.rotate_hoisted(ct, gks)
+        ev = c.rotate_hoisted(ct, gks, eval_form=True)
+        for r in range(len(gks)):
+            want = to_numpy(c.ring.ntt(coef[r].data.reshape(-1, l, N)))
+            assert np.array_equal(to_numpy(ev[r].data).reshape(-1, l, N), want), (N, l, r)
