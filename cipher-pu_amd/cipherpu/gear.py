@@ -55,8 +55,17 @@ def ciphertext_features(ck, ct, prev_gear: int, layer: int = 0) -> torch.Tensor:
     limbs 0..3 (8 values), mean(c')/q of c0 and c1 on limbs 0..1 (4), then
     level / L, log2(scale) / 64, the previous gear / 3 and the layer index.
     Limbs above the ciphertext's level read 0."""
-    B, l = ct.batch, ct.level
-    m = ck.ring.noise_moments(ct.data).view(B, 2, l, 4)
+    return features_many(ck, [ct], prev_gear, layer)
+
+
+def features_many(ck, cts, prev_gear: int, layer: int = 0) -> torch.Tensor:
+    """ciphertext_features of several ciphertexts of one level and scale,
+    rows concatenated, computed in one pass."""
+    l, scale = cts[0].level, cts[0].scale
+    if any(ct.level != l or ct.scale != scale for ct in cts):
+        return torch.cat([features_many(ck, [ct], prev_gear, layer) for ct in cts])
+    B = sum(ct.batch for ct in cts)
+    m = ck.ring.noise_moments_many([ct.data for ct in cts]).view(B, 2, l, 4)
     f = torch.zeros(B, N_FEATURES, dtype=torch.float64, device=m.device)
     k = min(l, 4)
     f[:, 0:k] = m[:, 0, :k, 1]
@@ -65,7 +74,7 @@ def ciphertext_features(ck, ct, prev_gear: int, layer: int = 0) -> torch.Tensor:
     f[:, 8:8 + k2] = m[:, 0, :k2, 0]
     f[:, 10:10 + k2] = m[:, 1, :k2, 0]
     f[:, 12] = l / ck.L
-    f[:, 13] = math.log2(ct.scale) / 64.0
+    f[:, 13] = math.log2(scale) / 64.0
     f[:, 14] = prev_gear / 3.0
     f[:, 15] = float(layer)
     return f.float()
@@ -358,27 +367,33 @@ def plain_monitor_loop(mlp, X: np.ndarray, target_db: float, chunk_cts: int = 64
     return LoopResult(res, gears, evals, mon, 0, dt, X.shape[0])
 
 
+def _gear_table(mlp, net: GearNet, chunks) -> np.ndarray:
+    """[previous gear, chunk, gear] mean predicted SNR: one batched
+    prediction for every chunk and every previous-gear value (the only
+    feature the decisions feed back into), with a single device sync."""
+    feats = features_many(mlp.ck, [ct for _, ct in chunks], 0)
+    n = feats.shape[0]
+    f = feats.repeat(N_GEARS, 1)
+    f[:, 14] = torch.arange(N_GEARS, device=f.device, dtype=f.dtype).repeat_interleave(n) / 3.0
+    pred = net.predict(f).view(N_GEARS, n, N_GEARS)
+    sizes = torch.tensor([ct.batch for _, ct in chunks], device=f.device)
+    seg = torch.repeat_interleave(torch.arange(len(chunks), device=f.device), sizes)
+    sums = torch.zeros(N_GEARS, len(chunks), N_GEARS, device=f.device, dtype=pred.dtype).index_add_(1, seg, pred)
+    return (sums / sizes.view(1, -1, 1).to(pred.dtype)).cpu().numpy()
+
+
 def learned_loop(mlp, net: GearNet, X: np.ndarray, target_db: float, chunk_cts: int = 64,
                  margin_db: float = MARGIN_DB) -> LoopResult:
     """The learned gear loop: per chunk, on-device features -> predicted SNR
     of every gear -> the cheapest gear that clears target + margin; when no
     gear clears it, the plain monitor search decides the chunk."""
     chunks = _setup(mlp, X, chunk_cts)
-    net.predict(ciphertext_features(mlp.ck, chunks[0][1], 0))   # warm the feature and network kernels
+    _gear_table(mlp, net, chunks[:2])            # warm the feature and network kernels
     outs, gears, evals, mon, fb = [], [], 0, 0, 0
     prev = 0
     _sync(mlp)
     t0 = time.perf_counter()
-    # one batched prediction for every chunk and every previous-gear value
-    # (the only feature the decisions feed back into): a single device sync
-    feats = torch.cat([ciphertext_features(mlp.ck, ct, 0) for _, ct in chunks])
-    sizes = [ct.batch for _, ct in chunks]
-    table = []
-    for pg in range(N_GEARS):
-        f = feats.clone()
-        f[:, 14] = pg / 3.0
-        table.append(torch.stack([p.mean(0) for p in torch.split(net.predict(f), sizes)]))
-    table = torch.stack(table).cpu().numpy()              # [prev gear, chunk, gear]
+    table = _gear_table(mlp, net, chunks)
     for i, (Xc, ct) in enumerate(chunks):
         pred = table[prev, i]
         g = choose_gear(pred, target_db, margin_db)

@@ -194,14 +194,23 @@ class RingContext:
         rms c' / q, max c' / q, mean centred c / q], c' = min(c, q - c)
         (the monitor's energy E = sum c'^2 / N gives rms = sqrt(E);
         rtl/m4/noise_monitor.sv:44-78)."""
-        p, l = self._rows(x, prime0)
-        st = torch.empty(p * l, 5, dtype=torch.uint64, device=self.device)
-        call("ckks_noise_stats", self._h, _ptr(x), _ptr(st), p, l, prime0, self.stream())
+        return self.noise_moments_many([x], prime0)
+
+    def noise_moments_many(self, xs, prime0: int = 0) -> torch.Tensor:
+        """noise_moments of several tensors, rows concatenated in order: one
+        statistics launch per tensor into a shared buffer and one conversion
+        (the gear controller's features for many chunks at once)."""
+        shapes = [self._rows(x, prime0) for x in xs]
+        st = torch.empty(sum(p * l for p, l in shapes), 5, dtype=torch.uint64, device=self.device)
+        qs, off = [], 0
+        for x, (p, l) in zip(xs, shapes):
+            call("ckks_noise_stats", self._h, _ptr(x), _ptr(st[off:off + p * l]), p, l, prime0, self.stream())
+            qs += [float(self.primes[prime0 + i]) for i in range(l)] * p
+            off += p * l
         v = st.view(torch.int64).to(torch.float64)
         v = torch.where(v < 0, v + 2.0 ** 64, v)          # unsigned words
         sg = st[:, 4].view(torch.int64).to(torch.float64)  # signed sum
-        q = torch.tensor([float(self.primes[prime0 + i]) for i in range(l)], dtype=torch.float64,
-                         device=self.device).repeat(p)
+        q = torch.tensor(qs, dtype=torch.float64, device=self.device)
         n = float(self.N)
         out = torch.stack([v[:, 0] / n / q, torch.sqrt((v[:, 1] + v[:, 2] * 2.0 ** 64) / n) / q, v[:, 3] / q,
                            sg / n / q], 1)
