@@ -267,7 +267,7 @@ def graph_leg(n_ops: int, reps: int = 5):
     kinds = [o.kind for o in ops]
     return {"workload": "mixed N 4096/8192, levels 1..4, 60/30/10 hmult/rotate/ptmul", "ops": n_ops,
             "ops_per_s": round(n_ops / (graph_ms / 1e3), 1), "ms_per_batch": round(graph_ms, 3),
-            "eager_ms_per_batch": round(eager_ms, 3), "buckets": len(s.buckets),
+            "eager_ms_per_batch": round(eager_ms, 3), "buckets": len(s.buckets), "streams": s.n_streams,
             "kinds": {k: kinds.count(k) for k in ("hmult", "rotate", "ptmul")}, "tune_s": round(t_tune, 1)}
 
 

@@ -11,8 +11,12 @@ plaintext-ciphertext product.  The scheduler
   * chooses a launch shape per bucket with the tuner (threads per
     workgroup of the transform, tensor and key-switch kernels, i.e. the
     radix of the register passes);
+  * spreads the buckets over a few streams (greedy by their tuned time),
+    each with its own workspace, so that small buckets run side by side
+    instead of each leaving most of the GPU idle;
   * captures every bucket's launches, each under its own launch shape, into
-    one HIP graph, replayed as a single launch.
+    one HIP graph (the streams fork from and join back into the capture
+    stream), replayed as a single launch.
 
 Level semantics: a ciphertext at level l has l rescales left and l + 1 RNS
 limbs over the primes CHAIN5 (cipherpu.params), plus the special prime for
@@ -75,6 +79,7 @@ class Bucket:
     out: torch.Tensor = None
     shape: Tuple[int, int, int] = (0, 0, 0)  # (ntt, ks, tensor) threads
     ms: float = 0.0
+    stream: int = 0                         # index of the scheduler stream it runs on
 
     @property
     def limbs(self) -> int:
@@ -86,8 +91,10 @@ class Bucket:
 
 
 class BatchScheduler:
-    def __init__(self, ops: List[Op], seed: int = 0, device: int = 0):
+    def __init__(self, ops: List[Op], seed: int = 0, device: int = 0, streams: int = 4):
         self.ops = ops
+        self.n_streams = max(1, streams)
+        self._streams: List[torch.cuda.Stream] = []
         self.ck: Dict[int, CKKS] = {}
         self.sk: Dict[int, SecretKey] = {}
         self.rlk: Dict[int, SwitchKey] = {}
@@ -104,7 +111,8 @@ class BatchScheduler:
             groups.setdefault((o.N, o.level, o.kind, o.step), []).append(i)
         self.buckets = [Bucket(N, lv, k, st, idx) for (N, lv, k, st), idx in sorted(groups.items())]
         self.graph: Optional[torch.cuda.CUDAGraph] = None
-        self._ws: Dict[int, torch.Tensor] = {}
+        self._ws: Dict[Tuple[int, int], torch.Tensor] = {}
+        self.assign_streams()
 
     # -- data -------------------------------------------------------------
     def fill_random(self, seed: int = 1) -> None:
@@ -124,34 +132,66 @@ class BatchScheduler:
                 bk.b = pt.unsqueeze(1).expand(B, 2, l, bk.N).contiguous().view(B * 2, l, bk.N)
                 bk.out = torch.empty_like(bk.a)
 
-    def _workspace(self, N: int) -> torch.Tensor:
-        if N not in self._ws:
+    def _workspace(self, N: int, stream: int = 0) -> torch.Tensor:
+        """One workspace per (ring degree, stream): buckets that may run at the
+        same time never share one."""
+        if (N, stream) not in self._ws:
             ck = self.ck[N]
             need = max(call("ckks_workspace_bytes", ck.ring.handle, bk.batch, bk.limbs)
                        for bk in self.buckets if bk.N == N)
-            self._ws[N] = torch.empty((need + 7) // 8, dtype=torch.uint64, device=ck.device)
-        return self._ws[N]
+            self._ws[(N, stream)] = torch.empty((need + 7) // 8, dtype=torch.uint64, device=ck.device)
+        return self._ws[(N, stream)]
+
+    def assign_streams(self) -> None:
+        """Longest-first greedy placement of the buckets on the streams, by
+        tuned time (or, before tuning, by an estimate of the work)."""
+        load = [0.0] * self.n_streams
+        cost = lambda bk: bk.ms if bk.ms > 0 else bk.batch * bk.N * bk.limbs ** 2
+        for bk in sorted(self.buckets, key=cost, reverse=True):
+            k = min(range(self.n_streams), key=load.__getitem__)
+            bk.stream = k
+            load[k] += cost(bk)
 
     # -- launches ---------------------------------------------------------
-    def _launch(self, bk: Bucket) -> None:
+    def _launch(self, bk: Bucket, stream: int = 0) -> None:
         ck = self.ck[bk.N]
         r = ck.ring
         n, k, t = bk.shape
         r.set_threads(n, k, t)
         l, B, K = bk.limbs, bk.batch, ck.L + 1
+        ws = self._workspace(bk.N, stream)
         if bk.kind == "hmult":
             call("ckks_hmult", r.handle, _ptr(bk.a), _ptr(bk.b), _ptr(self.rlk[bk.N].prepared), K, _ptr(bk.out), B,
-                 l, ck.sp, _ptr(self._workspace(bk.N)), r.stream())
+                 l, ck.sp, _ptr(ws), r.stream())
         elif bk.kind == "rotate":
             gk = self.gk[(bk.N, bk.step)]
             call("ckks_rotate", r.handle, _ptr(bk.a), _ptr(gk.prepared), K, _ptr(bk.out), B, l, ck.sp, gk.galois,
-                 _ptr(self._workspace(bk.N)), r.stream())
+                 _ptr(ws), r.stream())
         else:
             call("ckks_poly_mul", r.handle, _ptr(bk.a), _ptr(bk.b), _ptr(bk.out), B * 2, l, 0, r.stream())
 
+    def _side_streams(self) -> List[torch.cuda.Stream]:
+        if len(self._streams) != self.n_streams:
+            dev = self.ck[next(iter(self.ck))].device
+            self._streams = [torch.cuda.Stream(dev) for _ in range(self.n_streams)]   # default priority
+        return self._streams
+
     def run_eager(self) -> None:
+        """Every bucket on its stream: the streams fork from the current
+        stream and join back into it."""
+        if self.n_streams == 1:
+            for bk in self.buckets:
+                self._launch(bk)
+            return
+        cur = torch.cuda.current_stream()
+        side = self._side_streams()
+        for s in side:
+            s.wait_stream(cur)
         for bk in self.buckets:
-            self._launch(bk)
+            with torch.cuda.stream(side[bk.stream]):
+                self._launch(bk, bk.stream)
+        for s in side:
+            cur.wait_stream(s)
 
     # -- tuning -----------------------------------------------------------
     def tune(self, reps: int = 3) -> None:
@@ -174,12 +214,15 @@ class BatchScheduler:
                 if best is None or ms < best[1]:
                     best = (c, ms)
             bk.shape, bk.ms = best
+        self.assign_streams()
 
     # -- graph ------------------------------------------------------------
     def capture(self) -> torch.cuda.CUDAGraph:
         """Record every bucket's launches into one HIP graph."""
         for N in self.ck:
-            self._workspace(N)
+            for k in range(self.n_streams):
+                self._workspace(N, k)
+        self._side_streams()
         s = torch.cuda.Stream(self.ck[next(iter(self.ck))].device)
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):          # warm-up outside capture
