@@ -42,6 +42,7 @@ _SIGS = {
     "ckks_ctx_root": (_I, [_P, _I, ctypes.POINTER(_U)]),
     "ckks_ctx_set_threads": (_I, [_P, _I, _I, _I]),
     "ckks_ctx_set_front": (_I, [_P, _I]),
+    "ckks_ctx_set_hoist_order": (_I, [_P, _I, _I]),
     "ckks_version": (ctypes.c_char_p, []),
     "ckks_ntt_fwd": (_I, [_P, _P, _I, _I, _I, _I, _P]),
     "ckks_ntt_inv": (_I, [_P, _P, _I, _I, _I, _I, _P]),

@@ -79,6 +79,10 @@ class RingContext:
     def set_front(self, fused: bool = True):
         call("ckks_ctx_set_front", self._h, 1 if fused else 0)
 
+    def set_hoist_order(self, btile: int = 0, xcd: bool = False):
+        """Workgroup placement of the hoisted key switch (ckks_ctx_set_hoist_order)."""
+        call("ckks_ctx_set_hoist_order", self._h, btile, 1 if xcd else 0)
+
     # -- helpers ---------------------------------------------------------
     def stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)

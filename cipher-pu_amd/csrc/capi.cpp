@@ -93,6 +93,7 @@ struct ckks_ctx {
     int* slot_k = nullptr;        // (5^j mod 2N - 1)/4
     int ntt_threads = 0, ks_threads = 0, tensor_threads = 0;
     int front_fused = 0;   // HMult front: one fused kernel (1) or 2 transforms + tensor (0)
+    HoistOrder hoist = {0, 0};   // hoisted key-switch placement; btile 0 = default for the output form
     // Optional per-launch timing (ckks_timing_start/stop): an event pair
     // around every kernel the entry points launch, tagged by kernel kind.
     struct Timer {
@@ -357,6 +358,13 @@ int ckks_ctx_root(const ckks_ctx* c, int prime, uint64_t* root) {
 int ckks_ctx_set_front(ckks_ctx* c, int fused) {
     if (!c) return CKKS_ERR_NULL;
     c->front_fused = fused ? 1 : 0;
+    return CKKS_OK;
+}
+
+int ckks_ctx_set_hoist_order(ckks_ctx* c, int btile, int xcd) {
+    if (!c) return CKKS_ERR_NULL;
+    if (btile < 0) return CKKS_ERR_BAD_ARG;
+    c->hoist = HoistOrder{btile, xcd ? 1 : 0};
     return CKKS_OK;
 }
 
@@ -690,7 +698,7 @@ int ckks_rotate_hoisted(const ckks_ctx* c, const uint64_t* ct, const uint64_t* k
     // one prepared key per rotation: [dnum = key_limbs - 1][2][key_limbs][N] pairs
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
     TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,
-                                                 l, m, special, R, 0, c->ks_threads, S(stream)));
+                                                 l, m, special, R, 0, c->hoist, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch * R, l, m, special, mc, false,
                                                     S(stream)));
     return CKKS_OK;
@@ -721,7 +729,7 @@ int ckks_rotate_hoisted_eval(const ckks_ctx* c, const uint64_t* ct, const uint64
     TIMED(CKKS_KIND_NTT, launch_ntt(t, cn, ct, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
     TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,
-                                                 l, m, special, R, 1, c->ks_threads, S(stream)));
+                                                 l, m, special, R, 1, c->hoist, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_eval(t, acc, cn, out, batch, l, m, special, mc, gl, R, c->ks_threads,
                                                  S(stream)));
     return CKKS_OK;

@@ -263,9 +263,12 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
 // (d0, d1), then optional rescale by the last prime (verify_rescale.py:38-53
 // on the CRT value, in the per-limb form
 //   ((y_i + h) - [(y_L + h) mod q_L]) q_L^-1 mod q_i,  h = (q_L - 1)/2).
+// narrow = 1: every prime is below twice every other one, so a residue mod
+// one prime is reduced mod another by one conditional subtraction.
 __global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64_t* __restrict__ dcoef, int dmask,
                                   uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map,
-                                  int l, int pidx_p, ModDownConst mc, int rescale, int logN, size_t total) {
+                                  int l, int pidx_p, ModDownConst mc, int rescale, int narrow, int logN,
+                                  size_t total) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const size_t N = (size_t)1 << logN;
@@ -279,7 +282,8 @@ __global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64
     const uint64_t* dc = (dcoef && ((dmask >> (bp & 1)) & 1)) ? dcoef + bp * l * N + n : nullptr;
     auto moddown = [&](int i) -> uint64_t {
         const PrimeConst& pc = pcs[map.idx[i]];
-        uint64_t v = sub_mod(add_mod(in[(size_t)i * N], mc.hp[i], pc.q), reduce64(tp, pc), pc.q);
+        const uint64_t tpi = narrow ? csub(tp, pc.q) : reduce64(tp, pc);
+        uint64_t v = sub_mod(add_mod(in[(size_t)i * N], mc.hp[i], pc.q), tpi, pc.q);
         v = csub(mul_shoup(v, mc.pinv[i], mc.pinv_s[i], pc.q), pc.q);
         if (dc) v = add_mod(v, dc[(size_t)i * N], pc.q);
         return v;
@@ -292,15 +296,16 @@ __global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64
     const uint64_t tl = csub(moddown(l - 1) + (pl.q >> 1), pl.q);
     for (int i = 0; i < l - 1; ++i) {
         const PrimeConst& pc = pcs[map.idx[i]];
-        uint64_t v = sub_mod(add_mod(moddown(i), mc.hq[i], pc.q), reduce64(tl, pc), pc.q);
+        const uint64_t tli = narrow ? csub(tl, pc.q) : reduce64(tl, pc);
+        uint64_t v = sub_mod(add_mod(moddown(i), mc.hq[i], pc.q), tli, pc.q);
         o[(size_t)i * N] = csub(mul_shoup(v, mc.qinv[i], mc.qinv_s[i], pc.q), pc.q);
     }
 }
 
 // Rescale by the last limb (verify_rescale.py:38-57, RNS per-limb form).
 __global__ void k_rescale(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
-                          const PrimeConst* __restrict__ pcs, LimbMap map, int l, ModDownConst mc, int logN,
-                          size_t total) {
+                          const PrimeConst* __restrict__ pcs, LimbMap map, int l, ModDownConst mc, int narrow,
+                          int logN, size_t total) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const size_t N = (size_t)1 << logN;
@@ -311,7 +316,8 @@ __global__ void k_rescale(const uint64_t* __restrict__ in, uint64_t* __restrict_
     const uint64_t tl = csub(src[(size_t)(l - 1) * N] + (pl.q >> 1), pl.q);
     for (int i = 0; i < l - 1; ++i) {
         const PrimeConst& pc = pcs[map.idx[i]];
-        uint64_t v = sub_mod(add_mod(src[(size_t)i * N], mc.hq[i], pc.q), reduce64(tl, pc), pc.q);
+        const uint64_t tli = narrow ? csub(tl, pc.q) : reduce64(tl, pc);
+        uint64_t v = sub_mod(add_mod(src[(size_t)i * N], mc.hq[i], pc.q), tli, pc.q);
         dst[(size_t)i * N] = csub(mul_shoup(v, mc.qinv[i], mc.qinv_s[i], pc.q), pc.q);
     }
 }
@@ -630,7 +636,7 @@ hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const
     const size_t total = ((size_t)batch * 2) << t.logN;
     if (!total) return hipSuccess;
     hipLaunchKernelGGL(k_moddown_rescale, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, acc, dcoef, dmask, out,
-                       t.pc, map, l, pidx_special, mc, rescale ? 1 : 0, t.logN, total);
+                       t.pc, map, l, pidx_special, mc, rescale ? 1 : 0, t.narrow, t.logN, total);
     return hipGetLastError();
 }
 
@@ -639,7 +645,7 @@ hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out,
     const size_t total = (size_t)n_polys << t.logN;
     if (!total) return hipSuccess;
     hipLaunchKernelGGL(k_rescale, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, map, l, mc,
-                       t.logN, total);
+                       t.narrow, t.logN, total);
     return hipGetLastError();
 }
 

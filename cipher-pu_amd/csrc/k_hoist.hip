@@ -136,11 +136,8 @@ __device__ __forceinline__ void permute_inverse2(uint64_t (&a0)[C::E], uint64_t 
     ntt::store_coalesced<C>(a1, out1, tid);
 }
 
-// One workgroup per (target t, ciphertext b, rotation r), the ciphertexts of
-// one (t, r) adjacent: workgroups in flight share a few key slices (1 MB
-// each at N = 8192, l = 4), which stay in L2 and are read by every
-// ciphertext, while each workgroup's digits (l rows) are read once.
-// keys: [R][dnum][2][key_limbs][N]
+// One workgroup per (target t, ciphertext b, rotation r), placed by
+// hoist_slot.  keys: [R][dnum][2][key_limbs][N]
 // (value, companion) pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N]
 // coefficient form, the layout ckks_relin's ModDown reads.
 template <int LOGN, int T, bool EVAL>
@@ -148,13 +145,12 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
                                                   size_t key_stride, int key_limbs, GaloisList galois,
                                                   uint64_t* __restrict__ acc, const ulonglong2* __restrict__ tw_all,
                                                   const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
-                                                  int batch, int R) {
+                                                  int batch, int R, HoistOrder order) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
-    const int b = blockIdx.x % batch;
-    const int r = (blockIdx.x / batch) % R;
-    const int t = blockIdx.x / (R * batch);
+    const TRB s = hoist_slot(blockIdx.x, gridDim.x, R, batch, order);
+    const int b = s.b, r = s.r, t = s.t;
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeConst pc = pcs[pt];
@@ -206,7 +202,7 @@ template <int LOGN, int T>
 __global__ void __launch_bounds__(T) k_moddown_eval(const uint64_t* __restrict__ acc, const uint64_t* __restrict__ Cn,
                                                     uint64_t* __restrict__ out, const ulonglong2* __restrict__ tw_all,
                                                     const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
-                                                    ModDownConst mc, GaloisList galois, int batch) {
+                                                    ModDownConst mc, GaloisList galois, int batch, int narrow) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
@@ -221,7 +217,10 @@ __global__ void __launch_bounds__(T) k_moddown_eval(const uint64_t* __restrict__
     uint64_t x[C::E];
     ntt::load_coalesced<C>(x, arow + (size_t)l * C::N, tid);
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) x[e] = sub_mod(reduce64(csub(x[e] + (qp >> 1), qp), pc), hp, pc.q);
+    for (int e = 0; e < C::E; ++e) {
+        const uint64_t v = csub(x[e] + (qp >> 1), qp);
+        x[e] = sub_mod(narrow ? csub(v, pc.q) : reduce64(v, pc), hp, pc.q);
+    }
     ntt::fwd_ntt<C>(x, lds, tw_all + (size_t)p * 2 * C::N, pc.q, pc.two_q, tid);
 #pragma unroll
     for (int e = 0; e < C::E; ++e) x[e] = reduce64(x[e], pc);
@@ -271,14 +270,14 @@ hipError_t run_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int ba
 template <int LOGN, int T>
 hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
                           int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l, const LimbMap& map,
-                          int pidx_p, int R, int eval_out, hipStream_t s) {
+                          int pidx_p, int R, int eval_out, HoistOrder order, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
     auto k = eval_out ? k_ks_hoisted<LOGN, T, true> : k_ks_hoisted<LOGN, T, false>;
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, keys, key_stride, key_limbs, galois, acc,
-                       t.tw, t.pc, map, l, pidx_p, batch, R);
+                       t.tw, t.pc, map, l, pidx_p, batch, R, order);
     return hipGetLastError();
 }
 
@@ -290,7 +289,7 @@ hipError_t run_moddown_eval(const DevTables& t, const uint64_t* acc, const uint6
     hipError_t e = set_lds(k_moddown_eval<LOGN, T>, C::LDS_BYTES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_moddown_eval<LOGN, T>), dim3(R * batch * 2 * l), dim3(T), C::LDS_BYTES, s, acc, Cn, out,
-                       t.tw, t.pc, map, l, pidx_p, mc, galois, batch);
+                       t.tw, t.pc, map, l, pidx_p, mc, galois, batch, t.narrow);
     return hipGetLastError();
 }
 
@@ -313,16 +312,26 @@ hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int
 
 hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
-                             const LimbMap& map, int pidx_p, int R, int eval_out, int threads, hipStream_t s) {
+                             const LimbMap& map, int pidx_p, int R, int eval_out, HoistOrder order, int threads,
+                             hipStream_t s) {
     if (!batch || !R) return hipSuccess;
+    if (order.btile == 0) {
+        // Measured at N = 8192, l = 4, 64 ciphertexts: with the inverse
+        // transforms in the workgroup (coefficient form), tiles of 4
+        // ciphertexts dealt out per XCD beat one key slice across the batch
+        // by 8-10% (24 and 15 rotations); without them (evaluation form)
+        // the key slices dominate and the whole-batch walk is best.
+        order = eval_out ? HoistOrder{batch, 0} : HoistOrder{4, 1};
+    }
+    if (order.btile > batch) order.btile = batch;
     switch (t.logN * 10000 + (threads ? threads : (t.logN == 12 ? 256 : t.logN == 13 ? 1024 : 512))) {
-        case 120256: return run_ks_hoisted<12, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
-        case 120512: return run_ks_hoisted<12, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
-        case 130256: return run_ks_hoisted<13, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
-        case 130512: return run_ks_hoisted<13, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
-        case 131024: return run_ks_hoisted<13, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
-        case 140512: return run_ks_hoisted<14, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
-        case 141024: return run_ks_hoisted<14, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, s);
+        case 120256: return run_ks_hoisted<12, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, order, s);
+        case 120512: return run_ks_hoisted<12, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, order, s);
+        case 130256: return run_ks_hoisted<13, 256>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, order, s);
+        case 130512: return run_ks_hoisted<13, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, order, s);
+        case 131024: return run_ks_hoisted<13, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, order, s);
+        case 140512: return run_ks_hoisted<14, 512>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, order, s);
+        case 141024: return run_ks_hoisted<14, 1024>(t, D, keys, key_stride, key_limbs, galois, acc, batch, l, map, pidx_p, R, eval_out, order, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -116,14 +116,50 @@ struct GaloisList {
     uint32_t g[HOIST_MAX];
 };
 
+// Workgroup placement of the hoisted key switch, one workgroup per (target
+// t, rotation r, ciphertext b).  Each (r, t) key slice (1 MB at N = 8192,
+// l = 4) is read by every ciphertext and each (b, t) digit slice (l rows) by
+// every rotation, so the order decides what L2 keeps.  Logical order: t
+// outermost; inside one t, tiles of btile ciphertexts, r-major and b-minor
+// inside a tile.  btile = batch walks one key slice across all ciphertexts;
+// btile = 1 walks one ciphertext's digits across all rotations.  With xcd
+// set and the grid G a multiple of 8, the logical order is dealt out in 8
+// contiguous ranges, one per XCD: the hardware hands workgroup i to XCD
+// i mod 8, so workgroup i takes logical index (i mod 8) G/8 + i/8 and each
+// XCD's L2 sees one contiguous stretch.  For 1 <= btile <= batch every
+// placement is a bijection from [0, G) onto [0, l+1) x [0, R) x [0, batch)
+// (tests/hoist_order_check.cpp).
+struct HoistOrder {
+    int btile;
+    int xcd;
+};
+struct TRB {
+    int t, r, b;
+};
+MODARITH_FN TRB hoist_slot(int i, int G, int R, int batch, HoistOrder o) {
+    const int L = (o.xcd && (G & 7) == 0) ? (i & 7) * (G >> 3) + (i >> 3) : i;
+    const int RB = R * batch;
+    const int t = L / RB;
+    const int rem = L - t * RB;
+    const int tile = rem / (R * o.btile);
+    const int b0 = tile * o.btile;
+    const int nb = batch - b0 < o.btile ? batch - b0 : o.btile;
+    const int inner = rem - tile * R * o.btile;
+    const int r = inner / nb;
+    return TRB{t, r, b0 + inner - r * nb};
+}
+
 hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int batch, int l, const LimbMap& map,
                         int pidx_p, int threads, hipStream_t s);
 // eval_out = 1: the chain-limb accumulators are stored in evaluation form
 // (pi_g applied, no inverse transform) for launch_moddown_eval; the special
 // limb's in coefficient form as always.
+// order: workgroup placement (hoist_slot); btile 0 picks the measured
+// default for the output form, btile above batch means batch.
 hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
-                             const LimbMap& map, int pidx_p, int R, int eval_out, int threads, hipStream_t s);
+                             const LimbMap& map, int pidx_p, int R, int eval_out, HoistOrder order, int threads,
+                             hipStream_t s);
 hipError_t launch_moddown_eval(const DevTables& t, const uint64_t* acc, const uint64_t* Cn, uint64_t* out, int batch,
                                int l, const LimbMap& map, int pidx_p, const ModDownConst& mc,
                                const GaloisList& galois, int R, int threads, hipStream_t s);

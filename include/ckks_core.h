@@ -274,6 +274,15 @@ int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads, int ten
  * forward transforms, the tensor product and INTT(d2); 0 (default) =
  * separate transform launches then the tensor kernel.  Same results. */
 int ckks_ctx_set_front(ckks_ctx *ctx, int fused);
+/* Workgroup placement of the hoisted key switch (ckks_rotate_hoisted and
+ * its evaluation-form variant): inside one target limb, tiles of btile
+ * ciphertexts with every rotation of a tile before the next tile (btile =
+ * batch: one rotation's key slice across all ciphertexts); xcd = 1 gives
+ * each of the 8 XCDs one contiguous stretch of that order.  btile = 0
+ * (default) picks per call: tiles of 4 per XCD for coefficient-form
+ * output, the whole batch for evaluation form.  Results do not depend on
+ * it.  No reference counterpart (a launch-shape setting). */
+int ckks_ctx_set_hoist_order(ckks_ctx *ctx, int btile, int xcd);
 
 /* Device sampling for encryption (the reference samples on the host:
  * uniform a and the rounded Box-Muller dgauss of

@@ -524,6 +524,75 @@ def test_every_launch_shape_gives_the_same_words(gpu, N):
         assert all(np.array_equal(g, r) for g, r in zip(got[2], ref[2])), ("hoisted",) + shape
 
 
+WIDE = [CHAIN4[0], 2199023288321, CHAIN4[2], 4398047232001]   # 2^44.., 2^41.., 2^44.., 2^42.. (= 1 mod 2^15)
+
+
+def test_wide_chain_bit_exact(gpu):
+    """A chain whose primes differ by more than a factor of two (2^41 to
+    2^44, special prime 2^44): a residue mod one prime is no longer below
+    twice another, so the digit lift, ModDown and rescale take their full
+    reductions instead of one conditional subtraction.  HMult at two levels,
+    a rotation and hoisted rotations against the oracle, bit for bit; the
+    evaluation-form hoisted rotation against the transform of the
+    coefficient form."""
+    from cipherpu.ckks import CKKS, CKKSParams, Ciphertext
+    from cipherpu.ring import to_numpy
+    for q in WIDE[1::2]:
+        assert all(pow(a, q - 1, q) == 1 for a in (2, 3, 5, 7)) and q % (1 << 15) == 1
+    c = CKKS(CKKSParams(N=4096, primes=WIDE), seed=51)
+    assert max(c.primes) >= 2 * min(c.primes)
+    sk = c.keygen()
+    rlk = c.relin_key(sk)
+    pr, p, ps, pp = _oracle_args(c)
+    key = to_numpy(rlk.coeff)
+    rng = np.random.default_rng(51)
+    for l in (4, 3):
+        mk = lambda: np.stack([rng.integers(0, q, (2, 2, c.N), dtype=np.uint64) for q in c.primes[:l]], axis=2)
+        a, b = mk(), mk()
+        out = to_numpy(c.hmult(Ciphertext(c.ring.tensor(a), 1.0), Ciphertext(c.ring.tensor(b), 1.0), rlk).data)
+        for i in range(2):
+            assert np.array_equal(out[i], OC.hmult(a[i], b[i], key, pr[:l], p, ps[:l], pp)), (l, i)
+    ct = c.encrypt(c.encode(rng.uniform(-1, 1, (2, c.N // 2))), sk)
+    gks = [c.galois_key(sk, s) for s in (1, 9)]
+    rot = to_numpy(c.rotate(ct, gks[0]).data)
+    for i in range(2):
+        want = OC.rotate(to_numpy(ct.data[i]), to_numpy(gks[0].coeff), pr, p, ps, pp, gks[0].galois)
+        assert np.array_equal(rot[i], want), i
+    coef = c.rotate_hoisted(ct, gks)
+    ev = c.rotate_hoisted(ct, gks, eval_form=True)
+    for gk, co, e in zip(gks, coef, ev):
+        got = to_numpy(co.data)
+        for i in range(2):
+            want = OC.rotate_hoisted(to_numpy(ct.data[i]), to_numpy(gk.coeff), pr, p, ps, pp, gk.galois)
+            assert np.array_equal(got[i], want), (gk.galois, i)
+        assert np.array_equal(to_numpy(e.data).reshape(-1, 4, c.N),
+                              to_numpy(c.ring.ntt(co.data.reshape(-1, 4, c.N)))), gk.galois
+
+
+def test_hoisted_placement_never_changes_a_word(gpu):
+    """The hoisted key switch under every workgroup placement (tile sizes 1
+    to the whole batch, ragged last tiles, with and without the per-XCD
+    deal, on grids that are and are not multiples of 8) gives the words of
+    the default placement, in both output forms."""
+    from cipherpu.ckks import CKKS, CKKSParams
+    from cipherpu.ring import to_numpy
+    c = CKKS(CKKSParams(N=8192), seed=41)
+    sk = c.keygen()
+    rng = np.random.default_rng(41)
+    for B, steps in ((5, (1, 2, 7)), (4, (3, 9))):          # grids 5*3*5 = 75 and 5*2*4 = 40
+        gks = [c.galois_key(sk, s) for s in steps]
+        ct = c.encrypt(c.encode(rng.uniform(-1, 1, (B, c.N // 2))), sk)
+        c.ring.set_hoist_order(0, False)
+        ref = [[to_numpy(r.data) for r in c.rotate_hoisted(ct, gks, eval_form=ev)] for ev in (False, True)]
+        for btile in range(1, B + 2):
+            for xcd in (False, True):
+                c.ring.set_hoist_order(btile, xcd)
+                for ev in (False, True):
+                    got = [to_numpy(r.data) for r in c.rotate_hoisted(ct, gks, eval_form=ev)]
+                    assert all(np.array_equal(g, r) for g, r in zip(got, ref[ev])), (B, btile, xcd, ev)
+        c.ring.set_hoist_order(0, False)
+
+
 @pytest.mark.parametrize("N", [4096, 8192, 16384])
 def test_rotate_hoisted_eval_form_is_ntt_of_coefficient_form(gpu, N):
     """Hoisted rotations with ModDown in the evaluation domain give the

@@ -1,4 +1,4 @@
-"""The device modular arithmetic (modarith.h) compiled for the host and
+"""Device integer code compiled for the host.  The modular arithmetic (modarith.h)
 checked against exact 128-bit arithmetic: Shoup products (exact and lazy
 quotient), the lazy high half, Barrett, 64-bit reduction, the special-form
 product and both lazy butterflies on their documented bounds, for the chain
@@ -27,5 +27,19 @@ def test_modarith_against_128_bit_arithmetic(tmp_path):
     subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(HERE, "modarith_check.cpp"), "-o", exe],
                    check=True, capture_output=True, text=True)
     r = subprocess.run([exe, "200000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith(", 0 failures"), r.stdout
+
+
+def test_hoist_placement_is_a_bijection(tmp_path):
+    """hoist_slot (launch.h) maps every workgroup of the hoisted key switch to
+    a distinct (target, rotation, ciphertext) for every tile size and XCD
+    setting (tests/hoist_order_check.cpp).  The checker was seen to fail when
+    the rotation index of a ragged last tile was divided by the tile size
+    instead of the tile's length."""
+    exe = str(tmp_path / "hoist_order_check")
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(HERE, "hoist_order_check.cpp"), "-o", exe],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith(", 0 failures"), r.stdout
