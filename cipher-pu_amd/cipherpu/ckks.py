@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -277,6 +277,30 @@ class CKKS:
         ws = self.workspace(B, l)
         call("ckks_rotate", self.ring.handle, _ptr(ct.data), _ptr(gk.prepared), self.L + 1, _ptr(out), B, l,
              self.sp, gk.galois, _ptr(ws), self.ring.stream())
+        return Ciphertext(out, ct.scale)
+
+    @staticmethod
+    def rotation_tables(gks: Sequence[SwitchKey], device) -> Tuple[torch.Tensor, torch.Tensor]:
+        """Device tables for rotate_many: the prepared key address and the
+        Galois element of each ciphertext's rotation.  The keys must outlive
+        every call that uses the tables."""
+        keys = torch.tensor([g.prepared.data_ptr() for g in gks], dtype=torch.int64).to(device)
+        gal = torch.tensor([g.galois for g in gks], dtype=torch.int64).to(device)
+        return keys, gal
+
+    def rotate_many(self, ct: Ciphertext, gks: Sequence[SwitchKey],
+                    tables: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> Ciphertext:
+        """Ciphertext b rotated by the step gks[b] was made for, all in one
+        call (ckks_rotate_many); tables may hold rotation_tables(gks) for
+        reuse."""
+        B, l = ct.batch, ct.level
+        if len(gks) != B:
+            raise ValueError("one Galois key per ciphertext")
+        keys, gal = tables if tables is not None else self.rotation_tables(gks, self.device)
+        out = torch.empty_like(ct.data)
+        ws = self.workspace(B, l)
+        call("ckks_rotate_many", self.ring.handle, _ptr(ct.data), _ptr(keys), self.L + 1, _ptr(gal), _ptr(out), B,
+             l, self.sp, _ptr(ws), self.ring.stream())
         return Ciphertext(out, ct.scale)
 
     def hoist_key(self, gk: SwitchKey) -> torch.Tensor:

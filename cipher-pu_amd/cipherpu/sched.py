@@ -5,9 +5,11 @@ ring degrees and levels: HMult + relinearise + rescale with a partner
 ciphertext, rotation by a power-of-two slot step with its Galois key, or a
 plaintext-ciphertext product.  The scheduler
 
-  * buckets the operations by (N, level, kind, step) -- every bucket is one
+  * buckets the operations by (N, level, kind) -- every bucket is one
     batched call of the C ABI on contiguous device buffers that the
     scheduler owns (inputs are written bucket-major, so no gather pass);
+    rotations by different steps share a bucket, each ciphertext with its
+    own Galois key (ckks_rotate_many);
   * chooses a launch shape per bucket with the tuner (threads per
     workgroup of the transform, tensor and key-switch kernels, i.e. the
     radix of the register passes);
@@ -72,8 +74,9 @@ class Bucket:
     N: int
     level: int
     kind: str
-    step: int
     ops: List[int]                          # indices into the workload
+    steps: List[int] = field(default_factory=list)   # rotation step of each op (rotate buckets)
+    tables: Tuple[torch.Tensor, torch.Tensor] = None  # per-ciphertext key addresses and Galois elements
     a: torch.Tensor = None                  # [B, 2, l+1, N] inputs
     b: torch.Tensor = None                  # partner ciphertexts (hmult) or plaintexts [B*2, l+1, N] (ptmul)
     out: torch.Tensor = None
@@ -108,8 +111,12 @@ class BatchScheduler:
             self.gk[(N, step)] = self.ck[N].galois_key(self.sk[N], step)
         groups: Dict[tuple, List[int]] = {}
         for i, o in enumerate(ops):
-            groups.setdefault((o.N, o.level, o.kind, o.step), []).append(i)
-        self.buckets = [Bucket(N, lv, k, st, idx) for (N, lv, k, st), idx in sorted(groups.items())]
+            groups.setdefault((o.N, o.level, o.kind), []).append(i)
+        self.buckets = [Bucket(N, lv, k, idx, [ops[i].step for i in idx] if k == "rotate" else [])
+                        for (N, lv, k), idx in sorted(groups.items())]
+        for bk in self.buckets:
+            if bk.kind == "rotate":
+                bk.tables = CKKS.rotation_tables([self.gk[(bk.N, st)] for st in bk.steps], self.ck[bk.N].device)
         self.graph: Optional[torch.cuda.CUDAGraph] = None
         self._ws: Dict[Tuple[int, int], torch.Tensor] = {}
         self.assign_streams()
@@ -164,8 +171,8 @@ class BatchScheduler:
             call("ckks_hmult", r.handle, _ptr(bk.a), _ptr(bk.b), _ptr(self.rlk[bk.N].prepared), K, _ptr(bk.out), B,
                  l, ck.sp, _ptr(ws), r.stream())
         elif bk.kind == "rotate":
-            gk = self.gk[(bk.N, bk.step)]
-            call("ckks_rotate", r.handle, _ptr(bk.a), _ptr(gk.prepared), K, _ptr(bk.out), B, l, ck.sp, gk.galois,
+            keys, gal = bk.tables
+            call("ckks_rotate_many", r.handle, _ptr(bk.a), _ptr(keys), K, _ptr(gal), _ptr(bk.out), B, l, ck.sp,
                  _ptr(ws), r.stream())
         else:
             call("ckks_poly_mul", r.handle, _ptr(bk.a), _ptr(bk.b), _ptr(bk.out), B * 2, l, 0, r.stream())
@@ -242,6 +249,6 @@ class BatchScheduler:
         return [bk.out for bk in self.buckets]
 
     def summary(self) -> List[dict]:
-        return [{"N": bk.N, "level": bk.level, "kind": bk.kind, "step": bk.step, "batch": bk.batch,
+        return [{"N": bk.N, "level": bk.level, "kind": bk.kind, "steps": sorted(set(bk.steps)), "batch": bk.batch,
                  "threads": dict(zip(("ntt", "ks", "tensor"), bk.shape)), "tuned_ms": round(bk.ms, 4)}
                 for bk in self.buckets]

@@ -651,6 +651,30 @@ int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int
     return CKKS_OK;
 }
 
+int ckks_rotate_many(const ckks_ctx* c, const uint64_t* ct, const uint64_t* key_table, int key_limbs,
+                     const uint64_t* galois_table, uint64_t* out, int batch, int l, int special, void* workspace,
+                     void* stream) {
+    int st = ks_common_check(c, key_table, key_limbs, out, batch, l, special, workspace);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!ct || !galois_table) return CKKS_ERR_NULL;
+    if (ct == out) return CKKS_ERR_BAD_ARG;
+    if (misaligned(ct) || misaligned(galois_table)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (c->N < 128 || c->N > 16384) return CKKS_ERR_BAD_N;   // the per-ciphertext map runs in LDS
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const DevTables t = c->tables();
+    const LimbMap m = limb_map(0, l);
+    const ModDownConst mc = moddown_const(c, l, special);
+    const size_t LN = (size_t)l * c->N, B = (size_t)batch;
+    uint64_t* sct = (uint64_t*)workspace;   // sigma_b(c0), sigma_b(c1) per ciphertext b
+    uint64_t* acc = sct + B * 2 * LN;
+    TIMED(CKKS_KIND_AUTOMORPHISM, launch_automorphism_groups(t, ct, sct, batch * 2, l, (size_t)l * c->N, m, 1,
+                                                             S(stream), galois_table, 2));
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, nullptr, key_limbs, acc, batch,
+                                                l, m, special, c->ks_threads, S(stream), key_table));
+    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream)));
+    return CKKS_OK;
+}
+
 uint64_t ckks_workspace_hoisted_bytes(const ckks_ctx* c, int batch, int l, int R) {
     if (!c || batch < 0 || l < 1 || R < 1) return 0;
     return (uint64_t)batch * c->N * 8 * (uint64_t)(l * (l + 1) + R * (4 * l + 2));

@@ -174,14 +174,17 @@ __device__ __forceinline__ void key_mac(uint64_t (&a0)[C::E], uint64_t (&a1)[C::
 // ("ModUp") is the reduction of the digit's coefficients mod the target
 // prime on load; the digit NTTs, the key products and the inverse
 // transforms never leave the workgroup.
-template <int LOGN, int T>
+// KEYTAB: ciphertext b switches with its own key, key_table[b] (a device
+// address), instead of the one key of the launch (rotations by different
+// steps in one launch).
+template <int LOGN, int T, bool KEYTAB = false>
 __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2c, size_t d2c_bstride,
                                                  const uint64_t* __restrict__ d2e,
                                                  const uint64_t* __restrict__ d01, const ulonglong2* __restrict__ key,
                                                  int key_limbs, uint64_t* __restrict__ acc,
                                                  const ulonglong2* __restrict__ tw_all,
                                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
-                                                 int batch, int reduce_digits) {
+                                                 int batch, int reduce_digits, const uint64_t* __restrict__ key_table) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
@@ -189,6 +192,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     // limb, so each XCD's L2 holds one limb's key slice (2 dnum N 16 B) and
     // one prime's twiddles rather than the whole key.
     const int t = blockIdx.x / batch, b = blockIdx.x % batch;
+    if constexpr (KEYTAB) key = reinterpret_cast<const ulonglong2*>(key_table[b]);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeConst pc = pcs[pt];
@@ -352,13 +356,23 @@ __global__ void k_automorphism(const uint64_t* __restrict__ in, uint64_t* __rest
 // also safe in place: a row is fully staged before any of it is written.)
 // Rows are (group, limb) with group g's limb i at g group_stride + i N in
 // both tensors.
+// gal_table (optional): group g uses the Galois element gal_table[g / gal_div]
+// instead of the launch's, inverted here.
+__device__ __forceinline__ uint64_t galois_inverse(uint64_t g, int logN) {
+    uint64_t inv = 1;   // Newton's iteration for the inverse of an odd g modulo 2^64
+    for (int k = 0; k < 6; ++k) inv = inv * (2 - g * inv);
+    return inv & ((2ull << logN) - 1);
+}
+
 __global__ void __launch_bounds__(1024) k_automorphism_row(const uint64_t* in, uint64_t* out,
                                                            const PrimeConst* __restrict__ pcs, LimbMap map,
                                                            int n_limbs, size_t group_stride, uint64_t ginv,
-                                                           int logN) {
+                                                           int logN, const uint64_t* __restrict__ gal_table,
+                                                           int gal_div) {
     extern __shared__ uint64_t lds[];
     const int N = 1 << logN;
     const int r = blockIdx.x;
+    if (gal_table) ginv = galois_inverse(gal_table[r / n_limbs / gal_div], logN);
     const size_t off = (size_t)(r / n_limbs) * group_stride + (size_t)(r % n_limbs) * N;
     const uint64_t* src = in + off;
     for (int k = threadIdx.x; k < N; k += blockDim.x) lds[k] = src[k];
@@ -561,13 +575,14 @@ hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, u
 template <int LOGN, int T>
 hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uint64_t* d2e, const uint64_t* d01,
                   const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
-                  int pidx_p, hipStream_t s) {
+                  int pidx_p, const uint64_t* key_table, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
-    hipError_t e = set_lds(k_keyswitch<LOGN, T>, lds);
+    auto k = key_table ? k_keyswitch<LOGN, T, true> : k_keyswitch<LOGN, T, false>;
+    hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_keyswitch<LOGN, T>), dim3(batch * (l + 1)), dim3(T), lds, s, d2c, d2s, d2e, d01, key,
-                       key_limbs, acc, t.tw, t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1);
+    hipLaunchKernelGGL(k, dim3(batch * (l + 1)), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.tw,
+                       t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table);
     return hipGetLastError();
 }
 
@@ -614,17 +629,17 @@ hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01,
                             const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l,
-                            const LimbMap& map, int pidx_p, int threads, hipStream_t s) {
+                            const LimbMap& map, int pidx_p, int threads, hipStream_t s, const uint64_t* key_table) {
     if (batch * l == 0) return hipSuccess;
     if (!threads) threads = default_threads(t.logN);
     switch (t.logN * 10000 + threads) {
-        case 120256: return run_ks<12, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
-        case 120512: return run_ks<12, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
-        case 130256: return run_ks<13, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
-        case 130512: return run_ks<13, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
-        case 131024: return run_ks<13, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
-        case 140512: return run_ks<14, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
-        case 141024: return run_ks<14, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, s);
+        case 120256: return run_ks<12, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
+        case 120512: return run_ks<12, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
+        case 130256: return run_ks<13, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
+        case 130512: return run_ks<13, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
+        case 131024: return run_ks<13, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
+        case 140512: return run_ks<14, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
+        case 141024: return run_ks<14, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -651,7 +666,7 @@ hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out,
 
 hipError_t launch_automorphism_groups(const DevTables& t, const uint64_t* in, uint64_t* out, int n_groups,
                                      int n_limbs, size_t group_stride, const LimbMap& map, uint64_t galois,
-                                     hipStream_t s) {
+                                     hipStream_t s, const uint64_t* gal_table, int gal_div) {
     if (n_groups == 0 || n_limbs == 0) return hipSuccess;
     const int N = 1 << t.logN;
     const uint64_t M = 2ull << t.logN;
@@ -665,9 +680,11 @@ hipError_t launch_automorphism_groups(const DevTables& t, const uint64_t* in, ui
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_automorphism_row, dim3((unsigned)(n_groups * n_limbs)), dim3(N >= 2048 ? 1024 : N / 2),
-                           lds, s, in, out, t.pc, map, n_limbs, group_stride, inv, t.logN);
+                           lds, s, in, out, t.pc, map, n_limbs, group_stride, inv, t.logN, gal_table,
+                           gal_div > 0 ? gal_div : 1);
         return hipGetLastError();
     }
+    if (gal_table) return hipErrorInvalidValue;   // per-group elements need the LDS kernel
     // rows too long for LDS: the gather kernel, one launch per group
     const size_t total = (size_t)n_limbs << t.logN;
     for (int g = 0; g < n_groups; ++g) {

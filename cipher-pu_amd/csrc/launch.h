@@ -60,10 +60,12 @@ hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint
 hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
                         uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, int threads,
                         hipStream_t s);
+// key_table (optional): ciphertext b uses the key at device address
+// key_table[b] instead of key.
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01, const ulonglong2* key, int key_limbs, uint64_t* acc,
                             int batch, int l, const LimbMap& map, int pidx_special, int threads,
-                            hipStream_t s);
+                            hipStream_t s, const uint64_t* key_table = nullptr);
 hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
                                   uint64_t* out, int batch, int l, const LimbMap& map, int pidx_special,
                                   const ModDownConst& mc, bool rescale, hipStream_t s);
@@ -72,9 +74,11 @@ hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out,
 hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys,
                                int n_limbs, const LimbMap& map, uint64_t galois, hipStream_t s);
 // The automorphism over n_groups groups of n_limbs rows, group g at g group_stride.
+// gal_table (optional, device): group g uses gal_table[g / gal_div] instead
+// of galois; needs N in [128, 16384].
 hipError_t launch_automorphism_groups(const DevTables& t, const uint64_t* in, uint64_t* out, int n_groups,
                                       int n_limbs, size_t group_stride, const LimbMap& map, uint64_t galois,
-                                      hipStream_t s);
+                                      hipStream_t s, const uint64_t* gal_table = nullptr, int gal_div = 1);
 hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
                              int ell, hipStream_t s);
 hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,

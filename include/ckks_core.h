@@ -151,6 +151,18 @@ int ckks_automorphism(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, in
 int ckks_rotate(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *gkey, int key_limbs, uint64_t *out,
                 int batch, int l, int special, uint64_t galois, void *workspace, void *stream);
 
+/* ckks_rotate with a step per ciphertext, in one call: ciphertext b is
+ * rotated by galois_table[b] with the prepared Galois key at device address
+ * key_table[b].  Both tables are device arrays of batch uint64 words; the
+ * table pointer and every key it names must be 16-byte aligned (only the
+ * table pointer is checked on the host).  Output word for word that of
+ * ckks_rotate per ciphertext.  Same workspace as ckks_rotate; N in [128,
+ * 16384].  No reference counterpart: it lets a batch of rotations by mixed
+ * steps (one key each) fill the GPU as one launch per kernel. */
+int ckks_rotate_many(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *key_table, int key_limbs,
+                     const uint64_t *galois_table, uint64_t *out, int batch, int l, int special, void *workspace,
+                     void *stream);
+
 /* Hoisted rotations (no counterpart in the reference, which rotates one
  * step at a time: exp_e24_rotation_bootstrap.py:143-176; the method is
  * Halevi-Shoup, CRYPTO 2018).  R <= 32 rotations of the same ciphertexts

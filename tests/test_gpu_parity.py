@@ -320,6 +320,30 @@ def test_rotate_bit_exact_and_slots_move(ck):
         assert np.max(np.abs(dec.real - np.roll(z, -step, axis=1))) < 1e-5
 
 
+@pytest.mark.parametrize("N", [4096, 8192])
+def test_rotate_many_equals_one_rotation_per_ciphertext(gpu, N):
+    """ckks_rotate_many with a different step (and key) per ciphertext gives,
+    word for word, what ckks_rotate gives for each ciphertext alone, at two
+    levels, with repeated and distinct steps; one of them against the
+    oracle."""
+    from cipherpu.ckks import CKKS, CKKSParams, Ciphertext
+    from cipherpu.ring import to_numpy
+    c = CKKS(CKKSParams(N=N), seed=71)
+    sk = c.keygen()
+    steps = [1, 8, 1, 256, 3, N // 4 - 1]
+    gks = {s: c.galois_key(sk, s) for s in set(steps)}
+    rng = np.random.default_rng(N + 71)
+    pr, p, ps, pp = _oracle_args(c)
+    for l in (4, 2):
+        ct = c.encrypt(c.encode(rng.uniform(-1, 1, (len(steps), N // 2)), level=l), sk)
+        got = to_numpy(c.rotate_many(ct, [gks[s] for s in steps]).data)
+        for b, s in enumerate(steps):
+            one = to_numpy(c.rotate(Ciphertext(ct.data[b:b + 1].contiguous(), ct.scale), gks[s]).data)[0]
+            assert np.array_equal(got[b], one), (l, b, s)
+        want = OC.rotate(to_numpy(ct.data[3]), to_numpy(gks[256].coeff), pr[:l], p, ps[:l], pp, gks[256].galois)
+        assert np.array_equal(got[3], want), l
+
+
 def test_encode_decode_vs_fp64_oracle(ck):
     from cipherpu.ring import to_numpy
     c, sk = ck
