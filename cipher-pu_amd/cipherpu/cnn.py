@@ -20,8 +20,9 @@ pool is (1 + rot_1)(1 + rot_16): the sum of a 2x2 window lands on slot
 32 a + 2 b.  FC output k is sum_c W_(k,c) * S_c with W_(k,c)[32 a + 2 b] =
 F[k][36 c + 6 a + b] / 4, one evaluation-domain ckks_lincomb call for all
 10 outputs; outputs 4p..4p+3 are shifted into the four 256-slot windows of
-one ciphertext and summed over each window by rotate-and-sum (128 ... 1),
-every rotation batched over all the packed ciphertexts that need it.
+one ciphertext (all shifts, by three different steps, in one call) and
+summed over each window by rotate-and-sum (128 ... 1), every rotation
+batched over all the packed ciphertexts.
 Output k ends on slot 256 (k mod 4) of packed ciphertext k // 4.
 """
 from __future__ import annotations
@@ -198,17 +199,17 @@ class EncryptedCNN(EncryptedNet):
         self.ops["ptmul_rows"] += N_CH * N_OUT * B
         ck.ring.intt_(acc.view(-1, l, ck.N))
         outs = ck.rescale(Ciphertext(acc.view(N_OUT * B, 2, l, ck.N), ct.scale * scale))
-        out = lambda k: outs.data[k * B:(k + 1) * B]
+        # output k goes to window k mod 4 of packed ciphertext k // 4: every
+        # output shifted by -256 (k mod 4) in one call
+        sh = self.rot_many(outs, [-WINDOW * (k % 4) for k in range(N_OUT) for _ in range(B)]).data
+        out = lambda k: sh[k * B:(k + 1) * B]
         groups = list(range(0, N_OUT, 4))
-        # window m of every packed ciphertext: one rotation by -256 m over all
-        # the groups that have an output there, as one batch
         z = torch.cat([out(p) for p in groups])
         for m in range(1, 4):
-            have = [i for i, p in enumerate(groups) if p + m < N_OUT]
-            shifted = self.rot(Ciphertext(torch.cat([out(groups[i] + m) for i in have]), outs.scale), -WINDOW * m)
-            for j, i in enumerate(have):
-                zi = z[i * B:(i + 1) * B]
-                ck.ring.add(zi, shifted.data[j * B:(j + 1) * B], out=zi)
+            for i, p in enumerate(groups):
+                if p + m < N_OUT:
+                    zi = z[i * B:(i + 1) * B]
+                    ck.ring.add(zi, out(p + m), out=zi)
         # rotate-and-sum over each 256-slot window, all packed ciphertexts at once
         zc = Ciphertext(z, outs.scale)
         for k in range(1, 9):

@@ -142,6 +142,8 @@ class EncryptedNet:
         self._hkeys: Dict[tuple, torch.Tensor] = {}
         self.hoist = True          # baby steps / convolution taps as hoisted rotations
         self.eval_rotations = True  # baby steps leave the hoisted call in evaluation form (MLP)
+        self.mixed_rotations = True  # rotations of different rows by different steps in one call
+        self._rtab: Dict[tuple, tuple] = {}
         self.ops = {"rot": 0, "ptmul_rows": 0, "hmult": 0, "dare": 0}
         self.gears = (0, 0)
 
@@ -157,6 +159,44 @@ class EncryptedNet:
             return ct
         self.ops["rot"] += ct.batch
         return self.ck.rotate(ct, self.gk(step))
+
+    def rot_many(self, ct: Ciphertext, steps: List[int]) -> Ciphertext:
+        """Row b of ct rotated by steps[b] (a multiple of N/2 leaves it as it
+        is): the rows that move in one ckks_rotate_many call, each with its
+        own Galois key, or one rotation call per distinct step when
+        mixed_rotations is off.  Same words either way."""
+        ck = self.ck
+        half = ck.N // 2
+        move = [b for b, st in enumerate(steps) if st % half]
+        if not move:
+            return ct
+        n, lo = len(move), move[0]
+        contiguous = move == list(range(lo, lo + n))
+        idx = None if contiguous else torch.tensor(move, device=ck.device)
+        i64 = lambda t: t.view(torch.int64)       # row gathers and scatters on the same words
+        src = Ciphertext(ct.data[lo:lo + n] if contiguous else
+                         i64(ct.data).index_select(0, idx).view(torch.uint64), ct.scale)
+        key = tuple(steps[b] % half for b in move)
+        if self.mixed_rotations:
+            gks = [self.gk(st) for st in key]
+            if key not in self._rtab:
+                self._rtab[key] = ck.rotation_tables(gks, ck.device)   # the keys stay alive in self._gk
+            self.ops["rot"] += n
+            moved = ck.rotate_many(src, gks, tables=self._rtab[key]).data
+        else:
+            moved = torch.empty_like(src.data)
+            for st in sorted(set(key)):
+                rows = torch.tensor([i for i, k in enumerate(key) if k == st], device=ck.device)
+                part = Ciphertext(i64(src.data).index_select(0, rows).view(torch.uint64), ct.scale)
+                i64(moved).index_copy_(0, rows, i64(self.rot(part, st).data))
+        if n == ct.batch:
+            return Ciphertext(moved, ct.scale)
+        out = ct.data.clone()
+        if contiguous:
+            out[lo:lo + n] = moved
+        else:
+            i64(out).index_copy_(0, idx, i64(moved))
+        return Ciphertext(out, ct.scale)
 
     def _slots(self, v: np.ndarray) -> np.ndarray:
         """A 1024-slot block pattern repeated for the S samples of a ciphertext."""
@@ -324,11 +364,13 @@ class EncryptedMLP(EncryptedNet):
         call("ckks_lincomb", ck.ring.handle, _ptr(E), _ptr(p.D), _ptr(acc), nb, ng, B, l, ck.ring.stream())
         self.ops["ptmul_rows"] += nb * ng * B
         ck.ring.intt_(acc.view(-1, l, ck.N))
-        z = None
-        for gi, g in enumerate(p.used_g):
-            part = ck.rescale(Ciphertext(acc[gi], ct.scale * self.w_scale))
-            part = self.rot(part, p.baby * g)
-            z = part if z is None else ck.add(z, part)
+        # every giant step rescaled and rotated at once, each by its own step
+        parts = ck.rescale(Ciphertext(acc.view(ng * B, 2, l, ck.N), ct.scale * self.w_scale))
+        parts = self.rot_many(parts, [p.baby * g for g in p.used_g for _ in range(B)])
+        P = parts.data.view(ng, B, 2, parts.level, ck.N)
+        z = Ciphertext(P[0], parts.scale)
+        for gi in range(1, ng):
+            z = ck.add(z, Ciphertext(P[gi], parts.scale))
         for s in p.folds:
             z = ck.add(z, self.rot(z, s))
         return z

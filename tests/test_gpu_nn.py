@@ -181,3 +181,31 @@ def test_device_samplers(mlp_env):
     assert ck.sampler == "device"
     dec = to_numpy(ck.decode(ck.decrypt(ct, sk), ct.scale)).real
     assert np.max(np.abs(dec - z)) < 1e-7
+
+
+def test_mixed_step_rotations_keep_every_word(mlp_env):
+    """The MLP's giant steps and the CNN's output shifts rotate rows by
+    different steps in one ckks_rotate_many call; both linear layers of the
+    MLP and the CNN's output stage give the same words as with one rotation
+    call per step (the stages are compared on the same inputs, since the
+    DARE refresh between them draws fresh encryption randomness)."""
+    import torch
+    from cipherpu import cnn, nn
+    ck, sk, W1, W2 = mlp_env
+    X = nn.synthetic_images(8, seed=9)
+    mlp = nn.EncryptedMLP(ck, sk, W1, W2)
+    ct = mlp.encrypt_inputs(X)
+    h = mlp.activation(mlp.dare(mlp.layer1(ct)))
+    K, F = cnn.make_cnn(0)
+    net = cnn.EncryptedCNN(ck, sk, K, F)
+    y = net.pool(net.activation(net.dare(net.conv(net.encrypt_inputs(X)))))
+    stages = [(mlp, lambda: [mlp.layer1(ct)]), (mlp, lambda: [mlp.layer2(h)]), (net, lambda: net.fc(y, ct.batch))]
+    for model, stage in stages:
+        got = []
+        for mixed in (True, False):
+            model.mixed_rotations = mixed
+            model.ops["rot"] = 0
+            got.append(([o.data.clone() for o in stage()], model.ops["rot"]))
+        model.mixed_rotations = True
+        assert got[0][1] == got[1][1] > 0
+        assert all(torch.equal(a, b) for a, b in zip(got[0][0], got[1][0])), type(model).__name__
