@@ -201,8 +201,8 @@ class EncryptedCNN(EncryptedNet):
         outs = ck.rescale(Ciphertext(acc.view(N_OUT * B, 2, l, ck.N), ct.scale * scale))
         # output k goes to window k mod 4 of packed ciphertext k // 4: every
         # output shifted by -256 (k mod 4) in one call
-        sh = self.rot_many(outs, [-WINDOW * (k % 4) for k in range(N_OUT) for _ in range(B)]).data
-        out = lambda k: sh[k * B:(k + 1) * B]
+        sh = self.rot_blocks(outs, B, [-WINDOW * (k % 4) for k in range(N_OUT)])
+        out = lambda k: sh[k].data
         groups = list(range(0, N_OUT, 4))
         z = torch.cat([out(p) for p in groups])
         for m in range(1, 4):

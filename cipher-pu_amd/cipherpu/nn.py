@@ -160,43 +160,36 @@ class EncryptedNet:
         self.ops["rot"] += ct.batch
         return self.ck.rotate(ct, self.gk(step))
 
-    def rot_many(self, ct: Ciphertext, steps: List[int]) -> Ciphertext:
-        """Row b of ct rotated by steps[b] (a multiple of N/2 leaves it as it
-        is): the rows that move in one ckks_rotate_many call, each with its
-        own Galois key, or one rotation call per distinct step when
-        mixed_rotations is off.  Same words either way."""
+    def rot_blocks(self, ct: Ciphertext, B: int, steps: List[int]) -> List[Ciphertext]:
+        """ct holds len(steps) blocks of B rows; block i rotated by steps[i]
+        (a multiple of N/2 leaves it as it is).  The blocks that move go
+        through one ckks_rotate_many call, each row with its block's Galois
+        key, or through one rotation call per distinct step when
+        mixed_rotations is off; same words either way.  Returned blocks are
+        views of ct or of the rotation output (nothing is copied back)."""
         ck = self.ck
         half = ck.N // 2
-        move = [b for b, st in enumerate(steps) if st % half]
-        if not move:
-            return ct
-        n, lo = len(move), move[0]
-        contiguous = move == list(range(lo, lo + n))
-        idx = None if contiguous else torch.tensor(move, device=ck.device)
-        i64 = lambda t: t.view(torch.int64)       # row gathers and scatters on the same words
-        src = Ciphertext(ct.data[lo:lo + n] if contiguous else
-                         i64(ct.data).index_select(0, idx).view(torch.uint64), ct.scale)
-        key = tuple(steps[b] % half for b in move)
-        if self.mixed_rotations:
-            gks = [self.gk(st) for st in key]
-            if key not in self._rtab:
-                self._rtab[key] = ck.rotation_tables(gks, ck.device)   # the keys stay alive in self._gk
-            self.ops["rot"] += n
-            moved = ck.rotate_many(src, gks, tables=self._rtab[key]).data
-        else:
-            moved = torch.empty_like(src.data)
-            for st in sorted(set(key)):
-                rows = torch.tensor([i for i, k in enumerate(key) if k == st], device=ck.device)
-                part = Ciphertext(i64(src.data).index_select(0, rows).view(torch.uint64), ct.scale)
-                i64(moved).index_copy_(0, rows, i64(self.rot(part, st).data))
-        if n == ct.batch:
-            return Ciphertext(moved, ct.scale)
-        out = ct.data.clone()
-        if contiguous:
-            out[lo:lo + n] = moved
-        else:
-            i64(out).index_copy_(0, idx, i64(moved))
-        return Ciphertext(out, ct.scale)
+        blocks = [ct.data[i * B:(i + 1) * B] for i in range(len(steps))]
+        moving = [i for i, st in enumerate(steps) if st % half]
+        if moving:
+            m0 = moving[0]
+            if moving == list(range(m0, m0 + len(moving))):
+                src = ct.data[m0 * B:(m0 + len(moving)) * B]
+            else:
+                src = torch.cat([blocks[i] for i in moving])
+            if self.mixed_rotations:
+                key = tuple(steps[i] % half for i in moving for _ in range(B))
+                gks = [self.gk(st) for st in key]
+                if key not in self._rtab:
+                    self._rtab[key] = ck.rotation_tables(gks, ck.device)   # the keys stay alive in self._gk
+                self.ops["rot"] += len(key)
+                moved = ck.rotate_many(Ciphertext(src, ct.scale), gks, tables=self._rtab[key]).data
+                for j, i in enumerate(moving):
+                    blocks[i] = moved[j * B:(j + 1) * B]
+            else:
+                for j, i in enumerate(moving):
+                    blocks[i] = self.rot(Ciphertext(src[j * B:(j + 1) * B], ct.scale), steps[i]).data
+        return [Ciphertext(b, ct.scale) for b in blocks]
 
     def _slots(self, v: np.ndarray) -> np.ndarray:
         """A 1024-slot block pattern repeated for the S samples of a ciphertext."""
@@ -366,11 +359,10 @@ class EncryptedMLP(EncryptedNet):
         ck.ring.intt_(acc.view(-1, l, ck.N))
         # every giant step rescaled and rotated at once, each by its own step
         parts = ck.rescale(Ciphertext(acc.view(ng * B, 2, l, ck.N), ct.scale * self.w_scale))
-        parts = self.rot_many(parts, [p.baby * g for g in p.used_g for _ in range(B)])
-        P = parts.data.view(ng, B, 2, parts.level, ck.N)
-        z = Ciphertext(P[0], parts.scale)
-        for gi in range(1, ng):
-            z = ck.add(z, Ciphertext(P[gi], parts.scale))
+        blocks = self.rot_blocks(parts, B, [p.baby * g for g in p.used_g])
+        z = blocks[0]
+        for part in blocks[1:]:
+            z = ck.add(z, part)
         for s in p.folds:
             z = ck.add(z, self.rot(z, s))
         return z
