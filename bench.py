@@ -20,7 +20,8 @@ batch).  ``cpu_baseline`` times the CPU oracle on a bounded sample.
 decrypt-based monitor gear loop and under the learned gear controller;
 ``cnn`` does the same for the encrypted CNN (conv 5x5x8 stride 2, x^2,
 average pool, FC 288-10); ``batch_graph`` runs 2048 mixed operations (N = 4096 / 8192, levels 1..4,
-HMult / rotation / plaintext product) as one captured HIP graph.
+HMult / rotation / plaintext product) as one captured HIP graph; ``plain_product`` times 4096 negacyclic
+products at N = 4096 over one 45-bit prime.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256] [--no-mlp] [--no-cnn] [--no-graph]
     python bench.py --copy-probe      # plain copy of known size (counter scaling)
@@ -170,6 +171,67 @@ def modmul_probe(rows: int = 1024, reps: int = 10) -> None:
     print(json.dumps(res))
 
 
+Q1, PSI1 = 35184371884033, 7388705119   # largest 45-bit prime = 1 mod 8192, smallest primitive 8192-th root
+PRODUCT_BYTES = 3 * 4096 * 8             # two operands in, one product out
+
+
+def product_leg(count: int = 4096, reps: int = 20) -> dict:
+    """Plain negacyclic products at N = 4096 over the single 45-bit prime:
+    `count` products (forward transforms, point-wise product, inverse
+    transform, one fused kernel per product) with operands uniform in [0, q),
+    resident in HBM.  Every built thread count is timed; the fastest is
+    reported, priced at 98,304 algorithmic bytes a product."""
+    import numpy as np
+    import torch
+    from cipherpu.ring import RingContext
+    from cipherpu.tune import BUILT
+    ring = RingContext(4096, [Q1], roots=[PSI1])
+    rng = np.random.default_rng(11)
+    a_h = rng.integers(0, Q1, (count, 1, 4096), dtype=np.uint64)
+    b_h = rng.integers(0, Q1, (count, 1, 4096), dtype=np.uint64)
+    a, b = ring.tensor(a_h), ring.tensor(b_h)
+    out = torch.empty_like(a)
+    shapes = {}
+    for t in BUILT[12]:
+        ring.set_threads(t, 0, 0)
+        for _ in range(3):
+            ring.poly_mul(a, b, out=out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ring.poly_mul(a, b, out=out)
+        e1.record()
+        torch.cuda.synchronize()
+        shapes[t] = e0.elapsed_time(e1) / reps
+    best = min(shapes, key=shapes.get)
+    ms = shapes[best]
+    achieved = PRODUCT_BYTES * count / ms / 1e6
+    return {"workload": "negacyclic product, N=4096, one 45-bit prime", "products": count,
+            "products_per_s": round(count / ms * 1e3, 1), "ms": round(ms, 4), "threads": best,
+            "ms_by_threads": {str(k): round(v, 4) for k, v in shapes.items()},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": f"k_polymul<12,{best}>",
+                         "algorithmic_bytes_per_product": PRODUCT_BYTES}}
+
+
+def cpu_product_baseline(products: int = 2048) -> dict:
+    """The C oracle's transform product (N = 4096, the single 45-bit prime)
+    on one host core, for the plain-product leg."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle_c as OC
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, Q1, (products, 4096), dtype=np.uint64)
+    b = rng.integers(0, Q1, (products, 4096), dtype=np.uint64)
+    t0 = time.perf_counter()
+    for i in range(products):
+        OC.polymul_ntt(a[i], b[i], Q1, PSI1)
+    el = time.perf_counter() - t0
+    return {"value": products / el, "unit": "products/s", "cores": 1, "kind": "port",
+            "sample": f"{products} products (N=4096, one 45-bit prime) in the C oracle, {el:.2f} s"}
+
+
 def rotation_probe(ck, sk, batch: int = 64, steps=tuple(range(1, 16)), reps: int = 5) -> dict:
     """15 rotations of the same ciphertexts (a baby-step set): one at a time
     against hoisted (one shared digit decomposition)."""
@@ -294,6 +356,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--modmul-probe", action="store_true", help="Barrett vs special-form point-wise product")
     ap.add_argument("--no-graph", action="store_true", help="skip the uneven-batch graph leg")
+    ap.add_argument("--no-product", action="store_true", help="skip the N=4096 single-prime product leg")
     ap.add_argument("--graph-ops", type=int, default=2048)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -400,6 +463,8 @@ def main():
     }
     if not args.no_rotation_probe:
         res["rotation_probe"] = rotation_probe(ck, sk)
+    if not args.no_product:
+        res["plain_product"] = product_leg()
     if not args.no_mlp:
         res["mlp"] = mlp_leg(args.mlp_samples, args.target_snr, args.chunk_cts)
     if not args.no_cnn:
@@ -411,6 +476,8 @@ def main():
                              to_numpy(rlk.coeff), ck.primes, ck.ring.roots)
         res["cpu_baseline"] = py
         res["cpu_baseline_c"] = c
+        if not args.no_product:
+            res["cpu_baseline_product"] = cpu_product_baseline()
         res["cpu_model"] = cpu_model()
         res["cpu_count"] = os.cpu_count()
     print(json.dumps(res))
