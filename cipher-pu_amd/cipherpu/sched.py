@@ -69,6 +69,30 @@ def make_workload(n: int = 2048, seed: int = 0, degrees=(4096, 8192), levels=(1,
     return ops
 
 
+def bucket_ops(ops: List[Op]) -> List[Tuple[int, int, str, List[int], List[int]]]:
+    """Group operations by (N, level, kind), sorted: (N, level, kind, op
+    indices in workload order, rotation step of each op -- empty for the
+    other kinds).  Rotations by different steps share a group."""
+    groups: Dict[tuple, List[int]] = {}
+    for i, o in enumerate(ops):
+        groups.setdefault((o.N, o.level, o.kind), []).append(i)
+    return [(N, lv, k, idx, [ops[i].step for i in idx] if k == "rotate" else [])
+            for (N, lv, k), idx in sorted(groups.items())]
+
+
+def place(costs: List[float], n_streams: int) -> List[int]:
+    """Longest-first greedy placement of items with the given costs on
+    n_streams streams: each item goes to the stream with the least load so
+    far.  Returns the stream of each item."""
+    load = [0.0] * n_streams
+    where = [0] * len(costs)
+    for i in sorted(range(len(costs)), key=lambda i: costs[i], reverse=True):
+        k = min(range(n_streams), key=load.__getitem__)
+        where[i] = k
+        load[k] += costs[i]
+    return where
+
+
 @dataclass
 class Bucket:
     N: int
@@ -109,11 +133,7 @@ class BatchScheduler:
             self.rlk[N] = ck.relin_key(self.sk[N])
         for N, step in sorted({(o.N, o.step) for o in ops if o.kind == "rotate"}):
             self.gk[(N, step)] = self.ck[N].galois_key(self.sk[N], step)
-        groups: Dict[tuple, List[int]] = {}
-        for i, o in enumerate(ops):
-            groups.setdefault((o.N, o.level, o.kind), []).append(i)
-        self.buckets = [Bucket(N, lv, k, idx, [ops[i].step for i in idx] if k == "rotate" else [])
-                        for (N, lv, k), idx in sorted(groups.items())]
+        self.buckets = [Bucket(N, lv, k, idx, steps) for N, lv, k, idx, steps in bucket_ops(ops)]
         for bk in self.buckets:
             if bk.kind == "rotate":
                 bk.tables = CKKS.rotation_tables([self.gk[(bk.N, st)] for st in bk.steps], self.ck[bk.N].device)
@@ -152,12 +172,9 @@ class BatchScheduler:
     def assign_streams(self) -> None:
         """Longest-first greedy placement of the buckets on the streams, by
         tuned time (or, before tuning, by an estimate of the work)."""
-        load = [0.0] * self.n_streams
         cost = lambda bk: bk.ms if bk.ms > 0 else bk.batch * bk.N * bk.limbs ** 2
-        for bk in sorted(self.buckets, key=cost, reverse=True):
-            k = min(range(self.n_streams), key=load.__getitem__)
+        for bk, k in zip(self.buckets, place([cost(bk) for bk in self.buckets], self.n_streams)):
             bk.stream = k
-            load[k] += cost(bk)
 
     # -- launches ---------------------------------------------------------
     def _launch(self, bk: Bucket, stream: int = 0) -> None:

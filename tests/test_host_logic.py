@@ -24,6 +24,41 @@ def test_workload_generator_mix_and_levels():
     assert make_workload(64, seed=3) == make_workload(64, seed=3)
 
 
+def test_buckets_group_by_degree_level_kind():
+    """Every operation lands in exactly one bucket; a bucket holds one
+    (N, level, kind) in workload order; rotations by different steps share
+    a bucket and keep their own steps."""
+    from cipherpu.sched import bucket_ops, make_workload
+    ops = make_workload(2048, seed=0)
+    groups = bucket_ops(ops)
+    assert len(groups) == 24                     # 2 degrees x 4 levels x 3 kinds
+    seen = sorted(i for *_, idx, _ in groups for i in idx)
+    assert seen == list(range(len(ops)))
+    for N, lv, kind, idx, steps in groups:
+        assert idx == sorted(idx)
+        assert all((ops[i].N, ops[i].level, ops[i].kind) == (N, lv, kind) for i in idx)
+        assert steps == ([ops[i].step for i in idx] if kind == "rotate" else [])
+    assert any(len(set(steps)) > 1 for *_, steps in groups)
+
+
+def test_longest_first_placement():
+    """Greedy placement: the largest items go to distinct streams first and
+    no stream ends more than the largest item above the lightest."""
+    from cipherpu.sched import place
+    costs = [5.0, 1.0, 4.0, 3.0, 3.0, 2.0, 2.0, 9.0]
+    where = place(costs, 3)
+    load = [sum(c for c, w in zip(costs, where) if w == k) for k in range(3)]
+    assert sorted(load) == [9.0, 10.0, 10.0]
+    assert len({where[7], where[0], where[2]}) == 3
+    assert max(load) - min(load) <= max(costs)
+    assert place(costs, 1) == [0] * len(costs)
+    rng = np.random.default_rng(1)
+    c = rng.uniform(0, 1, 200).tolist()
+    w = place(c, 4)
+    loads = [sum(x for x, k in zip(c, w) if k == j) for j in range(4)]
+    assert max(loads) - min(loads) <= max(c)
+
+
 def test_choose_gear_margin_and_fallback():
     from cipherpu.gear import choose_gear
     # cheapest gear first: gear 3 clears 10 + 3 dB
