@@ -306,6 +306,73 @@ __global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64
     }
 }
 
+// The same ModDown (+ rescale) with the limb count L fixed at compile time,
+// for the chains in use (L <= 8).  Each thread takes coefficients n and
+// n + N/2 of one row, so the per-limb constants (uniform, in SGPRs) and the
+// index arithmetic serve two coefficients, and the loops unroll.  The
+// modular steps keep lazy sums: y + h + q - t < 3q enters the Shoup product
+// (exact for any input below 2^62), whose result in [0, 2q) is brought to
+// [0, q) by one subtraction -- the same canonical words as the general
+// kernel above.
+template <int L, bool RESCALE>
+__global__ void __launch_bounds__(256) k_moddown_rescale_l(const uint64_t* __restrict__ acc,
+                                                           const uint64_t* __restrict__ dcoef, int dmask,
+                                                           uint64_t* __restrict__ out,
+                                                           const PrimeConst* __restrict__ pcs, LimbMap map,
+                                                           int pidx_p, ModDownConst mc, int narrow, int logN,
+                                                           size_t total_half) {
+    constexpr int V = 2;
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total_half) return;
+    const size_t N = (size_t)1 << logN, H = N >> 1;
+    const size_t n = g & (H - 1);
+    const size_t bp = g >> (logN - 1);        // (batch, poly) row
+    const uint64_t* in = acc + bp * (L + 1) * N + n;
+    const PrimeConst& pp = pcs[pidx_p];
+    uint64_t tp[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) tp[v] = csub(in[(size_t)L * N + v * H] + (pp.q >> 1), pp.q);
+    constexpr int LO = RESCALE ? L - 1 : L;
+    uint64_t* o = out + bp * LO * N + n;
+    const uint64_t* dc = (dcoef && ((dmask >> (bp & 1)) & 1)) ? dcoef + bp * L * N + n : nullptr;
+    uint64_t md[L][V];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const PrimeConst& pc = pcs[map.idx[i]];
+        const uint64_t q = pc.q;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const uint64_t tpi = narrow ? csub(tp[v], q) : reduce64(tp[v], pc);
+            const uint64_t u = in[(size_t)i * N + v * H] + mc.hp[i] + q - tpi;   // < 3q
+            uint64_t r = csub(mul_shoup(u, mc.pinv[i], mc.pinv_s[i], q), q);
+            if (dc) r = add_mod(r, dc[(size_t)i * N + v * H], q);
+            md[i][v] = r;
+        }
+    }
+    if constexpr (!RESCALE) {
+#pragma unroll
+        for (int i = 0; i < L; ++i)
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[(size_t)i * N + v * H] = md[i][v];
+    } else {
+        const PrimeConst& pl = pcs[map.idx[L - 1]];
+        uint64_t tl[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) tl[v] = csub(md[L - 1][v] + (pl.q >> 1), pl.q);
+#pragma unroll
+        for (int i = 0; i < L - 1; ++i) {
+            const PrimeConst& pc = pcs[map.idx[i]];
+            const uint64_t q = pc.q;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const uint64_t tli = narrow ? csub(tl[v], q) : reduce64(tl[v], pc);
+                const uint64_t u = md[i][v] + mc.hq[i] + q - tli;                // < 3q
+                o[(size_t)i * N + v * H] = csub(mul_shoup(u, mc.qinv[i], mc.qinv_s[i], q), q);
+            }
+        }
+    }
+}
+
 // Rescale by the last limb (verify_rescale.py:38-57, RNS per-limb form).
 __global__ void k_rescale(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                           const PrimeConst* __restrict__ pcs, LimbMap map, int l, ModDownConst mc, int narrow,
@@ -650,6 +717,31 @@ hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const
                                   bool rescale, hipStream_t s) {
     const size_t total = ((size_t)batch * 2) << t.logN;
     if (!total) return hipSuccess;
+    if (l >= 1 && l <= 8 && t.logN >= 1) {
+        const size_t half = total >> 1;
+        const dim3 grid((unsigned)((half + 255) / 256));
+#define CKKS_MD_CASE(LL)                                                                                          \
+    case LL:                                                                                                      \
+        if (rescale)                                                                                              \
+            hipLaunchKernelGGL((k_moddown_rescale_l<LL, true>), grid, dim3(256), 0, s, acc, dcoef, dmask, out,     \
+                               t.pc, map, pidx_special, mc, t.narrow, t.logN, half);                              \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_moddown_rescale_l<LL, false>), grid, dim3(256), 0, s, acc, dcoef, dmask, out,    \
+                               t.pc, map, pidx_special, mc, t.narrow, t.logN, half);                              \
+        return hipGetLastError();
+        switch (l) {
+            CKKS_MD_CASE(1)
+            CKKS_MD_CASE(2)
+            CKKS_MD_CASE(3)
+            CKKS_MD_CASE(4)
+            CKKS_MD_CASE(5)
+            CKKS_MD_CASE(6)
+            CKKS_MD_CASE(7)
+            CKKS_MD_CASE(8)
+            default: break;
+        }
+#undef CKKS_MD_CASE
+    }
     hipLaunchKernelGGL(k_moddown_rescale, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, acc, dcoef, dmask, out,
                        t.pc, map, l, pidx_special, mc, rescale ? 1 : 0, t.narrow, t.logN, total);
     return hipGetLastError();

@@ -593,6 +593,51 @@ def test_wide_chain_bit_exact(gpu):
                               to_numpy(c.ring.ntt(co.data.reshape(-1, 4, c.N)))), gk.galois
 
 
+def _is_prime(n):
+    if n < 2:
+        return False
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d, s = d // 2, s + 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def test_long_chain_hmult_bit_exact(gpu):
+    """HMult on a chain of 10 primes (2^44 + 2^15 j + 1) at levels 10, 9 and
+    8 against the oracle: ModDown + rescale runs its compile-time limb-count
+    kernel for up to 8 chain limbs and the general kernel above that, and
+    both must give the oracle's words."""
+    from cipherpu.ckks import CKKS, CKKSParams, Ciphertext
+    from cipherpu.ring import to_numpy
+    primes = [q for q in ((1 << 44) + (j << 15) + 1 for j in range(1, 400)) if _is_prime(q)][:11]
+    assert len(primes) == 11
+    chain, special = primes[:10], primes[10]
+    c = CKKS(CKKSParams(N=4096, primes=chain, special=special), seed=61)
+    sk = c.keygen()
+    rlk = c.relin_key(sk)
+    key = to_numpy(rlk.coeff)
+    roots = c.ring.roots
+    rng = np.random.default_rng(61)
+    for l in (10, 9, 8):
+        mk = lambda: np.stack([rng.integers(0, q, (2, 2, c.N), dtype=np.uint64) for q in chain[:l]], axis=2)
+        a, b = mk(), mk()
+        out = to_numpy(c.hmult(Ciphertext(c.ring.tensor(a), 1.0), Ciphertext(c.ring.tensor(b), 1.0), rlk).data)
+        assert out.shape == (2, 2, l - 1, c.N)
+        for i in range(2):
+            want = OC.hmult(a[i], b[i], key, chain[:l], special, roots[:l], roots[10])
+            assert np.array_equal(out[i], want), (l, i)
+
+
 def test_hoisted_placement_never_changes_a_word(gpu):
     """The hoisted key switch under every workgroup placement (tile sizes 1
     to the whole batch, ragged last tiles, with and without the per-XCD
