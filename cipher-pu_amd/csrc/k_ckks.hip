@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
             o2[idx] = d2;
             x0[k] = d2;
         }
-        ntt::inv_ntt<C>(x0, lds, tw + C::N, pc, tid);
+        ntt::inv_ntt_db<C>(x0, lds, tw + C::N, pc, tid);
 #pragma unroll
         for (int k = 0; k < C::E; ++k) x0[k] = csub(x0[k], pc.q);
         ntt::store_coalesced<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
@@ -228,10 +228,13 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
                 for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
             }
             // left unreduced (below 2^51): the key product is exact for any such input
-            ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
+            if constexpr (ntt::pair_pays<C>()) ntt::fwd_ntt_db<C>(x, lds, tw, pc.q, pc.two_q, tid);
+            else ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
         }
         key_mac<C>(a0, a1, x, key, j, key_limbs, kt, nq, tid);
     }
+    // the double-buffered digit transforms leave reads of the LDS in flight
+    if constexpr (ntt::pair_pays<C>()) __syncthreads();
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
         a0[k] = reduce64(a0[k], pc);

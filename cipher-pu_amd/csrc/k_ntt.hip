@@ -89,7 +89,9 @@ __global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, c
     }
 #pragma unroll
     for (int e = 0; e < C::E; ++e) x[e] = mul_barrett(reduce64(x[e], pc), reduce64(y[e], pc), pc);
-    ntt::inv_ntt<C>(x, lds, tw + C::N, pc, tid);
+    // the pair's two exchange buffers serve the inverse as alternating ones
+    if constexpr (ntt::pair_pays<C>()) ntt::inv_ntt_db<C>(x, lds, tw + C::N, pc, tid);
+    else ntt::inv_ntt<C>(x, lds, tw + C::N, pc, tid);
 #pragma unroll
     for (int e = 0; e < C::E; ++e) x[e] = csub(x[e], pc.q);
     ntt::store_coalesced<C>(x, out + off, tid);

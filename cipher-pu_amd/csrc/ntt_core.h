@@ -248,6 +248,43 @@ __device__ __forceinline__ void fwd_ntt(uint64_t (&x)[C::E], uint64_t* lds,
     fwd_rest<C, 1>(x, lds, tw, q, two_q, tid);
 }
 
+// Forward passes with two alternating exchange buffers (pass P uses buffer
+// P & 1), for kernels that own 2 LDS_WORDS of LDS.  An exchange then needs
+// only its write -> read ordering: the next write into a buffer comes two
+// exchanges later, and every wave has finished reading it by the time it
+// crosses the workgroup barrier of the exchange in between.  Where that
+// exchange is wave-local, a workgroup barrier is put before the write.
+template <class C, int P>
+__device__ __forceinline__ void fwd_rest_db(uint64_t (&x)[C::E], uint64_t* lds,
+                                            const ulonglong2* __restrict__ tw, uint64_t q,
+                                            uint64_t two_q, int tid) {
+    if constexpr (P < C::NP) {
+        constexpr bool W = wave_local<C, P>();
+        if constexpr (P == 1) __syncthreads();
+        else if constexpr (P >= 3 && wave_local<C, P - 1>()) __syncthreads();
+        uint64_t* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
+        to_lds<C, P - 1>(x, buf, tid);
+        exchange_sync<W>();
+        from_lds<C, P>(x, buf, tid);
+        fwd_pass<C, P>(x, tw, q, two_q, tid);
+        fwd_rest_db<C, P + 1>(x, lds, tw, q, two_q, tid);
+    }
+}
+
+// fwd_ntt through two alternating exchange buffers (2 LDS_WORDS of LDS):
+// same entry and exit conditions, but on exit other waves may still be
+// reading the LDS, so the caller puts a workgroup barrier before any other
+// LDS write (a following fwd_ntt_db starts with one).
+template <class C>
+__device__ __forceinline__ void fwd_ntt_db(uint64_t (&x)[C::E], uint64_t* lds,
+                                           const ulonglong2* __restrict__ tw, uint64_t q,
+                                           uint64_t two_q, int tid) {
+    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
+    opaque(tw, tid);
+    fwd_pass<C, 0>(x, tw, q, two_q, tid);
+    fwd_rest_db<C, 1>(x, lds, tw, q, two_q, tid);
+}
+
 template <class C, int P>
 __device__ __forceinline__ void inv_rest(uint64_t (&x)[C::E], uint64_t* lds,
                                          const ulonglong2* __restrict__ itw, const PrimeConst& pc,
@@ -261,6 +298,41 @@ __device__ __forceinline__ void inv_rest(uint64_t (&x)[C::E], uint64_t* lds,
         inv_pass<C, P>(x, itw, pc, tid);
         inv_rest<C, P - 1>(x, lds, itw, pc, tid);
     }
+}
+
+// Inverse passes with two alternating exchange buffers (the exchange into
+// pass P's layout uses buffer P & 1); the same ordering argument as
+// fwd_rest_db, read in the other direction.  The first exchange starts with
+// a workgroup barrier, since what used the LDS before is not known here.
+template <class C, int P>
+__device__ __forceinline__ void inv_rest_db(uint64_t (&x)[C::E], uint64_t* lds,
+                                            const ulonglong2* __restrict__ itw, const PrimeConst& pc,
+                                            int tid) {
+    if constexpr (P >= 0) {
+        constexpr bool W = wave_local<C, P + 1>();
+        if constexpr (P == C::NP - 2) __syncthreads();
+        else if constexpr (P <= C::NP - 4 && wave_local<C, P + 2>()) __syncthreads();
+        uint64_t* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
+        to_lds<C, P + 1>(x, buf, tid);
+        exchange_sync<W>();
+        from_lds<C, P>(x, buf, tid);
+        inv_pass<C, P>(x, itw, pc, tid);
+        inv_rest_db<C, P - 1>(x, lds, itw, pc, tid);
+    }
+}
+
+// inv_ntt through two alternating exchange buffers (2 LDS_WORDS of LDS):
+// same entry and exit conditions; on exit other waves may still be reading
+// the LDS, so the caller puts a workgroup barrier before any further LDS
+// write.
+template <class C>
+__device__ __forceinline__ void inv_ntt_db(uint64_t (&x)[C::E], uint64_t* lds,
+                                           const ulonglong2* __restrict__ itw, const PrimeConst& pc,
+                                           int tid) {
+    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
+    opaque(itw, tid);
+    inv_pass<C, C::NP - 1>(x, itw, pc, tid);
+    inv_rest_db<C, C::NP - 2>(x, lds, itw, pc, tid);
 }
 
 // Full inverse transform: x holds the last forward pass's layout on entry
