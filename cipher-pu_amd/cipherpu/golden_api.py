@@ -13,11 +13,18 @@ Reference surface -> here:
                                        of the toy rescale, see below)
   scripts/exp_e24_rotation_bootstrap.py automorphism_exact, automorphism_key_exact
   scripts/verify_multi_slot_simd.py    neg_fft, neg_ifft, encode, decode
+  scripts/verify_m4_h_multilayer.py    noise_monitor_energy
+  scripts/verify_m4h2_prod_calibration.py  compute_snr, snr_to_gear
+  scripts/exp_e11_ckks_e2e_real.py     estimate_r_hat, select_gear
+  scripts/verify_project_b_mlp_reference.py  AdaptiveGearCtrl
+  rtl/m4/adaptive_gear_ctrl.sv         RtlGearCtrl (register transfer)
 """
 from __future__ import annotations
 
 from functools import lru_cache
 from typing import Sequence
+
+import math
 
 import numpy as np
 import torch
@@ -250,3 +257,132 @@ def encode(v, scale, n):
 def decode(m, scale, n, nslot):
     """verify_multi_slot_simd.py:87-91"""
     return neg_fft(np.asarray(m, dtype=np.float64)).real[:nslot] / scale
+
+
+# ---------------------------------------------------------------------------
+# Noise monitor and gear policy (the monitor's energy runs on the GPU; the
+# policies are scalar host logic, as in the reference).
+
+def noise_monitor_energy(coeffs, q):
+    """(E, r_hat) with E = sum min(c, q - c)^2 / count and r_hat = sqrt(E)/q
+    (scripts/verify_m4_h_multilayer.py:162-174; rtl/m4/noise_monitor.sv).
+    The sum of squares is the statistics kernel's exact 128-bit row sum; the
+    row is zero-padded to a power of two (zeros add nothing).  q must admit
+    a transform context at the padded length (the chain primes do); other
+    moduli raise ValueError."""
+    c = np.asarray(coeffs, dtype=np.uint64).reshape(-1)
+    n = c.size
+    if n == 0:
+        raise ZeroDivisionError("noise_monitor_energy of an empty coefficient list")
+    if int(c.max()) >= q:
+        raise ValueError("coefficients must lie in [0, q)")
+    N = max(8, 1 << (n - 1).bit_length())
+    ctx = None
+    for cyclic in (False, True):
+        try:
+            ctx = _ring(N, int(q), None, cyclic)
+            break
+        except Exception:
+            continue
+    if ctx is None:
+        raise ValueError(f"no transform context for q = {q} at length {N}")
+    row = np.zeros((1, N), dtype=np.uint64)
+    row[0, :n] = c
+    st = ctx.noise_stats(ctx.tensor(row))
+    E = st[0][1] / n
+    return E, math.sqrt(E) / q
+
+
+def compute_snr(signal, noise):
+    """10 log10(sum |signal|^2 / sum |noise|^2) in dB, inf for zero noise
+    (scripts/verify_m4h2_prod_calibration.py:64-69)."""
+    sp = float(np.sum(np.abs(np.asarray(signal)) ** 2))
+    npw = float(np.sum(np.abs(np.asarray(noise)) ** 2))
+    return math.inf if npw == 0 else 10.0 * math.log10(sp / npw)
+
+
+def snr_to_gear(snr_db, threshold_db=15.0):
+    """'Gear 3' above 20 dB, 'Gear 2' above threshold_db, 'Gear 1' above
+    10 dB, else 'Gear 0' -- the reference's labels
+    (scripts/verify_m4h2_prod_calibration.py:72-82)."""
+    if snr_db > 20:
+        return "Gear 3"
+    if snr_db > threshold_db:
+        return "Gear 2"
+    if snr_db > 10:
+        return "Gear 1"
+    return "Gear 0"
+
+
+# Largest noise ratio r_hat each gear tolerates, gear 0 (strict) to 3
+# (scripts/exp_e11_ckks_e2e_real.py:182-187,
+# scripts/verify_project_b_mlp_reference.py:245-250).
+GEAR_R_MAX = (0.10, 0.20, 0.35, 0.55)
+
+
+def estimate_r_hat(decrypted_val, true_val):
+    """Relative decryption error, capped at 1 (exp_e11_ckks_e2e_real.py:169-176)."""
+    return float(min(abs(decrypted_val - true_val) / (abs(true_val) + 1e-8), 1.0))
+
+
+def select_gear(r_hat, safety_first=True):
+    """Safety first: the strictest gear whose r_max tolerates r_hat, gear 0 if
+    none does; otherwise the loosest such gear (exp_e11_ckks_e2e_real.py:189-200)."""
+    ok = [g for g, r in enumerate(GEAR_R_MAX) if r >= r_hat]
+    if not ok:
+        return 0
+    return ok[0] if safety_first else ok[-1]
+
+
+class AdaptiveGearCtrl:
+    """Safety-first gear choice with hysteresis: a new target gear must be
+    seen on `hyst` + 1 consecutive steps before the gear changes; starts at
+    the loosest gear (scripts/verify_project_b_mlp_reference.py:279-317)."""
+
+    def __init__(self, hyst: int = 2, gear: int = 3):
+        self.gear, self.hyst = gear, hyst
+        self.hyst_cnt, self.hyst_target = 0, -1
+
+    def step(self, r_hat: float) -> int:
+        target = select_gear(r_hat, True)
+        if target != self.gear:
+            if target == self.hyst_target:
+                self.hyst_cnt += 1
+            else:
+                self.hyst_target, self.hyst_cnt = target, 0
+            if self.hyst_cnt >= self.hyst:
+                self.gear, self.hyst_cnt, self.hyst_target = target, 0, -1
+        return self.gear
+
+
+class RtlGearCtrl:
+    """The register transfer of rtl/m4/adaptive_gear_ctrl.sv:42-100, one
+    step per flush_valid (hysteresis on, no manual gear).  Thresholds
+    t1 > t2 > t3 map the monitor's energy to a raw gear (>= t1 -> 0,
+    >= t2 -> 1, >= t3 -> 2, else 3).  A stricter raw gear is taken at once.
+    A looser one is a candidate: the same candidate again counts up, a new
+    one restarts the count at 1, and the gear moves to the reading when the
+    count before this step had reached `hyst` (the switch test reads the
+    register's old value).  A reading at the current gear clears the count."""
+
+    def __init__(self, t1: float, t2: float, t3: float, hyst: int = 3):
+        self.t = (t1, t2, t3)
+        self.hyst = hyst
+        self.gear, self.cnt, self.cand = 3, 0, 3      # reset values
+
+    def raw(self, energy: float) -> int:
+        return next((g for g, t in enumerate(self.t) if energy >= t), 3)
+
+    def step(self, energy: float) -> int:
+        r = self.raw(energy)
+        gear, cnt = self.gear, self.cnt
+        if r < gear:
+            self.gear, self.cnt, self.cand = r, 0, r
+        elif r == gear:
+            self.cnt, self.cand = 0, r
+        else:
+            self.cnt = cnt + 1 if r == self.cand else 1
+            self.cand = r
+            if cnt >= self.hyst:
+                self.gear, self.cnt, self.cand = r, 0, gear
+        return self.gear

@@ -623,15 +623,21 @@ class AdaptiveGearCtrl:
 
 @dataclass
 class RtlGearCtrl:
-    """Energy-threshold gear selection with immediate down-shift and
-    up-shift after ``hyst`` consecutive candidates
-    (rtl/m4/adaptive_gear_ctrl.sv:1-17,42-100)."""
+    """One flush_valid step of rtl/m4/adaptive_gear_ctrl.sv:42-100 with
+    hysteresis_en = 1 and manual_en = 0 (the register transfer, non-blocking
+    semantics: the switch test reads the counter's value before the step).
+    raw: energy >= t1 -> 0, >= t2 -> 1, >= t3 -> 2, else 3 (:44-50).
+    A stricter raw gear is taken at once (:77-80); a looser one moves the
+    candidate/counter (:66-71) and the gear switches to it when the previous
+    counter had reached hyst (:72-76), i.e. on the (hyst+1)-th consecutive
+    reading of one candidate.  Reset: gear 3, counter 0, candidate 3."""
     t1: float
     t2: float
     t3: float
     hyst: int = 3
     gear: int = 3
     cnt: int = 0
+    cand: int = 3
 
     def raw(self, energy: float) -> int:
         if energy >= self.t1:
@@ -644,14 +650,18 @@ class RtlGearCtrl:
 
     def step(self, energy: float) -> int:
         r = self.raw(energy)
-        if r < self.gear:
-            self.gear, self.cnt = r, 0
-        elif r > self.gear:
-            self.cnt += 1
-            if self.cnt >= self.hyst:
-                self.gear, self.cnt = r, 0
+        old_gear, old_cnt = self.gear, self.cnt
+        if r > old_gear:
+            if r == self.cand:
+                self.cnt = old_cnt + 1
+            else:
+                self.cand, self.cnt = r, 1
+            if old_cnt >= self.hyst:
+                self.gear, self.cnt, self.cand = r, 0, old_gear
+        elif r < old_gear:
+            self.gear, self.cnt, self.cand = r, 0, r
         else:
-            self.cnt = 0
+            self.cnt, self.cand = 0, r
         return self.gear
 
 

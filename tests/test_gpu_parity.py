@@ -205,6 +205,22 @@ def test_noise_stats(gpu):
             assert row[1] / N == E
 
 
+def test_noise_monitor_energy_drop_in(gpu):
+    """golden_api.noise_monitor_energy (the statistics kernel's exact sums)
+    against the oracle's restatement of verify_m4_h_multilayer.py:162-174 on
+    ragged lengths, edge values and a prime of each chain."""
+    from cipherpu import golden_api as G
+    rng = np.random.default_rng(12)
+    for q, n in ((CHAIN4[3], 1), (CHAIN4[3], 5), (CHAIN4[0], 1000), (CHAIN4[1], 4096), (CHAIN4[2], 8191)):
+        c = [int(v) for v in rng.integers(0, q, n, dtype=np.uint64)]
+        c[0] = q - 1
+        if n > 2:
+            c[1], c[2] = 0, q // 2
+        assert G.noise_monitor_energy(c, q) == O.noise_monitor_energy(c, q), (q, n)
+    with pytest.raises(ValueError):
+        G.noise_monitor_energy([CHAIN4[0]], CHAIN4[0])
+
+
 # ---------------------------------------------------------------- CKKS ops
 
 @pytest.fixture(scope="module")

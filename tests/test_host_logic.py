@@ -115,3 +115,30 @@ def test_wave_local_exchanges_table():
                     for tid in range(64 * w, 64 * w + 64) for g in range(G) for e in range(M) for f in [g * T + tid]}
         got = [all(slots(P - 1, w) == slots(P, w) for w in range(T // 64)) for P in range(1, NP)]
         assert got == table, (LOGN, T, got)
+
+
+def test_reference_gear_policies_match_the_oracle():
+    """The drop-in monitor/gear policy functions (host logic) against the
+    oracle's restatements, which are pinned to the reference's thresholds
+    and hand-traced register sequences, on random reading sequences."""
+    import ckks_oracle as O
+    from cipherpu import golden_api as G
+    rng = np.random.default_rng(7)
+    for x in list(rng.uniform(-5, 30, 200)) + [20, 15, 10, 10.0000001]:
+        assert G.snr_to_gear(x) == f"Gear {O.snr_to_gear(x)}"
+        assert G.snr_to_gear(x, 12.0) == f"Gear {O.snr_to_gear(x, 12.0)}"
+    for r in list(rng.uniform(0, 0.7, 200)) + [0.10, 0.20, 0.35, 0.55, 0.551]:
+        assert G.select_gear(r) == O.AdaptiveGearCtrl.best_gear(r)
+    assert [G.select_gear(r, safety_first=False) for r in (0.05, 0.15, 0.3, 0.5, 0.9)] == [3, 3, 3, 3, 0]
+    for _ in range(20):
+        rs = rng.choice([0.05, 0.15, 0.3, 0.5, 0.9], size=60)
+        g, o = G.AdaptiveGearCtrl(), O.AdaptiveGearCtrl()
+        assert [g.step(r) for r in rs] == [o.step(r) for r in rs]
+        es = rng.choice([600, 300, 100, 10], size=80)
+        g, o = G.RtlGearCtrl(500, 200, 50), O.RtlGearCtrl(500, 200, 50)
+        assert [g.step(e) for e in es] == [o.step(e) for e in es]
+    s, n = rng.normal(size=64), rng.normal(size=64) * 1e-3
+    assert G.compute_snr(s, n) == O.compute_snr(s, n)
+    assert math.isinf(G.compute_snr(s, np.zeros(4)))
+    assert G.estimate_r_hat(1.1, 1.0) == min(abs(1.1 - 1.0) / (1.0 + 1e-8), 1.0)
+    assert G.estimate_r_hat(5.0, 0.0) == 1.0
