@@ -18,6 +18,9 @@ Reference surface -> here:
   scripts/exp_e11_ckks_e2e_real.py     estimate_r_hat, select_gear
   scripts/verify_project_b_mlp_reference.py  AdaptiveGearCtrl
   rtl/m4/adaptive_gear_ctrl.sv         RtlGearCtrl (register transfer)
+  scripts/exp_e24_rotation_bootstrap.py neg_conv_modq, decrypt_exact_int,
+                                       rlwe_encrypt_mint, reencrypt (DARE)
+  scripts/exp_e11_ckks_e2e_real.py     rlwe_encrypt, rlwe_decrypt
 """
 from __future__ import annotations
 
@@ -386,3 +389,78 @@ class RtlGearCtrl:
             if cnt >= self.hyst:
                 self.gear, self.cnt, self.cand = r, 0, gear
         return self.gear
+
+
+# ---------------------------------------------------------------------------
+# Single-modulus RLWE helpers of the reference's experiments
+# (scripts/exp_e24_rotation_bootstrap.py, scripts/exp_e11_ckks_e2e_real.py).
+# The negacyclic products run on the GPU and are exact; the mask a is drawn
+# from numpy's global generator with the reference's own calls, so a run
+# seeded like the reference's gets the same a.  q needs 2n | q - 1.
+
+def _residues(x, q) -> np.ndarray:
+    return np.array([int(v) % q for v in np.asarray(x).reshape(-1)], dtype=np.uint64)
+
+
+def _negconv(x, y, q) -> np.ndarray:
+    n = len(x)
+    if not _ntt_ok(n, q):
+        raise ValueError(f"the GPU product needs a prime with 2n | q-1 (n={n}, q={q})")
+    ctx = _ring(n, q, None, False)
+    xs = ctx.tensor(_residues(x, q).reshape(1, n))
+    ys = ctx.tensor(_residues(y, q).reshape(1, n))
+    return to_numpy(ctx.poly_mul(xs, ys)).reshape(n).astype(np.int64)
+
+
+def neg_conv_modq(x, y, q):
+    """Exact negacyclic product mod (X^n + 1, q), int64
+    (exp_e24_rotation_bootstrap.py:80-106)."""
+    return _negconv(x, y, q)
+
+
+def decrypt_exact_int(ct, s, q):
+    """(b - a s) mod q centred to (-q/2, q/2], int64
+    (exp_e24_rotation_bootstrap.py:126-133)."""
+    d = (_residues(ct[0], q).astype(np.int64) - _negconv(ct[1], s, q)) % q
+    return np.where(d > q // 2, d - q, d).astype(np.int64)
+
+
+def _draw_a(n: int, q: int) -> np.ndarray:
+    # exp_e24_rotation_bootstrap.py:217-220 / exp_e11_ckks_e2e_real.py:79-82
+    if q <= 2 ** 31 - 1:
+        return np.random.randint(0, q, size=n).astype(np.int64)
+    return np.floor(np.random.random(size=n) * q).astype(np.int64)
+
+
+def rlwe_encrypt_mint(m_int, s, e, q):
+    """(b, a) with b = a s + e + m mod q for an integer message, int64
+    (exp_e24_rotation_bootstrap.py:209-222)."""
+    a = _draw_a(len(s), q)
+    extra = (_residues(e, q).astype(np.int64) + _residues(m_int, q).astype(np.int64)) % q
+    return (_negconv(a, s, q) + extra) % q, a
+
+
+def reencrypt(ct, s_key, q, scale, rng):
+    """DARE refresh: the decrypted integer message (scale kept, no rounding)
+    encrypted afresh with ternary noise drawn from rng; returns
+    ((b, a), m_new) (exp_e24_rotation_bootstrap.py:287-304)."""
+    m_new = decrypt_exact_int(ct, s_key, q)
+    e_new = rng.integers(-1, 2, size=len(s_key)).astype(np.int64)
+    return rlwe_encrypt_mint(m_new, s_key, e_new, q), m_new
+
+
+def rlwe_encrypt(m_poly, s, e, q, scale):
+    """(b, a) with b = a s + e + scale m mod q, the fractional message part
+    kept in float64 (exp_e11_ckks_e2e_real.py:72-88).  a s is exact and
+    reduced mod q before the float64 sum, where the reference sums its
+    float64 convolution unreduced: equal up to float64 rounding."""
+    a = _draw_a(len(s), q)
+    total = _negconv(a, s, q).astype(np.float64) + np.asarray(e, dtype=np.float64)
+    return np.mod(total + scale * np.asarray(m_poly, dtype=np.float64), q), a
+
+
+def rlwe_decrypt(ct, s, q):
+    """(b - a s) mod q centred to [-q/2, q/2) in float64, i.e. scale m + e
+    (exp_e11_ckks_e2e_real.py:66-70,91-99)."""
+    v = np.mod(np.asarray(ct[0], dtype=np.float64) - _negconv(ct[1], s, q).astype(np.float64), q)
+    return np.where(v > q / 2, v - q, v)

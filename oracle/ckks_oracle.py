@@ -554,6 +554,71 @@ def reencrypt_message(dec_limbs):
 
 
 # ---------------------------------------------------------------------------
+# single-modulus RLWE helpers of the reference's experiments (exact integers)
+# ---------------------------------------------------------------------------
+
+def neg_conv_modq(x, y, q):
+    """Exact negacyclic product mod (X^n + 1, q) as int64
+    (exp_e24_rotation_bootstrap.py:80-106; the reference splits operands
+    into 10-bit pieces so float64 stays exact -- here: big integers)."""
+    return np.array(poly_mult([int(v) % q for v in x], [int(v) % q for v in y], q), dtype=np.int64)
+
+
+def decrypt_exact_int(ct, s, q):
+    """(b - a s) mod q centred to (-q/2, q/2] as int64
+    (exp_e24_rotation_bootstrap.py:126-133)."""
+    b = [int(v) % q for v in ct[0]]
+    a_s = neg_conv_modq(ct[1], s, q)
+    d = [(x - int(y)) % q for x, y in zip(b, a_s)]
+    return np.array([v - q if v > q // 2 else v for v in d], dtype=np.int64)
+
+
+def draw_uniform_a(n, q):
+    """The reference's draw of the mask a from numpy's global generator:
+    randint below 2^31 - 1, floor(random * q) above
+    (exp_e24_rotation_bootstrap.py:217-220, exp_e11_ckks_e2e_real.py:79-82)."""
+    if q <= 2 ** 31 - 1:
+        return np.random.randint(0, q, size=n).astype(np.int64)
+    return np.floor(np.random.random(size=n) * q).astype(np.int64)
+
+
+def rlwe_encrypt_mint(m_int, s, e, q):
+    """b = a s + e + m mod q for an integer message (exp_e24:209-222)."""
+    a = draw_uniform_a(len(s), q)
+    a_s = neg_conv_modq(a, s, q)
+    b = [(int(x) + int(ei) + int(mi)) % q for x, ei, mi in zip(a_s, e, m_int)]
+    return np.array(b, dtype=np.int64), a
+
+
+def reencrypt(ct, s_key, q, scale, rng):
+    """DARE refresh: decrypt to the scale-carrying integer message and
+    encrypt it afresh with ternary noise from rng (exp_e24:287-304).
+    Returns ((b, a), m_new)."""
+    m_new = decrypt_exact_int(ct, s_key, q)
+    e_new = rng.integers(-1, 2, size=len(s_key)).astype(np.int64)
+    return rlwe_encrypt_mint(m_new, s_key, e_new, q), m_new
+
+
+def rlwe_encrypt(m_poly, s, e, q, scale):
+    """b = a s + e + scale m mod q with a fractional message part kept in
+    float64 (exp_e11_ckks_e2e_real.py:72-88).  a s is exact here and taken
+    mod q before the float64 sum (the reference sums it unreduced)."""
+    a = draw_uniform_a(len(s), q)
+    a_s = neg_conv_modq(a, s, q).astype(np.float64)
+    b = np.mod(a_s + np.asarray(e, dtype=np.float64) + scale * np.asarray(m_poly, dtype=np.float64), q)
+    return b, a
+
+
+def rlwe_decrypt(ct, s, q):
+    """(b - a s) mod q centred to [-q/2, q/2) in float64: scale m + e
+    (exp_e11_ckks_e2e_real.py:66-70,91-99)."""
+    b = np.asarray(ct[0], dtype=np.float64)
+    a_s = neg_conv_modq(ct[1], s, q).astype(np.float64)
+    v = np.mod(b - a_s, q)
+    return np.where(v > q / 2, v - q, v)
+
+
+# ---------------------------------------------------------------------------
 # noise monitor and gear policy
 # ---------------------------------------------------------------------------
 

@@ -697,3 +697,38 @@ def test_rotate_hoisted_eval_form_is_ntt_of_coefficient_form(gpu, N):
         for r in range(len(gks)):
             want = to_numpy(c.ring.ntt(coef[r].data.reshape(-1, l, N)))
             assert np.array_equal(to_numpy(ev[r].data).reshape(-1, l, N), want), (N, l, r)
+
+
+def test_single_modulus_rlwe_drop_ins(gpu):
+    """golden_api's exp_e24/exp_e11 RLWE helpers (products on the GPU)
+    against the oracle under the same numpy seeds: bit-exact for the integer
+    forms, equal floats for the float64 pair; both mask-draw branches
+    (q below and above 2^31) at the reference's N = 1024."""
+    from cipherpu import golden_api as G
+    for q in (998244353, CHAIN4[0]):
+        n = 1024
+        rng = np.random.default_rng(q % 977)
+        s = rng.integers(-1, 2, n)
+        e = rng.integers(-3, 4, n)
+        m = rng.integers(-10 ** 6, 10 ** 6, n)
+        assert np.array_equal(G.neg_conv_modq(m, s, q), O.neg_conv_modq(m, s, q))
+        np.random.seed(21)
+        ct = G.rlwe_encrypt_mint(m, s, e, q)
+        np.random.seed(21)
+        want = O.rlwe_encrypt_mint(m, s, e, q)
+        assert np.array_equal(ct[0], want[0]) and np.array_equal(ct[1], want[1])
+        assert np.array_equal(G.decrypt_exact_int(ct, s, q), O.decrypt_exact_int(want, s, q))
+        np.random.seed(22)
+        (c2, m2) = G.reencrypt(ct, s, q, 2.0 ** 20, np.random.default_rng(3))
+        np.random.seed(22)
+        (w2, wm2) = O.reencrypt(want, s, q, 2.0 ** 20, np.random.default_rng(3))
+        assert np.array_equal(m2, wm2) and np.array_equal(c2[0], w2[0]) and np.array_equal(c2[1], w2[1])
+        mf = rng.uniform(-1, 1, n)
+        np.random.seed(23)
+        cf = G.rlwe_encrypt(mf, s, e, q, 2.0 ** 12)
+        np.random.seed(23)
+        wf = O.rlwe_encrypt(mf, s, e, q, 2.0 ** 12)
+        assert np.array_equal(cf[0], wf[0]) and np.array_equal(cf[1], wf[1])
+        assert np.array_equal(G.rlwe_decrypt(cf, s, q), O.rlwe_decrypt(wf, s, q))
+    with pytest.raises(ValueError):
+        G.neg_conv_modq([1] * 1024, [1] * 1024, 1000003)

@@ -535,3 +535,34 @@ def test_rotate_hoisted_oracle_decrypts():
     assert np.max(np.abs(dec(one) - mrot)) < 2000
     assert np.max(np.abs(dec(hoisted) - mrot)) < 2000
     assert not np.array_equal(one, hoisted)
+
+
+def test_single_modulus_rlwe_helpers():
+    """exp_e24/exp_e11 RLWE helpers: the mask is numpy's global draw (same
+    seed, same a), b - a s recovers e + m exactly (a dropped term or sign
+    fails), DARE's refresh decrypts to the old decryption plus ternary
+    noise, and the float64 pair round-trips scale m + e."""
+    for q, n in ((998244353, 64), (17592186634241, 32)):
+        rng = np.random.default_rng(q % 1000)
+        s = rng.integers(-1, 2, n)
+        e = rng.integers(-3, 4, n)
+        m = rng.integers(-10 ** 6, 10 ** 6, n)
+        np.random.seed(5)
+        b, a = O.rlwe_encrypt_mint(m, s, e, q)
+        np.random.seed(5)
+        want_a = (np.random.randint(0, q, size=n) if q <= 2 ** 31 - 1
+                  else np.floor(np.random.random(size=n) * q)).astype(np.int64)
+        assert np.array_equal(a, want_a)
+        a_s = O.poly_mult([int(v) for v in a], [int(v) % q for v in s], q)
+        assert [(int(x) - y) % q for x, y in zip(b, a_s)] == [int(v) % q for v in e + m]
+        assert np.array_equal(O.decrypt_exact_int((b, a), s, q), e + m)
+        (b2, a2), m_new = O.reencrypt((b, a), s, q, 2.0 ** 20, np.random.default_rng(9))
+        assert np.array_equal(m_new, e + m)
+        noise = O.decrypt_exact_int((b2, a2), s, q) - m_new
+        assert set(np.unique(noise)) <= {-1, 0, 1}
+        assert np.array_equal(noise, np.random.default_rng(9).integers(-1, 2, size=n))
+        scale = 2.0 ** 12
+        mf = rng.uniform(-1, 1, n)
+        bf, af = O.rlwe_encrypt(mf, s, e, q, scale)
+        # float64 sums of values up to q carry rounding of q 2^-52 per step
+        assert np.max(np.abs(O.rlwe_decrypt((bf, af), s, q) - (scale * mf + e))) < q * 2.0 ** -49
