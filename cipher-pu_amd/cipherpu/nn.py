@@ -103,6 +103,29 @@ def block_keep_mask(W: np.ndarray, frac: float, block: int = BLOCK) -> np.ndarra
     return np.repeat(keep, block)[: flat.size].reshape(np.shape(W))
 
 
+def apply_block_sparse_mask(weight_matrix, threshold_val, B):
+    """The reference's threshold rule: every full run of B flattened weights
+    whose largest |w| is below threshold_val is zeroed; a ragged tail run is
+    never pruned.  Returns (masked weights, skipped blocks, full blocks,
+    kept share) (scripts/exp_e10_sparsity_sweep_v2.py:102-124)."""
+    flat = np.array(weight_matrix, dtype=np.float64).reshape(-1)
+    nb = flat.size // B
+    drop = np.abs(flat[: nb * B]).reshape(nb, B).max(axis=1) < threshold_val
+    out = flat.copy()
+    out[: nb * B].reshape(nb, B)[drop] = 0.0
+    skipped = int(drop.sum())
+    return out.reshape(np.shape(weight_matrix)), skipped, nb, (nb - skipped) / nb
+
+
+def block_sparse_mask(weight_matrix, threshold_frac, B):
+    """The same rule with the threshold threshold_frac times the largest
+    |w|; returns (masked weights, skipped blocks, full blocks)
+    (scripts/exp_e7_sparsity_sweep.py:72-90)."""
+    t = np.abs(np.asarray(weight_matrix, dtype=np.float64)).max() * threshold_frac
+    masked, skipped, nb, _ = apply_block_sparse_mask(weight_matrix, t, B)
+    return masked, skipped, nb
+
+
 def weights_from_diagonals(D: np.ndarray, rows: int, cols: int, shape) -> np.ndarray:
     """Inverse of extended_diagonals (to evaluate pruned networks in FP64)."""
     P = np.zeros((rows, cols))

@@ -756,6 +756,29 @@ def fast_red(x: int, q: int, k: int = 44) -> int:
     return r
 
 
+def apply_block_sparse_mask(weight_matrix, threshold_val, B):
+    """Zero every full run of B flattened weights whose max |w| is below
+    threshold_val; a ragged tail is never pruned.  Returns (masked,
+    skipped, blocks, kept share) (exp_e10_sparsity_sweep_v2.py:102-124)."""
+    flat = np.array(weight_matrix, dtype=np.float64).ravel()
+    out = flat.copy()
+    nb = flat.size // B
+    skip = 0
+    for i in range(nb):
+        if np.max(np.abs(flat[i * B:(i + 1) * B])) < threshold_val:
+            out[i * B:(i + 1) * B] = 0.0
+            skip += 1
+    return out.reshape(np.shape(weight_matrix)), skip, nb, (nb - skip) / nb
+
+
+def block_sparse_mask(weight_matrix, threshold_frac, B):
+    """The same rule with the threshold threshold_frac * max |w|; returns
+    (masked, skipped, blocks) (exp_e7_sparsity_sweep.py:72-90)."""
+    t = float(np.max(np.abs(np.asarray(weight_matrix, dtype=np.float64)))) * threshold_frac
+    w, skip, nb, _ = apply_block_sparse_mask(weight_matrix, t, B)
+    return w, skip, nb
+
+
 def block_prune_mask(w: np.ndarray, frac: float, block: int = 64) -> np.ndarray:
     """Zero the ``frac`` share of size-``block`` blocks (row-major runs of
     the flattened matrix) with the smallest max-abs -- the quantile form of

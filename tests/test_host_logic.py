@@ -142,3 +142,29 @@ def test_reference_gear_policies_match_the_oracle():
     assert math.isinf(G.compute_snr(s, np.zeros(4)))
     assert G.estimate_r_hat(1.1, 1.0) == min(abs(1.1 - 1.0) / (1.0 + 1e-8), 1.0)
     assert G.estimate_r_hat(5.0, 0.0) == 1.0
+
+
+def test_reference_block_sparsity_rules():
+    """nn.apply_block_sparse_mask / block_sparse_mask (the reference's
+    threshold rules) against a hand example and the oracle's loops."""
+    import ckks_oracle as O
+    from cipherpu import nn
+    w = np.array([[0.1, -0.2, 0.05, 0.0, 3.0, -0.1], [0.01, 0.02, -0.03, 0.9, 0.5, 0.4]])
+    # B = 4 over 12 weights: blocks [0.1 -0.2 0.05 0] (max 0.2), [3 -0.1 0.01 0.02] (3),
+    # [-0.03 0.9 0.5 0.4] (0.9); threshold 0.5 drops the first only
+    m, skip, nb, kept = nn.apply_block_sparse_mask(w, 0.5, 4)
+    assert (skip, nb) == (1, 3) and abs(kept - 2 / 3) < 1e-15
+    assert not m.ravel()[:4].any() and np.array_equal(m.ravel()[4:], w.ravel()[4:])
+    # B = 5: two full blocks and a ragged tail of 2 that is never pruned
+    m, skip, nb, _ = nn.apply_block_sparse_mask(w, 10.0, 5)
+    assert (skip, nb) == (2, 2) and not m.ravel()[:10].any() and np.array_equal(m.ravel()[10:], w.ravel()[10:])
+    m, skip, nb = nn.block_sparse_mask(w, 0.25, 4)          # threshold 0.75: blocks 0 drops, 1 and 2 stay
+    assert (skip, nb) == (1, 3)
+    rng = np.random.default_rng(3)
+    for shape, B in (((784, 128), 64), ((128, 10), 64), ((30, 7), 16)):
+        W = rng.normal(0, 1 / np.sqrt(shape[0]), shape)
+        for t in (0.0, 0.05, 0.1, 0.2, 1.0):
+            got, want = nn.apply_block_sparse_mask(W, t, B), O.apply_block_sparse_mask(W, t, B)
+            assert np.array_equal(got[0], want[0]) and got[1:] == want[1:]
+            got, want = nn.block_sparse_mask(W, t, B), O.block_sparse_mask(W, t, B)
+            assert np.array_equal(got[0], want[0]) and got[1:] == want[1:]
