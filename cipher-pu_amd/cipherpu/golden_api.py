@@ -464,3 +464,54 @@ def rlwe_decrypt(ct, s, q):
     (exp_e11_ckks_e2e_real.py:66-70,91-99)."""
     v = np.mod(np.asarray(ct[0], dtype=np.float64) - _negconv(ct[1], s, q).astype(np.float64), q)
     return np.where(v > q / 2, v - q, v)
+
+
+def decrypt_exact(ct, s, q):
+    """decrypt_exact_int as float64 (exp_e24_rotation_bootstrap.py:109-123)."""
+    return decrypt_exact_int(ct, s, q).astype(np.float64)
+
+
+def mul_plain_exact(ct, p_poly, q):
+    """(b p, a p) mod q, exact, int64 (exp_e24_rotation_bootstrap.py:136-140)."""
+    return _negconv(ct[0], p_poly, q), _negconv(ct[1], p_poly, q)
+
+
+def poly_x_power(nu, n):
+    """The monomial X^(nu mod n) (exp_e24_rotation_bootstrap.py:225-229)."""
+    p = np.zeros(n, dtype=np.int64)
+    p[nu % n] = 1
+    return p
+
+
+def rotate_monomial(ct, nu, q):
+    """ct times X^nu, i.e. a negacyclic shift of both polynomials: the
+    reference's coefficient-domain `rotate` (exp_e24_rotation_bootstrap.py:232-238),
+    named apart from the slot rotation of the CKKS API."""
+    return mul_plain_exact(ct, poly_x_power(nu, len(ct[0])), q)
+
+
+def scatter_w(msg_indices, w, scale, q, n):
+    """Integer plaintext that puts round(scale w_j) at position n - msg_j
+    (negated when msg_j = 0 mod n), summing collisions, reduced mod q
+    (exp_e24_rotation_bootstrap.py:241-262)."""
+    acc = np.zeros(n, dtype=object)
+    for j, mi in enumerate(msg_indices):
+        v = int(round(scale * float(w[j])))
+        k = int(mi) % n
+        acc[(n - k) % n] += -v if k == 0 else v
+    return np.array([int(x) % q for x in acc], dtype=np.int64)
+
+
+def inner_product(ct, w, s_key, q, scale, msg_indices=None):
+    """sum_j x_(msg_j) w_j from one plaintext product and the constant term
+    of its decryption, -dec[0] / scale^2 (exp_e24_rotation_bootstrap.py:265-275)."""
+    w = np.asarray(w, dtype=np.float64)
+    idx = range(len(w)) if msg_indices is None else msg_indices
+    prod = mul_plain_exact(ct, scatter_w(idx, w, scale, q, len(ct[0])), q)
+    return -float(decrypt_exact(prod, s_key, q)[0]) / (scale * scale)
+
+
+def coef_sum(ct, dim, s_key, q, scale):
+    """sum_(i < dim) x_i: the inner product with ones
+    (exp_e24_rotation_bootstrap.py:278-284)."""
+    return inner_product(ct, np.ones(dim), s_key, q, scale)

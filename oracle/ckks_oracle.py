@@ -599,6 +599,57 @@ def reencrypt(ct, s_key, q, scale, rng):
     return rlwe_encrypt_mint(m_new, s_key, e_new, q), m_new
 
 
+def decrypt_exact(ct, s, q):
+    """decrypt_exact_int as float64 (exp_e24_rotation_bootstrap.py:109-123)."""
+    return decrypt_exact_int(ct, s, q).astype(np.float64)
+
+
+def mul_plain_exact(ct, p_poly, q):
+    """(b p, a p) mod q, exact (exp_e24_rotation_bootstrap.py:136-140)."""
+    return neg_conv_modq(ct[0], p_poly, q), neg_conv_modq(ct[1], p_poly, q)
+
+
+def poly_x_power(nu, n):
+    """The monomial X^(nu mod n) (exp_e24_rotation_bootstrap.py:225-229)."""
+    p = np.zeros(n, dtype=np.int64)
+    p[nu % n] = 1
+    return p
+
+
+def rotate_monomial(ct, nu, q):
+    """ct times X^nu: a negacyclic coefficient shift
+    (exp_e24_rotation_bootstrap.py:232-238, `rotate`)."""
+    return mul_plain_exact(ct, poly_x_power(nu, len(ct[0])), q)
+
+
+def scatter_w(msg_indices, w, scale, q, n):
+    """Integer plaintext with round(scale w_j) at n - msg_j (negated at 0),
+    colliding positions summed, reduced mod q
+    (exp_e24_rotation_bootstrap.py:241-262)."""
+    m = [0] * n
+    for j, mi in enumerate(msg_indices):
+        v = int(round(scale * float(w[j])))
+        if mi % n == 0:
+            m[0] -= v
+        else:
+            m[n - mi % n] += v
+    return np.array([x % q for x in m], dtype=np.int64)
+
+
+def inner_product(ct, w, s_key, q, scale, msg_indices=None):
+    """-dec(ct * scatter_w)[0] / scale^2 = sum_j x_(msg_j) w_j
+    (exp_e24_rotation_bootstrap.py:265-275)."""
+    w = np.asarray(w, dtype=np.float64)
+    idx = list(range(len(w))) if msg_indices is None else msg_indices
+    m_w = scatter_w(idx, w, scale, q, len(ct[0]))
+    return -float(decrypt_exact(mul_plain_exact(ct, m_w, q), s_key, q)[0]) / (scale * scale)
+
+
+def coef_sum(ct, dim, s_key, q, scale):
+    """sum_(i < dim) x_i (exp_e24_rotation_bootstrap.py:278-284)."""
+    return inner_product(ct, np.ones(dim), s_key, q, scale)
+
+
 def rlwe_encrypt(m_poly, s, e, q, scale):
     """b = a s + e + scale m mod q with a fractional message part kept in
     float64 (exp_e11_ckks_e2e_real.py:72-88).  a s is exact here and taken

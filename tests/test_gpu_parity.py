@@ -732,3 +732,30 @@ def test_single_modulus_rlwe_drop_ins(gpu):
         assert np.array_equal(G.rlwe_decrypt(cf, s, q), O.rlwe_decrypt(wf, s, q))
     with pytest.raises(ValueError):
         G.neg_conv_modq([1] * 1024, [1] * 1024, 1000003)
+
+
+def test_coefficient_encoding_drop_ins(gpu):
+    """golden_api's exp_e24 coefficient-encoding primitives against the
+    oracle at the reference's N = 1024, q = 998244353 and at a chain prime."""
+    from cipherpu import golden_api as G
+    for q, scale in ((998244353, 2.0 ** 8), (CHAIN4[1], 2.0 ** 14)):
+        n = 1024
+        rng = np.random.default_rng(q % 331)
+        s = rng.integers(-1, 2, n)
+        e = rng.integers(-3, 4, n)
+        x = rng.uniform(-1, 1, n)
+        np.random.seed(8)
+        ct = O.rlwe_encrypt_mint(np.round(scale * x).astype(np.int64), s, e, q)
+        p = rng.integers(-50, 50, n)
+        got, want = G.mul_plain_exact(ct, p, q), O.mul_plain_exact(ct, p, q)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        for nu in (1, 77, 1023, 2047):
+            got, want = G.rotate_monomial(ct, nu, q), O.rotate_monomial(ct, nu, q)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), nu
+        assert np.array_equal(G.decrypt_exact(ct, s, q), O.decrypt_exact(ct, s, q))
+        w = rng.uniform(-0.1, 0.1, 100)
+        idx = list(rng.integers(0, n, 100))
+        assert np.array_equal(G.scatter_w(idx, w, scale, q, n), O.scatter_w(idx, w, scale, q, n))
+        assert G.inner_product(ct, w, s, q, scale) == O.inner_product(ct, w, s, q, scale)
+        assert G.inner_product(ct, w, s, q, scale, idx) == O.inner_product(ct, w, s, q, scale, idx)
+        assert G.coef_sum(ct, 64, s, q, scale) == O.coef_sum(ct, 64, s, q, scale)

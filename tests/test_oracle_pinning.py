@@ -566,3 +566,33 @@ def test_single_modulus_rlwe_helpers():
         bf, af = O.rlwe_encrypt(mf, s, e, q, scale)
         # float64 sums of values up to q carry rounding of q 2^-52 per step
         assert np.max(np.abs(O.rlwe_decrypt((bf, af), s, q) - (scale * mf + e))) < q * 2.0 ** -49
+
+
+def test_coefficient_encoding_primitives():
+    """exp_e24's coefficient-encoding primitives: X^nu multiplication is the
+    negacyclic shift, scatter_w places and sums weights as traced by hand,
+    and inner_product / coef_sum decrypt to the plain dot product and sum
+    (an index or sign slip in the placement fails them)."""
+    q, n = 17592186634241, 32
+    rng = np.random.default_rng(11)
+    c = (rng.integers(0, q, n), rng.integers(0, q, n))
+    for nu in (0, 1, 5, 31, 33):
+        r = O.rotate_monomial(c, nu, q)
+        for src, dst in ((c[0], r[0]), (c[1], r[1])):
+            want = [0] * n
+            for j in range(n):
+                k = j + nu % n
+                want[k % n] = int(src[j]) if k < n else (-int(src[j])) % q
+            assert list(map(int, dst)) == want
+    assert list(O.scatter_w([0, 1, 1, 3], [0.5, 0.25, 0.25, -1.0], 4.0, 97, 8)) == [95, 0, 0, 0, 0, 93, 0, 2]
+    scale = 2.0 ** 14
+    s = rng.integers(-1, 2, n)
+    e = rng.integers(-3, 4, n)
+    x = rng.uniform(-1, 1, n)
+    np.random.seed(4)
+    ct = O.rlwe_encrypt_mint(np.round(scale * x).astype(np.int64), s, e, q)
+    w = rng.uniform(-1, 1, 20)
+    assert abs(O.inner_product(ct, w, s, q, scale) - float(x[:20] @ w)) < 0.02
+    idx = [3, 7, 7, 0, 31]
+    assert abs(O.inner_product(ct, w[:5], s, q, scale, idx) - float(sum(x[i] * wj for i, wj in zip(idx, w[:5])))) < 0.02
+    assert abs(O.coef_sum(ct, 25, s, q, scale) - float(x[:25].sum())) < 0.02
