@@ -397,7 +397,7 @@ def main():
         ck.ring.set_threads(args.ntt_threads, args.ks_threads, args.tensor_threads)
     else:
         from cipherpu.tune import tune_hmult
-        tuned = tune_hmult(ck, cta, ctb, rlk, reps=5, rounds=3)
+        tuned = tune_hmult(ck, cta, ctb, rlk, reps=10, rounds=4)
     out = ck.hmult(cta, ctb, rlk)
     torch.cuda.synchronize()
     # decrypted product vs the FP64 slot-wise product (all pairs)
@@ -424,6 +424,12 @@ def main():
     shape = ({k: tuned[k] for k in ("front", "ntt", "tensor", "ks")} if tuned else
              {"front": int(args.front), "ntt": args.ntt_threads, "tensor": args.tensor_threads,
               "ks": args.ks_threads})
+    if tuned:
+        # the tuner's best HMult batch time for each front, as it measured them
+        for f, name in ((0, "tuned_ms_separate_front"), (1, "tuned_ms_fused_front")):
+            ts = [v for k, v in tuned["grid"].items() if k.startswith(f"{f}/")]
+            if ts:
+                shape[name] = round(min(ts), 4)
     dom = max(kern, key=lambda k: kern[k][0])
     dom_ms = kern[dom][0] / kern[dom][1]
     assert dom == "keyswitch", f"dominant kernel is {dom}; the byte model covers the key switch"

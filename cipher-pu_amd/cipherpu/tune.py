@@ -52,16 +52,19 @@ def tune_hmult(ck, a, b, rlk, reps: int = 5, ntt: Iterable[int] = None, tensor: 
     ntt = tuple(ntt or BUILT[logN])
     tensor = tuple(tensor or BUILT_FUSED[logN])
     ks = tuple(ks or BUILT_FUSED[logN])
+    # the fused front does not use the transform kernel's thread count
+    settings = [(f, n, t, k) for f in fused
+                for n, t, k in itertools.product(ntt if not f else ntt[:1], tensor, ks)]
     grid = {}
-    for _ in range(max(1, rounds)):
-        for f in fused:
+    for rnd in range(max(1, rounds)):
+        # every other pass walks the grid backwards, so that a drift in the
+        # clocks during a pass favours no end of it
+        for f, n, t, k in (settings if rnd % 2 == 0 else settings[::-1]):
             ck.ring.set_front(bool(f))
-            # the fused front does not use the transform kernel's thread count
-            for n, t, k in itertools.product(ntt if not f else ntt[:1], tensor, ks):
-                ck.ring.set_threads(n, k, t)
-                ms = _time(lambda: ck.hmult(a, b, rlk), reps)
-                name = f"{f}/{n}/{t}/{k}"
-                grid[name] = min(ms, grid.get(name, ms))
+            ck.ring.set_threads(n, k, t)
+            ms = _time(lambda: ck.hmult(a, b, rlk), reps)
+            name = f"{f}/{n}/{t}/{k}"
+            grid[name] = min(ms, grid.get(name, ms))
     best = min(grid, key=grid.get)
     f, n, t, k = (int(x) for x in best.split("/"))
     ck.ring.set_front(bool(f))
