@@ -9,8 +9,9 @@ Reference surface -> here:
                                        poly_mult_via_ntt, cyclic_conv_direct
   scripts/verify_keyswitch.py          poly_add, poly_sub, poly_mult, relin
   scripts/verify_rescale.py            rescale_coeff, rescale_poly
-  scripts/verify_e2e_mult.py           e2e_mult (exact modulus switch instead
-                                       of the toy rescale, see below)
+  scripts/verify_e2e_mult.py           e2e_mult, rescale (the toy rescale with
+                                       scale_bits; the exact modulus switch
+                                       when scale_bits is None)
   scripts/exp_e24_rotation_bootstrap.py automorphism_exact, automorphism_key_exact
   scripts/verify_multi_slot_simd.py    neg_fft, neg_ifft, encode, decode
   scripts/verify_m4_h_multilayer.py    noise_monitor_energy
@@ -24,14 +25,16 @@ Reference surface -> here:
 """
 from __future__ import annotations
 
+import ctypes
+import math
 from functools import lru_cache
 from typing import Sequence
-
-import math
 
 import numpy as np
 import torch
 
+from ._lib import call
+from .params import CHAIN4
 from .ring import RingContext, bitrev_, mod_switch, to_numpy
 
 
@@ -47,6 +50,14 @@ def _ring(N: int, q: int, root, cyclic: bool) -> RingContext:
 
 def _t(a, ctx: RingContext):
     return ctx.tensor(np.asarray(a, dtype=np.uint64).reshape(1, -1))
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 def _list(t) -> list:
@@ -81,10 +92,9 @@ def poly_mult_via_ntt(a, b, N, q, omega, omega_inv) -> list:
 
 
 def cyclic_conv_direct(a, b, N, q) -> list:
-    """Cyclic convolution (gen_poly_mult_vectors_n12.py:110-117); computed
-    through the transform with the default N-th root of q."""
-    ctx = _ring(N, q, None, True)
-    return _list(ctx.poly_mul(_t(a, ctx), _t(b, ctx)))
+    """Cyclic convolution mod (X^N - 1, q) for any modulus q < 2^45
+    (gen_poly_mult_vectors_n12.py:110-117)."""
+    return _list(_prod(_residues(a, q), _residues(b, q), q, cyclic=True))
 
 
 def poly_add(a, b, q) -> list:
@@ -97,34 +107,64 @@ def poly_sub(a, b, q) -> list:
     return _list(_ew(a, b, q, 2))
 
 
-def _ntt_ok(n, q):
-    return n >= 2 and n & (n - 1) == 0 and (q - 1) % (2 * n) == 0
+def _ntt_ok(n, q, cyclic=False):
+    return n >= 2 and n & (n - 1) == 0 and (q - 1) % (n if cyclic else 2 * n) == 0 and _isprime(q)
 
 
 def _ew(a, b, q, op):
     n = len(a)
-    ctx = _ring(n, q, None, not _ntt_ok(n, q))
-    return ctx._pw(_t(a, ctx), _t(b, ctx) if b is not None else None, op, 0, None)
+    if _ntt_ok(n, q) or _ntt_ok(n, q, True):
+        ctx = _ring(n, q, None, not _ntt_ok(n, q))
+        return ctx._pw(_t(a, ctx), _t(b, ctx) if b is not None else None, op, 0, None)
+    # element-wise add/sub needs no transform: exact on the GPU for any q
+    x = torch.from_numpy(_residues(a, q).astype(np.int64)).to(_dev())
+    y = torch.from_numpy(_residues(b, q).astype(np.int64)).to(_dev())
+    return torch.remainder(x + y if op == 1 else x - y, q).to(torch.uint64)
+
+
+@lru_cache(maxsize=16)
+def _crt_ring(n: int, cyclic: bool) -> RingContext:
+    """Three 45-bit chain primes (2^15 | q - 1): the exact product over Z
+    of two polynomials mod any q < 2^45, n <= 16384, fits their product."""
+    return RingContext(n, list(CHAIN4[:3]), cyclic=cyclic, device=torch.cuda.current_device())
+
+
+def _prod(x, y, q, cyclic=False) -> torch.Tensor:
+    """Ring product of residue vectors (numpy uint64 in [0, q), [n] or
+    [rows, n]) mod (X^n -+ 1, q) on the GPU: the single-prime transform
+    product where q admits one, else the exact product through three primes
+    and CRT (ckks_poly_mul_modq)."""
+    x = np.asarray(x, dtype=np.uint64)
+    n = x.shape[-1]
+    if _ntt_ok(n, q, cyclic):
+        ctx = _ring(n, q, None, cyclic)
+        return ctx.poly_mul(ctx.tensor(x.reshape(-1, n)), ctx.tensor(np.asarray(y, np.uint64).reshape(-1, n)))
+    ctx = _crt_ring(n, cyclic)
+    X = ctx.tensor(x.reshape(-1, n))
+    Y = ctx.tensor(np.asarray(y, np.uint64).reshape(-1, n))
+    rows = X.shape[0]
+    out = torch.empty(rows, n, dtype=torch.uint64, device=X.device)
+    ws = torch.empty(9 * rows * n, dtype=torch.uint64, device=X.device)
+    call("ckks_poly_mul_modq", ctx.handle, _ptr(X), _ptr(Y), _ptr(out), rows, int(q), _ptr(ws),
+         ctx.stream())
+    return out
 
 
 def poly_mult(a, b, q) -> list:
-    """Negacyclic product mod (X^n + 1, q) (verify_keyswitch.py:48-59).
-    Needs 2n | q - 1 (an NTT-friendly prime)."""
-    n = len(a)
-    if not _ntt_ok(n, q):
-        raise ValueError(f"poly_mult on the GPU needs a prime with 2n | q-1 (n={n}, q={q})")
-    ctx = _ring(n, q, None, False)
-    return _list(ctx.poly_mul(_t(a, ctx), _t(b, ctx)))
+    """Negacyclic product mod (X^n + 1, q) for any modulus q < 2^45
+    (verify_keyswitch.py:48-59, verify_e2e_mult.py:56-67)."""
+    return _list(_prod(_residues(a, q), _residues(b, q), q))
 
 
 def relin(e0, e1, e2, a, b, q):
-    """f0 = e0 + e2*b, f1 = e1 - e2*a (verify_keyswitch.py:62-68)."""
+    """f0 = e0 + e2*b, f1 = e1 - e2*a (verify_keyswitch.py:62-68), any q."""
     n = len(e0)
-    ctx = _ring(n, q, None, False)
-    E2 = _t(e2, ctx)
-    f0 = ctx.add(_t(e0, ctx), ctx.poly_mul(E2, _t(b, ctx)))
-    f1 = ctx.sub(_t(e1, ctx), ctx.poly_mul(E2, _t(a, ctx)))
-    return _list(f0), _list(f1)
+    E2 = _residues(e2, q)
+    pb = _prod(np.stack([E2, E2]), np.stack([_residues(b, q), _residues(a, q)]), q)
+    e01 = torch.from_numpy(np.stack([_residues(e0, q), _residues(e1, q)]).astype(np.int64)).to(pb.device)
+    f = torch.remainder(torch.stack([e01[0] + pb[0].to(torch.int64), e01[1] - pb[1].to(torch.int64)]), q)
+    f = to_numpy(f.to(torch.uint64))
+    return [int(v) for v in f[0].reshape(n)], [int(v) for v in f[1].reshape(n)]
 
 
 def rescale_coeff(c: int, q_in: int, q_out: int) -> int:
@@ -138,22 +178,44 @@ def rescale_poly(c, q_in, q_out) -> list:
     return _list(mod_switch(x, q_in, q_out))
 
 
+def rescale(s, q_in, q_out, scale_bits):
+    """The reference's toy rescale floor((s q_out + floor(q_in / 2)) /
+    (q_in 2^scale_bits)) mod q_out (verify_e2e_mult.py:70-75) of one value
+    or a list of values in [0, 2^64)."""
+    scalar = np.ndim(s) == 0
+    x = torch.from_numpy(np.asarray([s] if scalar else s, dtype=np.uint64).reshape(-1).copy()).to(_dev())
+    out = _rescale_toy(x, q_in, q_out, scale_bits)
+    return out[0] if scalar else out
+
+
+def _rescale_toy(x: torch.Tensor, q_in, q_out, scale_bits) -> list:
+    out = torch.empty_like(x)
+    call("ckks_mod_switch_scaled", _ptr(x), _ptr(out), x.numel(), int(q_in), int(q_out), int(scale_bits),
+         _stream())
+    return _list(out)
+
+
 def e2e_mult(c0_1, c1_1, c0_2, c1_2, a, b, q_in, q_out, scale_bits=None):
-    """Tensor product -> relinearise -> rescale on single-prime ciphertexts
-    (verify_e2e_mult.py:78-102).  The rescale is the exact modulus switch
-    q_in -> q_out of verify_rescale.py; the toy division by 2^scale_bits
-    (verify_e2e_mult.py:70-75) is not reproduced, so scale_bits must be None."""
-    if scale_bits is not None:
-        raise ValueError("e2e_mult uses the exact modulus switch; pass scale_bits=None")
+    """Tensor product -> relinearise -> rescale on single-modulus ciphertexts
+    (verify_e2e_mult.py:78-102), for any modulus q_in < 2^45.  With
+    scale_bits (the reference's positional call, :173,:205,:220) the rescale
+    is the reference's toy division by q_in 2^scale_bits (:70-75), word for
+    word; with scale_bits=None it is the exact modulus switch q_in -> q_out
+    of verify_rescale.py."""
     n = len(c0_1)
-    ctx = _ring(n, q_in, None, False)
-    A0, A1, B0, B1 = (_t(v, ctx) for v in (c0_1, c1_1, c0_2, c1_2))
-    m0 = ctx.poly_mul(A0, B0)
-    m1 = ctx.add(ctx.poly_mul(A0, B1), ctx.poly_mul(A1, B0))
-    m2 = ctx.poly_mul(A1, B1)
-    f0 = ctx.add(m0, ctx.poly_mul(m2, _t(b, ctx)))
-    f1 = ctx.sub(m1, ctx.poly_mul(m2, _t(a, ctx)))
-    return _list(mod_switch(f0, q_in, q_out)), _list(mod_switch(f1, q_in, q_out))
+    R = lambda v: _residues(v, q_in)
+    A0, A1, B0, B1, KA, KB = (R(v) for v in (c0_1, c1_1, c0_2, c1_2, a, b))
+    # m0 = A0 B0, m1 = A0 B1 + A1 B0, m2 = A1 B1 in one batched call
+    m = _prod(np.stack([A0, A0, A1, A1]), np.stack([B0, B1, B0, B1]), q_in).to(torch.int64)
+    m0, m1, m2 = m[0], torch.remainder(m[1] + m[2], q_in), m[3]
+    m2n = to_numpy(m2.to(torch.uint64))
+    e = _prod(np.stack([m2n, m2n]), np.stack([KB, KA]), q_in).to(torch.int64)   # m2 b, m2 a
+    f = torch.remainder(torch.stack([m0 + e[0], m1 - e[1]]), q_in).to(torch.uint64).reshape(-1).contiguous()
+    if scale_bits is None:
+        g = _list(mod_switch(f, q_in, q_out))
+    else:
+        g = _rescale_toy(f, q_in, q_out, scale_bits)
+    return g[:n], g[n:]
 
 
 def automorphism_exact(ct, nu, n, q):
@@ -234,15 +296,31 @@ def neg_fft(m):
     return to_numpy(out).reshape(m.shape[:-1] + (n,))
 
 
+def _neg_ifft_real(H: np.ndarray) -> np.ndarray:
+    """Inverse of ``neg_fft`` for Hermitian rows H (H[n-1-k] = conj H[k]):
+    the real polynomial, from the FFT-order half of H."""
+    n = H.shape[-1]
+    ctx = _emb_ring(n)
+    fft_order = H[:, 0::2]   # k = 2k' has e = 4k' + 1
+    return to_numpy(ctx.embed_inv(torch.from_numpy(np.ascontiguousarray(fft_order)), order=1))
+
+
 def neg_ifft(V):
-    """Inverse of ``neg_fft`` for Hermitian V (V[n-1-k] = conj V[k]), i.e.
-    real polynomials (verify_multi_slot_simd.py:42-52)."""
+    """Inverse of ``neg_fft``: m_j = (1/n) sum_k V_k Phi^(-(2k+1) j), complex
+    (verify_multi_slot_simd.py:42-52).  Any V splits into two Hermitian
+    vectors, H = (V + V*)/2 and A = (V - V*)/(2i) with V*_k = conj V_(n-1-k),
+    whose inverses are the real and imaginary parts of m; for a Hermitian V
+    (real polynomials, encode with nslot <= n/2) A is zero."""
     V = np.asarray(V, dtype=np.complex128)
     n = V.shape[-1]
-    ctx = _emb_ring(n)
-    fft_order = V.reshape(-1, n)[:, 0::2]   # k = 2k' has e = 4k' + 1
-    m = ctx.embed_inv(torch.from_numpy(np.ascontiguousarray(fft_order)), order=1)
-    return to_numpy(m).reshape(V.shape[:-1] + (n,)).astype(np.complex128)
+    rows = V.reshape(-1, n)
+    Vs = np.conj(rows[:, ::-1])
+    H = (rows + Vs) / 2
+    A = (rows - Vs) / 2j
+    both = _neg_ifft_real(np.concatenate([H, A]))
+    r = rows.shape[0]
+    m = both[:r] + 1j * both[r:]
+    return m.reshape(V.shape[:-1] + (n,))
 
 
 def encode(v, scale, n):
@@ -268,31 +346,21 @@ def decode(m, scale, n, nslot):
 
 def noise_monitor_energy(coeffs, q):
     """(E, r_hat) with E = sum min(c, q - c)^2 / count and r_hat = sqrt(E)/q
-    (scripts/verify_m4_h_multilayer.py:162-174; rtl/m4/noise_monitor.sv).
-    The sum of squares is the statistics kernel's exact 128-bit row sum; the
-    row is zero-padded to a power of two (zeros add nothing).  q must admit
-    a transform context at the padded length (the chain primes do); other
-    moduli raise ValueError."""
+    (scripts/verify_m4_h_multilayer.py:162-174; rtl/m4/noise_monitor.sv),
+    for any modulus q < 2^45 and any count (the reference calls it with
+    q = 113 on 32 coefficients, :260).  The sum of squares is the statistics
+    kernel's exact 128-bit row sum."""
     c = np.asarray(coeffs, dtype=np.uint64).reshape(-1)
     n = c.size
     if n == 0:
         raise ZeroDivisionError("noise_monitor_energy of an empty coefficient list")
     if int(c.max()) >= q:
         raise ValueError("coefficients must lie in [0, q)")
-    N = max(8, 1 << (n - 1).bit_length())
-    ctx = None
-    for cyclic in (False, True):
-        try:
-            ctx = _ring(N, int(q), None, cyclic)
-            break
-        except Exception:
-            continue
-    if ctx is None:
-        raise ValueError(f"no transform context for q = {q} at length {N}")
-    row = np.zeros((1, N), dtype=np.uint64)
-    row[0, :n] = c
-    st = ctx.noise_stats(ctx.tensor(row))
-    E = st[0][1] / n
+    x = torch.from_numpy(c.copy()).to(_dev())
+    st = torch.empty(5, dtype=torch.uint64, device=x.device)
+    call("ckks_noise_stats_q", _ptr(x), _ptr(st), 1, n, int(q), _stream())
+    w = [int(v) for v in to_numpy(st)]
+    E = (w[1] + (w[2] << 64)) / n
     return E, math.sqrt(E) / q
 
 
@@ -396,7 +464,7 @@ class RtlGearCtrl:
 # (scripts/exp_e24_rotation_bootstrap.py, scripts/exp_e11_ckks_e2e_real.py).
 # The negacyclic products run on the GPU and are exact; the mask a is drawn
 # from numpy's global generator with the reference's own calls, so a run
-# seeded like the reference's gets the same a.  q needs 2n | q - 1.
+# seeded like the reference's gets the same a.  Any modulus q < 2^45.
 
 def _residues(x, q) -> np.ndarray:
     return np.array([int(v) % q for v in np.asarray(x).reshape(-1)], dtype=np.uint64)
@@ -404,16 +472,11 @@ def _residues(x, q) -> np.ndarray:
 
 def _negconv(x, y, q) -> np.ndarray:
     n = len(x)
-    if not _ntt_ok(n, q):
-        raise ValueError(f"the GPU product needs a prime with 2n | q-1 (n={n}, q={q})")
-    ctx = _ring(n, q, None, False)
-    xs = ctx.tensor(_residues(x, q).reshape(1, n))
-    ys = ctx.tensor(_residues(y, q).reshape(1, n))
-    return to_numpy(ctx.poly_mul(xs, ys)).reshape(n).astype(np.int64)
+    return to_numpy(_prod(_residues(x, q), _residues(y, q), q)).reshape(n).astype(np.int64)
 
 
 def neg_conv_modq(x, y, q):
-    """Exact negacyclic product mod (X^n + 1, q), int64
+    """Exact negacyclic product mod (X^n + 1, q) for any q < 2^45, int64
     (exp_e24_rotation_bootstrap.py:80-106)."""
     return _negconv(x, y, q)
 

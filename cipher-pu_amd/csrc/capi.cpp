@@ -812,6 +812,69 @@ int ckks_mod_switch(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t q_in
     return CKKS_OK;
 }
 
+int ckks_mod_switch_scaled(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t q_in, uint64_t q_out,
+                           int scale_bits, void* stream) {
+    if (n == 0) return CKKS_OK;
+    if (!in || !out) return CKKS_ERR_NULL;
+    if (misaligned(in) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (q_in < 2 || q_out < 2 || q_in >= (1ull << 45) || q_out >= (1ull << 45)) return CKKS_ERR_BAD_MODULUS;
+    if (scale_bits < 0 || scale_bits > 127) return CKKS_ERR_BAD_ARG;
+    HIPCHK(launch_mod_switch_toy(in, out, n, q_in, q_out, scale_bits, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_poly_mul_modq(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, uint64_t* out, int n_polys,
+                       uint64_t q, void* workspace, void* stream) {
+    int st = check_rows(c, n_polys, 3, 0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (q < 2 || q >= (1ull << 45)) return CKKS_ERR_BAD_MODULUS;
+    // every product coefficient lies in (-N q^2, N q^2); Garner's sign test
+    // needs it inside (-P/4, P/4), P = p0 p1 p2
+    const int pbits = bitlen(c->primes[0]) + bitlen(c->primes[1]) + bitlen(c->primes[2]) - 3;
+    if (pbits < 2 * bitlen(q - 1) + c->logN + 3) return CKKS_ERR_BAD_MODULUS;
+    if (!a || !b || !out || !workspace) return CKKS_ERR_NULL;
+    if (misaligned(a) || misaligned(b) || misaligned(out) || misaligned(workspace)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    const uint64_t p0 = c->primes[0], p1 = c->primes[1], p2 = c->primes[2];
+    Crt3Const cc;
+    memset(&cc, 0, sizeof(cc));
+    cc.p0inv_p1 = invmod(p0 % p1, p1);
+    cc.p0inv_p1_s = shoup(cc.p0inv_p1, p1);
+    cc.p0_p2 = p0 % p2;
+    cc.p0_p2_s = shoup(cc.p0_p2, p2);
+    cc.p01inv_p2 = invmod(mulmod(p0 % p2, p1 % p2, p2), p2);
+    cc.p01inv_p2_s = shoup(cc.p01inv_p2, p2);
+    cc.p0_q = p0 % q;
+    cc.p01_q = mulmod(cc.p0_q, p1 % q, q);
+    cc.P_q = mulmod(cc.p01_q, p2 % q, q);
+    PrimeConst& Q = cc.Pq;
+    Q.q = q;
+    Q.two_q = 2 * q;
+    Q.ell = bitlen(q);
+    Q.mu = (uint64_t)(((u128)1 << (2 * Q.ell)) / q);
+    Q.mu64 = (uint64_t)(~0ull / q);
+    const DevTables t = c->tables();
+    const size_t rows3 = (size_t)n_polys * 3 * c->N;
+    uint64_t* A3 = (uint64_t*)workspace;
+    uint64_t* B3 = A3 + rows3;
+    uint64_t* C3 = B3 + rows3;
+    HIPCHK(launch_lift3(t, a, A3, n_polys, Q, S(stream)));
+    HIPCHK(launch_lift3(t, b, B3, n_polys, Q, S(stream)));
+    TIMED(CKKS_KIND_POLYMUL, launch_polymul(t, A3, B3, C3, n_polys, 3, limb_map(0, 3), c->ntt_threads, S(stream)));
+    HIPCHK(launch_crt3_modq(t, C3, out, n_polys, cc, S(stream)));
+    return CKKS_OK;
+}
+
+int ckks_noise_stats_q(const uint64_t* data, uint64_t* stats, int rows, uint64_t n, uint64_t q, void* stream) {
+    if (rows < 0 || n > (1ull << 30)) return CKKS_ERR_BAD_SHAPE;
+    if (rows == 0 || n == 0) return CKKS_OK;
+    if (q < 2 || q >= (1ull << 45)) return CKKS_ERR_BAD_MODULUS;
+    if (!data || !stats) return CKKS_ERR_NULL;
+    if (misaligned(data) || misaligned(stats)) return CKKS_ERR_BAD_ALIGNMENT;
+    HIPCHK(launch_noise_stats_q(data, stats, rows, n, q, S(stream)));
+    return CKKS_OK;
+}
+
 int ckks_embed(const ckks_ctx* c, const double* coeffs, double* slots, int n_vec, int order, void* stream) {
     if (!c || !coeffs || !slots) return CKKS_ERR_NULL;
     if (!c->fft_tw) return CKKS_ERR_BAD_ARG;

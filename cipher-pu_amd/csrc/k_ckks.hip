@@ -468,16 +468,12 @@ __global__ void __launch_bounds__(1024) k_automorphism_row(const uint64_t* in, u
 // reports sum c', sum c'^2 (128-bit), max c' and the sum of the centered
 // values, all exact.  stats: [row][5] = {sum_abs, sumsq_lo, sumsq_hi,
 // max_abs, sum_centered (two's complement)}.
-__global__ void __launch_bounds__(256) k_noise_stats(const uint64_t* __restrict__ in, uint64_t* __restrict__ stats,
-                                                     const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs,
-                                                     int logN) {
+// One row of the monitor: block-wide exact sums over n coefficients mod q.
+__device__ __forceinline__ void noise_row(const uint64_t* __restrict__ src, int n, uint64_t q,
+                                          uint64_t* __restrict__ o) {
     __shared__ uint64_t s_abs[256], s_lo[256], s_hi[256], s_max[256], s_sgn[256];
-    const int N = 1 << logN;
-    const size_t row = blockIdx.x;
-    const uint64_t q = pcs[map.idx[row % n_limbs]].q;
-    const uint64_t* src = in + row * N;
     uint64_t sa = 0, lo = 0, hi = 0, mx = 0, sg = 0;
-    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const uint64_t c = src[k];
         const bool neg = c > q - c;
         const uint64_t a = neg ? q - c : c;
@@ -503,9 +499,100 @@ __global__ void __launch_bounds__(256) k_noise_stats(const uint64_t* __restrict_
         __syncthreads();
     }
     if (t == 0) {
-        uint64_t* o = stats + row * 5;
         o[0] = s_abs[0]; o[1] = s_lo[0]; o[2] = s_hi[0]; o[3] = s_max[0]; o[4] = s_sgn[0];
     }
+}
+
+__global__ void __launch_bounds__(256) k_noise_stats(const uint64_t* __restrict__ in, uint64_t* __restrict__ stats,
+                                                     const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs,
+                                                     int logN) {
+    const int N = 1 << logN;
+    const size_t row = blockIdx.x;
+    noise_row(in + row * N, N, pcs[map.idx[row % n_limbs]].q, stats + row * 5);
+}
+
+// The same monitor for rows of any length n under one modulus q, with no
+// transform context: the statistics need only q (the reference's call sites
+// use toy moduli such as q = 113 on 32 coefficients).
+__global__ void __launch_bounds__(256) k_noise_stats_q(const uint64_t* __restrict__ in, uint64_t* __restrict__ stats,
+                                                       size_t n, uint64_t q) {
+    const size_t row = blockIdx.x;
+    noise_row(in + row * n, (int)n, q, stats + row * 5);
+}
+
+// The reference's toy rescale (scripts/verify_e2e_mult.py:70-75, the same
+// formula in gen_tb_e2e_vec.py:59-63 and gen_tb_rescale_vec.py:28-33):
+//   out = floor((c q_out + floor(q_in / 2)) / (q_in 2^sb)) mod q_out
+// for any 64-bit c.  floor(X / (q_in 2^sb)) = floor(floor(X / 2^sb) / q_in),
+// and the 128-bit Y = floor(X / 2^sb) is divided by q_in sixteen bits at a
+// time: the running remainder stays below q_in < 2^45, so (r << 16) | digit
+// fits 64 bits, and each quotient digit (< 2^16) is folded into the result
+// modulo q_out (Q 2^16 + d < 2^62).  Exact for 2 <= q_in, q_out < 2^45.
+__global__ void k_mod_switch_toy(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, size_t n,
+                                 uint64_t q_in, uint64_t q_out, int sb) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint64_t c = in[g];
+    uint64_t lo = c * q_out, hi = __umul64hi(c, q_out);
+    const uint64_t lo2 = lo + (q_in >> 1);
+    hi += lo2 < lo ? 1 : 0;
+    lo = lo2;
+    if (sb >= 64) {
+        lo = sb >= 128 ? 0 : hi >> (sb - 64);
+        hi = 0;
+    } else if (sb > 0) {
+        lo = (lo >> sb) | (hi << (64 - sb));
+        hi >>= sb;
+    }
+    uint64_t r = 0, Q = 0;
+    for (int k = 7; k >= 0; --k) {
+        const uint64_t digit = ((k >= 4 ? hi : lo) >> (16 * (k & 3))) & 0xffffu;
+        const uint64_t cur = (r << 16) | digit;
+        const uint64_t d = cur / q_in;
+        r = cur - d * q_in;
+        Q = ((Q << 16) + d) % q_out;
+    }
+    out[g] = Q;
+}
+
+// Exact product modulo any q through three transform primes p0, p1, p2:
+// the lift takes x mod q (any 64-bit x) and writes its residues modulo the
+// three primes, [rows][N] -> [rows][3][N].
+__global__ void k_lift3(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                        const PrimeConst* __restrict__ pcs, PrimeConst Pq, int logN, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const size_t N = (size_t)1 << logN;
+    const size_t n = g & (N - 1), row = g >> logN;
+    const uint64_t x = reduce64(in[g], Pq);
+    uint64_t* o = out + row * 3 * N + n;
+    for (int i = 0; i < 3; ++i) o[(size_t)i * N] = reduce64(x, pcs[i]);
+}
+
+// Garner's recombination of the three residues of a product over Z whose
+// coefficients lie in (-P/4, P/4), P = p0 p1 p2:
+//   t1 = (r1 - r0) p0^-1 mod p1,  t2 = (r2 - r0 - p0 t1) (p0 p1)^-1 mod p2,
+//   x = r0 + p0 t1 + p0 p1 t2 in [0, P),  the value is x - P when t2 > p2 / 2,
+// and the output is that value modulo q, [rows][3][N] -> [rows][N].
+__global__ void k_crt3_modq(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                            const PrimeConst* __restrict__ pcs, Crt3Const cc, int logN, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const size_t N = (size_t)1 << logN;
+    const size_t n = g & (N - 1), row = g >> logN;
+    const uint64_t* src = in + row * 3 * N + n;
+    const uint64_t p1 = pcs[1].q, p2 = pcs[2].q;
+    const uint64_t r0 = src[0], r1 = src[N], r2 = src[2 * N];
+    const uint64_t t1 = csub(mul_shoup(sub_mod(r1, reduce64(r0, pcs[1]), p1), cc.p0inv_p1, cc.p0inv_p1_s, p1), p1);
+    const uint64_t u = csub(mul_shoup(t1, cc.p0_p2, cc.p0_p2_s, p2), p2);            // p0 t1 mod p2
+    const uint64_t v = sub_mod(sub_mod(r2, reduce64(r0, pcs[2]), p2), u, p2);
+    const uint64_t t2 = csub(mul_shoup(v, cc.p01inv_p2, cc.p01inv_p2_s, p2), p2);
+    const PrimeConst& Q = cc.Pq;
+    uint64_t acc = reduce64(r0, Q);
+    acc = add_mod(acc, mul_barrett(reduce64(t1, Q), cc.p0_q, Q), Q.q);
+    acc = add_mod(acc, mul_barrett(reduce64(t2, Q), cc.p01_q, Q), Q.q);
+    if (t2 > (p2 >> 1)) acc = sub_mod(acc, cc.P_q, Q.q);
+    out[g] = acc;
 }
 
 // Single-prime modulus switch round-half-up(c q_out / q_in) mod q_out
@@ -801,6 +888,38 @@ hipError_t launch_noise_stats(const DevTables& t, const uint64_t* in, uint64_t* 
     const int rows = n_polys * n_limbs;
     if (!rows) return hipSuccess;
     hipLaunchKernelGGL(k_noise_stats, dim3(rows), dim3(256), 0, s, in, stats, t.pc, map, n_limbs, t.logN);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_stats_q(const uint64_t* in, uint64_t* stats, int rows, size_t n, uint64_t q, hipStream_t s) {
+    if (!rows || !n) return hipSuccess;
+    hipLaunchKernelGGL(k_noise_stats_q, dim3(rows), dim3(256), 0, s, in, stats, n, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_mod_switch_toy(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, int sb,
+                                 hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_mod_switch_toy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n, q_in, q_out,
+                       sb);
+    return hipGetLastError();
+}
+
+hipError_t launch_lift3(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, const PrimeConst& Pq,
+                        hipStream_t s) {
+    const size_t total = (size_t)rows << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_lift3, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, Pq, t.logN,
+                       total);
+    return hipGetLastError();
+}
+
+hipError_t launch_crt3_modq(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, const Crt3Const& cc,
+                            hipStream_t s) {
+    const size_t total = (size_t)rows << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_crt3_modq, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, cc,
+                       t.logN, total);
     return hipGetLastError();
 }
 

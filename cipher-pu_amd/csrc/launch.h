@@ -87,6 +87,23 @@ hipError_t launch_lift_small(const DevTables& t, const uint64_t* in, uint64_t* o
                              const LimbMap& map, hipStream_t s);
 hipError_t launch_noise_stats(const DevTables& t, const uint64_t* in, uint64_t* stats, int n_polys,
                               int n_limbs, const LimbMap& map, hipStream_t s);
+// The monitor for rows of any length n under one modulus q (no context).
+hipError_t launch_noise_stats_q(const uint64_t* in, uint64_t* stats, int rows, size_t n, uint64_t q, hipStream_t s);
+// The reference's toy rescale floor((c q_out + floor(q_in/2)) / (q_in 2^sb)) mod q_out.
+hipError_t launch_mod_switch_toy(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, int sb,
+                                 hipStream_t s);
+// Exact product modulo any q < 2^45 through the context's primes 0..2.
+struct Crt3Const {
+    uint64_t p0inv_p1, p0inv_p1_s;     // p0^-1 mod p1, Shoup companion
+    uint64_t p0_p2, p0_p2_s;           // p0 mod p2
+    uint64_t p01inv_p2, p01inv_p2_s;   // (p0 p1)^-1 mod p2
+    uint64_t p0_q, p01_q, P_q;         // p0, p0 p1, p0 p1 p2 modulo q
+    PrimeConst Pq;                     // q, mu, mu64, ell (Barrett / 64-bit reduction mod q)
+};
+hipError_t launch_lift3(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, const PrimeConst& Pq,
+                        hipStream_t s);
+hipError_t launch_crt3_modq(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, const Crt3Const& cc,
+                            hipStream_t s);
 
 // fft kernels (k_fft.hip)
 #define CRT_MAX_WORDS 8

@@ -213,6 +213,37 @@ int ckks_bitrev(uint64_t *data, int n_rows, uint64_t N, void *stream);
  * bitlen(q_in) < 46. */
 int ckks_mod_switch(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t q_in, uint64_t q_out, void *stream);
 
+/* The reference's toy rescale: out = floor((in * q_out + floor(q_in / 2)) /
+ * (q_in * 2^scale_bits)) mod q_out for n 64-bit values, exactly as
+ * rescale(s, q_in, q_out, scale_bits) of scripts/verify_e2e_mult.py:70-75
+ * (also scripts/gen_tb_e2e_vec.py:59-63, gen_tb_rescale_vec.py:28-33)
+ * computes it on Python integers.  2 <= q_in, q_out < 2^45, 0 <= scale_bits
+ * <= 127. */
+int ckks_mod_switch_scaled(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t q_in, uint64_t q_out,
+                           int scale_bits, void *stream);
+
+/* Product modulo ANY q (2 <= q < 2^45, prime or not, no root of unity
+ * needed): a, b [n_polys][N] in [0, q) -> out [n_polys][N], in the
+ * context's ring (negacyclic, or cyclic for a CKKS_CYCLIC context).  The
+ * exact product over Z is formed from its residues modulo the context's
+ * primes 0..2 (their product must exceed 8 N q^2, else
+ * CKKS_ERR_BAD_MODULUS; the 45-bit chain primes serve any q < 2^45 up to
+ * N = 16384) and reduced modulo q.  workspace: 9 n_polys N words.
+ * Replaces the schoolbook poly_mult (scripts/verify_keyswitch.py:48-59,
+ * verify_e2e_mult.py:56-67) and neg_conv_modq
+ * (scripts/exp_e24_rotation_bootstrap.py:80-106), which accept every q,
+ * and cyclic_conv_direct (tb/ref/gen_poly_mult_vectors_n12.py:110-117). */
+int ckks_poly_mul_modq(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, int n_polys,
+                       uint64_t q, void *workspace, void *stream);
+
+/* The plain noise monitor of ckks_noise_stats for rows of any length n
+ * under one modulus q (2 <= q < 2^45), with no context: data [rows][n] in
+ * [0, q) -> stats [rows][5].  The monitor needs only q, and the
+ * reference's call site uses a toy modulus
+ * (noise_monitor_energy(f0p + f1p, TOY_Q=113) on 32 coefficients,
+ * scripts/verify_m4_h_multilayer.py:162-174,260). */
+int ckks_noise_stats_q(const uint64_t *data, uint64_t *stats, int rows, uint64_t n, uint64_t q, void *stream);
+
 /* Raw canonical embedding of real coefficients (doubles [n_vec][N]) to
  * complex slots [n_vec][N/2]: order 0 = slot j at zeta^(5^j), order 1 =
  * slot k at zeta^(4k+1).  ckks_embed_inv is its inverse (no rounding).

@@ -28,6 +28,12 @@ Pinning (what checks each function; see tests/test_oracle_pinning.py):
                               rtl/m5/tw/*.hex
   poly_mult (schoolbook) .... X^N = -1 monomial identities, NTT product
   rescale_coeff ............. verify_rescale.py invariants + big-int division
+  rescale_toy ............... equals rescale_coeff at scale_bits 0 (odd q_in),
+                              rescale_coeff >> scale_bits below q_in, exact
+                              fractions on small inputs
+  e2e_mult .................. composition of the pinned products; with the
+                              E33 key (b = a s + e + s^2) it decrypts to the
+                              rescaled plaintext product
   rns_rescale ............... CRT big-int rescale_coeff (independent path)
   keyswitch_hybrid .......... dnum=1,P=1 reduces to relin(); decrypt invariant
   automorphism .............. monomial closed form, decode(rot) = roll(decode)
@@ -287,6 +293,29 @@ def rescale_poly(c, q_in, q_out):
     return [rescale_coeff(x, q_in, q_out) for x in c]
 
 
+def rescale_toy(s: int, q_in: int, q_out: int, scale_bits: int) -> int:
+    """The toy rescale floor((s q_out + floor(q_in/2)) / (q_in 2^scale_bits))
+    mod q_out (scripts/verify_e2e_mult.py:70-75, also
+    gen_tb_e2e_vec.py:59-63, gen_tb_rescale_vec.py:28-33)."""
+    return (s * q_out + q_in // 2) // (q_in << scale_bits) % q_out
+
+
+def e2e_mult(c0_1, c1_1, c0_2, c1_2, a, b, q_in, q_out, scale_bits=None):
+    """Tensor product, relinearisation f0 = m0 + m2 b, f1 = m1 - m2 a, then
+    the rescale of every coefficient (scripts/verify_e2e_mult.py:78-102):
+    the toy rescale with scale_bits, the exact rescale_coeff when it is
+    None.  Negacyclic schoolbook products, any q_in."""
+    m0 = poly_mult(c0_1, c0_2, q_in)
+    m1 = poly_add(poly_mult(c0_1, c1_2, q_in), poly_mult(c1_1, c0_2, q_in), q_in)
+    m2 = poly_mult(c1_1, c1_2, q_in)
+    f0 = poly_add(m0, poly_mult(m2, b, q_in), q_in)
+    f1 = poly_sub(m1, poly_mult(m2, a, q_in), q_in)
+    if scale_bits is None:
+        return rescale_poly(f0, q_in, q_out), rescale_poly(f1, q_in, q_out)
+    return ([rescale_toy(x, q_in, q_out, scale_bits) for x in f0],
+            [rescale_toy(x, q_in, q_out, scale_bits) for x in f1])
+
+
 def crt_compose(limbs: Sequence[Sequence[int]], primes: Sequence[int]) -> List[int]:
     """CRT value in [0, prod(primes)) for each coefficient."""
     Q = math.prod(primes)
@@ -526,31 +555,12 @@ def decode_rns(limbs, scale, N, primes):
 # RLWE primitives (exp_e33:75-85,151-191; exp_e24:209-222,287-304)
 # ---------------------------------------------------------------------------
 
-def dgauss(rng: random.Random, sigma: float, n: int) -> List[int]:
-    """Rounded Box-Muller Gaussian (exp_e33_real_ciphertext.py:75-85)."""
-    if sigma <= 0:
-        return [0] * n
-    out = []
-    for _ in range(n):
-        u1 = rng.uniform(1e-9, 1.0)
-        u2 = rng.uniform(1e-9, 1.0)
-        out.append(int(round(math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2) * sigma)))
-    return out
-
-
 def decrypt(ct, s_limbs, primes, psis):
     """c0 + c1*s per limb (E33 convention, exp_e33:138-141)."""
     out = []
     for i, qi in enumerate(primes):
         out.append(poly_add(ct[0][i], poly_mult_ntt(ct[1][i], s_limbs[i], psis[i], qi), qi))
     return out
-
-
-def reencrypt_message(dec_limbs):
-    """DARE refresh: the decrypted, scale-carrying integer message is kept
-    as is (no division by the scale) and encrypted afresh
-    (exp_e24:287-304).  Returns the message to encrypt."""
-    return [list(l) for l in dec_limbs]
 
 
 # ---------------------------------------------------------------------------

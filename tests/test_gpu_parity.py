@@ -3,7 +3,8 @@
 Integer work is compared bit-exactly; encode/decode (the only floating
 point) within stated tolerances.  Sizes are chosen so the oracle finishes
 in seconds; full-size batches are checked through size-independent
-properties (decrypt error bounds) in test_gpu_ckks.py."""
+properties (decrypt error bounds) in test_gpu_fullsize.py, and the drop-in
+functions at the reference's own call shapes in test_gpu_dropin.py."""
 import random
 
 import numpy as np
@@ -730,8 +731,8 @@ def test_single_modulus_rlwe_drop_ins(gpu):
         wf = O.rlwe_encrypt(mf, s, e, q, 2.0 ** 12)
         assert np.array_equal(cf[0], wf[0]) and np.array_equal(cf[1], wf[1])
         assert np.array_equal(G.rlwe_decrypt(cf, s, q), O.rlwe_decrypt(wf, s, q))
-    with pytest.raises(ValueError):
-        G.neg_conv_modq([1] * 1024, [1] * 1024, 1000003)
+    x = np.arange(1024) - 512
+    assert np.array_equal(G.neg_conv_modq(x, x[::-1], 1000003), O.neg_conv_modq(x, x[::-1], 1000003))
 
 
 def test_coefficient_encoding_drop_ins(gpu):

@@ -211,6 +211,65 @@ def test_rescale_coeff_invariants(i):
         assert O.rescale_coeff(a, q_in, q_out) <= O.rescale_coeff(b, q_in, q_out)
 
 
+def test_rescale_toy_against_rescale_coeff():
+    """The toy rescale of verify_e2e_mult.py:70-75 against what fixes it
+    independently of its formula: at scale_bits 0 and odd q_in it is the
+    pinned rescale_coeff (floor((x + floor(q/2)) / q) rounds x / q half up for
+    odd and even q alike); below q_in it is rescale_coeff >> scale_bits,
+    since floor(floor(x / a) / b) = floor(x / (a b)) and nothing wraps mod
+    q_out.  A dropped q_in // 2 or a shift taken before the division fails."""
+    rng = random.Random(11)
+    for i in range(3):
+        q_in, q_out = CHAIN4[i], CHAIN4[i + 1]
+        for s in [0, 1, q_in // 2, q_in - 1] + [rng.randrange(q_in) for _ in range(500)]:
+            r = O.rescale_coeff(s, q_in, q_out)
+            assert O.rescale_toy(s, q_in, q_out, 0) == r
+            for sb in (1, 13, 30, 44):
+                assert O.rescale_toy(s, q_in, q_out, sb) == r >> sb
+    for q_in, q_out in ((2, 3), (4, 7), (113, 113), (6, 5)):
+        for s in range(3 * q_in):
+            assert O.rescale_toy(s, q_in, q_out, 0) == O.rescale_coeff(s, q_in, q_out)
+
+
+def test_e2e_mult_decrypts_to_the_rescaled_product():
+    """verify_e2e_mult.py:78-102 composed by the oracle: with the E33 key
+    b = a s + s^2 (no error, so the single-modulus relinearisation is exact)
+    and E33 ciphertexts, f0' + f1' s decrypts to round(m1 m2 q_out / q_in)
+    within the rounding of f0', f1' (1/2 + ||s||_1 / 2).  The exact
+    (scale_bits None) and toy (0) rescales agree at odd q_in.  A flipped
+    relinearisation sign or a swapped key half fails by ~q."""
+    rng = random.Random(17)
+    n, q_in, q_out = 16, CHAIN4[0], CHAIN4[1]
+    s = [rng.randrange(-1, 2) for _ in range(n)]
+    sq = [v % q_in for v in s]
+    s2 = O.poly_mult(sq, sq, q_in)
+    a_k = [rng.randrange(q_in) for _ in range(n)]
+    b_k = O.poly_add(O.poly_mult(a_k, sq, q_in), s2, q_in)
+
+    def enc(m):
+        a = [rng.randrange(q_in) for _ in range(n)]
+        c0 = O.poly_add(O.poly_mult(a, sq, q_in), [v % q_in for v in m], q_in)
+        return c0, [(-v) % q_in for v in a]
+
+    for _ in range(4):
+        m1 = [rng.randrange(-2 ** 16, 2 ** 16) for _ in range(n)]
+        m2 = [rng.randrange(-2 ** 16, 2 ** 16) for _ in range(n)]
+        (c01, c11), (c02, c12) = enc(m1), enc(m2)
+        f0, f1 = O.e2e_mult(c01, c11, c02, c12, a_k, b_k, q_in, q_out, None)
+        assert (f0, f1) == O.e2e_mult(c01, c11, c02, c12, a_k, b_k, q_in, q_out, 0)
+        d = O.poly_add(f0, O.poly_mult(f1, [v % q_out for v in s], q_out), q_out)
+        prod = O.poly_mult([v % q_in for v in m1], [v % q_in for v in m2], q_in)
+        for got, p in zip(d, prod):
+            p = p - q_in if p > q_in // 2 else p
+            got = got - q_out if got > q_out // 2 else got
+            assert abs(got - p * q_out / q_in) <= 0.5 + sum(map(abs, s)) / 2 + 1e-6
+        bad = O.poly_add(O.poly_mult(c11, c12, q_in), [0] * n, q_in)
+        f1_bad = O.poly_add(O.poly_add(O.poly_mult(c01, c12, q_in), O.poly_mult(c11, c02, q_in), q_in),
+                            O.poly_mult(bad, a_k, q_in), q_in)
+        assert f1_bad != O.relin([0] * n, O.poly_add(O.poly_mult(c01, c12, q_in), O.poly_mult(c11, c02, q_in),
+                                                      q_in), bad, a_k, b_k, q_in)[1]
+
+
 def test_rns_rescale_identity():
     rng = random.Random(3)
     N = 64
