@@ -88,7 +88,7 @@ class CKKS:
         self.sampler = sampler
         self._seed = int(seed) & ((1 << 64) - 1)
         self._draws = 0
-        self._ws = None
+        self._ws = {}      # stream handle -> workspace
 
     # -- sampling (host) ------------------------------------------------------
     def ternary(self, *shape) -> np.ndarray:
@@ -222,12 +222,21 @@ class CKKS:
         return self.ring.add(d[:, 0].contiguous(), self.ring.poly_mul(d[:, 1].contiguous(), S))
 
     # -- evaluation -----------------------------------------------------------------
-    def workspace(self, batch: int, l: int) -> torch.Tensor:
-        nbytes = call("ckks_workspace_bytes", self.ring.handle, batch, l)
+    def workspace(self, batch: int, l: int, nbytes: Optional[int] = None) -> torch.Tensor:
+        """Scratch for one call on the current stream (ckks_workspace_bytes,
+        or nbytes).  Every stream has its own buffer, allocated while that
+        stream is current: calls on different streams never share scratch,
+        and a buffer replaced by a larger one is released in the order of
+        the stream whose kernels used it."""
+        if nbytes is None:
+            nbytes = call("ckks_workspace_bytes", self.ring.handle, batch, l)
         words = (nbytes + 7) // 8
-        if self._ws is None or self._ws.numel() < words:
-            self._ws = torch.empty(words, dtype=torch.uint64, device=self.device)
-        return self._ws
+        key = torch.cuda.current_stream(self.device).cuda_stream
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < words:
+            ws = torch.empty(words, dtype=torch.uint64, device=self.device)
+            self._ws[key] = ws
+        return ws
 
     def hmult(self, a: Ciphertext, b: Ciphertext, rlk: SwitchKey) -> Ciphertext:
         """HMult + relinearise + rescale (fused; verify_e2e_mult.py:78-102)."""
@@ -331,12 +340,10 @@ class CKKS:
             out = torch.empty(R, B, 2, l, self.N, dtype=torch.uint64, device=self.device)
         elif tuple(out.shape) != (R, B, 2, l, self.N) or not out.is_contiguous():
             raise ValueError("out must be a contiguous [R, B, 2, l, N] tensor")
-        words = (call("ckks_workspace_hoisted_bytes", self.ring.handle, B, l, R) + 7) // 8
-        if self._ws is None or self._ws.numel() < words:
-            self._ws = torch.empty(words, dtype=torch.uint64, device=self.device)
+        ws = self.workspace(B, l, call("ckks_workspace_hoisted_bytes", self.ring.handle, B, l, R))
         gal = (ctypes.c_uint64 * R)(*[int(g.galois) for g in gks])
         call("ckks_rotate_hoisted_eval" if eval_form else "ckks_rotate_hoisted", self.ring.handle, _ptr(ct.data),
-             _ptr(keys), self.L + 1, gal, R, _ptr(out), B, l, self.sp, _ptr(self._ws), self.ring.stream())
+             _ptr(keys), self.L + 1, gal, R, _ptr(out), B, l, self.sp, _ptr(ws), self.ring.stream())
         return [Ciphertext(out[r], ct.scale) for r in range(R)]
 
     def add(self, a: Ciphertext, b: Ciphertext) -> Ciphertext:
