@@ -138,7 +138,9 @@ def _prod(x, y, q, cyclic=False) -> torch.Tensor:
     n = x.shape[-1]
     if _ntt_ok(n, q, cyclic):
         ctx = _ring(n, q, None, cyclic)
-        return ctx.poly_mul(ctx.tensor(x.reshape(-1, n)), ctx.tensor(np.asarray(y, np.uint64).reshape(-1, n)))
+        X = ctx.tensor(x.reshape(-1, 1, n))
+        Y = ctx.tensor(np.asarray(y, np.uint64).reshape(-1, 1, n))
+        return ctx.poly_mul(X, Y).reshape(-1, n)
     ctx = _crt_ring(n, cyclic)
     X = ctx.tensor(x.reshape(-1, n))
     Y = ctx.tensor(np.asarray(y, np.uint64).reshape(-1, n))
