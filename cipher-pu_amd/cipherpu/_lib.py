@@ -53,6 +53,7 @@ _SIGS = {
     "ckks_key_prepare": (_I, [_P, _P, _P, _I, _I, _I, _P]),
     "ckks_relin": (_I, [_P, _P, _P, _P, _I, _P, _I, _I, _I, _P, _P]),
     "ckks_hmult": (_I, [_P, _P, _P, _P, _I, _P, _I, _I, _I, _P, _P]),
+    "ckks_hmult_noalloc": (_I, [_P, _P, _P, _P, _I, _P, _I, _I, _I, _P]),
     "ckks_automorphism": (_I, [_P, _P, _P, _I, _I, _I, _U, _P]),
     "ckks_rotate": (_I, [_P, _P, _P, _I, _P, _I, _I, _I, _U, _P, _P]),
     "ckks_rotate_many": (_I, [_P, _P, _P, _I, _P, _P, _I, _I, _I, _P, _P]),

@@ -4,10 +4,18 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "ckks_core.h"
 #include "launch.h"
+
+// Scratch of ckks_hmult_noalloc.  A device variable: every GPU the library
+// runs on holds its own copy, at its own address, until the process ends.
+// In host code this name is only a stand-in for the symbol; the device
+// address comes from hipGetSymbolAddress (scratch_on_device), and only that
+// address is ever handed to a kernel.
+static __device__ __align__(256) uint64_t scratch[CKKS_NOALLOC_SCRATCH_BYTES / 8];
 
 namespace {
 
@@ -104,6 +112,7 @@ struct ckks_ctx {
     };
     mutable Timer timer;
     int narrow = 0;        // every prime below twice every other one
+    void* scratch = nullptr;   // this device's copy of the no-alloc scratch array
     DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN, narrow}; }
 };
 
@@ -160,6 +169,23 @@ void timer_end(const ckks_ctx* c, int slot, hipStream_t s) {
     } while (0)
 
 int set_device(const ckks_ctx* ctx) { return hipSetDevice(ctx->device) == hipSuccess ? CKKS_OK : CKKS_ERR_HIP; }
+
+// Device address of the scratch array on the current device, looked up once
+// per device (with that device current) and kept.
+const int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_addr[SCRATCH_DEVICES] = {};
+
+void* scratch_on_device(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_addr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        scratch_addr[device] = p;
+    }
+    return scratch_addr[device];
+}
 
 // constants for ModDown by the special prime sp and rescale by limb l-1
 ModDownConst moddown_const(const ckks_ctx* c, int l, int sp) {
@@ -331,6 +357,9 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
         ckks_ctx_destroy(c);
         return CKKS_ERR_HIP;
     }
+    // set-up of the no-alloc entry point, with the device current; a failed
+    // lookup leaves scratch NULL and only ckks_hmult_noalloc reports it
+    c->scratch = scratch_on_device(device);
     *out = c;
     return CKKS_OK;
 }
@@ -613,6 +642,18 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, nullptr, 0, out, batch, l, m, special, mc, true,
                                                     S(stream)));
     return CKKS_OK;
+}
+
+int ckks_hmult_noalloc(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const uint64_t* key, int key_limbs,
+                       uint64_t* out, int batch, int l, int special, void* stream) {
+    if (!c) return CKKS_ERR_NULL;
+    if (batch < 0 || l < 1) return CKKS_ERR_BAD_SHAPE;
+    if (ckks_workspace_bytes(c, batch, l) > CKKS_NOALLOC_SCRATCH_BYTES) return CKKS_ERR_BAD_SHAPE;
+    if (!c->scratch) return CKKS_ERR_HIP;
+    // the same kernels as ckks_hmult, with the device's scratch array as the
+    // workspace; nothing in that path allocates, and no part of the
+    // workspace needs to start cleared (every word is written before it is read)
+    return ckks_hmult(c, a, b, key, key_limbs, out, batch, l, special, c->scratch, stream);
 }
 
 int ckks_automorphism(const ckks_ctx* c, const uint64_t* in, uint64_t* out, int n_polys, int n_limbs, int prime0,

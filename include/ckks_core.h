@@ -139,6 +139,20 @@ int ckks_relin(const ckks_ctx *ctx, const uint64_t *d01, const uint64_t *d2, con
 int ckks_hmult(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, const uint64_t *key, int key_limbs,
                uint64_t *out, int batch, int l, int special, void *workspace, void *stream);
 
+/* ckks_hmult without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a library-owned device array of CKKS_NOALLOC_SCRATCH_BYTES
+ * per GPU (held until the process ends); the call allocates nothing.
+ * batch * ckks_workspace_bytes(ctx, 1, l) must fit in it, else
+ * CKKS_ERR_BAD_SHAPE (at N = 8192, l = 4: at most 24 pairs).  Every call on
+ * one GPU shares that GPU's array, so calls that may overlap on one GPU
+ * (other streams, other host threads) race: issue them on one stream.
+ * Results are word for word those of ckks_hmult. */
+#define CKKS_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int ckks_hmult_noalloc(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, const uint64_t *key,
+                       int key_limbs, uint64_t *out, int batch, int l, int special, void *stream);
+
 /* Galois automorphism f(X) -> f(X^g), g odd, coefficient form, out of
  * place.  Replaces automorphism_exact (exp_e24_rotation_bootstrap.py:143-162). */
 int ckks_automorphism(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, int n_polys, int n_limbs,
