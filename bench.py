@@ -9,13 +9,18 @@ dnum = 4, scale 2^40, inputs already resident in HBM; 256 pairs encrypting
 Prints one JSON line.  ``value`` = HMult/s over the timed steps.  The
 roofline entry is for the dominant kernel of the step (the key switch),
 timed per launch with HIP events on the call's stream over the timed
-region, priced at its algorithmic bytes: per ciphertext the coefficient
-digits d2 (l limbs), d2 of the own limb in evaluation form (l limbs),
-d0/d1 in evaluation form (2 l limbs) and the accumulators written
-(2 (l + 1) limbs); per launch the relinearisation key once (l digits x
-2 x (l + 1) limbs, 8 B a coefficient).  The whole step is reported
-beside it at 1,452,032 B per HMult (inputs, outputs and the key once per
-batch).  ``cpu_baseline`` times the CPU oracle on a bounded sample.
+region.  Its ``achieved`` prices one launch at the algorithmic bytes of the
+units it processes: 1,452,032 B per HMult (two input ciphertexts of 2 x 4
+limb-polys, one output of 2 x 3, the 2.5 MiB relinearisation key once per
+batch of 256) x the batch.  ``kernel_model`` prices the same launch at the
+bytes the key-switch kernel itself must move (per ciphertext: the
+coefficient digits d2, l limbs; d2 of the own limb in evaluation form, l;
+d0/d1 in evaluation form, 2 l; the accumulators written, 2 (l + 1); the
+key once per launch).  ``traffic`` and ``valu_issue`` come from rocprofv3
+counter runs of this command for the same kernel, batch and launch shape
+(the file is named in ``source``).  ``cpu_baseline`` runs the oracle's
+pure-Python HMult (the reference's big-integer path) over every host core
+with multiprocessing; the one-core figure and the C oracle are beside it.
 ``mlp`` runs the encrypted MLP on 4096 synthetic images under the plain
 decrypt-based monitor gear loop and under the learned gear controller;
 ``cnn`` does the same for the encrypted CNN (conv 5x5x8 stride 2, x^2,
@@ -44,6 +49,9 @@ HBM_PEAK_GBS = 8000.0
 N, L = 8192, 4
 
 
+BYTES_PER_HMULT = 1452032   # (2 x 2 x 4 + 2 x 3) limb-polys of 65,536 B + the 2,621,440 B key / 256
+
+
 def algorithmic_bytes(batch: int, l: int = L) -> int:
     """Whole HMult step: inputs, outputs, the relinearisation key once."""
     limb = N * 8
@@ -63,16 +71,17 @@ def keyswitch_bytes(batch: int, l: int = L) -> int:
     return batch * per_ct + key
 
 
-# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
-# this bench, scaled by the same counters on the --copy-probe copy (the
-# summaries and commands are under profiles/).  Keyed by (kernel kind,
-# batch, threads); None where not measured for the running shape.
-MEASURED_TRAFFIC = {("keyswitch", 256, 1024): 692131469}
-
-# Wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU of
-# the same command, profiles/), same keys.  The issue peak counts one wave64
-# VALU instruction per SIMD every 4 cycles: 1024 SIMDs x 2.4 GHz / 4.
-MEASURED_VALU_INSTR = {("keyswitch", 256, 1024): 154361856}
+# Counter figures per launch of the key switch, from rocprofv3 --pmc runs of
+# this bench (commands and tables in PROFILE_SOURCE): HBM bytes from
+# FETCH_SIZE / WRITE_SIZE scaled by the same counters on the --copy-probe
+# copy, and wave-level VALU instructions (SQ_INSTS_VALU).  Keyed by (batch,
+# key-switch threads); the bench only runs shapes listed here.
+PROFILE_SOURCE = "profiles/keyswitch_hmult_n8192_l4_b256.md"
+MEASURED_TRAFFIC = {(256, 1024): 692131469}
+MEASURED_VALU_INSTR = {(256, 1024): 154361856}
+PROFILED_KS_THREADS = (1024,)
+# The issue peak counts one wave64 VALU instruction per SIMD every 4 cycles:
+# 1024 SIMDs x 2.4 GHz / 4.
 VALU_ISSUE_PEAK_G = 1024 * 2.4 / 4
 
 
@@ -96,30 +105,49 @@ def _c_oracle_worker(args):
     return time.perf_counter() - t
 
 
-def cpu_baseline(sample_s: float, cta, ctb, key_coeff, primes, roots):
-    """The oracle's pure-Python HMult (big-int restatement of the
-    reference's golden path) on whole ciphertext pairs for about sample_s
-    seconds on one core; also the C restatement over all cores."""
+_PY_JOB = {}
+
+
+def _py_oracle_worker(i):
+    """One pure-Python HMult of pair i of the shared job (fork-inherited)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ckks_oracle as O
-    import numpy as np
+    cta, ctb, keyl, pr, p, psis = (_PY_JOB[k] for k in ("cta", "ctb", "key", "pr", "p", "psis"))
+    a = [[list(map(int, cta[i % len(cta), c, j])) for j in range(L)] for c in range(2)]
+    b = [[list(map(int, ctb[i % len(ctb), c, j])) for j in range(L)] for c in range(2)]
+    t = time.perf_counter()
+    O.hmult(a, b, keyl, pr, p, psis, N)
+    return time.perf_counter() - t
+
+
+def cpu_baseline(sample_s: float, cta, ctb, key_coeff, primes, roots):
+    """The oracle's pure-Python HMult (big-int restatement of the
+    reference's golden path) on whole ciphertext pairs: one process per host
+    core (at most 16) over independent pairs, two rounds of pairs; the same
+    on one core for about sample_s seconds; and the C restatement over all
+    cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
     pr, p, ps, pp = primes[:L], primes[L], roots[:L], roots[L]
     keyl = [([list(map(int, key_coeff[j, 0, i])) for i in range(L + 1)],
              [list(map(int, key_coeff[j, 1, i])) for i in range(L + 1)]) for j in range(L)]
-    done, t0 = 0, time.perf_counter()
-    while True:
-        a = [[list(map(int, cta[done % len(cta), c, i])) for i in range(L)] for c in range(2)]
-        b = [[list(map(int, ctb[done % len(ctb), c, i])) for i in range(L)] for c in range(2)]
-        O.hmult(a, b, keyl, pr, p, ps + [pp], N)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= sample_s or done >= len(cta):
-            break
-    py = {"value": done / el, "unit": "HMult/s", "cores": 1, "kind": "port",
-          "sample": f"{done} HMult (N=8192, L=4, dnum=4) in pure-Python big-int arithmetic, {el:.1f} s"}
-    # C restatement, one process per core over independent pairs
+    _PY_JOB.update(cta=cta, ctb=ctb, key=keyl, pr=pr, p=p, psis=ps + [pp])
     cores = min(os.cpu_count() or 1, 16)
+    n_all = 2 * cores
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        pool.map(_py_oracle_worker, range(n_all), chunksize=1)
+    el_all = time.perf_counter() - t0
+    py_all = {"value": n_all / el_all, "unit": "HMult/s", "cores": cores, "kind": "port",
+              "sample": f"{n_all} HMult (N=8192, L=4, dnum=4) in pure-Python big-int arithmetic, "
+                        f"{cores} processes, {el_all:.1f} s"}
+    done, busy = 0, 0.0
+    while busy < sample_s and done < len(cta):
+        busy += _py_oracle_worker(done)
+        done += 1
+    py = {"value": done / busy, "unit": "HMult/s", "cores": 1, "kind": "port",
+          "sample": f"{done} HMult (N=8192, L=4, dnum=4) in pure-Python big-int arithmetic, {busy:.1f} s"}
+    # C restatement, one process per core over independent pairs
     n = max(cores, 8)
     jobs = [(cta[i % len(cta)], ctb[i % len(ctb)], key_coeff, pr, p, ps, pp) for i in range(n)]
     t0 = time.perf_counter()
@@ -128,7 +156,7 @@ def cpu_baseline(sample_s: float, cta, ctb, key_coeff, primes, roots):
     el = time.perf_counter() - t0
     c = {"value": n / el, "unit": "HMult/s", "cores": cores, "kind": "port",
          "sample": f"{n} HMult in the C oracle (unsigned __int128), {cores} processes, {el:.1f} s"}
-    return py, c
+    return py_all, py, c
 
 
 def copy_probe(nbytes: int = 1 << 28, reps: int = 5) -> None:
@@ -397,7 +425,8 @@ def main():
         ck.ring.set_threads(args.ntt_threads, args.ks_threads, args.tensor_threads)
     else:
         from cipherpu.tune import tune_hmult
-        tuned = tune_hmult(ck, cta, ctb, rlk, reps=10, rounds=4)
+        ks = PROFILED_KS_THREADS if B in {b for b, _ in MEASURED_TRAFFIC} else None
+        tuned = tune_hmult(ck, cta, ctb, rlk, reps=10, rounds=4, ks=ks)
     out = ck.hmult(cta, ctb, rlk)
     torch.cuda.synchronize()
     # decrypted product vs the FP64 slot-wise product (all pairs)
@@ -434,14 +463,19 @@ def main():
     dom_ms = kern[dom][0] / kern[dom][1]
     assert dom == "keyswitch", f"dominant kernel is {dom}; the byte model covers the key switch"
     kbytes = keyswitch_bytes(B)
-    k_achieved = kbytes / (dom_ms / 1e3) / 1e9
-    traffic = MEASURED_TRAFFIC.get((dom, B, shape.get("ks")))
-    valu_instr = MEASURED_VALU_INSTR.get((dom, B, shape.get("ks")))
-    valu = None
-    if valu_instr:
-        rate = valu_instr / (dom_ms / 1e3) / 1e9
-        valu = {"wave_instructions": valu_instr, "achieved": round(rate, 1), "peak": VALU_ISSUE_PEAK_G,
-                "unit": "G wave-instr/s", "frac": round(rate / VALU_ISSUE_PEAK_G, 4)}
+    k_achieved = nbytes / (dom_ms / 1e3) / 1e9
+    m_achieved = kbytes / (dom_ms / 1e3) / 1e9
+    prof_key = (B, shape.get("ks"))
+    if prof_key in MEASURED_TRAFFIC:
+        traffic = MEASURED_TRAFFIC[prof_key]
+        rate = MEASURED_VALU_INSTR[prof_key] / (dom_ms / 1e3) / 1e9
+        valu = {"wave_instructions": MEASURED_VALU_INSTR[prof_key], "achieved": round(rate, 1),
+                "peak": VALU_ISSUE_PEAK_G, "unit": "G wave-instr/s", "frac": round(rate / VALU_ISSUE_PEAK_G, 4),
+                "source": PROFILE_SOURCE}
+    elif B == 256 and not (args.ntt_threads or args.ks_threads or args.tensor_threads or args.no_tune):
+        raise RuntimeError(f"no counter profile for the key switch at batch {B}, {shape.get('ks')} threads")
+    else:   # an explicit shape or batch outside the profiled ones: say so
+        traffic, valu = None, {"source": f"not profiled at batch {B}, {shape.get('ks')} threads"}
 
     res = {
         "metric": "HMult+relin+rescale/s at N=2^13 L=4",
@@ -462,7 +496,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(k_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k_achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"k_keyswitch<13,{shape.get('ks') or 'default'}>",
-                     "avg_launch_ms": round(dom_ms, 4), "algorithmic_bytes_per_launch": kbytes},
+                     "avg_launch_ms": round(dom_ms, 4), "algorithmic_bytes_per_launch": nbytes,
+                     "units_per_launch": B, "bytes_per_unit": BYTES_PER_HMULT,
+                     "traffic_source": PROFILE_SOURCE if traffic else None,
+                     "kernel_model": {"bytes_per_launch": kbytes, "achieved": round(m_achieved, 1),
+                                      "frac": round(m_achieved / HBM_PEAK_GBS, 4)}},
         "valu_issue": valu,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kern.items()},
         "step_roofline": {"achieved": round(nbytes / (ms / 1e3) / 1e9, 1), "unit": "GB/s",
@@ -484,9 +522,10 @@ def main():
     if not args.no_graph:
         res["batch_graph"] = graph_leg(args.graph_ops)
     if not args.no_cpu_baseline:
-        py, c = cpu_baseline(args.cpu_sample_s, to_numpy(cta.data[:8]), to_numpy(ctb.data[:8]),
-                             to_numpy(rlk.coeff), ck.primes, ck.ring.roots)
-        res["cpu_baseline"] = py
+        py_all, py, c = cpu_baseline(args.cpu_sample_s, to_numpy(cta.data[:32]), to_numpy(ctb.data[:32]),
+                                     to_numpy(rlk.coeff), ck.primes, ck.ring.roots)
+        res["cpu_baseline"] = py_all
+        res["cpu_baseline_1core"] = py
         res["cpu_baseline_c"] = c
         if not args.no_product:
             res["cpu_baseline_product"] = cpu_product_baseline()
