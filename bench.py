@@ -61,14 +61,32 @@ def algorithmic_bytes(batch: int, l: int = L) -> int:
     return batch * (inputs + outputs) + key
 
 
-def keyswitch_bytes(batch: int, l: int = L) -> int:
-    """One key-switch launch inside HMult: per ciphertext d2 in coefficient
-    form (l limbs) and its own-limb evaluation form (l limbs), d0/d1 in
-    evaluation form (2 l), accumulators out (2 (l + 1)); the key once."""
+KERNEL_NAMES = {"front": "k_front<13,{}>", "tensor": "k_tensor_intt<13,{}>", "ntt": "k_ntt_fwd<13,{}>",
+                "keyswitch": "k_keyswitch<13,{},false,false> (limb l-1 and the special prime)",
+                "keyswitch_fused": "k_keyswitch<13,{},false,true> (limbs 0..l-2, ModDown and rescale fused)",
+                "moddown": "k_moddown_rescale_l<4,true>"}
+
+
+def kernel_model_bytes(kind: str, batch: int, l: int = L):
+    """Bytes one launch of an HMult kernel must move, from its inputs and
+    outputs (limb-polys of N 8-byte words; key rows are 16-byte value and
+    companion pairs, read once per launch):
+      front            per (pair, limb): a0, a1, b0, b1 in; p d0, p d1, d2
+                       (evaluation form) and d2 (coefficient form) out;
+      keyswitch        targets l-1 and p, per pair: the other limbs' digits
+                       (l - 1 and l), the own digit and p d0, p d1 for limb
+                       l-1, the 2 x 2 accumulators out; key rows of both;
+      keyswitch_fused  targets 0..l-2, per (pair, target): l - 1 digits, the
+                       own digit, p d0, p d1, the accumulators of limb l-1
+                       and of p (4 rows), 2 output limbs; their key rows."""
     limb = N * 8
-    per_ct = (l + l + 2 * l + 2 * (l + 1)) * limb
-    key = l * 2 * (l + 1) * limb
-    return batch * per_ct + key
+    if kind == "front":
+        return batch * l * 8 * limb
+    if kind == "keyswitch":
+        return batch * ((l - 1 + 1 + 2 + 2) + (l + 2)) * limb + 2 * l * 2 * 2 * limb
+    if kind == "keyswitch_fused":
+        return batch * (l - 1) * ((l - 1) + 1 + 2 + 4 + 2) * limb + (l - 1) * l * 2 * 2 * limb
+    return None
 
 
 # Counter figures per launch of the key switch, from rocprofv3 --pmc runs of
@@ -76,10 +94,12 @@ def keyswitch_bytes(batch: int, l: int = L) -> int:
 # FETCH_SIZE / WRITE_SIZE scaled by the same counters on the --copy-probe
 # copy, and wave-level VALU instructions (SQ_INSTS_VALU).  Keyed by (batch,
 # key-switch threads); the bench only runs shapes listed here.
-PROFILE_SOURCE = "profiles/keyswitch_hmult_n8192_l4_b256.md"
-MEASURED_TRAFFIC = {(256, 1024): 692131469}
-MEASURED_VALU_INSTR = {(256, 1024): 154361856}
-PROFILED_KS_THREADS = (1024,)
+PROFILE_SOURCE = "profiles/hmult_kernels_n8192_l4_b256.md"
+MEASURED_TRAFFIC = {("front", 256, 1024): 556527155, ("keyswitch_fused", 256, 1024): 636715487,
+                    ("keyswitch", 256, 1024): 256456465}
+MEASURED_VALU_INSTR = {("front", 256, 1024): 133808128, ("keyswitch_fused", 256, 1024): 114634752,
+                       ("keyswitch", 256, 1024): 64462848}
+PROFILED_THREADS = (1024,)   # key-switch and front/tensor threads of the profiled shapes
 # The issue peak counts one wave64 VALU instruction per SIMD every 4 cycles:
 # 1024 SIMDs x 2.4 GHz / 4.
 VALU_ISSUE_PEAK_G = 1024 * 2.4 / 4
@@ -425,8 +445,11 @@ def main():
         ck.ring.set_threads(args.ntt_threads, args.ks_threads, args.tensor_threads)
     else:
         from cipherpu.tune import tune_hmult
-        ks = PROFILED_KS_THREADS if B in {b for b, _ in MEASURED_TRAFFIC} else None
-        tuned = tune_hmult(ck, cta, ctb, rlk, reps=10, rounds=4, ks=ks)
+        # at a profiled batch the tuner stays on the profiled key-switch and
+        # front shapes, so the counter figures belong to the kernels timed
+        prof = B in {b for _, b, _ in MEASURED_TRAFFIC}
+        tuned = tune_hmult(ck, cta, ctb, rlk, reps=10, rounds=4, ks=PROFILED_THREADS if prof else None,
+                           tensor=PROFILED_THREADS if prof else None)
     out = ck.hmult(cta, ctb, rlk)
     torch.cuda.synchronize()
     # decrypted product vs the FP64 slot-wise product (all pairs)
@@ -459,13 +482,15 @@ def main():
             ts = [v for k, v in tuned["grid"].items() if k.startswith(f"{f}/")]
             if ts:
                 shape[name] = round(min(ts), 4)
-    dom = max(kern, key=lambda k: kern[k][0])
-    dom_ms = kern[dom][0] / kern[dom][1]
-    assert dom == "keyswitch", f"dominant kernel is {dom}; the byte model covers the key switch"
-    kbytes = keyswitch_bytes(B)
+    # the dominant kernel: the longest launch of the step
+    per_launch = {k: v[0] / v[1] for k, v in kern.items()}
+    dom = max(per_launch, key=per_launch.get)
+    dom_ms = per_launch[dom]
+    threads = {"front": shape.get("tensor"), "tensor": shape.get("tensor"), "keyswitch": shape.get("ks"),
+               "keyswitch_fused": shape.get("ks"), "ntt": shape.get("ntt"), "moddown": 256}.get(dom)
+    kbytes = kernel_model_bytes(dom, B)
     k_achieved = nbytes / (dom_ms / 1e3) / 1e9
-    m_achieved = kbytes / (dom_ms / 1e3) / 1e9
-    prof_key = (B, shape.get("ks"))
+    prof_key = (dom, B, threads)
     if prof_key in MEASURED_TRAFFIC:
         traffic = MEASURED_TRAFFIC[prof_key]
         rate = MEASURED_VALU_INSTR[prof_key] / (dom_ms / 1e3) / 1e9
@@ -473,9 +498,9 @@ def main():
                 "peak": VALU_ISSUE_PEAK_G, "unit": "G wave-instr/s", "frac": round(rate / VALU_ISSUE_PEAK_G, 4),
                 "source": PROFILE_SOURCE}
     elif B == 256 and not (args.ntt_threads or args.ks_threads or args.tensor_threads or args.no_tune):
-        raise RuntimeError(f"no counter profile for the key switch at batch {B}, {shape.get('ks')} threads")
+        raise RuntimeError(f"no counter profile for {dom} at batch {B}, {threads} threads")
     else:   # an explicit shape or batch outside the profiled ones: say so
-        traffic, valu = None, {"source": f"not profiled at batch {B}, {shape.get('ks')} threads"}
+        traffic, valu = None, {"source": f"not profiled: {dom} at batch {B}, {threads} threads"}
 
     res = {
         "metric": "HMult+relin+rescale/s at N=2^13 L=4",
@@ -495,14 +520,16 @@ def main():
                    "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": round(k_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k_achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_keyswitch<13,{shape.get('ks') or 'default'}>",
+                     "kernel": KERNEL_NAMES[dom].format(threads),
                      "avg_launch_ms": round(dom_ms, 4), "algorithmic_bytes_per_launch": nbytes,
                      "units_per_launch": B, "bytes_per_unit": BYTES_PER_HMULT,
                      "traffic_source": PROFILE_SOURCE if traffic else None,
-                     "kernel_model": {"bytes_per_launch": kbytes, "achieved": round(m_achieved, 1),
-                                      "frac": round(m_achieved / HBM_PEAK_GBS, 4)}},
+                     "kernel_model": kbytes and {"bytes_per_launch": kbytes,
+                                                 "achieved": round(kbytes / (dom_ms / 1e3) / 1e9, 1),
+                                                 "frac": round(kbytes / (dom_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}},
         "valu_issue": valu,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kern.items()},
+        "launches_per_step": {k: v[1] // args.steps for k, v in kern.items()},
         "step_roofline": {"achieved": round(nbytes / (ms / 1e3) / 1e9, 1), "unit": "GB/s",
                           "frac": round(nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_step": nbytes},

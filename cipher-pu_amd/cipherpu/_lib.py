@@ -88,7 +88,7 @@ _SIGS = {
 }
 
 # Kernel kinds of ckks_timing_stop (include/ckks_core.h CKKS_KIND_*)
-KERNEL_KINDS = ("ntt", "tensor", "front", "keyswitch", "moddown", "automorphism", "polymul")
+KERNEL_KINDS = ("ntt", "tensor", "front", "keyswitch", "moddown", "automorphism", "polymul", "keyswitch_fused")
 
 EXPORTED = tuple(_SIGS)
 

@@ -637,10 +637,19 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
         TIMED(CKKS_KIND_TENSOR,
               launch_tensor_intt(t, ea, eb, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
     }
+    // the key switch of limb l-1 and of the special prime first, then the
+    // other chain limbs with ModDown and the rescale fused in
+    KsTargets tail, rest;
+    tail.first = l - 1;
+    tail.count = 2;
+    rest.first = 0;
+    rest.count = l - 1;
+    rest.out = out;
+    rest.mc = &mc;
     TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc, batch,
-                                                l, m, special, c->ks_threads, S(stream)));
-    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, nullptr, 0, out, batch, l, m, special, mc, true,
-                                                    S(stream)));
+                                                l, m, special, c->ks_threads, S(stream), nullptr, tail));
+    TIMED(CKKS_KIND_KEYSWITCH_FUSED, launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc,
+                                                      batch, l, m, special, c->ks_threads, S(stream), nullptr, rest));
     return CKKS_OK;
 }
 

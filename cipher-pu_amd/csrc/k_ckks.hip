@@ -9,6 +9,8 @@
 //   forward transform)
 //   key  [digit][2][key_limbs][N] of (value, Shoup companion) pairs,
 //        evaluation order; digit j = (b_j, a_j), last limb = special prime
+#include <string.h>
+
 #include "launch.h"
 #include "ntt_core.h"
 
@@ -164,6 +166,42 @@ __device__ __forceinline__ void key_mac(uint64_t (&a0)[C::E], uint64_t (&a1)[C::
     }
 }
 
+// ModDown by p and rescale by q_(l-1) of one coefficient of chain limb t <
+// l - 1, from the coefficient-form accumulators of limb t (xt), of limb
+// l - 1 (xl) and of the special prime (xp), all reduced: the per-limb steps
+// of k_moddown_rescale_l, word for word.
+__device__ __forceinline__ uint64_t moddown_rescale_one(uint64_t xt, uint64_t xl, uint64_t xp, int t, int lm1,
+                                                        const PrimeConst& pc, const PrimeConst& pl,
+                                                        const PrimeConst& pp, const ModDownConst& mc, int narrow) {
+    const uint64_t tp = csub(xp + (pp.q >> 1), pp.q);
+    const uint64_t ql = pl.q, q = pc.q;
+    const uint64_t tpl = narrow ? csub(tp, ql) : reduce64(tp, pl);
+    const uint64_t yl = csub(mul_shoup(xl + mc.hp[lm1] + ql - tpl, mc.pinv[lm1], mc.pinv_s[lm1], ql), ql);
+    const uint64_t tl = csub(yl + (ql >> 1), ql);
+    const uint64_t tpt = narrow ? csub(tp, q) : reduce64(tp, pc);
+    const uint64_t yt = csub(mul_shoup(xt + mc.hp[t] + q - tpt, mc.pinv[t], mc.pinv_s[t], q), q);
+    const uint64_t tlt = narrow ? csub(tl, q) : reduce64(tl, pc);
+    return csub(mul_shoup(yt + mc.hq[t] + q - tlt, mc.qinv[t], mc.qinv_s[t], q), q);
+}
+
+// The fused epilogue for poly c of target t: x holds the accumulator in
+// coefficient form (coalesced layout, reduced); the special-prime and limb
+// l - 1 accumulators come from an earlier launch through acc_c ([l+1][N]).
+template <class C>
+__device__ __forceinline__ void fused_out(const uint64_t (&x)[C::E], const uint64_t* __restrict__ acc_c,
+                                          uint64_t* __restrict__ out_c, int t, int l,
+                                          const PrimeConst* __restrict__ pcs, const LimbMap& map, int pidx_p,
+                                          const ModDownConst& mc, int narrow, int tid) {
+    const PrimeConst pc = pcs[map.idx[t]], pl = pcs[map.idx[l - 1]], pp = pcs[pidx_p];
+    const uint64_t* sp = acc_c + (size_t)l * C::N;
+    const uint64_t* sl = acc_c + (size_t)(l - 1) * C::N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+        const int j = e * C::T + tid;
+        out_c[j] = moddown_rescale_one(x[e], sl[j], sp[j], t, l - 1, pc, pl, pp, mc, narrow);
+    }
+}
+
 // Hybrid key switch for one (ciphertext, target limb t) pair, t = 0..l where
 // t = l is the special prime p.  Generalises the single-digit relinearisation
 // f0 = e0 + e2 b, f1 = e1 - e2 a (verify_keyswitch.py:62-68,
@@ -177,21 +215,28 @@ __device__ __forceinline__ void key_mac(uint64_t (&a0)[C::E], uint64_t (&a1)[C::
 // KEYTAB: ciphertext b switches with its own key, key_table[b] (a device
 // address), instead of the one key of the launch (rotations by different
 // steps in one launch).
-template <int LOGN, int T, bool KEYTAB = false>
+// Targets t0 .. t0 + gridDim.x / batch - 1 in one launch.  FUSED (HMult,
+// chain limbs t < l - 1 only): instead of writing its accumulators, the
+// workgroup finishes ModDown and the rescale for its limb, reading the
+// coefficient-form accumulators of limb l - 1 and of the special prime
+// that an earlier launch (targets l - 1 and l) left in acc, and writes the
+// output ciphertext limb [batch][2][l-1][N] to out.
+template <int LOGN, int T, bool KEYTAB = false, bool FUSED = false>
 __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2c, size_t d2c_bstride,
                                                  const uint64_t* __restrict__ d2e,
                                                  const uint64_t* __restrict__ d01, const ulonglong2* __restrict__ key,
                                                  int key_limbs, uint64_t* __restrict__ acc,
                                                  const ulonglong2* __restrict__ tw_all,
                                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
-                                                 int batch, int reduce_digits, const uint64_t* __restrict__ key_table) {
+                                                 int batch, int reduce_digits, const uint64_t* __restrict__ key_table,
+                                                 int t0, uint64_t* __restrict__ out_ct, ModDownConst mc, int narrow) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
     // Target-major order: workgroups in flight together share one target
     // limb, so each XCD's L2 holds one limb's key slice (2 dnum N 16 B) and
     // one prime's twiddles rather than the whole key.
-    const int t = blockIdx.x / batch, b = blockIdx.x % batch;
+    const int t = t0 + (int)(blockIdx.x / batch), b = blockIdx.x % batch;
     if constexpr (KEYTAB) key = reinterpret_cast<const ulonglong2*>(key_table[b]);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
@@ -242,6 +287,8 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
         a1[k] = v ? pc.q - v : 0;
     }
     uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
+    const uint64_t* acc_b = acc + (size_t)b * 2 * (l + 1) * C::N;
+    uint64_t* out_b = FUSED ? out_ct + ((size_t)b * 2 * (l - 1) + t) * C::N : nullptr;
     const ulonglong2* itw = tw + C::N;
     if constexpr (ntt::pair_pays<C>()) {
         // both accumulators as one transform pair
@@ -251,19 +298,29 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
             a0[k] = csub(a0[k], pc.q);
             a1[k] = csub(a1[k], pc.q);
         }
-        ntt::store_coalesced<C>(a0, out, tid);
-        ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
+        if constexpr (FUSED) {
+            fused_out<C>(a0, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
+            fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map,
+                         pidx_p, mc, narrow, tid);
+        } else {
+            ntt::store_coalesced<C>(a0, out, tid);
+            ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
+        }
         return;
     }
     ntt::inv_ntt<C>(a0, lds, itw, pc, tid);
 #pragma unroll
     for (int k = 0; k < C::E; ++k) a0[k] = csub(a0[k], pc.q);
-    ntt::store_coalesced<C>(a0, out, tid);
+    if constexpr (FUSED) fused_out<C>(a0, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
+    else ntt::store_coalesced<C>(a0, out, tid);
     __syncthreads();
     ntt::inv_ntt<C>(a1, lds, itw, pc, tid);
 #pragma unroll
     for (int k = 0; k < C::E; ++k) a1[k] = csub(a1[k], pc.q);
-    ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
+    if constexpr (FUSED)
+        fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map, pidx_p, mc,
+                     narrow, tid);
+    else ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
 }
 
 // ModDown by p (round-half-up of X/p), optional addition of coefficient-form
@@ -732,14 +789,19 @@ hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, u
 template <int LOGN, int T>
 hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uint64_t* d2e, const uint64_t* d01,
                   const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
-                  int pidx_p, const uint64_t* key_table, hipStream_t s) {
+                  int pidx_p, const uint64_t* key_table, const KsTargets& tg, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
-    auto k = key_table ? k_keyswitch<LOGN, T, true> : k_keyswitch<LOGN, T, false>;
+    auto k = tg.out ? k_keyswitch<LOGN, T, false, true>
+                    : (key_table ? k_keyswitch<LOGN, T, true> : k_keyswitch<LOGN, T, false>);
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(batch * (l + 1)), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.tw,
-                       t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table);
+    const int nt = tg.count ? tg.count : l + 1;
+    ModDownConst mc;
+    if (tg.mc) mc = *tg.mc;
+    else memset(&mc, 0, sizeof(mc));
+    hipLaunchKernelGGL(k, dim3(batch * nt), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.tw,
+                       t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table, tg.first, tg.out, mc, t.narrow);
     return hipGetLastError();
 }
 
@@ -786,17 +848,20 @@ hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01,
                             const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l,
-                            const LimbMap& map, int pidx_p, int threads, hipStream_t s, const uint64_t* key_table) {
+                            const LimbMap& map, int pidx_p, int threads, hipStream_t s, const uint64_t* key_table,
+                            const KsTargets& tg) {
     if (batch * l == 0) return hipSuccess;
+    if (tg.out && (key_table || tg.first < 0 || tg.first + tg.count > l - 1 || !tg.mc)) return hipErrorInvalidValue;
+    if (tg.count < 0 || tg.first < 0 || tg.first + (tg.count ? tg.count : l + 1) > l + 1) return hipErrorInvalidValue;
     if (!threads) threads = default_threads(t.logN);
     switch (t.logN * 10000 + threads) {
-        case 120256: return run_ks<12, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
-        case 120512: return run_ks<12, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
-        case 130256: return run_ks<13, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
-        case 130512: return run_ks<13, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
-        case 131024: return run_ks<13, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
-        case 140512: return run_ks<14, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
-        case 141024: return run_ks<14, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, s);
+        case 120256: return run_ks<12, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
+        case 120512: return run_ks<12, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
+        case 130256: return run_ks<13, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
+        case 130512: return run_ks<13, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
+        case 131024: return run_ks<13, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
+        case 140512: return run_ks<14, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
+        case 141024: return run_ks<14, 1024>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -60,12 +60,23 @@ hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint
 hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
                         uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, int threads,
                         hipStream_t s);
+// Which target limbs one key-switch launch covers: first .. first + count - 1
+// (count 0: all l + 1).  With out set (HMult only; first + count <= l - 1),
+// the workgroups finish ModDown and the rescale by the last chain prime
+// themselves, from the accumulators of limb l - 1 and of the special prime
+// that an earlier launch left in acc, and write [batch][2][l-1][N] to out.
+struct KsTargets {
+    int first = 0, count = 0;
+    uint64_t* out = nullptr;
+    const ModDownConst* mc = nullptr;
+};
+
 // key_table (optional): ciphertext b uses the key at device address
 // key_table[b] instead of key.
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01, const ulonglong2* key, int key_limbs, uint64_t* acc,
                             int batch, int l, const LimbMap& map, int pidx_special, int threads,
-                            hipStream_t s, const uint64_t* key_table = nullptr);
+                            hipStream_t s, const uint64_t* key_table = nullptr, const KsTargets& tg = KsTargets());
 hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
                                   uint64_t* out, int batch, int l, const LimbMap& map, int pidx_special,
                                   const ModDownConst& mc, bool rescale, hipStream_t s);

@@ -372,7 +372,8 @@ enum {
     CKKS_KIND_MODDOWN = 4,
     CKKS_KIND_AUTOMORPHISM = 5,
     CKKS_KIND_POLYMUL = 6,
-    CKKS_KIND_COUNT = 7
+    CKKS_KIND_KEYSWITCH_FUSED = 7,   /* HMult's key switch of limbs 0..l-2 with ModDown and rescale fused */
+    CKKS_KIND_COUNT = 8
 };
 int ckks_timing_start(ckks_ctx *ctx, int max_launches);
 int ckks_timing_stop(ckks_ctx *ctx, double *ms, int *launches, int n_kinds);
