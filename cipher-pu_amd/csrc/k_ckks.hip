@@ -95,6 +95,8 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
         ntt::fwd_ntt2<C>(y0, y1, lds, tw, pc.q, pc.two_q, tid);
         uint64_t* o0 = d01 + c0;
         uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
+        // the inverse transform's first twiddles, loaded before the stores
+        const auto pre = ntt::inv_first<C, C::NP - 1>(tw + C::N, tid);
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
             const uint64_t a0 = reduce64(x0[k], pc), a1 = reduce64(x1[k], pc);
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
             o2[idx] = d2;
             x0[k] = d2;
         }
-        ntt::inv_ntt_db<C>(x0, lds, tw + C::N, pc, tid);
+        ntt::inv_ntt_db<C>(x0, lds, tw + C::N, pc, tid, pre);
 #pragma unroll
         for (int k = 0; k < C::E; ++k) x0[k] = csub(x0[k], pc.q);
         ntt::store_coalesced<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, tid);

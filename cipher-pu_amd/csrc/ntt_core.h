@@ -89,9 +89,28 @@ __device__ __forceinline__ int group_of(int g, int tid) {
     else return fh;
 }
 
+// The twiddles of forward pass P's first stage (bit HI), one per register
+// group: every butterfly of group g in that stage uses entry
+// 2^(LOGN-1-HI) + fh(g).  Loaded before the exchange into the pass, their
+// latency overlaps the exchange instead of stalling the pass's first stage.
 template <class C, int P>
-__device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ tw,
-                                         uint64_t q, uint64_t two_q, int tid) {
+struct FwdFirst {
+    ulonglong2 w[C::E >> C::rsize(P)];
+};
+
+template <class C, int P>
+__device__ __forceinline__ FwdFirst<C, P> fwd_first(const ulonglong2* __restrict__ tw, int tid) {
+    constexpr int G = C::E >> C::rsize(P);
+    constexpr int base = 1 << (C::LOGN - 1 - C::hi(P));
+    FwdFirst<C, P> f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) f.w[g] = tw[base + group_of<C, P>(g, tid)];
+    return f;
+}
+
+template <class C, int P, bool PRE>
+__device__ __forceinline__ void fwd_pass_t(uint64_t (&x)[C::E], const ulonglong2* __restrict__ tw,
+                                           uint64_t q, uint64_t two_q, int tid, const FwdFirst<C, P>& pre) {
     const uint64_t nq = (uint64_t)0 - q, four_q = two_q << 1;
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
@@ -106,7 +125,7 @@ __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* 
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
                 const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                const ulonglong2 w = tw[base + i];
+                const ulonglong2 w = (PRE && b == HI) ? pre.w[g] : tw[base + i];
                 ct_bfly_lazy(x[g * M + e], x[g * M + e + h], w.x, w.y, nq, four_q);
             }
         }
@@ -114,8 +133,39 @@ __device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* 
 }
 
 template <class C, int P>
+__device__ __forceinline__ void fwd_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ tw,
+                                         uint64_t q, uint64_t two_q, int tid) {
+    fwd_pass_t<C, P, false>(x, tw, q, two_q, tid, FwdFirst<C, P>{});
+}
+
+// The twiddles of inverse pass P's first stage (bit LO): M / 2 per register
+// group, entry 2^(LOGN-1-LO) + (fh(g) 2^(HI-LO) | e / 2) for even e.  As for
+// the forward passes, loaded before the exchange into the pass.
+template <class C, int P>
+struct InvFirst {
+    ulonglong2 w[C::E / 2];
+};
+
+template <class C, int P>
+__device__ __forceinline__ InvFirst<C, P> inv_first(const ulonglong2* __restrict__ itw, int tid) {
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+    constexpr int base = 1 << (C::LOGN - 1 - LO);
+    InvFirst<C, P> f;
+    if constexpr (LO != C::LOGN - 1) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int fh = group_of<C, P>(g, tid);
+#pragma unroll
+            for (int e = 0; e < M; e += 2) f.w[g * (M / 2) + e / 2] = itw[base + ((fh << (HI - LO)) | (e >> 1))];
+        }
+    }
+    return f;
+}
+
+template <class C, int P, bool PRE = false>
 __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* __restrict__ itw,
-                                         const PrimeConst& pc, int tid) {
+                                         const PrimeConst& pc, int tid, const InvFirst<C, P>& pre = InvFirst<C, P>{}) {
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
     const uint64_t q = pc.q, nq = (uint64_t)0 - q;
@@ -139,7 +189,7 @@ __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* 
                     Y = mul_shoup_nq(u + bound - v, pc.ninv_w1, pc.ninv_w1_s, nq);
                 } else {
                     const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                    const ulonglong2 w = itw[base + i];
+                    const ulonglong2 w = (PRE && b == LO) ? pre.w[g * (M / 2) + e / 2] : itw[base + i];
                     gs_bfly_lazy(X, Y, w.x, w.y, nq, bound);
                 }
             }
@@ -226,12 +276,13 @@ __device__ __forceinline__ void fwd_rest(uint64_t (&x)[C::E], uint64_t* lds,
                                          uint64_t two_q, int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = wave_local<C, P>();
+        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
         if constexpr (P == 1) __syncthreads();
         to_lds<C, P - 1>(x, lds, tid);
         exchange_sync<W>();
         from_lds<C, P>(x, lds, tid);
         exchange_sync<W>();
-        fwd_pass<C, P>(x, tw, q, two_q, tid);
+        fwd_pass_t<C, P, true>(x, tw, q, two_q, tid, pre);
         fwd_rest<C, P + 1>(x, lds, tw, q, two_q, tid);
     }
 }
@@ -263,10 +314,11 @@ __device__ __forceinline__ void fwd_rest_db(uint64_t (&x)[C::E], uint64_t* lds,
         if constexpr (P == 1) __syncthreads();
         else if constexpr (P >= 3 && wave_local<C, P - 1>()) __syncthreads();
         uint64_t* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
+        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
         to_lds<C, P - 1>(x, buf, tid);
         exchange_sync<W>();
         from_lds<C, P>(x, buf, tid);
-        fwd_pass<C, P>(x, tw, q, two_q, tid);
+        fwd_pass_t<C, P, true>(x, tw, q, two_q, tid, pre);
         fwd_rest_db<C, P + 1>(x, lds, tw, q, two_q, tid);
     }
 }
@@ -291,11 +343,12 @@ __device__ __forceinline__ void inv_rest(uint64_t (&x)[C::E], uint64_t* lds,
                                          int tid) {
     if constexpr (P >= 0) {
         constexpr bool W = wave_local<C, P + 1>();
+        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
         to_lds<C, P + 1>(x, lds, tid);
         exchange_sync<W>();
         from_lds<C, P>(x, lds, tid);
         exchange_sync<W>();
-        inv_pass<C, P>(x, itw, pc, tid);
+        inv_pass<C, P, true>(x, itw, pc, tid, pre);
         inv_rest<C, P - 1>(x, lds, itw, pc, tid);
     }
 }
@@ -313,10 +366,11 @@ __device__ __forceinline__ void inv_rest_db(uint64_t (&x)[C::E], uint64_t* lds,
         if constexpr (P == C::NP - 2) __syncthreads();
         else if constexpr (P <= C::NP - 4 && wave_local<C, P + 2>()) __syncthreads();
         uint64_t* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
+        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
         to_lds<C, P + 1>(x, buf, tid);
         exchange_sync<W>();
         from_lds<C, P>(x, buf, tid);
-        inv_pass<C, P>(x, itw, pc, tid);
+        inv_pass<C, P, true>(x, itw, pc, tid, pre);
         inv_rest_db<C, P - 1>(x, lds, itw, pc, tid);
     }
 }
@@ -332,6 +386,20 @@ __device__ __forceinline__ void inv_ntt_db(uint64_t (&x)[C::E], uint64_t* lds,
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     opaque(itw, tid);
     inv_pass<C, C::NP - 1>(x, itw, pc, tid);
+    inv_rest_db<C, C::NP - 2>(x, lds, itw, pc, tid);
+}
+
+// inv_ntt_db with the first pass's first-stage twiddles already loaded
+// (inv_first<C, NP-1>): a caller that stores results just before the
+// transform loads them first, since a wait for a load also waits for every
+// memory operation issued before it.
+template <class C>
+__device__ __forceinline__ void inv_ntt_db(uint64_t (&x)[C::E], uint64_t* lds,
+                                           const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid,
+                                           const InvFirst<C, C::NP - 1>& pre) {
+    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
+    opaque(itw, tid);
+    inv_pass<C, C::NP - 1, true>(x, itw, pc, tid, pre);
     inv_rest_db<C, C::NP - 2>(x, lds, itw, pc, tid);
 }
 
@@ -363,10 +431,10 @@ constexpr bool pair_fits() { return 2 * C::LDS_BYTES <= 160 * 1024; }
 template <class C>
 constexpr bool pair_pays() { return pair_fits<C>() && C::E <= 8; }
 
-template <class C, int P>
+template <class C, int P, bool PRE = false>
 __device__ __forceinline__ void fwd_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::E],
                                           const ulonglong2* __restrict__ tw, uint64_t q, uint64_t two_q,
-                                          int tid) {
+                                          int tid, const FwdFirst<C, P>& pre = FwdFirst<C, P>{}) {
     const uint64_t nq = (uint64_t)0 - q, four_q = two_q << 1;
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
@@ -381,7 +449,7 @@ __device__ __forceinline__ void fwd_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
                 const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                const ulonglong2 w = tw[base + i];
+                const ulonglong2 w = (PRE && b == HI) ? pre.w[g] : tw[base + i];
                 ct_bfly_lazy(x[g * M + e], x[g * M + e + h], w.x, w.y, nq, four_q);
                 ct_bfly_lazy(y[g * M + e], y[g * M + e + h], w.x, w.y, nq, four_q);
             }
@@ -389,9 +457,10 @@ __device__ __forceinline__ void fwd_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::
     }
 }
 
-template <class C, int P>
+template <class C, int P, bool PRE = false>
 __device__ __forceinline__ void inv_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::E],
-                                          const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid) {
+                                          const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid,
+                                          const InvFirst<C, P>& pre = InvFirst<C, P>{}) {
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
     const uint64_t q = pc.q, nq = (uint64_t)0 - q;
@@ -418,7 +487,7 @@ __device__ __forceinline__ void inv_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::
                     V = mul_shoup_nq(u0 + bound - u1, pc.ninv_w1, pc.ninv_w1_s, nq);
                 } else {
                     const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                    const ulonglong2 w = itw[base + i];
+                    const ulonglong2 w = (PRE && b == LO) ? pre.w[g * (M / 2) + e / 2] : itw[base + i];
                     gs_bfly_lazy(X, Y, w.x, w.y, nq, bound);
                     gs_bfly_lazy(U, V, w.x, w.y, nq, bound);
                 }
@@ -443,9 +512,10 @@ __device__ __forceinline__ void fwd_rest2(uint64_t (&x)[C::E], uint64_t (&y)[C::
                                           const ulonglong2* __restrict__ tw, uint64_t q, uint64_t two_q,
                                           int tid) {
     if constexpr (P < C::NP) {
+        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
         if constexpr (P == 1) __syncthreads();
         exchange2<C, P - 1, P, wave_local<C, P>()>(x, y, lds, tid);
-        fwd_pass2<C, P>(x, y, tw, q, two_q, tid);
+        fwd_pass2<C, P, true>(x, y, tw, q, two_q, tid, pre);
         fwd_rest2<C, P + 1>(x, y, lds, tw, q, two_q, tid);
     }
 }
@@ -465,8 +535,9 @@ template <class C, int P>
 __device__ __forceinline__ void inv_rest2(uint64_t (&x)[C::E], uint64_t (&y)[C::E], uint64_t* lds,
                                           const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid) {
     if constexpr (P >= 0) {
+        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
         exchange2<C, P + 1, P, wave_local<C, P + 1>()>(x, y, lds, tid);
-        inv_pass2<C, P>(x, y, itw, pc, tid);
+        inv_pass2<C, P, true>(x, y, itw, pc, tid, pre);
         inv_rest2<C, P - 1>(x, y, lds, itw, pc, tid);
     }
 }
