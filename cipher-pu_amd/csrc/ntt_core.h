@@ -52,8 +52,27 @@ struct Cfg {
     static constexpr int LDS_BYTES = LDS_WORDS * 8;
 };
 
+template <class C, int K = C::PS>
+__device__ __forceinline__ int pad(int j) { return j + (j >> K); }
+
+// Padding shift of the exchange between the layouts of passes P - 1 and P
+// (either direction).  Every exchange may pad differently: its two sides
+// are its own.  Where a pass's lanes cover a long contiguous index run, the
+// default pad(j) = j + (j >> PS) spreads 32 lanes of 8-byte words over more
+// than the LDS's banks and doubles the cycles of every access; a coarser
+// shift keeps the run compact.  The shifts below come from a bank model of
+// every layout pair of the hot shapes (32 lanes of 8-byte words per cycle
+// over 64 four-byte banks, and 16 over 32); others keep PS.  The shift must
+// leave each layout's register field wholly above or below it, so that the
+// slots stay affine in the registers (Lay).
 template <class C>
-__device__ __forceinline__ int pad(int j) { return j + (j >> C::PS); }
+constexpr int xshift(int P) {
+    if (C::LOGN == 13 && C::T == 1024) return P == 1 ? 5 : P == 2 ? 7 : C::PS;
+    if ((C::LOGN == 13 || C::LOGN == 12) && C::T == 512) return P == 1 ? 5 : C::PS;
+    if (C::LOGN == 14 && C::T == 1024) return P == 1 ? 5 : C::PS;
+    if (C::LOGN == 14 && C::T == 512) return P == 1 ? 9 : C::PS;
+    return C::PS;
+}
 
 // Hide the twiddle table (a workgroup-uniform pointer) and the thread index
 // from the optimiser at the start of every transform.  Without this, a kernel
@@ -204,32 +223,36 @@ __device__ __forceinline__ void inv_pass(uint64_t (&x)[C::E], const ulonglong2* 
 // the padding does not see e at all.  (T is a multiple of 2^LO in every
 // built configuration.)  Affine slots let the LDS instructions carry the
 // per-register part as an immediate offset.
-template <class C, int P>
+template <class C, int P, int K = C::PS>
 struct Lay {
     static constexpr int LO = C::lo(P), HI = C::hi(P);
     static_assert(C::T % (1 << LO) == 0, "threads must cover the pass's low bits");
-    static constexpr int SE = (1 << LO) + (LO >= C::PS ? (1 << (LO - C::PS)) : 0);
-    static constexpr int SG = ((C::T >> LO) << (HI + 1)) + (((C::T >> LO) << (HI + 1)) >> C::PS);
+    static_assert(K >= C::PS, "padding beyond the allocated LDS words");
+    static_assert(LO >= K || HI < K, "the register field straddles the padding shift");
+    static constexpr int RAW_G = (C::T >> LO) << (HI + 1);
+    static_assert(C::E == (1 << C::rsize(P)) || RAW_G % (1 << K) == 0, "group stride not a multiple of 2^K");
+    static constexpr int SE = (1 << LO) + (LO >= K ? (1 << (LO - K)) : 0);
+    static constexpr int SG = RAW_G + (RAW_G >> K);
 };
 
-template <class C, int P>
+template <class C, int P, int K = C::PS>
 __device__ __forceinline__ void to_lds(const uint64_t (&x)[C::E], uint64_t* lds, int tid) {
     constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
-    uint64_t* base = lds + pad<C>(index_of<C, P>(0, 0, tid));
+    uint64_t* base = lds + pad<C, K>(index_of<C, P>(0, 0, tid));
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int e = 0; e < M; ++e) base[g * Lay<C, P>::SG + e * Lay<C, P>::SE] = x[g * M + e];
+        for (int e = 0; e < M; ++e) base[g * Lay<C, P, K>::SG + e * Lay<C, P, K>::SE] = x[g * M + e];
 }
 
-template <class C, int P>
+template <class C, int P, int K = C::PS>
 __device__ __forceinline__ void from_lds(uint64_t (&x)[C::E], const uint64_t* lds, int tid) {
     constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
-    const uint64_t* base = lds + pad<C>(index_of<C, P>(0, 0, tid));
+    const uint64_t* base = lds + pad<C, K>(index_of<C, P>(0, 0, tid));
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int e = 0; e < M; ++e) x[g * M + e] = base[g * Lay<C, P>::SG + e * Lay<C, P>::SE];
+        for (int e = 0; e < M; ++e) x[g * M + e] = base[g * Lay<C, P, K>::SG + e * Lay<C, P, K>::SE];
 }
 
 // True when the exchange between the layouts of passes P - 1 and P stays
@@ -266,6 +289,18 @@ __device__ __forceinline__ void exchange_sync() {
     }
 }
 
+// An exchange that writes into the buffer the previous exchange used needs a
+// workgroup barrier first when that previous exchange was wave-local (it
+// ends with a wave-level fence only) and the two pad differently: each wave
+// then writes slots that other waves may not have read yet.  With the same
+// padding a wave rewrites exactly the slots it read itself.  Exchange XP
+// (layouts XP - 1 and XP) was the previous one, exchange XN is next.
+template <class C, int XP, int XN>
+constexpr bool relaid_after_wave_local() {
+    if constexpr (XP < 1 || XP >= C::NP) return false;
+    else return wave_local<C, XP>() && xshift<C>(XP) != xshift<C>(XN);
+}
+
 // Forward passes 1..NP-1 (pass 0 already applied), exchanging through LDS.
 // The first exchange starts with a workgroup barrier: a preceding transform
 // in the same kernel may end with a wave-local exchange, and pass 0's slots
@@ -278,9 +313,10 @@ __device__ __forceinline__ void fwd_rest(uint64_t (&x)[C::E], uint64_t* lds,
         constexpr bool W = wave_local<C, P>();
         const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
         if constexpr (P == 1) __syncthreads();
-        to_lds<C, P - 1>(x, lds, tid);
+        else if constexpr (relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
+        to_lds<C, P - 1, xshift<C>(P)>(x, lds, tid);
         exchange_sync<W>();
-        from_lds<C, P>(x, lds, tid);
+        from_lds<C, P, xshift<C>(P)>(x, lds, tid);
         exchange_sync<W>();
         fwd_pass_t<C, P, true>(x, tw, q, two_q, tid, pre);
         fwd_rest<C, P + 1>(x, lds, tw, q, two_q, tid);
@@ -315,9 +351,9 @@ __device__ __forceinline__ void fwd_rest_db(uint64_t (&x)[C::E], uint64_t* lds,
         else if constexpr (P >= 3 && wave_local<C, P - 1>()) __syncthreads();
         uint64_t* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
         const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
-        to_lds<C, P - 1>(x, buf, tid);
+        to_lds<C, P - 1, xshift<C>(P)>(x, buf, tid);
         exchange_sync<W>();
-        from_lds<C, P>(x, buf, tid);
+        from_lds<C, P, xshift<C>(P)>(x, buf, tid);
         fwd_pass_t<C, P, true>(x, tw, q, two_q, tid, pre);
         fwd_rest_db<C, P + 1>(x, lds, tw, q, two_q, tid);
     }
@@ -344,9 +380,10 @@ __device__ __forceinline__ void inv_rest(uint64_t (&x)[C::E], uint64_t* lds,
     if constexpr (P >= 0) {
         constexpr bool W = wave_local<C, P + 1>();
         const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
-        to_lds<C, P + 1>(x, lds, tid);
+        if constexpr (relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
+        to_lds<C, P + 1, xshift<C>(P + 1)>(x, lds, tid);
         exchange_sync<W>();
-        from_lds<C, P>(x, lds, tid);
+        from_lds<C, P, xshift<C>(P + 1)>(x, lds, tid);
         exchange_sync<W>();
         inv_pass<C, P, true>(x, itw, pc, tid, pre);
         inv_rest<C, P - 1>(x, lds, itw, pc, tid);
@@ -367,9 +404,9 @@ __device__ __forceinline__ void inv_rest_db(uint64_t (&x)[C::E], uint64_t* lds,
         else if constexpr (P <= C::NP - 4 && wave_local<C, P + 2>()) __syncthreads();
         uint64_t* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
         const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
-        to_lds<C, P + 1>(x, buf, tid);
+        to_lds<C, P + 1, xshift<C>(P + 1)>(x, buf, tid);
         exchange_sync<W>();
-        from_lds<C, P>(x, buf, tid);
+        from_lds<C, P, xshift<C>(P + 1)>(x, buf, tid);
         inv_pass<C, P, true>(x, itw, pc, tid, pre);
         inv_rest_db<C, P - 1>(x, lds, itw, pc, tid);
     }
@@ -499,11 +536,12 @@ __device__ __forceinline__ void inv_pass2(uint64_t (&x)[C::E], uint64_t (&y)[C::
 // One exchange of both register sets from pass A's layout to pass B's.
 template <class C, int A, int B, bool W>
 __device__ __forceinline__ void exchange2(uint64_t (&x)[C::E], uint64_t (&y)[C::E], uint64_t* lds, int tid) {
-    to_lds<C, A>(x, lds, tid);
-    to_lds<C, A>(y, lds + C::LDS_WORDS, tid);
+    constexpr int K = xshift<C>(A > B ? A : B);
+    to_lds<C, A, K>(x, lds, tid);
+    to_lds<C, A, K>(y, lds + C::LDS_WORDS, tid);
     exchange_sync<W>();
-    from_lds<C, B>(x, lds, tid);
-    from_lds<C, B>(y, lds + C::LDS_WORDS, tid);
+    from_lds<C, B, K>(x, lds, tid);
+    from_lds<C, B, K>(y, lds + C::LDS_WORDS, tid);
     exchange_sync<W>();
 }
 
@@ -514,6 +552,7 @@ __device__ __forceinline__ void fwd_rest2(uint64_t (&x)[C::E], uint64_t (&y)[C::
     if constexpr (P < C::NP) {
         const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
         if constexpr (P == 1) __syncthreads();
+        else if constexpr (relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
         exchange2<C, P - 1, P, wave_local<C, P>()>(x, y, lds, tid);
         fwd_pass2<C, P, true>(x, y, tw, q, two_q, tid, pre);
         fwd_rest2<C, P + 1>(x, y, lds, tw, q, two_q, tid);
@@ -536,6 +575,7 @@ __device__ __forceinline__ void inv_rest2(uint64_t (&x)[C::E], uint64_t (&y)[C::
                                           const ulonglong2* __restrict__ itw, const PrimeConst& pc, int tid) {
     if constexpr (P >= 0) {
         const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
+        if constexpr (relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
         exchange2<C, P + 1, P, wave_local<C, P + 1>()>(x, y, lds, tid);
         inv_pass2<C, P, true>(x, y, itw, pc, tid, pre);
         inv_rest2<C, P - 1>(x, y, lds, itw, pc, tid);

@@ -760,3 +760,31 @@ def test_coefficient_encoding_drop_ins(gpu):
         assert G.inner_product(ct, w, s, q, scale) == O.inner_product(ct, w, s, q, scale)
         assert G.inner_product(ct, w, s, q, scale, idx) == O.inner_product(ct, w, s, q, scale, idx)
         assert G.coef_sum(ct, 64, s, q, scale) == O.coef_sum(ct, 64, s, q, scale)
+
+
+@pytest.mark.parametrize("N,threads", [(4096, 256), (4096, 512), (8192, 256), (8192, 512), (8192, 1024),
+                                       (16384, 512), (16384, 1024)])
+def test_transforms_and_products_over_many_rows(gpu, N, threads):
+    """Every built transform shape over 512 rows at once (enough workgroups
+    that every CU runs several in turn): the forward-inverse round trip
+    returns every row, sampled forward rows and products equal the C oracle.
+    Catches LDS ordering faults between exchanges, which show only when
+    waves of one workgroup drift apart."""
+    from cipherpu.ring import RingContext, bitrev_, to_numpy
+    q = CHAIN4[0]
+    ring = RingContext(N, [q])
+    ring.set_threads(threads, 0, 0)
+    rng = np.random.default_rng(N + threads)
+    rows = 512
+    a = rng.integers(0, q, (rows, 1, N), dtype=np.uint64)
+    b = rng.integers(0, q, (rows, 1, N), dtype=np.uint64)
+    A = ring.tensor(a)
+    X = ring.ntt_(A.clone())
+    assert np.array_equal(to_numpy(ring.intt_(X.clone())), a)
+    psi = ring.roots[0]
+    Xn = to_numpy(bitrev_(X.view(rows, N).clone()))
+    for r in (0, rows // 2, rows - 1):
+        assert np.array_equal(Xn[r], OC.ntt_negacyclic(a[r, 0], N, q, psi)), r
+    P = to_numpy(ring.poly_mul(A, ring.tensor(b)))
+    for r in (0, 137, rows - 1):
+        assert np.array_equal(P[r, 0], OC.polymul_ntt(a[r, 0], b[r, 0], q, psi)), r
