@@ -32,6 +32,52 @@ def load_chain_json(path: str) -> List[dict]:
     return chain
 
 
+def chain_primes_and_roots(chain: List[dict], N: int):
+    """(primes, roots) of a loaded chain for the negacyclic ring of degree
+    N: each entry's omega is a primitive entry["N"]-th root, and the ring
+    needs one of order 2N, omega^(entry["N"] / 2N) (the chain file's
+    N = 16384 gives psi itself at N = 8192)."""
+    primes, roots = [], []
+    for e in chain:
+        q, w, n = int(e["q"]), int(e["omega"]), int(e["N"])
+        if n % (2 * N):
+            raise ValueError(f"chain root of order {n} has no power of order {2 * N}")
+        primes.append(q)
+        roots.append(pow(w, n // (2 * N), q))
+    return primes, roots
+
+
+def default_root(q: int, order: int) -> int:
+    """g^((q - 1) / order) for the smallest generator g of Z_q^*: the root
+    the library derives when given none (scripts/gen_modulus_chain.py:128-130
+    builds the chain's roots the same way)."""
+    n, fs, d = q - 1, [], 2
+    while d * d <= n:
+        if n % d == 0:
+            fs.append(d)
+            while n % d == 0:
+                n //= d
+        d += 1 if d == 2 else 2
+    if n > 1:
+        fs.append(n)
+    g = 2
+    while any(pow(g, (q - 1) // f, q) == 1 for f in fs):
+        g += 1
+    return pow(g, (q - 1) // order, q)
+
+
+def root_from_rom(values: List[int], q: int) -> int:
+    """The root a twiddle ROM tabulates (values[k] = w^k,
+    scripts/gen_omega_tables.py:96-106), checked entry by entry."""
+    w = int(values[1]) % q
+    acc = 1
+    for k, v in enumerate(values):
+        if int(v) != acc:
+            raise ValueError(f"ROM entry {k} is not the {k}-th power of {w} mod {q}")
+        acc = acc * w % q
+    return w
+
+
 def save_chain_json(path: str, entries: List[dict]) -> None:
     with open(path, "w", encoding="utf-8") as f:
         json.dump(entries, f, indent=2)

@@ -272,8 +272,8 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
             if (powmod(r, order, q) != 1 || powmod(r, order / 2, q) != q - 1) return CKKS_ERR_BAD_MODULUS;
         }
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return CKKS_ERR_NO_DEVICE;
+    // host tables first, then the device: the table construction is plain
+    // host code (checked under AddressSanitizer on machines without a GPU)
     ckks_ctx* c = new ckks_ctx();
     c->device = device;
     c->N = N;
@@ -323,15 +323,13 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
         pc.ninv_w1 = mulmod(pc.ninv, iv[1].x, q);
         pc.ninv_w1_s = shoup(pc.ninv_w1, q);
     }
-    bool ok = hipSetDevice(device) == hipSuccess;
-    ok = ok && hipMalloc(&c->dpc, sizeof(PrimeConst) * n_primes) == hipSuccess;
-    ok = ok && hipMalloc(&c->dtw, tw.size() * sizeof(ulonglong2)) == hipSuccess;
-    ok = ok && hipMemcpy(c->dpc, c->hpc, sizeof(PrimeConst) * n_primes, hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpy(c->dtw, tw.data(), tw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice) == hipSuccess;
-    if (ok && !cyclic && N >= 4) {
-        const uint64_t n = N / 2;
-        std::vector<double2> ft(n / 2 > 0 ? n / 2 : 1), zt(n);
-        std::vector<int> sk(n);
+    // canonical-embedding tables (negacyclic rings): the length-N/2 FFT's
+    // twiddles, the twist zeta^j and the 5^j slot order
+    const bool emb = !cyclic && N >= 4;
+    const uint64_t n = N / 2;
+    std::vector<double2> ft(emb && n / 2 > 0 ? n / 2 : 1), zt(emb ? n : 1);
+    std::vector<int> sk(emb ? n : 1);
+    if (emb) {
         const long double PI = 3.141592653589793238462643383279502884L;
         for (uint64_t k = 0; k < n / 2; ++k) {
             const long double a = 2.0L * PI * (long double)k / (long double)n;
@@ -346,6 +344,18 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
             sk[j] = (int)((e - 1) / 4);
             e = e * 5 % (2 * N);
         }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        delete c;
+        return CKKS_ERR_NO_DEVICE;
+    }
+    bool ok = hipSetDevice(device) == hipSuccess;
+    ok = ok && hipMalloc(&c->dpc, sizeof(PrimeConst) * n_primes) == hipSuccess;
+    ok = ok && hipMalloc(&c->dtw, tw.size() * sizeof(ulonglong2)) == hipSuccess;
+    ok = ok && hipMemcpy(c->dpc, c->hpc, sizeof(PrimeConst) * n_primes, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->dtw, tw.data(), tw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && emb) {
         ok = ok && hipMalloc(&c->fft_tw, ft.size() * sizeof(double2)) == hipSuccess;
         ok = ok && hipMalloc(&c->fft_zeta, zt.size() * sizeof(double2)) == hipSuccess;
         ok = ok && hipMalloc(&c->slot_k, sk.size() * sizeof(int)) == hipSuccess;

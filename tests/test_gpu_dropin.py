@@ -204,3 +204,22 @@ def test_one_ckks_object_on_two_streams(gpu):
     torch.cuda.synchronize()
     for h, h2, r in outs:
         assert torch.equal(h.data, want_h) and torch.equal(h2.data, want_h2) and torch.equal(r.data, want_r)
+
+
+def test_ckks_from_the_reference_chain_file(gpu, fixtures, tmp_path):
+    """CKKSParams.from_chain_json on the reference's chain schema gives a
+    context whose HMult words equal those of the built-in parameters."""
+    import json
+    from cipherpu.ckks import CKKS, CKKSParams
+    path = tmp_path / "modulus_chain_chain4.json"
+    path.write_text(json.dumps(fixtures["chain4"]))
+    a, b = CKKS(CKKSParams(), seed=3), CKKS(CKKSParams.from_chain_json(str(path)), seed=3)
+    assert a.ring.roots == b.ring.roots
+    outs = []
+    for ck in (a, b):
+        sk = ck.keygen()
+        rlk = ck.relin_key(sk)
+        z = np.random.default_rng(4).uniform(-1, 1, (2, 4096))
+        ct = ck.encrypt(ck.encode(z), sk)
+        outs.append(ck.hmult(ct, ct, rlk).data.cpu())
+    assert (outs[0] == outs[1]).all()
