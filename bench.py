@@ -305,11 +305,11 @@ def rotation_probe(ck, sk, batch: int = 64, steps=tuple(range(1, 16)), reps: int
     return res
 
 
-def mlp_leg(samples: int, target_db: float, chunk_cts: int, kind: str = "mlp"):
-    """Encrypted MLP (or CNN) samples/s at a fixed output SNR: the plain
-    decrypt-based monitor gear loop against the learned gear controller,
-    plus static gear 0 without any monitor."""
-    import numpy as np
+def mlp_leg(samples: int, targets, chunk_cts: int, kind: str = "mlp"):
+    """Encrypted MLP (or CNN) samples/s at a fixed output SNR, for each
+    target: the plain decrypt-based monitor gear loop against the learned
+    gear controller; static gear 0 without any monitor beside them (the
+    headline comparison for the learned loop)."""
     import torch
     from cipherpu import gear, nn
     from cipherpu.ckks import CKKS, CKKSParams
@@ -325,28 +325,37 @@ def mlp_leg(samples: int, target_db: float, chunk_cts: int, kind: str = "mlp"):
     net = gear.GearNet(seed=0)
     trained = gear.load_trained(net, gear.TRAINED_PATHS[kind])
     if not trained:
-        F, T = gear.sweep(mlp, 512, seed=1)
-        net.fit(F, T, epochs=60, batch=256)
+        raise RuntimeError(f"no committed controller weights for the {kind} leg")
     X = nn.synthetic_images(samples, seed=4096)
     ref = mlp.reference_outputs(X)
     out = {"metric": f"enc-{kind.upper()} samples/s at fixed output SNR", "network": desc,
-           "samples": samples, "target_snr_db": target_db, "chunk_ciphertexts": chunk_cts,
-           "samples_per_ciphertext": mlp.S,
-           "controller_weights": trained or "trained in this run on a 512-ciphertext sweep"}
-    loops = (("plain_monitor", lambda X: gear.plain_monitor_loop(mlp, X, target_db, chunk_cts)),
-             ("learned", lambda X: gear.learned_loop(mlp, net, X, target_db, chunk_cts)),
-             ("static_gear0", lambda X: gear.static_loop(mlp, X, 0, chunk_cts)))
+           "samples": samples, "target_snr_db": list(targets), "chunk_ciphertexts": chunk_cts,
+           "samples_per_ciphertext": mlp.S, "controller_weights": trained,
+           "predicted_snr_db_per_gear": None}
+
+    def summary(r):
+        return {"samples_per_s": round(r.samples_per_s, 1), "snr_db": round(gear.snr_db(ref, r.outputs), 2),
+                "evaluations": r.evaluations, "monitor_calls": r.monitor_calls, "fallbacks": r.fallbacks,
+                "gears": {str(g): r.gears.count(g) for g in sorted(set(r.gears))}}
     # one untimed chunk through every loop first, so that first-launch costs
     # (code-object loading, allocator growth) fall on none of the timed loops
-    for _, fn in loops:
-        fn(X[:chunk_cts * mlp.S])
+    Xw = X[:chunk_cts * mlp.S]
+    gear.static_loop(mlp, Xw, 0, chunk_cts)
+    gear.plain_monitor_loop(mlp, Xw, targets[0], chunk_cts)
+    gear.learned_loop(mlp, net, Xw, targets[0], chunk_cts)
     torch.cuda.synchronize()
-    for name, fn in loops:
-        r = fn(X)
-        out[name] = {"samples_per_s": round(r.samples_per_s, 1), "snr_db": round(gear.snr_db(ref, r.outputs), 2),
-                     "evaluations": r.evaluations, "monitor_calls": r.monitor_calls, "fallbacks": r.fallbacks,
-                     "gears": {str(g): r.gears.count(g) for g in sorted(set(r.gears))}}
-    out["learned_over_plain"] = round(out["learned"]["samples_per_s"] / out["plain_monitor"]["samples_per_s"], 3)
+    chunks = gear._setup(mlp, X[:chunk_cts * mlp.S], chunk_cts)
+    out["predicted_snr_db_per_gear"] = [round(float(v), 2) for v in gear._gear_table(mlp, net, chunks)[0, 0]]
+    out["static_gear0"] = summary(gear.static_loop(mlp, X, 0, chunk_cts))
+    by = {}
+    for t in targets:
+        plain = summary(gear.plain_monitor_loop(mlp, X, t, chunk_cts))
+        learned = summary(gear.learned_loop(mlp, net, X, t, chunk_cts))
+        by[str(t)] = {"plain_monitor": plain, "learned": learned,
+                      "learned_over_plain": round(learned["samples_per_s"] / plain["samples_per_s"], 3),
+                      "learned_over_static_gear0": round(learned["samples_per_s"] /
+                                                         out["static_gear0"]["samples_per_s"], 3)}
+    out["by_target"] = by
     return out
 
 
@@ -406,7 +415,9 @@ def main():
     ap.add_argument("--mlp-samples", type=int, default=4096)
     ap.add_argument("--chunk-cts", type=int, default=64,
                     help="ciphertexts per gear decision in the encrypted-network legs")
-    ap.add_argument("--target-snr", type=float, default=3.0)
+    ap.add_argument("--target-snr", type=float, nargs="+", default=[3.0, 10.0, 15.0, 20.0],
+                    help="output SNR targets (dB) of the encrypted-network legs: 3 dB and the reference's "
+                         "gear thresholds 10 / 15 / 20 dB")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--modmul-probe", action="store_true", help="Barrett vs special-form point-wise product")
     ap.add_argument("--no-graph", action="store_true", help="skip the uneven-batch graph leg")

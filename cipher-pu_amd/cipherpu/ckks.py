@@ -35,10 +35,24 @@ class CKKSParams:
     special: int = SPECIAL_P
     scale: float = 2.0 ** 40
     sigma: float = 3.2
+    # roots of order 2N for primes + [special]; None derives them from the
+    # smallest generator (which is what the reference's chain file holds)
+    roots: Optional[Sequence[int]] = None
 
     @property
     def L(self) -> int:
         return len(self.primes)
+
+    @classmethod
+    def from_chain_json(cls, path: str, N: int = 8192, special: int = SPECIAL_P, **kw) -> "CKKSParams":
+        """Parameters from a modulus chain in the reference's JSON schema
+        (rtl/m2/modulus_chain_chain4.json): its primes, with the roots of
+        order 2N taken from its omega column (params.chain_primes_and_roots);
+        the special prime's root is the one the library derives."""
+        from .params import chain_primes_and_roots, default_root, load_chain_json
+        primes, roots = chain_primes_and_roots(load_chain_json(path), N)
+        return cls(N=N, primes=tuple(primes), special=special,
+                   roots=tuple(roots) + (default_root(special, 2 * N),), **kw)
 
 
 @dataclass
@@ -82,7 +96,7 @@ class CKKS:
         self.L = params.L
         self.primes = [int(q) for q in params.primes] + [int(params.special)]
         self.sp = self.L                      # ring index of the special prime
-        self.ring = RingContext(self.N, self.primes, device=device)
+        self.ring = RingContext(self.N, self.primes, roots=params.roots, device=device)
         self.device = self.ring.device
         self.rng = np.random.default_rng(seed)
         self.sampler = sampler

@@ -58,6 +58,15 @@ def make_cnn(seed: int = 0) -> Tuple[np.ndarray, np.ndarray]:
     return K, F
 
 
+def make_cnn_family(seed: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Conv [8][5][5] and FC [10][288] weights of family
+    WEIGHT_FAMILIES[seed % 5] (nn.family_weights), seeded."""
+    from .nn import WEIGHT_FAMILIES, family_weights
+    rng = np.random.default_rng(20_000 + seed)
+    fam = WEIGHT_FAMILIES[seed % len(WEIGHT_FAMILIES)]
+    return (family_weights((N_CH, KS, KS), KS * KS, fam, rng), family_weights((N_OUT, 288), 288, fam, rng))
+
+
 def cnn_forward(K: np.ndarray, F: np.ndarray, X: np.ndarray) -> np.ndarray:
     """The FP64 network: conv 5x5 stride 2 (no padding) -> x^2 -> 2x2 average
     pool -> FC, FC inputs flattened as [channel][row][col]."""
@@ -106,6 +115,16 @@ class EncryptedCNN(EncryptedNet):
 
     def reference_outputs(self, X: np.ndarray) -> np.ndarray:
         return cnn_forward(self.K, self.F, X)
+
+    def weight_matrices(self):
+        """The conv weights as [channel][25] and the FC weights, as the gears
+        prune them."""
+        return self.K.reshape(N_CH, -1), self.F
+
+    def plain_outputs_at_gear(self, X: np.ndarray, gear: int) -> np.ndarray:
+        """The FP64 network with both layers pruned as `gear` prunes them."""
+        K = self.K * block_keep_mask(self.K.reshape(N_CH, -1), GEAR_FRACTIONS[gear]).reshape(self.K.shape)
+        return cnn_forward(K, self.F * block_keep_mask(self.F, GEAR_FRACTIONS[gear]), X)
 
     # -- gears ------------------------------------------------------------
     def set_gears(self, gear1: int, gear2: int):

@@ -13,8 +13,12 @@ Plain monitor (the reference's policy and the arbiter of predictions):
 
 Learned controller: a 16 -> 64 -> 64 -> 4 network (FP32 weights, BF16
 activations, forward/backward/Adam in HIP, include/ckks_core.h
-ckks_gear_*) maps ciphertext statistics, level and gear history to the
-predicted output SNR of each gear.  It picks the cheapest gear whose
+ckks_gear_*) maps the network's per-gear pruning statistics, ciphertext
+statistics, level and gear history to the predicted output SNR of each
+gear.  It is trained across networks of several weight families, so the
+pruning statistics are what tell one network's gears from another's (the
+ciphertext coefficients are uniform modulo q and carry nothing about the
+message).  It picks the cheapest gear whose
 prediction clears the target by the margin (3 dB); when no gear does, the
 plain monitor decides.  The predictor never touches the modular
 arithmetic, and the plain monitor measures every reported SNR.
@@ -49,30 +53,52 @@ def _stream(dev):
 
 # ---------------------------------------------------------------- features
 
-def ciphertext_features(ck, ct, prev_gear: int, layer: int = 0) -> torch.Tensor:
-    """[B, 16] float32 features on the device, from per-limb coefficient
-    moments of the ciphertext (ckks_noise_stats): rms(c')/q of c0 and c1 on
-    limbs 0..3 (8 values), mean(c')/q of c0 and c1 on limbs 0..1 (4), then
-    level / L, log2(scale) / 64, the previous gear / 3 and the layer index.
-    Limbs above the ciphertext's level read 0."""
-    return features_many(ck, [ct], prev_gear, layer)
+PROBE_IMAGES = 64     # synthetic images of the plaintext probe (fixed seed)
 
 
-def features_many(ck, cts, prev_gear: int, layer: int = 0) -> torch.Tensor:
+def network_features(net) -> np.ndarray:
+    """The network's pruning statistics, 9 values, computed once per network
+    in plaintext from its weights: the weight SNR in dB (nn.weight_snr_db,
+    capped at 60) of each of its two weight matrices under gears 1, 2 and 3
+    (6 values), then the output SNR of the pruned FP64 network against the
+    dense one at gears 1, 2 and 3 on a fixed probe batch of synthetic
+    images (3 values; capped at 60), all divided by 60.  No ciphertext and
+    no input of the current batch enters them."""
+    from .nn import synthetic_images, weight_snr_db
+    cached = getattr(net, "_gear_features", None)
+    if cached is None:
+        mats = net.weight_matrices()
+        w = [weight_snr_db(W, g) for W in mats for g in (1, 2, 3)]
+        Xp = synthetic_images(PROBE_IMAGES, seed=777)
+        ref = net.reference_outputs(Xp)
+        o = [min(60.0, snr_db(ref, net.plain_outputs_at_gear(Xp, g))) for g in (1, 2, 3)]
+        cached = np.array(w + o) / 60.0
+        net._gear_features = cached
+    return cached
+
+
+def ciphertext_features(ck, ct, prev_gear: int, layer: int = 0, netf=None) -> torch.Tensor:
+    """[B, 16] float32 features on the device: rms(c')/q of c0 and c1 on
+    limb 0 (ckks_noise_stats moments; 2 values), the network's pruning
+    statistics (network_features; 9), mean(c')/q of c0 on limb 0 (1), then
+    level / L, log2(scale) / 64, the previous gear / 3 and the layer index."""
+    return features_many(ck, [ct], prev_gear, layer, netf)
+
+
+def features_many(ck, cts, prev_gear: int, layer: int = 0, netf=None) -> torch.Tensor:
     """ciphertext_features of several ciphertexts of one level and scale,
     rows concatenated, computed in one pass."""
     l, scale = cts[0].level, cts[0].scale
     if any(ct.level != l or ct.scale != scale for ct in cts):
-        return torch.cat([features_many(ck, [ct], prev_gear, layer) for ct in cts])
+        return torch.cat([features_many(ck, [ct], prev_gear, layer, netf) for ct in cts])
     B = sum(ct.batch for ct in cts)
     m = ck.ring.noise_moments_many([ct.data for ct in cts]).view(B, 2, l, 4)
     f = torch.zeros(B, N_FEATURES, dtype=torch.float64, device=m.device)
-    k = min(l, 4)
-    f[:, 0:k] = m[:, 0, :k, 1]
-    f[:, 4:4 + k] = m[:, 1, :k, 1]
-    k2 = min(l, 2)
-    f[:, 8:8 + k2] = m[:, 0, :k2, 0]
-    f[:, 10:10 + k2] = m[:, 1, :k2, 0]
+    f[:, 0] = m[:, 0, 0, 1]
+    f[:, 1] = m[:, 1, 0, 1]
+    if netf is not None:
+        f[:, 2:11] = torch.as_tensor(np.asarray(netf, dtype=np.float64), device=m.device)
+    f[:, 11] = m[:, 0, 0, 0]
     f[:, 12] = l / ck.L
     f[:, 13] = math.log2(scale) / 64.0
     f[:, 14] = prev_gear / 3.0
@@ -221,9 +247,9 @@ class GearNet:
             self.load_state({k: z[k] for k in z.files})
 
 
-# Controller weights trained by cipherpu.gear_train on monitor sweeps of the
-# 784-128-10 network of nn.make_mlp(0) and the CNN of cnn.make_cnn(0)
-# (synthetic images, every gear).
+# Controller weights trained by cipherpu.gear_train on monitor sweeps of 40
+# networks of each architecture (784-128-10 MLPs of nn.make_mlp_family and
+# CNNs of cnn.make_cnn_family, synthetic images, every gear).
 _DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 TRAINED_PATH = os.path.join(_DATA, "gearnet_mlp784.npz")
 TRAINED_PATHS = {"mlp": TRAINED_PATH, "cnn": os.path.join(_DATA, "gearnet_cnn.npz")}
@@ -284,7 +310,7 @@ def sweep(mlp, n_ct: int, seed: int = 0, chunk_cts: int = 64, gears: Sequence[in
         X = synthetic_images(nc * S, seed=int(rng.integers(1 << 62)))
         ct = mlp.encrypt_inputs(X)
         prev = int(rng.integers(N_GEARS))
-        F.append(ciphertext_features(mlp.ck, ct, prev).cpu().numpy())
+        F.append(ciphertext_features(mlp.ck, ct, prev, netf=network_features(mlp)).cpu().numpy())
         lab = np.zeros((nc, N_GEARS))
         for g in gears:
             mlp.set_gears(g, g)
@@ -371,7 +397,7 @@ def _gear_table(mlp, net: GearNet, chunks) -> np.ndarray:
     """[previous gear, chunk, gear] mean predicted SNR: one batched
     prediction for every chunk and every previous-gear value (the only
     feature the decisions feed back into), with a single device sync."""
-    feats = features_many(mlp.ck, [ct for _, ct in chunks], 0)
+    feats = features_many(mlp.ck, [ct for _, ct in chunks], 0, netf=network_features(mlp))
     n = feats.shape[0]
     f = feats.repeat(N_GEARS, 1)
     f[:, 14] = torch.arange(N_GEARS, device=f.device, dtype=f.dtype).repeat_interleave(n) / 3.0

@@ -73,6 +73,60 @@ def make_mlp(seed: int = 0) -> Tuple[np.ndarray, np.ndarray]:
     return W1, W2
 
 
+# Weight families for training the gear controller across networks: the
+# pruning error of a gear, hence the output SNR the monitor measures,
+# depends on how a network's weight energy is spread over its 64-weight
+# blocks.  Every family is rescaled to the variance 1 / fan_in of the
+# bench's Gaussian networks.
+WEIGHT_FAMILIES = ("gaussian", "laplace", "student3", "rowscaled", "blocksparse")
+
+
+def family_weights(shape, fan_in: int, family: str, rng: np.random.Generator) -> np.ndarray:
+    """Weights of `family`, variance 1 / fan_in:
+      gaussian     N(0, 1);
+      laplace      Laplace(0, 1) (heavier tails);
+      student3     Student t with 3 degrees of freedom (heavy tails);
+      rowscaled    N(0, 1) rows scaled by exp(N(0, 1)) (uneven output units);
+      blocksparse  N(0, 1) with every other 64-weight block of the flattened
+                   matrix zeroed at random (probability 1/2)."""
+    shape = tuple(shape)
+    if family == "gaussian":
+        w = rng.normal(size=shape)
+    elif family == "laplace":
+        w = rng.laplace(size=shape)
+    elif family == "student3":
+        w = rng.standard_t(3, size=shape)
+    elif family == "rowscaled":
+        w = rng.normal(size=shape) * np.exp(rng.normal(size=(shape[0],) + (1,) * (len(shape) - 1)))
+    elif family == "blocksparse":
+        flat = rng.normal(size=int(np.prod(shape)))
+        nb = -(-flat.size // BLOCK)
+        keep = np.repeat(rng.random(nb) < 0.5, BLOCK)[: flat.size]
+        keep[:BLOCK] = True                                   # never all zero
+        w = (flat * keep).reshape(shape)
+    else:
+        raise ValueError(f"unknown weight family {family}")
+    return w / (w.std() * math.sqrt(fan_in))
+
+
+def make_mlp_family(seed: int) -> Tuple[np.ndarray, np.ndarray]:
+    """784-128-10 weights of family WEIGHT_FAMILIES[seed % 5], seeded."""
+    rng = np.random.default_rng(10_000 + seed)
+    fam = WEIGHT_FAMILIES[seed % len(WEIGHT_FAMILIES)]
+    return family_weights((128, 784), 784, fam, rng), family_weights((10, 128), 128, fam, rng)
+
+
+def weight_snr_db(W: np.ndarray, gear: int, cap: float = 60.0) -> float:
+    """10 log10(sum W^2 / sum (W - W_gear)^2) of one weight matrix under a
+    gear's block pruning, capped (a lossless gear reads `cap`)."""
+    W = np.asarray(W, dtype=np.float64)
+    lost = float(np.sum((W * ~block_keep_mask(W, GEAR_FRACTIONS[gear])) ** 2))
+    tot = float(np.sum(W ** 2))
+    if lost <= tot * 10.0 ** (-cap / 10):
+        return cap
+    return min(cap, 10.0 * math.log10(tot / lost))
+
+
 def plain_forward(W1, W2, X) -> np.ndarray:
     return (W2 @ ((W1 @ X.T) ** 2)).T
 
@@ -300,6 +354,15 @@ class EncryptedMLP(EncryptedNet):
     def reference_outputs(self, X: np.ndarray) -> np.ndarray:
         """The dense FP64 network: the monitor's reference."""
         return plain_forward(self.W1, self.W2, X)
+
+    def weight_matrices(self):
+        """The two weight matrices as the gears prune them."""
+        return self.W1, self.W2
+
+    def plain_outputs_at_gear(self, X: np.ndarray, gear: int) -> np.ndarray:
+        """The FP64 network with both layers pruned as `gear` prunes them."""
+        return plain_forward(self.W1 * block_keep_mask(self.W1, GEAR_FRACTIONS[gear]),
+                             self.W2 * block_keep_mask(self.W2, GEAR_FRACTIONS[gear]), X)
 
     # -- plans --------------------------------------------------------------
     def _plan(self, D: np.ndarray, rows, cols, baby, giant, folds, level) -> LayerPlan:

@@ -109,11 +109,14 @@ def test_monitor_features_and_gear_costs(mlp):
     import torch
     X = nn.synthetic_images(256, seed=21)
     ct = mlp.encrypt_inputs(X)
-    f = gear.ciphertext_features(mlp.ck, ct, prev_gear=2).cpu().numpy()
+    netf = gear.network_features(mlp)
+    f = gear.ciphertext_features(mlp.ck, ct, prev_gear=2, netf=netf).cpu().numpy()
     assert f.shape == (64, 16)
     # uniform residues: rms(c')/q -> 1/sqrt(12), mean(c')/q -> 1/4
-    assert np.allclose(f[:, 0:8], 1 / math.sqrt(12), atol=0.01)
-    assert np.allclose(f[:, 8:12], 0.25, atol=0.01)
+    assert np.allclose(f[:, [0, 1]], 1 / math.sqrt(12), atol=0.01)
+    assert np.allclose(f[:, 11], 0.25, atol=0.01)
+    assert np.allclose(f[:, 2:11], netf[None, :].astype(np.float32))
+    assert netf.shape == (9,) and np.all(netf[6:] <= 1.0)
     assert np.allclose(f[:, 12], 1.0) and np.allclose(f[:, 14], 2 / 3)
     snrs, costs = [], []
     for g in range(4):
@@ -158,3 +161,27 @@ def test_learned_controller_beats_plain_monitor_loop(mlp):
     assert s_learned >= s_plain - 0.1
     assert learned.evaluations <= plain.evaluations and learned.monitor_calls < plain.monitor_calls
     assert learned.samples_per_s > plain.samples_per_s
+
+
+@pytest.mark.parametrize("kind", ["mlp", "cnn"])
+def test_controller_beats_constant_predictor_on_heldout_networks(torch_gpu, kind):
+    """The committed controller on networks it never saw (seeds 2000..2004,
+    one per weight family): its mean absolute error on the monitor's
+    per-gear SNR (gears 1-3, where networks differ) is below that of the
+    constant predictor answering each gear's mean training label (the
+    controller's stored label mean), by at least a third."""
+    from cipherpu import gear
+    from cipherpu.ckks import CKKS, CKKSParams
+    from cipherpu.gear_train import sweep_networks
+    import time
+    ck = CKKS(CKKSParams(), seed=8)
+    sk = ck.keygen()
+    ctl = gear.GearNet(seed=0)
+    assert gear.load_trained(ctl, gear.TRAINED_PATHS[kind])
+    F, T = sweep_networks(kind, ck, sk, range(2000, 2005), 64, 64, "held-out", time.perf_counter())
+    pred = ctl.predict(F).cpu().numpy()
+    const = ctl.y_mu.cpu().numpy()[None, :]
+    mae = np.abs(pred - T)[:, 1:].mean()
+    mae_c = np.abs(const - T)[:, 1:].mean()
+    print(kind, "held-out MAE dB, controller vs constant:", round(float(mae), 2), round(float(mae_c), 2))
+    assert mae < 0.67 * mae_c

@@ -195,3 +195,29 @@ def test_cnn_forward_against_direct_convolution():
     a = y ** 2
     p = a.reshape(2, 8, 6, 2, 6, 2).mean(axis=(3, 5))
     assert np.allclose(cnn.cnn_forward(K, F, X), p.reshape(2, 288) @ F.T, atol=1e-12)
+
+
+def test_weight_families_and_pruning_statistics():
+    """The controller's training networks: every weight family has variance
+    1 / fan_in; a gear's weight SNR is 10 log10(|W|^2 / |W - W_g|^2) of the
+    block rule (checked against the oracle's mask), never rises with the
+    gear, and reads the cap when the pruned blocks are exactly zero (the
+    block-sparse family at gear 1)."""
+    from cipherpu import cnn, nn
+    for seed in range(len(nn.WEIGHT_FAMILIES)):
+        W1, W2 = nn.make_mlp_family(seed)
+        assert W1.shape == (128, 784) and W2.shape == (10, 128)
+        assert abs(W1.var() * 784 - 1) < 1e-9 and abs(W2.var() * 128 - 1) < 1e-9
+        K, F = cnn.make_cnn_family(seed)
+        assert K.shape == (8, 5, 5) and F.shape == (10, 288)
+        snrs = [nn.weight_snr_db(W1, g) for g in range(4)]
+        assert snrs[0] == 60.0 and snrs[1] >= snrs[2] >= snrs[3]
+        mask = block_prune_mask(W1, nn.GEAR_FRACTIONS[2], 64)
+        want = 10 * math.log10(np.sum(W1 ** 2) / np.sum((W1 * ~mask) ** 2))
+        assert abs(nn.weight_snr_db(W1, 2) - min(60.0, want)) < 1e-9
+    W1, _ = nn.make_mlp_family(4)                       # block-sparse family
+    zero_blocks = np.mean(np.abs(W1.reshape(-1, 64)).max(axis=1) == 0)
+    assert zero_blocks > 0.4 and nn.weight_snr_db(W1, 1) == 60.0
+    # families differ where the controller needs them to: gear-1 pruning
+    # costs a Gaussian network more than a heavy-tailed one
+    assert nn.weight_snr_db(nn.make_mlp_family(0)[0], 1) < nn.weight_snr_db(nn.make_mlp_family(2)[0], 1)
