@@ -31,6 +31,7 @@ products at N = 4096 over one 45-bit prime.
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256] [--no-mlp] [--no-cnn] [--no-graph]
     python bench.py --copy-probe      # plain copy of known size (counter scaling)
     python bench.py --modmul-probe    # Barrett vs special-form point-wise product
+    python bench.py --mfma-probe [--mfma-probe-reps R]   # 64-point step: radix-2 VALU vs int8 MFMA
 """
 from __future__ import annotations
 
@@ -217,6 +218,50 @@ def modmul_probe(rows: int = 1024, reps: int = 10) -> None:
         res[name] = {"ms": round(ms, 4), "G_products_per_s": round(a.numel() / ms / 1e6, 1),
                      "GB_per_s": round(3 * 8 * a.numel() / ms / 1e6, 1)}
     print(json.dumps(res))
+
+
+def mfma_probe(count: int = 1 << 18, reps_list=(1, 17), launches: int = 5) -> None:
+    """The matrix-core question, measured: one 64-point transform step (six
+    radix-2 stages) over `count` vectors of 45-bit residues, on the integer
+    VALU (ckks_probe_dft64 method 0) and as an exact int8 product on the
+    matrix cores (methods 1 and 2, the two candidate accumulator layouts).
+    Each kernel applies the step `reps` times to data held in registers; the
+    time per step is the difference between two repetition counts, so the
+    loads and stores cancel.  Outputs are compared word for word."""
+    import torch
+    from cipherpu import dft64_probe as P
+    from cipherpu.params import CHAIN4, default_root
+    q = CHAIN4[0]
+    T = P.build_tables(q, default_root(q, 128))
+    g = torch.Generator(device="cpu").manual_seed(5)
+    x = (torch.randint(0, 1 << 62, (64, count), generator=g, dtype=torch.int64) % q).cuda()
+    tab = torch.from_numpy(T.blob.copy()).cuda()
+    names = ("radix2", "mfma_layout0", "mfma_layout1", "mfma_layout0_interleaved")
+    ys = {m: torch.empty_like(x) for m in range(4)}
+    res = {"vectors": count, "elements": 64 * count, "q": q, "launches": launches, "by_reps": {}}
+    for reps in reps_list:
+        row = {}
+        for m in range(4):
+            P.launch(x, ys[m], tab, T, m, reps)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(launches):
+                P.launch(x, ys[m], tab, T, m, reps)
+            e1.record()
+            torch.cuda.synchronize()
+            row[names[m]] = round(e0.elapsed_time(e1) / launches, 4)
+        row["equal_to_radix2"] = [bool(torch.equal(ys[m], ys[0])) for m in (1, 2, 3)]
+        res["by_reps"][str(reps)] = row
+    if len(reps_list) == 2:
+        r0, r1 = (res["by_reps"][str(r)] for r in reps_list)
+        span = reps_list[1] - reps_list[0]
+        per = {k: (r1[k] - r0[k]) / span for k in names}
+        res["us_per_step"] = {k: round(v * 1e3, 2) for k, v in per.items()}
+        res["G_elements_per_s_per_step"] = {k: round(64 * count / v / 1e6, 1) for k, v in per.items()}
+        res["over_radix2"] = {k: round(per["radix2"] / per[k], 3)
+                              for k, ok in zip(names[1:], r1["equal_to_radix2"]) if ok}
+    print(json.dumps({"mfma_probe": res}))
 
 
 Q1, PSI1 = 35184371884033, 7388705119   # largest 45-bit prime = 1 mod 8192, smallest primitive 8192-th root
@@ -420,6 +465,10 @@ def main():
                          "gear thresholds 10 / 15 / 20 dB")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--modmul-probe", action="store_true", help="Barrett vs special-form point-wise product")
+    ap.add_argument("--mfma-probe", action="store_true",
+                    help="64-point transform step: radix-2 VALU stages vs the int8 matrix-core product")
+    ap.add_argument("--mfma-probe-reps", type=int, default=0,
+                    help="run the probe at this one repetition count only (for profiler runs)")
     ap.add_argument("--no-graph", action="store_true", help="skip the uneven-batch graph leg")
     ap.add_argument("--no-product", action="store_true", help="skip the N=4096 single-prime product leg")
     ap.add_argument("--graph-ops", type=int, default=2048)
@@ -433,6 +482,9 @@ def main():
         return
     if args.modmul_probe:
         modmul_probe()
+        return
+    if args.mfma_probe:
+        mfma_probe(reps_list=(args.mfma_probe_reps,) if args.mfma_probe_reps else (1, 17))
         return
 
     import numpy as np

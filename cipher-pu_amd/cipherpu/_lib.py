@@ -64,6 +64,7 @@ _SIGS = {
     "ckks_mod_switch_scaled": (_I, [_P, _P, _U, _U, _U, _I, _P]),
     "ckks_poly_mul_modq": (_I, [_P, _P, _P, _P, _I, _U, _P, _P]),
     "ckks_noise_stats_q": (_I, [_P, _P, _I, _U, _U, _P]),
+    "ckks_probe_dft64": (_I, [_P, _P, _P, _U, _U, _U, _I, _I, _P]),
     "ckks_embed": (_I, [_P, _P, _P, _I, _I, _P]),
     "ckks_embed_inv": (_I, [_P, _P, _P, _I, _I, _P]),
     "ckks_noise_stats": (_I, [_P, _P, _P, _I, _I, _I, _P]),

@@ -17,7 +17,8 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 INCLUDE = os.path.join(os.path.dirname(PKG_ROOT), "include")
 BUILD = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(HERE, "libcipherpu_hip.so")
-SOURCES = ["k_ntt.hip", "k_ckks.hip", "k_fft.hip", "k_gear.hip", "k_sample.hip", "k_hoist.hip", "capi.cpp"]
+SOURCES = ["k_ntt.hip", "k_ckks.hip", "k_fft.hip", "k_gear.hip", "k_sample.hip", "k_hoist.hip", "k_mfma_probe.hip",
+           "capi.cpp"]
 ARCH = os.environ.get("CIPHERPU_ARCH", "gfx950")
 
 
@@ -28,10 +29,16 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def flags():
+# The matrix-core probe keeps its int32 accumulators in VGPRs (the VGPR form
+# of the MFMA instructions): in AGPRs every recombined sum would first need
+# a v_accvgpr_read.  No other source uses the matrix cores.
+SOURCE_FLAGS = {"k_mfma_probe.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
+def flags(src: str | None = None):
     return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-command-line-argument", "-I", CSRC, "-I", INCLUDE]
+            "-Wno-unused-command-line-argument", "-I", CSRC, "-I", INCLUDE, *SOURCE_FLAGS.get(src, [])]
 
 
 def build(verbose: bool = False) -> str:
@@ -42,7 +49,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmd = [cc, *flags(), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *flags(src), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
     failed = []
     for src, p in procs:

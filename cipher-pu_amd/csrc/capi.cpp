@@ -925,6 +925,19 @@ int ckks_poly_mul_modq(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, 
     return CKKS_OK;
 }
 
+int ckks_probe_dft64(const uint64_t* x, uint64_t* y, const void* tables, uint64_t q, uint64_t ws40, size_t count,
+                     int method, int reps, void* stream) {
+    if (!x || !y || !tables) return CKKS_ERR_NULL;
+    if (count == 0 || count % 64 || count > (1ull << 26)) return CKKS_ERR_BAD_SHAPE;
+    if (method < 0 || method > 3 || reps < 1 || reps > 64) return CKKS_ERR_BAD_ARG;
+    // q < 2^45 keeps the radix-2 path's lazy values below 2^62 for 64 reps
+    // (q + 24 reps q) and the matrix path's below 2^48
+    if (q < 3 || !(q & 1) || q >= (1ull << 45)) return CKKS_ERR_BAD_MODULUS;
+    if (misaligned(x) || misaligned(y) || misaligned_pairs(tables)) return CKKS_ERR_BAD_ALIGNMENT;
+    HIPCHK(launch_dft64_probe(x, y, tables, q, ws40, count, method, reps, S(stream)));
+    return CKKS_OK;
+}
+
 int ckks_noise_stats_q(const uint64_t* data, uint64_t* stats, int rows, uint64_t n, uint64_t q, void* stream) {
     if (rows < 0 || n > (1ull << 30)) return CKKS_ERR_BAD_SHAPE;
     if (rows == 0 || n == 0) return CKKS_OK;

@@ -198,3 +198,11 @@ hipError_t launch_moddown_eval(const DevTables& t, const uint64_t* acc, const ui
 hipError_t launch_key_permute(const DevTables& t, const ulonglong2* in, ulonglong2* out, int rows, uint32_t g,
                               hipStream_t s);
 
+
+// Matrix-core probe (k_mfma_probe.hip): y = G^reps x for V length-64 vectors
+// stored [64][V]; method 0 = radix-2 VALU stages, 1 / 2 = int8 MFMA product
+// with accumulator layout candidate 0 / 1, 3 = candidate 0 software-pipelined.  tables: 64 twiddle pairs, row
+// constants (128 words: kq, (-BIAS) mod q, BIAS), digit operands
+// [2][4][6][64] x 16 bytes.
+hipError_t launch_dft64_probe(const uint64_t* X, uint64_t* Y, const void* tables, uint64_t q, uint64_t ws40,
+                              size_t V, int method, int reps, hipStream_t s);

@@ -378,6 +378,22 @@ enum {
 int ckks_timing_start(ckks_ctx *ctx, int max_launches);
 int ckks_timing_stop(ckks_ctx *ctx, double *ms, int *launches, int n_kinds);
 
+/* Matrix-core measurement probe (no counterpart in the reference; it
+ * settles whether an exact int8 matrix-core product can replace the six
+ * radix-2 stages of a 64-point transform step): y = G^reps x mod q for
+ * count vectors of 64 words stored element-major, x [64][count] in [0, q),
+ * G the 64-point negacyclic forward transform in bit-reversed output order.
+ * method 0 runs the radix-2 VALU stages, 1 and 2 the int8 product
+ * (v_mfma_i32_16x16x64_i8) under the two candidate accumulator layouts, 3
+ * the first layout with the next row tile's products interleaved into the
+ * recombination.
+ * tables (CKKS_PROBE_DFT64_TABLE_BYTES, device memory) and ws40 (the Shoup
+ * companion of 2^40) come from cipherpu.dft64_probe.  count is a positive
+ * multiple of 64, 1 <= reps <= 64, q an odd prime below 2^45. */
+#define CKKS_PROBE_DFT64_TABLE_BYTES 51200
+int ckks_probe_dft64(const uint64_t *x, uint64_t *y, const void *tables, uint64_t q, uint64_t ws40, size_t count,
+                     int method, int reps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

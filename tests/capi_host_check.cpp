@@ -164,6 +164,15 @@ static void check_entry_arguments() {
     CHECK(ckks_noise_stats_q(buf, buf, -1, 32, 113, nullptr) == CKKS_ERR_BAD_SHAPE);
     CHECK(ckks_noise_stats_q(nullptr, buf, 1, 32, 113, nullptr) == CKKS_ERR_NULL);
     CHECK(ckks_noise_stats_q(buf, buf, 1, 32, 1, nullptr) == CKKS_ERR_BAD_MODULUS);
+    const uint64_t q45 = 17592186765313ull;
+    CHECK(ckks_probe_dft64(nullptr, buf, buf, q45, 1, 64, 0, 1, nullptr) == CKKS_ERR_NULL);
+    CHECK(ckks_probe_dft64(buf, buf, nullptr, q45, 1, 64, 0, 1, nullptr) == CKKS_ERR_NULL);
+    for (size_t n : {(size_t)0, (size_t)32, (size_t)96}) CHECK(ckks_probe_dft64(buf, buf, buf, q45, 1, n, 0, 1, nullptr) == CKKS_ERR_BAD_SHAPE);
+    for (int m : {-1, 4}) CHECK(ckks_probe_dft64(buf, buf, buf, q45, 1, 64, m, 1, nullptr) == CKKS_ERR_BAD_ARG);
+    for (int r : {0, 65}) CHECK(ckks_probe_dft64(buf, buf, buf, q45, 1, 64, 1, r, nullptr) == CKKS_ERR_BAD_ARG);
+    for (uint64_t bq : {(uint64_t)2, q45 + 1, (uint64_t)1 << 45}) CHECK(ckks_probe_dft64(buf, buf, buf, bq, 1, 64, 1, 1, nullptr) == CKKS_ERR_BAD_MODULUS);
+    CHECK(ckks_probe_dft64(buf, buf, (char*)buf + 8, q45, 1, 64, 1, 1, nullptr) == CKKS_ERR_BAD_ALIGNMENT);
+    CHECK(ckks_probe_dft64(mis, buf, buf, q45, 1, 64, 1, 1, nullptr) == CKKS_ERR_BAD_ALIGNMENT);
     CHECK(ckks_noise_stats_q(buf, mis, 1, 32, 113, nullptr) == CKKS_ERR_BAD_ALIGNMENT);
     CHECK(ckks_poly_mul_modq(nullptr, buf, buf, buf, 1, 113, buf, nullptr) == CKKS_ERR_NULL);
     CHECK(ckks_bitrev(buf, 1, 12, nullptr) == CKKS_ERR_BAD_N);

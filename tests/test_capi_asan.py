@@ -33,7 +33,7 @@ def _kernel_objects():
             for f in os.listdir(B.CSRC) if f.endswith((".h", ".hip")))
         if stale:
             os.makedirs(B.BUILD, exist_ok=True)
-            subprocess.run([B.hipcc(), *B.flags(), "-x", "hip", "-c", os.path.join(B.CSRC, src), "-o", obj],
+            subprocess.run([B.hipcc(), *B.flags(src), "-x", "hip", "-c", os.path.join(B.CSRC, src), "-o", obj],
                            check=True, capture_output=True)
         objs.append(obj)
     return B, objs
