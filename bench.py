@@ -96,10 +96,10 @@ def kernel_model_bytes(kind: str, batch: int, l: int = L):
 # copy, and wave-level VALU instructions (SQ_INSTS_VALU).  Keyed by (batch,
 # key-switch threads); the bench only runs shapes listed here.
 PROFILE_SOURCE = "profiles/hmult_kernels_n8192_l4_b256.md"
-MEASURED_TRAFFIC = {("front", 256, 1024): 556527155, ("keyswitch_fused", 256, 1024): 636715487,
-                    ("keyswitch", 256, 1024): 256456465}
-MEASURED_VALU_INSTR = {("front", 256, 1024): 133808128, ("keyswitch_fused", 256, 1024): 114634752,
-                       ("keyswitch", 256, 1024): 64462848}
+MEASURED_TRAFFIC = {("front", 256, 1024): 558112345, ("keyswitch_fused", 256, 1024): 636649116,
+                    ("keyswitch", 256, 1024): 256452104}
+MEASURED_VALU_INSTR = {("front", 256, 1024): 134250496, ("keyswitch_fused", 256, 1024): 115593216,
+                       ("keyswitch", 256, 1024): 65171456}
 PROFILED_THREADS = (1024,)   # key-switch and front/tensor threads of the profiled shapes
 # The issue peak counts one wave64 VALU instruction per SIMD every 4 cycles:
 # 1024 SIMDs x 2.4 GHz / 4.
