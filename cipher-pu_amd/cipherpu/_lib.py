@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(_HERE, "libcipherpu_hip.so")
 
 CKKS_CYCLIC = 1
 CKKS_NATURAL_ORDER = 2
+CKKS_ARITH_INT64 = 0
+CKKS_ARITH_FP64 = 1
 
 _ERRORS = {
     -1: "bad ring degree N",
@@ -42,6 +44,7 @@ _SIGS = {
     "ckks_ctx_root": (_I, [_P, _I, ctypes.POINTER(_U)]),
     "ckks_ctx_set_threads": (_I, [_P, _I, _I, _I]),
     "ckks_ctx_set_front": (_I, [_P, _I]),
+    "ckks_ctx_set_arith": (_I, [_P, _I]),
     "ckks_ctx_set_hoist_order": (_I, [_P, _I, _I]),
     "ckks_version": (ctypes.c_char_p, []),
     "ckks_ntt_fwd": (_I, [_P, _P, _I, _I, _I, _I, _P]),

@@ -37,7 +37,7 @@ import torch
 from ._lib import call
 from .ckks import CKKS, Ciphertext, SecretKey
 from .nn import GEAR_FRACTIONS, SLOTS_PER_SAMPLE, EncryptedNet, block_keep_mask
-from .ring import to_numpy
+from .ring import prepared_pairs, to_numpy
 
 ROW = 16          # row stride of the polyphase grids and of the output grid
 PHASE = 256       # slots per polyphase component
@@ -146,7 +146,8 @@ class EncryptedCNN(EncryptedNet):
 
     # -- plaintext material -------------------------------------------------
     def _conv_weights(self, level: int, scale: float) -> torch.Tensor:
-        """[8][taps][level] (value, companion) residues of round(K * scale)."""
+        """[8][taps][level] prepared pairs (the doubles r and RN(r / q)) of the
+        residues r of round(K * scale)."""
         key = (self.gears[0], level, scale)
         if key not in self._conv:
             ck = self.ck
@@ -157,7 +158,7 @@ class EncryptedCNN(EncryptedNet):
                     for i in range(level):
                         q = ck.primes[i]
                         r = w % q
-                        D[c, b, i] = (r, (r << 63) // q)
+                        D[c, b, i] = prepared_pairs(r, q)
             self._conv[key] = ck.ring.tensor(D)
         return self._conv[key]
 

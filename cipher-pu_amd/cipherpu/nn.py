@@ -332,8 +332,8 @@ class EncryptedNet:
         return ck.encrypt(top, self.sk, scale=ct.scale)
 
     def prepared_pt(self, z: np.ndarray, level: int, scale: float) -> torch.Tensor:
-        """Slot vectors [rows, N/2] -> evaluation-form (value, companion) rows
-        [rows, level, N, 2] for ckks_lincomb."""
+        """Slot vectors [rows, N/2] -> evaluation-form prepared pairs (the
+        doubles w and RN(w / q)) [rows, level, N, 2] for ckks_lincomb."""
         ck = self.ck
         pt = ck.encode(np.asarray(z, dtype=np.complex128), level=level, scale=scale)
         prep = torch.empty(*pt.shape, 2, dtype=torch.uint64, device=ck.device)

@@ -79,6 +79,11 @@ class RingContext:
     def set_front(self, fused: bool = True):
         call("ckks_ctx_set_front", self._h, 1 if fused else 0)
 
+    def set_arith(self, fp64: bool = True):
+        """Transform kernels on the FP64 pipeline (True) or on 64-bit integer
+        words (False); the results are the same words."""
+        call("ckks_ctx_set_arith", self._h, 1 if fp64 else 0)
+
     def set_hoist_order(self, btile: int = 0, xcd: bool = False):
         """Workgroup placement of the hoisted key switch (ckks_ctx_set_hoist_order)."""
         call("ckks_ctx_set_hoist_order", self._h, btile, 1 if xcd else 0)
@@ -254,6 +259,15 @@ class RingContext:
         out = torch.empty(p, self.N // 2, dtype=torch.complex128, device=self.device)
         call("ckks_decode", self._h, _ptr(pt), _ptr(out), p, l, prime0, float(scale), _ptr(work), self.stream())
         return out
+
+
+def prepared_pairs(w, q) -> np.ndarray:
+    """Reduced words w < q -> the prepared-pair form that the key switches
+    and the plaintext products read: the IEEE doubles w and RN(w / q), as
+    uint64 bit patterns, shape w.shape + (2,).  q broadcasts against w."""
+    wd = np.asarray(w, dtype=np.uint64).astype(np.float64)   # exact below 2^53
+    qd = np.asarray(q, dtype=np.float64)
+    return np.ascontiguousarray(np.stack([wd, wd / qd], axis=-1)).view(np.uint64)
 
 
 def bitrev_(x: torch.Tensor) -> torch.Tensor:

@@ -96,6 +96,9 @@ struct ckks_ctx {
     PrimeConst hpc[CKKS_MAX_PRIMES] = {};
     PrimeConst* dpc = nullptr;
     ulonglong2* dtw = nullptr;
+    PrimeF* dpf = nullptr;        // FP64 pipeline: per-prime constants
+    double2* dtwf = nullptr;      // FP64 pipeline: (w, RN(w / q)) twiddles, order of dtw
+    int fp64 = 1;                 // transform arithmetic (ckks_ctx_set_arith)
     double2* fft_tw = nullptr;    // exp(2 pi i k / n), k < n/2, n = N/2
     double2* fft_zeta = nullptr;  // exp(i pi j / N), j < n
     int* slot_k = nullptr;        // (5^j mod 2N - 1)/4
@@ -113,7 +116,7 @@ struct ckks_ctx {
     mutable Timer timer;
     int narrow = 0;        // every prime below twice every other one
     void* scratch = nullptr;   // this device's copy of the no-alloc scratch array
-    DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN, narrow}; }
+    DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN, narrow, dpf, dtwf, fp64}; }
 };
 
 namespace {
@@ -201,6 +204,8 @@ ModDownConst moddown_const(const ckks_ctx* c, int l, int sp) {
             m.pinv_s[i] = shoup(m.pinv[i], qi);
             m.p_mod[i] = p % qi;
             m.p_mod_s[i] = shoup(m.p_mod[i], qi);
+            m.pmf[i] = (double)m.p_mod[i];
+            m.pmf_q[i] = (double)m.p_mod[i] / (double)qi;
         }
         if (i < l - 1) {
             m.hq[i] = (qL >> 1) % qi;
@@ -290,6 +295,8 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
     }
     const int logN = c->logN;
     std::vector<ulonglong2> tw((size_t)n_primes * 2 * N);
+    std::vector<double2> twf((size_t)n_primes * 2 * N);
+    std::vector<PrimeF> pf(n_primes);
     for (int p = 0; p < n_primes; ++p) {
         const uint64_t q = primes[p];
         const uint64_t r = roots ? roots[p] % q : powmod(primitive_root(q), (q - 1) / order, q);
@@ -322,6 +329,13 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
         pc.ninv_s = shoup(pc.ninv, q);
         pc.ninv_w1 = mulmod(pc.ninv, iv[1].x, q);
         pc.ninv_w1_s = shoup(pc.ninv_w1, q);
+        // the FP64 pipeline's constants and twiddles: integers below 2^45 are
+        // exact doubles, and IEEE division rounds w / q once
+        const double dq = (double)q;
+        pf[p] = PrimeF{dq, -dq, 1.0 / dq, (double)pc.ninv, (double)pc.ninv / dq, (double)pc.ninv_w1,
+                       (double)pc.ninv_w1 / dq, 0.0};
+        double2* ff = &twf[(size_t)p * 2 * N];
+        for (uint64_t k = 0; k < 2 * N; ++k) ff[k] = make_double2((double)f[k].x, (double)f[k].x / dq);
     }
     // canonical-embedding tables (negacyclic rings): the length-N/2 FFT's
     // twiddles, the twist zeta^j and the 5^j slot order
@@ -355,6 +369,10 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
     ok = ok && hipMalloc(&c->dtw, tw.size() * sizeof(ulonglong2)) == hipSuccess;
     ok = ok && hipMemcpy(c->dpc, c->hpc, sizeof(PrimeConst) * n_primes, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(c->dtw, tw.data(), tw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMalloc(&c->dpf, pf.size() * sizeof(PrimeF)) == hipSuccess;
+    ok = ok && hipMalloc(&c->dtwf, twf.size() * sizeof(double2)) == hipSuccess;
+    ok = ok && hipMemcpy(c->dpf, pf.data(), pf.size() * sizeof(PrimeF), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->dtwf, twf.data(), twf.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && emb) {
         ok = ok && hipMalloc(&c->fft_tw, ft.size() * sizeof(double2)) == hipSuccess;
         ok = ok && hipMalloc(&c->fft_zeta, zt.size() * sizeof(double2)) == hipSuccess;
@@ -380,6 +398,8 @@ int ckks_ctx_destroy(ckks_ctx* c) {
     for (hipEvent_t e : c->timer.ev) hipEventDestroy(e);
     if (c->dpc) hipFree(c->dpc);
     if (c->dtw) hipFree(c->dtw);
+    if (c->dpf) hipFree(c->dpf);
+    if (c->dtwf) hipFree(c->dtwf);
     if (c->fft_tw) hipFree(c->fft_tw);
     if (c->fft_zeta) hipFree(c->fft_zeta);
     if (c->slot_k) hipFree(c->slot_k);
@@ -391,6 +411,13 @@ int ckks_ctx_root(const ckks_ctx* c, int prime, uint64_t* root) {
     if (!c || !root) return CKKS_ERR_NULL;
     if (prime < 0 || prime >= c->K) return CKKS_ERR_BAD_SHAPE;
     *root = c->roots[prime];
+    return CKKS_OK;
+}
+
+int ckks_ctx_set_arith(ckks_ctx* c, int arith) {
+    if (!c) return CKKS_ERR_NULL;
+    if (arith != CKKS_ARITH_INT64 && arith != CKKS_ARITH_FP64) return CKKS_ERR_BAD_ARG;
+    c->fp64 = arith == CKKS_ARITH_FP64;
     return CKKS_OK;
 }
 
@@ -586,7 +613,7 @@ int ckks_key_prepare(const ckks_ctx* c, const uint64_t* key_coeff, uint64_t* key
     const DevTables t = c->tables();
     uint64_t* tmp = const_cast<uint64_t*>(key_coeff);   // transformed in place (documented)
     HIPCHK(launch_ntt(t, tmp, tmp, dnum * 2, key_limbs, m, false, true, c->ntt_threads, S(stream)));
-    HIPCHK(launch_shoup_pairs(t, tmp, (ulonglong2*)key_out, dnum * 2, key_limbs, m, S(stream)));
+    HIPCHK(launch_fp64_pairs(t, tmp, (double2*)key_out, dnum * 2, key_limbs, m, S(stream)));
     return CKKS_OK;
 }
 
@@ -615,7 +642,7 @@ int ckks_relin(const ckks_ctx* c, const uint64_t* d01, const uint64_t* d2, const
     const ModDownConst mc = moddown_const(c, l, special);
     uint64_t* acc = (uint64_t*)workspace;
     const size_t LN = (size_t)l * c->N;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2, LN, nullptr, nullptr, (const ulonglong2*)key, key_limbs, acc,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2, LN, nullptr, nullptr, (const double2*)key, key_limbs, acc,
                                                 batch, l, m, special, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, d01, 3, out, batch, l, m, special, mc, false, S(stream)));
     return CKKS_OK;
@@ -656,9 +683,9 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
     rest.count = l - 1;
     rest.out = out;
     rest.mc = &mc;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc, batch,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2c, LN, d2e, d01, (const double2*)key, key_limbs, acc, batch,
                                                 l, m, special, c->ks_threads, S(stream), nullptr, tail));
-    TIMED(CKKS_KIND_KEYSWITCH_FUSED, launch_keyswitch(t, d2c, LN, d2e, d01, (const ulonglong2*)key, key_limbs, acc,
+    TIMED(CKKS_KIND_KEYSWITCH_FUSED, launch_keyswitch(t, d2c, LN, d2e, d01, (const double2*)key, key_limbs, acc,
                                                       batch, l, m, special, c->ks_threads, S(stream), nullptr, rest));
     return CKKS_OK;
 }
@@ -705,7 +732,7 @@ int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int
     uint64_t* sct = (uint64_t*)workspace;   // sigma(c0), sigma(c1)
     uint64_t* acc = sct + B * 2 * LN;
     TIMED(CKKS_KIND_AUTOMORPHISM, launch_automorphism(t, ct, sct, batch * 2, l, m, galois, S(stream)));
-    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, (const ulonglong2*)gkey,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, (const double2*)gkey,
                                                 key_limbs, acc, batch, l, m, special, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream)));
     return CKKS_OK;
@@ -781,7 +808,7 @@ int ckks_rotate_hoisted(const ckks_ctx* c, const uint64_t* ct, const uint64_t* k
                                                                  2 * LN, m, galois[r], S(stream)));
     // one prepared key per rotation: [dnum = key_limbs - 1][2][key_limbs][N] pairs
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const double2*)keys, key_stride, key_limbs, gl, acc, batch,
                                                  l, m, special, R, 0, c->hoist, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch * R, l, m, special, mc, false,
                                                     S(stream)));
@@ -812,7 +839,7 @@ int ckks_rotate_hoisted_eval(const ckks_ctx* c, const uint64_t* ct, const uint64
     TIMED(CKKS_KIND_KEYSWITCH, launch_modup(t, ct, D, batch, l, m, special, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_NTT, launch_ntt(t, cn, ct, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const ulonglong2*)keys, key_stride, key_limbs, gl, acc, batch,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const double2*)keys, key_stride, key_limbs, gl, acc, batch,
                                                  l, m, special, R, 1, c->hoist, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_eval(t, acc, cn, out, batch, l, m, special, mc, gl, R, c->ks_threads,
                                                  S(stream)));
@@ -980,7 +1007,7 @@ int ckks_pt_prepare(const ckks_ctx* c, uint64_t* pt, uint64_t* out, int n_polys,
     const DevTables t = c->tables();
     const LimbMap m = limb_map(prime0, l);
     HIPCHK(launch_ntt(t, pt, pt, n_polys, l, m, false, true, c->ntt_threads, S(stream)));
-    HIPCHK(launch_shoup_pairs(t, pt, (ulonglong2*)out, n_polys, l, m, S(stream)));
+    HIPCHK(launch_fp64_pairs(t, pt, (double2*)out, n_polys, l, m, S(stream)));
     return CKKS_OK;
 }
 
@@ -992,7 +1019,7 @@ int ckks_lincomb(const ckks_ctx* c, const uint64_t* E, const uint64_t* D, uint64
     if (!E || !D || !out) return CKKS_ERR_NULL;
     if (misaligned(E) || misaligned_pairs(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
-    HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), 0, S(stream)));
+    HIPCHK(launch_lincomb(c->tables(), E, (const double2*)D, out, nb, ng, batch, l, limb_map(0, l), 0, S(stream)));
     return CKKS_OK;
 }
 
@@ -1004,7 +1031,7 @@ int ckks_lincomb_scalar(const ckks_ctx* c, const uint64_t* E, const uint64_t* D,
     if (!E || !D || !out) return CKKS_ERR_NULL;
     if (misaligned(E) || misaligned_pairs(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
-    HIPCHK(launch_lincomb(c->tables(), E, (const ulonglong2*)D, out, nb, ng, batch, l, limb_map(0, l), 1, S(stream)));
+    HIPCHK(launch_lincomb(c->tables(), E, (const double2*)D, out, nb, ng, batch, l, limb_map(0, l), 1, S(stream)));
     return CKKS_OK;
 }
 

@@ -13,158 +13,171 @@
 
 #include "launch.h"
 #include "ntt_core.h"
+#include "ntt_f64.h"
 
 namespace {
 
 using ntt::Cfg;
-
-__device__ __forceinline__ uint64_t reduce4q(uint64_t x, const PrimeConst& pc) {
-    return csub(csub(x, pc.two_q), pc.q);
-}
 
 // Tensor product in the evaluation domain (verify_e2e_mult.py:78-92 /
 // rtl/m4/e2e_ckks_mult_ctrl_m4.sv S_MUL_*):
 //   d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1.
 // Writes p*d0, p*d1 (evaluation form, ready to be folded into the key
 // switch accumulators), d2 in evaluation form, and d2 back in coefficient
-// form (the key-switch digits).
+// form (the key-switch digits).  FP64 pipeline (modf64.h); every word
+// written is canonical.  Operands below q, so each data product is exact.
 template <int LOGN, int T>
 __global__ void __launch_bounds__(T) k_tensor_intt(const uint64_t* __restrict__ ea, const uint64_t* __restrict__ eb,
                                                    uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
-                                                   uint64_t* __restrict__ d2c, const ulonglong2* __restrict__ tw_all,
-                                                   const PrimeConst* __restrict__ pcs, LimbMap map, int l,
+                                                   uint64_t* __restrict__ d2c, const double2* __restrict__ twf_all,
+                                                   const PrimeF* __restrict__ pfs, LimbMap map, int l,
                                                    ModDownConst mc) {
     using C = Cfg<LOGN, T>;
-    extern __shared__ uint64_t lds[];
+    extern __shared__ uint64_t lds_u[];
+    double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
     const int b = blockIdx.x / l, i = blockIdx.x % l;
     const int p = map.idx[i];
-    const PrimeConst pc = pcs[p];
+    const PrimeF pf = pfs[p];
+    const double qi = pf.qinv, nq = pf.nq;
     const size_t LN = (size_t)l * C::N;
     const size_t ct0 = (size_t)b * 2 * LN + (size_t)i * C::N;  // poly 0, limb i
-    const uint64_t pm = mc.p_mod[i], pms = mc.p_mod_s[i];
-    uint64_t x[C::E];
+    const double pm = mc.pmf[i], pmq = mc.pmf_q[i];
+    double x[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) {
         const size_t j = (size_t)e * C::T + tid;
-        const uint64_t a0 = ea[ct0 + j], a1 = ea[ct0 + LN + j];
-        const uint64_t b0 = eb[ct0 + j], b1 = eb[ct0 + LN + j];
-        const uint64_t d0 = mul_barrett(a0, b0, pc);
-        const uint64_t d1 = add_mod(mul_barrett(a0, b1, pc), mul_barrett(a1, b0, pc), pc.q);
-        const uint64_t d2 = mul_barrett(a1, b1, pc);
-        d01[ct0 + j] = csub(mul_shoup(d0, pm, pms, pc.q), pc.q);
-        d01[ct0 + LN + j] = csub(mul_shoup(d1, pm, pms, pc.q), pc.q);
-        d2e[(size_t)b * LN + (size_t)i * C::N + j] = d2;
+        const double a0 = u2d(ea[ct0 + j]), a1 = u2d(ea[ct0 + LN + j]);
+        const double b0 = u2d(eb[ct0 + j]), b1 = u2d(eb[ct0 + LN + j]);
+        const double d1 = muld(a0, b1, qi, nq) + muld(a1, b0, qi, nq);
+        const double d2 = muld(a1, b1, qi, nq);
+        d01[ct0 + j] = d2u(canon(mulw(muld(a0, b0, qi, nq), pm, pmq, nq), qi, nq));
+        d01[ct0 + LN + j] = d2u(canon(mulw(d1, pm, pmq, nq), qi, nq));
+        d2e[(size_t)b * LN + (size_t)i * C::N + j] = d2u(canon(d2, qi, nq));
         x[e] = d2;
     }
-    ntt::coalesced_to_last<C>(x, lds, tid);
-    ntt::inv_ntt<C>(x, lds, tw_all + ((size_t)p * 2 + 1) * C::N, pc, tid);
+    // to the last forward pass's layout, through LDS
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) x[e] = csub(x[e], pc.q);
-    ntt::store_coalesced<C>(x, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
+    for (int e = 0; e < C::E; ++e) lds[ntt::pad<C>(e * C::T + tid)] = x[e];
+    __syncthreads();
+    ntt::from_lds<C, C::NP - 1>(x, lds, tid);
+    __syncthreads();
+    nttf::inv_ntt<C>(x, lds, twf_all + ((size_t)p * 2 + 1) * C::N, pf, tid);
+    nttf::store_words<C>(x, d2c + (size_t)b * LN + (size_t)i * C::N, pf, tid);
+}
+
+// One (pair, limb) of the tensor step from the four forward transforms
+// (bounded by 16 q): p d0, p d1 and d2 written in evaluation order at
+// register k's slot; d2 returned (bounded by 3q/4) for the inverse transform.
+template <class C>
+__device__ __forceinline__ double tensor_one(double x0, double x1, double y0, double y1, uint64_t* __restrict__ o0,
+                                             uint64_t* __restrict__ o2, size_t LN, int idx, const PrimeF& pf,
+                                             double pm, double pmq) {
+#pragma clang fp contract(off)
+    const double qi = pf.qinv, nq = pf.nq;
+    const double a0 = rmod(x0, qi, nq), a1 = rmod(x1, qi, nq);
+    const double b0 = rmod(y0, qi, nq), b1 = rmod(y1, qi, nq);
+    o0[idx] = d2u(canon(mulw(muld(a0, b0, qi, nq), pm, pmq, nq), qi, nq));
+    const double d1 = muld(a1, b0, qi, nq) + muld(a0, b1, qi, nq);
+    o0[LN + idx] = d2u(canon(mulw(d1, pm, pmq, nq), qi, nq));
+    const double d2 = muld(a1, b1, qi, nq);
+    o2[idx] = d2u(canon(d2, qi, nq));
+    return d2;
 }
 
 // The whole front of HMult for one (pair, limb i), coefficient form in:
 // forward transforms of a0, a1, b0, b1, the tensor product, p*d0 and p*d1
 // and d2 written in evaluation form, d2 transformed back to coefficient form.
-// Limb-major grid: workgroups in flight share one prime's twiddles.
+// Limb-major grid: workgroups in flight share one prime's twiddles.  FP64
+// pipeline: the forward transforms leave values below (1 + 5 log2(N)/8) q,
+// reduced before the data products.
 template <int LOGN, int T>
 __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
                                              uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
-                                             uint64_t* __restrict__ d2c, const ulonglong2* __restrict__ tw_all,
-                                             const PrimeConst* __restrict__ pcs, LimbMap map, int l, int batch,
+                                             uint64_t* __restrict__ d2c, const double2* __restrict__ twf_all,
+                                             const PrimeF* __restrict__ pfs, LimbMap map, int l, int batch,
                                              ModDownConst mc) {
     using C = Cfg<LOGN, T>;
-    extern __shared__ uint64_t lds[];
+    extern __shared__ uint64_t lds_u[];
+    double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
     const int i = blockIdx.x / batch, b = blockIdx.x % batch;
     const int p = map.idx[i];
-    const PrimeConst pc = pcs[p];
-    const ulonglong2* tw = tw_all + (size_t)p * 2 * C::N;
+    const PrimeF pf = pfs[p];
+    const double2* tw = twf_all + (size_t)p * 2 * C::N;
     const size_t LN = (size_t)l * C::N;
     const size_t c0 = (size_t)b * 2 * LN + (size_t)i * C::N;   // poly 0 of pair b, limb i
-    const uint64_t pm = mc.p_mod[i], pms = mc.p_mod_s[i];
+    const double pm = mc.pmf[i], pmq = mc.pmf_q[i];
+    uint64_t* o0 = d01 + c0;
+    uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
     if constexpr (ntt::pair_pays<C>()) {
         // a0 with a1 and b0 with b1 as transform pairs
-        uint64_t x0[C::E], x1[C::E], y0[C::E], y1[C::E];
-        ntt::load_coalesced<C>(x0, A + c0, tid);
-        ntt::load_coalesced<C>(x1, A + c0 + LN, tid);
-        ntt::fwd_ntt2<C>(x0, x1, lds, tw, pc.q, pc.two_q, tid);
-        ntt::load_coalesced<C>(y0, B + c0, tid);
-        ntt::load_coalesced<C>(y1, B + c0 + LN, tid);
-        ntt::fwd_ntt2<C>(y0, y1, lds, tw, pc.q, pc.two_q, tid);
-        uint64_t* o0 = d01 + c0;
-        uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
+        double x0[C::E], x1[C::E], y0[C::E], y1[C::E];
+        nttf::load_words<C>(x0, A + c0, tid);
+        nttf::load_words<C>(x1, A + c0 + LN, tid);
+        nttf::fwd_ntt2<C>(x0, x1, lds, tw, pf.nq, tid);
+        nttf::load_words<C>(y0, B + c0, tid);
+        nttf::load_words<C>(y1, B + c0 + LN, tid);
+        nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf.nq, tid);
         // the inverse transform's first twiddles, loaded before the stores
-        const auto pre = ntt::inv_first<C, C::NP - 1>(tw + C::N, tid);
+        const auto pre = nttf::inv_first<C, C::NP - 1>(tw + C::N, tid);
 #pragma unroll
-        for (int k = 0; k < C::E; ++k) {
-            const uint64_t a0 = reduce64(x0[k], pc), a1 = reduce64(x1[k], pc);
-            const uint64_t b0 = reduce64(y0[k], pc), b1 = reduce64(y1[k], pc);
-            const int idx = ntt::eval_index<C>(k, tid);
-            o0[idx] = csub(mul_shoup(mul_barrett(a0, b0, pc), pm, pms, pc.q), pc.q);
-            const uint64_t d1 = add_mod(mul_barrett(a1, b0, pc), mul_barrett(a0, b1, pc), pc.q);
-            o0[LN + idx] = csub(mul_shoup(d1, pm, pms, pc.q), pc.q);
-            const uint64_t d2 = mul_barrett(a1, b1, pc);
-            o2[idx] = d2;
-            x0[k] = d2;
-        }
-        ntt::inv_ntt_db<C>(x0, lds, tw + C::N, pc, tid, pre);
-#pragma unroll
-        for (int k = 0; k < C::E; ++k) x0[k] = csub(x0[k], pc.q);
-        ntt::store_coalesced<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
+        for (int k = 0; k < C::E; ++k)
+            x0[k] = tensor_one<C>(x0[k], x1[k], y0[k], y1[k], o0, o2, LN, ntt::eval_index<C>(k, tid), pf, pm, pmq);
+        nttf::inv_ntt_db<C>(x0, lds, tw + C::N, pf, tid, pre);
+        nttf::store_words<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, pf, tid);
         return;
     }
-    uint64_t x0[C::E], x1[C::E], y[C::E], z[C::E];
-    ntt::load_coalesced<C>(x0, A + c0, tid);
-    ntt::fwd_ntt<C>(x0, lds, tw, pc.q, pc.two_q, tid);
-    ntt::load_coalesced<C>(x1, A + c0 + LN, tid);
-    ntt::fwd_ntt<C>(x1, lds, tw, pc.q, pc.two_q, tid);
-    ntt::load_coalesced<C>(y, B + c0, tid);
-    ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
-    uint64_t* o0 = d01 + c0;
+    double x0[C::E], x1[C::E], y[C::E], z[C::E];
+    const double qi = pf.qinv, nq = pf.nq;
+    nttf::load_words<C>(x0, A + c0, tid);
+    nttf::fwd_ntt<C>(x0, lds, tw, nq, tid);
+    nttf::load_words<C>(x1, A + c0 + LN, tid);
+    nttf::fwd_ntt<C>(x1, lds, tw, nq, tid);
+    nttf::load_words<C>(y, B + c0, tid);
+    nttf::fwd_ntt<C>(y, lds, tw, nq, tid);
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
-        const uint64_t a0 = reduce64(x0[k], pc), a1 = reduce64(x1[k], pc), b0 = reduce64(y[k], pc);
-        o0[ntt::eval_index<C>(k, tid)] = csub(mul_shoup(mul_barrett(a0, b0, pc), pm, pms, pc.q), pc.q);
-        z[k] = mul_barrett(a1, b0, pc);
+        const double a0 = rmod(x0[k], qi, nq), a1 = rmod(x1[k], qi, nq), b0 = rmod(y[k], qi, nq);
+        o0[ntt::eval_index<C>(k, tid)] = d2u(canon(mulw(muld(a0, b0, qi, nq), pm, pmq, nq), qi, nq));
+        z[k] = muld(a1, b0, qi, nq);
         x0[k] = a0;
         x1[k] = a1;
     }
-    ntt::load_coalesced<C>(y, B + c0 + LN, tid);
-    ntt::fwd_ntt<C>(y, lds, tw, pc.q, pc.two_q, tid);
-    uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
+    nttf::load_words<C>(y, B + c0 + LN, tid);
+    nttf::fwd_ntt<C>(y, lds, tw, nq, tid);
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
-        const uint64_t b1 = reduce64(y[k], pc);
+#pragma clang fp contract(off)
+        const double b1 = rmod(y[k], qi, nq);
         const int idx = ntt::eval_index<C>(k, tid);
-        const uint64_t d1 = add_mod(z[k], mul_barrett(x0[k], b1, pc), pc.q);
-        o0[LN + idx] = csub(mul_shoup(d1, pm, pms, pc.q), pc.q);
-        const uint64_t d2 = mul_barrett(x1[k], b1, pc);
-        o2[idx] = d2;
+        const double d1 = z[k] + muld(x0[k], b1, qi, nq);
+        o0[LN + idx] = d2u(canon(mulw(d1, pm, pmq, nq), qi, nq));
+        const double d2 = muld(x1[k], b1, qi, nq);
+        o2[idx] = d2u(canon(d2, qi, nq));
         x0[k] = d2;
     }
-    ntt::inv_ntt<C>(x0, lds, tw + C::N, pc, tid);
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) x0[k] = csub(x0[k], pc.q);
-    ntt::store_coalesced<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, tid);
+    nttf::inv_ntt<C>(x0, lds, tw + C::N, pf, tid);
+    nttf::store_words<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, pf, tid);
 }
 
-// a0 += v * b_j,t and a1 += v * a_j,t for digit j in evaluation order (key
-// rows of (value, companion) pairs).  Each product is below 4q: with at most
-// 15 digits and a start below q the sums stay below 2^51.
+// a0 += v * b_j,t and a1 -= v * a_j,t for digit j in evaluation order (key
+// rows of (value, RN(value / q)) FP64 pairs).  v below 2^51; each product is
+// bounded by 5q/8, so with at most 15 digits and a start below q the sums
+// stay below 11 q.
 template <class C>
-__device__ __forceinline__ void key_mac(uint64_t (&a0)[C::E], uint64_t (&a1)[C::E], const uint64_t (&v)[C::E],
-                                        const ulonglong2* __restrict__ key, int j, int key_limbs, int kt,
-                                        uint64_t nq, int tid) {
-    const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
-    const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+__device__ __forceinline__ void key_mac(double (&a0)[C::E], double (&a1)[C::E], const double (&v)[C::E],
+                                        const double2* __restrict__ key, int j, int key_limbs, int kt, double nq,
+                                        int tid) {
+    const double2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+    const double2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
+#pragma clang fp contract(off)
         const int idx = ntt::eval_index<C>(k, tid);
-        const ulonglong2 wb = kb[idx], wa = ka[idx];
-        a0[k] += mul_shoup4_nq(v[k], wb.x, wb.y, nq);
-        a1[k] += mul_shoup4_nq(v[k], wa.x, wa.y, nq);
+        const double2 wb = kb[idx], wa = ka[idx];
+        a0[k] = a0[k] + mulw(v[k], wb.x, wb.y, nq);
+        a1[k] = a1[k] - mulw(v[k], wa.x, wa.y, nq);
     }
 }
 
@@ -210,6 +223,7 @@ __device__ __forceinline__ void fused_out(const uint64_t (&x)[C::E], const uint6
 // rtl/m3/keyswitch_pipe.sv:90-100) to one digit per RNS prime:
 //   acc0_t = [p d0]_t + sum_j NTT_t([d2]_(q_j)) * b_j,t
 //   acc1_t = [p d1]_t - sum_j NTT_t([d2]_(q_j)) * a_j,t
+// (FP64 pipeline: signed residues, key rows of (value, RN(value / q)) pairs)
 // then both accumulators go back to coefficient form.  The digit lift
 // ("ModUp") is the reduction of the digit's coefficients mod the target
 // prime on load; the digit NTTs, the key products and the inverse
@@ -226,57 +240,57 @@ __device__ __forceinline__ void fused_out(const uint64_t (&x)[C::E], const uint6
 template <int LOGN, int T, bool KEYTAB = false, bool FUSED = false>
 __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2c, size_t d2c_bstride,
                                                  const uint64_t* __restrict__ d2e,
-                                                 const uint64_t* __restrict__ d01, const ulonglong2* __restrict__ key,
+                                                 const uint64_t* __restrict__ d01, const double2* __restrict__ key,
                                                  int key_limbs, uint64_t* __restrict__ acc,
-                                                 const ulonglong2* __restrict__ tw_all,
+                                                 const double2* __restrict__ twf_all, const PrimeF* __restrict__ pfs,
                                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
                                                  int batch, int reduce_digits, const uint64_t* __restrict__ key_table,
                                                  int t0, uint64_t* __restrict__ out_ct, ModDownConst mc, int narrow) {
     using C = Cfg<LOGN, T>;
-    extern __shared__ uint64_t lds[];
+    extern __shared__ uint64_t lds_u[];
+    double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
     // Target-major order: workgroups in flight together share one target
     // limb, so each XCD's L2 holds one limb's key slice (2 dnum N 16 B) and
     // one prime's twiddles rather than the whole key.
     const int t = t0 + (int)(blockIdx.x / batch), b = blockIdx.x % batch;
-    if constexpr (KEYTAB) key = reinterpret_cast<const ulonglong2*>(key_table[b]);
+    if constexpr (KEYTAB) key = reinterpret_cast<const double2*>(key_table[b]);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
-    const PrimeConst pc = pcs[pt];
-    const uint64_t nq = (uint64_t)0 - pc.q;
-    const ulonglong2* tw = tw_all + (size_t)pt * 2 * C::N;
+    const PrimeF pf = pfs[pt];
+    const double nq = pf.nq;
+    const double2* tw = twf_all + (size_t)pt * 2 * C::N;
     const size_t LN = (size_t)l * C::N;
-    uint64_t a0[C::E], a1[C::E], x[C::E];
+    double a0[C::E], a1[C::E], x[C::E];
     if (t < l && d01) {
         const uint64_t* s0 = d01 + (size_t)b * 2 * LN + (size_t)t * C::N;
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
             const int j = ntt::eval_index<C>(k, tid);
-            a0[k] = s0[j];
-            const uint64_t v = s0[LN + j];
-            a1[k] = v ? pc.q - v : 0;   // acc1 holds -(p d1) + sum d a, negated at the end
+            a0[k] = u2d(s0[j]);
+            a1[k] = u2d(s0[LN + j]);   // acc1 = p d1 - sum d a
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0;
+        for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
     }
     for (int j = 0; j < l; ++j) {
         if (j == t && d2e) {
             // the digit of the target's own prime is already in evaluation form
             const uint64_t* s = d2e + (size_t)b * LN + (size_t)j * C::N;
 #pragma unroll
-            for (int k = 0; k < C::E; ++k) x[k] = s[ntt::eval_index<C>(k, tid)];
+            for (int k = 0; k < C::E; ++k) x[k] = u2d(s[ntt::eval_index<C>(k, tid)]);
         } else {
-            ntt::load_coalesced<C>(x, d2c + (size_t)b * d2c_bstride + (size_t)j * C::N, tid);
+            nttf::load_words<C>(x, d2c + (size_t)b * d2c_bstride + (size_t)j * C::N, tid);
             // the digit's residues mod q_j enter the transform mod q_t as they
             // are when q_j < 2 q_t (reduce_digits = 0); else reduce them first
             if (reduce_digits) {
 #pragma unroll
-                for (int k = 0; k < C::E; ++k) x[k] = reduce64(x[k], pc);
+                for (int k = 0; k < C::E; ++k) x[k] = rmod(x[k], pf.qinv, nq);
             }
-            // left unreduced (below 2^51): the key product is exact for any such input
-            if constexpr (ntt::pair_pays<C>()) ntt::fwd_ntt_db<C>(x, lds, tw, pc.q, pc.two_q, tid);
-            else ntt::fwd_ntt<C>(x, lds, tw, pc.q, pc.two_q, tid);
+            // left unreduced (below 11 q): the key product is exact for any such input
+            if constexpr (ntt::pair_pays<C>()) nttf::fwd_ntt_db<C>(x, lds, tw, nq, tid);
+            else nttf::fwd_ntt<C>(x, lds, tw, nq, tid);
         }
         key_mac<C>(a0, a1, x, key, j, key_limbs, kt, nq, tid);
     }
@@ -284,45 +298,42 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     if constexpr (ntt::pair_pays<C>()) __syncthreads();
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
-        a0[k] = reduce64(a0[k], pc);
-        const uint64_t v = reduce64(a1[k], pc);
-        a1[k] = v ? pc.q - v : 0;
+        a0[k] = rmod(a0[k], pf.qinv, nq);
+        a1[k] = rmod(a1[k], pf.qinv, nq);
     }
     uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
     const uint64_t* acc_b = acc + (size_t)b * 2 * (l + 1) * C::N;
     uint64_t* out_b = FUSED ? out_ct + ((size_t)b * 2 * (l - 1) + t) * C::N : nullptr;
-    const ulonglong2* itw = tw + C::N;
+    const double2* itw = tw + C::N;
+    uint64_t u[C::E];
     if constexpr (ntt::pair_pays<C>()) {
         // both accumulators as one transform pair
-        ntt::inv_ntt2<C>(a0, a1, lds, itw, pc, tid);
+        nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid);
 #pragma unroll
-        for (int k = 0; k < C::E; ++k) {
-            a0[k] = csub(a0[k], pc.q);
-            a1[k] = csub(a1[k], pc.q);
-        }
-        if constexpr (FUSED) {
-            fused_out<C>(a0, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
-            fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map,
-                         pidx_p, mc, narrow, tid);
-        } else {
-            ntt::store_coalesced<C>(a0, out, tid);
-            ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
-        }
+        for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a0[k], pf.qinv, nq));
+        if constexpr (FUSED) fused_out<C>(u, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
+        else ntt::store_coalesced<C>(u, out, tid);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a1[k], pf.qinv, nq));
+        if constexpr (FUSED)
+            fused_out<C>(u, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map, pidx_p,
+                         mc, narrow, tid);
+        else ntt::store_coalesced<C>(u, out + (size_t)(l + 1) * C::N, tid);
         return;
     }
-    ntt::inv_ntt<C>(a0, lds, itw, pc, tid);
+    nttf::inv_ntt<C>(a0, lds, itw, pf, tid);
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) a0[k] = csub(a0[k], pc.q);
-    if constexpr (FUSED) fused_out<C>(a0, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
-    else ntt::store_coalesced<C>(a0, out, tid);
+    for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a0[k], pf.qinv, nq));
+    if constexpr (FUSED) fused_out<C>(u, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
+    else ntt::store_coalesced<C>(u, out, tid);
     __syncthreads();
-    ntt::inv_ntt<C>(a1, lds, itw, pc, tid);
+    nttf::inv_ntt<C>(a1, lds, itw, pf, tid);
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) a1[k] = csub(a1[k], pc.q);
+    for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a1[k], pf.qinv, nq));
     if constexpr (FUSED)
-        fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map, pidx_p, mc,
+        fused_out<C>(u, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map, pidx_p, mc,
                      narrow, tid);
-    else ntt::store_coalesced<C>(a1, out + (size_t)(l + 1) * C::N, tid);
+    else ntt::store_coalesced<C>(u, out + (size_t)(l + 1) * C::N, tid);
 }
 
 // ModDown by p (round-half-up of X/p), optional addition of coefficient-form
@@ -681,7 +692,7 @@ __global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restri
 // the diagonal (Halevi-Shoup) matrix-vector product:
 //   out[g][c][poly][i][n] = sum_b E[b][c][poly][i][n] * D[g][b][i][n]  (mod q_i)
 // E: [nb][B][2][l][N] evaluation-form ciphertext rows, D: [ng][nb][l][N]
-// (value, Shoup companion) plaintext rows shared by the whole batch.
+// (value, RN(value / q)) FP64 plaintext pairs shared by the whole batch.
 // With scalar = 1, D is [ng][nb][l]: one integer per (g, b, limb), the
 // same product in either domain (a constant plaintext), so E may then be in
 // coefficient form.
@@ -692,8 +703,8 @@ __global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restri
 // of the last block are skipped.
 template <int NG, int R>
 __global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict__ E,
-                                                     const ulonglong2* __restrict__ D, uint64_t* __restrict__ out,
-                                                     const PrimeConst* __restrict__ pcs, LimbMap map, int nb, int ng,
+                                                     const double2* __restrict__ D, uint64_t* __restrict__ out,
+                                                     const PrimeF* __restrict__ pfs, LimbMap map, int nb, int ng,
                                                      int rows, int l, int logN, size_t total, int scalar) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= total) return;
@@ -701,28 +712,31 @@ __global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict_
     const size_t n = gid & (N - 1), rest = gid >> logN;
     const int i = (int)(rest % l);
     const int r0 = (int)(rest / l) * R;
-    const PrimeConst& pc = pcs[map.idx[i]];
-    const uint64_t nq = (uint64_t)0 - pc.q;
+    const PrimeF& pf = pfs[map.idx[i]];
+    const double nq = pf.nq;
     const size_t dn = scalar ? 1 : N;
     const size_t dstride = (size_t)l * dn;
-    const ulonglong2* dbase = D + (size_t)i * dn + (scalar ? 0 : n);
+    const double2* dbase = D + (size_t)i * dn + (scalar ? 0 : n);
     const uint64_t* ebase = E + (size_t)r0 * LN + (size_t)i * N + n;
     const size_t brow = (size_t)rows * LN;   // one b (or g) block of rows
-    uint64_t acc[NG][R];
+    double acc[NG][R];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[g][r] = 0;
+        for (int r = 0; r < R; ++r) acc[g][r] = 0.0;
     for (int b = 0; b < nb; ++b) {
-        uint64_t e[R];
+        double e[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) e[r] = r0 + r < rows ? ebase[(size_t)b * brow + (size_t)r * LN] : 0;
+        for (int r = 0; r < R; ++r) e[r] = r0 + r < rows ? u2d(ebase[(size_t)b * brow + (size_t)r * LN]) : 0.0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             if (g < ng) {
-                const ulonglong2 w = dbase[((size_t)g * nb + b) * dstride];
+                const double2 w = dbase[((size_t)g * nb + b) * dstride];
 #pragma unroll
-                for (int r = 0; r < R; ++r) acc[g][r] += mul_shoup4_nq(e[r], w.x, w.y, nq);   // nb <= 32: < 2^52
+                for (int r = 0; r < R; ++r) {
+#pragma clang fp contract(off)
+                    acc[g][r] = acc[g][r] + mulw(e[r], w.x, w.y, nq);   // nb <= 32: below 20 q
+                }
             }
         }
     }
@@ -732,7 +746,8 @@ __global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict_
         if (g < ng) {
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                if (r0 + r < rows) obase[(size_t)g * brow + (size_t)r * LN] = reduce64(acc[g][r], pc);
+                if (r0 + r < rows)
+                    obase[(size_t)g * brow + (size_t)r * LN] = d2u(canon(rmod(acc[g][r], pf.qinv, nq), pf.qinv, nq));
         }
     }
 }
@@ -772,7 +787,7 @@ hipError_t run_tensor(const DevTables& t, const uint64_t* ea, const uint64_t* eb
     hipError_t e = set_lds(k_tensor_intt<LOGN, T>, C::LDS_BYTES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_tensor_intt<LOGN, T>), dim3(batch * l), dim3(T), C::LDS_BYTES, s, ea, eb, d01, d2e, d2c,
-                       t.tw, t.pc, map, l, mc);
+                       t.twf, t.pf, map, l, mc);
     return hipGetLastError();
 }
 
@@ -783,14 +798,14 @@ hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, u
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
     hipError_t e = set_lds(k_front<LOGN, T>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(batch * l), dim3(T), lds, s, a, b, d01, d2e, d2c, t.tw,
-                       t.pc, map, l, batch, mc);
+    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(batch * l), dim3(T), lds, s, a, b, d01, d2e, d2c, t.twf,
+                       t.pf, map, l, batch, mc);
     return hipGetLastError();
 }
 
 template <int LOGN, int T>
 hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uint64_t* d2e, const uint64_t* d01,
-                  const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
+                  const double2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
                   int pidx_p, const uint64_t* key_table, const KsTargets& tg, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
@@ -802,8 +817,8 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
     ModDownConst mc;
     if (tg.mc) mc = *tg.mc;
     else memset(&mc, 0, sizeof(mc));
-    hipLaunchKernelGGL(k, dim3(batch * nt), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.tw,
-                       t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table, tg.first, tg.out, mc, t.narrow);
+    hipLaunchKernelGGL(k, dim3(batch * nt), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.twf,
+                       t.pf, t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table, tg.first, tg.out, mc, t.narrow);
     return hipGetLastError();
 }
 
@@ -849,7 +864,7 @@ hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b
 
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01,
-                            const ulonglong2* key, int key_limbs, uint64_t* acc, int batch, int l,
+                            const double2* key, int key_limbs, uint64_t* acc, int batch, int l,
                             const LimbMap& map, int pidx_p, int threads, hipStream_t s, const uint64_t* key_table,
                             const KsTargets& tg) {
     if (batch * l == 0) return hipSuccess;
@@ -1001,17 +1016,17 @@ hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64
 namespace {
 
 template <int NG, int R>
-hipError_t run_lincomb_blk(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
+hipError_t run_lincomb_blk(const DevTables& t, const uint64_t* E, const double2* D, uint64_t* out, int nb, int ng,
                            int rows, int l, const LimbMap& map, int scalar, hipStream_t s) {
     const size_t total = ((size_t)((rows + R - 1) / R) * l) << t.logN;
     hipLaunchKernelGGL((k_lincomb_blk<NG, R>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, E, D, out,
-                       t.pc, map, nb, ng, rows, l, t.logN, total, scalar);
+                       t.pf, map, nb, ng, rows, l, t.logN, total, scalar);
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
+hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const double2* D, uint64_t* out, int nb, int ng,
                           int batch, int l, const LimbMap& map, int scalar, hipStream_t s) {
     const int rows = batch * 2;
     if (!rows || !ng || !l) return hipSuccess;
@@ -1020,7 +1035,7 @@ hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong
     // giant steps in chunks of at most 16
     for (int g0 = 0; g0 < ng; g0 += 16) {
         const int gc = ng - g0 < 16 ? ng - g0 : 16;
-        const ulonglong2* Dg = D + (size_t)g0 * nb * l * dn;
+        const double2* Dg = D + (size_t)g0 * nb * l * dn;
         uint64_t* og = out + (size_t)g0 * rows * LN;
         hipError_t e;
         if (gc <= 4) e = run_lincomb_blk<4, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);

@@ -20,6 +20,7 @@
 // 2 brv(j(i)) + 1 = g (2 brv(i) + 1) mod 2N.
 #include "launch.h"
 #include "ntt_core.h"
+#include "ntt_f64.h"
 
 using ntt::Cfg;
 
@@ -60,63 +61,53 @@ __global__ void __launch_bounds__(T) k_modup(const uint64_t* __restrict__ ct, ui
     ntt::store_eval<C>(x, lds, D + (((size_t)b * l + j) * (l + 1) + t) * C::N, tid);
 }
 
-// Reduce (and negate) an accumulator whose register k holds evaluation slot
-// k T + tid (the coalesced order the key products are summed in), apply
-// pi_g through LDS, inverse-transform and store it in coefficient form.
+// Reduce an accumulator whose register k holds evaluation slot k T + tid
+// (the coalesced order the key products are summed in; FP64, below 10 q),
+// apply pi_g through LDS, inverse-transform and store it in coefficient form.
 template <class C>
-__device__ __forceinline__ void permute_inverse(uint64_t (&a)[C::E], bool negate, uint32_t g, uint64_t* lds,
-                                                const ulonglong2* __restrict__ itw, const PrimeConst& pc,
+__device__ __forceinline__ void permute_inverse(double (&a)[C::E], uint32_t g, double* lds,
+                                                const double2* __restrict__ itw, const PrimeF& pf,
                                                 uint64_t* __restrict__ out, int tid) {
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-        const uint64_t v = reduce64(a[k], pc);
-        lds[ntt::pad<C>(k * C::T + tid)] = (negate && v) ? pc.q - v : v;
-    }
+    for (int k = 0; k < C::E; ++k) lds[ntt::pad<C>(k * C::T + tid)] = rmod(a[k], pf.qinv, pf.nq);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < C::E; ++k) a[k] = lds[ntt::pad<C>(eval_perm(ntt::eval_index<C>(k, tid), g, C::LOGN))];
     __syncthreads();
-    ntt::inv_ntt<C>(a, lds, itw, pc, tid);
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) a[k] = csub(a[k], pc.q);
-    ntt::store_coalesced<C>(a, out, tid);
+    nttf::inv_ntt<C>(a, lds, itw, pf, tid);
+    nttf::store_words<C>(a, out, pf, tid);
     __syncthreads();
 }
 
-// Reduce (and negate) an accumulator whose register k holds evaluation slot
-// k T + tid and
-// store pi_g of it in evaluation order: one LDS round trip, coalesced stores.
+// Reduce an accumulator whose register k holds evaluation slot k T + tid
+// and store pi_g of it in evaluation order: one LDS round trip, coalesced
+// stores.
 template <class C>
-__device__ __forceinline__ void permute_store_eval(uint64_t (&a)[C::E], bool negate, uint32_t g, uint64_t* lds,
-                                                   const PrimeConst& pc, uint64_t* __restrict__ out, int tid) {
+__device__ __forceinline__ void permute_store_eval(const double (&a)[C::E], uint32_t g, double* lds,
+                                                   const PrimeF& pf, uint64_t* __restrict__ out, int tid) {
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-        const uint64_t v = reduce64(a[k], pc);
-        lds[ntt::pad<C>(k * C::T + tid)] = (negate && v) ? pc.q - v : v;
-    }
+    for (int k = 0; k < C::E; ++k) lds[ntt::pad<C>(k * C::T + tid)] = rmod(a[k], pf.qinv, pf.nq);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < C::E; ++e) {
         const int j = e * C::T + tid;
-        out[j] = lds[ntt::pad<C>(eval_perm(j, g, C::LOGN))];
+        out[j] = d2u(canon(lds[ntt::pad<C>(eval_perm(j, g, C::LOGN))], pf.qinv, pf.nq));
     }
     __syncthreads();
 }
 
-// permute_inverse for both accumulators at once (the second negated): two
-// LDS buffers, one gather each, one transform pair.
+// permute_inverse for both accumulators at once: two LDS buffers, one
+// gather each, one transform pair.
 template <class C>
-__device__ __forceinline__ void permute_inverse2(uint64_t (&a0)[C::E], uint64_t (&a1)[C::E], uint32_t g,
-                                                 uint64_t* lds, const ulonglong2* __restrict__ itw,
-                                                 const PrimeConst& pc, uint64_t* __restrict__ out0,
-                                                 uint64_t* __restrict__ out1, int tid) {
-    uint64_t* lds1 = lds + C::LDS_WORDS;
+__device__ __forceinline__ void permute_inverse2(double (&a0)[C::E], double (&a1)[C::E], uint32_t g, double* lds,
+                                                 const double2* __restrict__ itw, const PrimeF& pf,
+                                                 uint64_t* __restrict__ out0, uint64_t* __restrict__ out1, int tid) {
+    double* lds1 = lds + C::LDS_WORDS;
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
         const int slot = ntt::pad<C>(k * C::T + tid);
-        lds[slot] = reduce64(a0[k], pc);
-        const uint64_t v = reduce64(a1[k], pc);
-        lds1[slot] = v ? pc.q - v : 0;
+        lds[slot] = rmod(a0[k], pf.qinv, pf.nq);
+        lds1[slot] = rmod(a1[k], pf.qinv, pf.nq);
     }
     __syncthreads();
 #pragma unroll
@@ -126,66 +117,63 @@ __device__ __forceinline__ void permute_inverse2(uint64_t (&a0)[C::E], uint64_t 
         a1[k] = lds1[src];
     }
     __syncthreads();
-    ntt::inv_ntt2<C>(a0, a1, lds, itw, pc, tid);
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-        a0[k] = csub(a0[k], pc.q);
-        a1[k] = csub(a1[k], pc.q);
-    }
-    ntt::store_coalesced<C>(a0, out0, tid);
-    ntt::store_coalesced<C>(a1, out1, tid);
+    nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid);
+    nttf::store_words<C>(a0, out0, pf, tid);
+    nttf::store_words<C>(a1, out1, pf, tid);
 }
 
 // One workgroup per (target t, ciphertext b, rotation r), placed by
-// hoist_slot.  keys: [R][dnum][2][key_limbs][N]
-// (value, companion) pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N]
-// coefficient form, the layout ckks_relin's ModDown reads.
+// hoist_slot.  keys: [R][dnum][2][key_limbs][N] (value, RN(value / q)) FP64
+// pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N] coefficient form, the
+// layout ckks_relin's ModDown reads.  acc1 = -(sum D a) directly.
 template <int LOGN, int T, bool EVAL>
-__global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D, const ulonglong2* __restrict__ keys,
+__global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D, const double2* __restrict__ keys,
                                                   size_t key_stride, int key_limbs, GaloisList galois,
-                                                  uint64_t* __restrict__ acc, const ulonglong2* __restrict__ tw_all,
-                                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
+                                                  uint64_t* __restrict__ acc, const double2* __restrict__ twf_all,
+                                                  const PrimeF* __restrict__ pfs, LimbMap map, int l, int pidx_p,
                                                   int batch, int R, HoistOrder order) {
     using C = Cfg<LOGN, T>;
-    extern __shared__ uint64_t lds[];
+    extern __shared__ uint64_t lds_u[];
+    double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
     const TRB s = hoist_slot(blockIdx.x, gridDim.x, R, batch, order);
     const int b = s.b, r = s.r, t = s.t;
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
-    const PrimeConst pc = pcs[pt];
-    const uint64_t nq = (uint64_t)0 - pc.q;
-    const ulonglong2* key = keys + (size_t)r * key_stride;
-    uint64_t a0[C::E], a1[C::E];
+    const PrimeF pf = pfs[pt];
+    const double nq = pf.nq;
+    const double2* key = keys + (size_t)r * key_stride;
+    double a0[C::E], a1[C::E];
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0;
+    for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
     for (int j = 0; j < l; ++j) {
         const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
-        const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
-        const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+        const double2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+        const double2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
+#pragma clang fp contract(off)
             const int idx = k * C::T + tid;          // coalesced: the sum is slot by slot
-            const uint64_t x = d[idx];
-            const ulonglong2 wb = kb[idx], wa = ka[idx];
-            a0[k] += mul_shoup4_nq(x, wb.x, wb.y, nq);   // < 4q each, l <= 15 digits
-            a1[k] += mul_shoup4_nq(x, wa.x, wa.y, nq);
+            const double x = u2d(d[idx]);
+            const double2 wb = kb[idx], wa = ka[idx];
+            a0[k] = a0[k] + mulw(x, wb.x, wb.y, nq);   // 5q/8 each, l <= 15 digits
+            a1[k] = a1[k] - mulw(x, wa.x, wa.y, nq);
         }
     }
     const uint32_t g = galois.g[r];
-    const ulonglong2* itw = tw_all + ((size_t)pt * 2 + 1) * C::N;
+    const double2* itw = twf_all + ((size_t)pt * 2 + 1) * C::N;
     uint64_t* out = acc + (((size_t)r * batch + b) * 2 * (l + 1) + t) * C::N;
     if (EVAL && t < l) {
         // chain limbs stay in evaluation form; ModDown finishes there
-        permute_store_eval<C>(a0, false, g, lds, pc, out, tid);
-        permute_store_eval<C>(a1, true, g, lds, pc, out + (size_t)(l + 1) * C::N, tid);
+        permute_store_eval<C>(a0, g, lds, pf, out, tid);
+        permute_store_eval<C>(a1, g, lds, pf, out + (size_t)(l + 1) * C::N, tid);
         return;
     }
     if constexpr (ntt::pair_pays<C>()) {
-        permute_inverse2<C>(a0, a1, g, lds, itw, pc, out, out + (size_t)(l + 1) * C::N, tid);
+        permute_inverse2<C>(a0, a1, g, lds, itw, pf, out, out + (size_t)(l + 1) * C::N, tid);
     } else {
-        permute_inverse<C>(a0, false, g, lds, itw, pc, out, tid);
-        permute_inverse<C>(a1, true, g, lds, itw, pc, out + (size_t)(l + 1) * C::N, tid);
+        permute_inverse<C>(a0, g, lds, itw, pf, out, tid);
+        permute_inverse<C>(a1, g, lds, itw, pf, out + (size_t)(l + 1) * C::N, tid);
     }
 }
 
@@ -268,7 +256,7 @@ hipError_t run_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int ba
 }
 
 template <int LOGN, int T>
-hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
+hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const double2* keys, size_t key_stride,
                           int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l, const LimbMap& map,
                           int pidx_p, int R, int eval_out, HoistOrder order, hipStream_t s) {
     using C = Cfg<LOGN, T>;
@@ -277,7 +265,7 @@ hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, keys, key_stride, key_limbs, galois, acc,
-                       t.tw, t.pc, map, l, pidx_p, batch, R, order);
+                       t.twf, t.pf, map, l, pidx_p, batch, R, order);
     return hipGetLastError();
 }
 
@@ -310,7 +298,7 @@ hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int
     }
 }
 
-hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
+hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const double2* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
                              const LimbMap& map, int pidx_p, int R, int eval_out, HoistOrder order, int threads,
                              hipStream_t s) {

@@ -5,6 +5,7 @@
 // map.idx[i].  One workgroup transforms one (poly, limb) row.
 #include "launch.h"
 #include "ntt_core.h"
+#include "ntt_f64.h"
 
 namespace {
 
@@ -95,6 +96,39 @@ __global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, c
 #pragma unroll
     for (int e = 0; e < C::E; ++e) x[e] = csub(x[e], pc.q);
     ntt::store_coalesced<C>(x, out + off, tid);
+}
+
+// k_polymul on the FP64 pipeline (modf64.h): the same three phases, the
+// same words out.  Operands below q; the point-wise products take both
+// transforms reduced to (q + 1) / 2, so every product is exact.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_polymul_f(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
+                                                 uint64_t* __restrict__ out, const double2* __restrict__ twf_all,
+                                                 const PrimeF* __restrict__ pfs, LimbMap map, int n_limbs) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    double* ldsf = reinterpret_cast<double*>(lds);
+    const int tid = threadIdx.x;
+    const int limb = blockIdx.x % n_limbs;
+    const int p = map.idx[limb];
+    const PrimeF pf = pfs[p];
+    const size_t off = (size_t)blockIdx.x * C::N;
+    const double2* tw = twf_all + (size_t)p * 2 * C::N;
+    double x[C::E], y[C::E];
+    nttf::load_words<C>(x, A + off, tid);
+    nttf::load_words<C>(y, B + off, tid);
+    if constexpr (ntt::pair_pays<C>()) {
+        nttf::fwd_ntt2<C>(x, y, ldsf, tw, pf.nq, tid);
+    } else {
+        nttf::fwd_ntt<C>(x, ldsf, tw, pf.nq, tid);
+        nttf::fwd_ntt<C>(y, ldsf, tw, pf.nq, tid);
+    }
+#pragma unroll
+    for (int e = 0; e < C::E; ++e)
+        x[e] = muld(rmod(x[e], pf.qinv, pf.nq), rmod(y[e], pf.qinv, pf.nq), pf.qinv, pf.nq);
+    if constexpr (ntt::pair_pays<C>()) nttf::inv_ntt_db<C>(x, ldsf, tw + C::N, pf, tid);
+    else nttf::inv_ntt<C>(x, ldsf, tw + C::N, pf, tid);
+    nttf::store_words<C>(x, out + off, pf, tid);
 }
 
 // Generic transform for small rings (N <= 2048): one stage per barrier.
@@ -210,24 +244,73 @@ __global__ void k_pointwise(const uint64_t* __restrict__ a, const uint64_t* __re
     out[g] = r;
 }
 
-// (w, floor(w 2^63 / q)) pairs for key material; w < q < 2^45, the quotient
-// by restoring division.
-__global__ void k_shoup_pairs(const uint64_t* __restrict__ in, ulonglong2* __restrict__ out,
-                              const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, int logN, size_t total) {
+// (w, RN(w / q)) FP64 pairs of reduced words w < q: the prepared form of
+// key and plaintext rows.  The double division is correctly rounded (IEEE
+// 754, as the gfx950 division sequence computes it).
+__global__ void k_fp64_pairs(const uint64_t* __restrict__ in, double2* __restrict__ out,
+                             const PrimeF* __restrict__ pfs, LimbMap map, int n_limbs, int logN, size_t total) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
-    const uint64_t q = pcs[map.idx[(g >> logN) % n_limbs]].q;
-    const uint64_t w = in[g];
-    uint64_t r = w, quo = 0;
-    for (int k = 0; k < 63; ++k) {
-        r <<= 1;
-        quo <<= 1;
-        if (r >= q) {
-            r -= q;
-            quo |= 1;
-        }
+    const double q = pfs[map.idx[(g >> logN) % n_limbs]].q;
+    const double w = u2d(in[g]);
+    out[g] = make_double2(w, w / q);
+}
+
+// Batched transforms on the FP64 pipeline, the same words as k_ntt_fwd /
+// k_ntt_inv.  reduce_in: inputs are any 64-bit words, reduced exactly by
+// the integer reduce64 first; otherwise below 2q, the integer kernels'
+// contract (the inverse reduces its inputs on load either way).
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_ntt_fwd_f(uint64_t* dst, const uint64_t* src, const double2* __restrict__ twf_all,
+                                                 const PrimeF* __restrict__ pfs, const PrimeConst* __restrict__ pcs,
+                                                 LimbMap map, int n_limbs, int reduce_in) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    double* ldsf = reinterpret_cast<double*>(lds);
+    const int tid = threadIdx.x;
+    const int p = map.idx[blockIdx.x % n_limbs];
+    const PrimeF pf = pfs[p];
+    const size_t off = (size_t)blockIdx.x * C::N;
+    uint64_t u[C::E];
+    ntt::load_coalesced<C>(u, src + off, tid);
+    if (reduce_in) {
+        const PrimeConst pc = pcs[p];
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) u[e] = reduce64(u[e], pc);
     }
-    out[g] = make_ulonglong2(w, quo);
+    double x[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = u2d(u[e]);
+    nttf::fwd_ntt<C>(x, ldsf, twf_all + (size_t)p * 2 * C::N, pf.nq, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) u[e] = d2u(canon(x[e], pf.qinv, pf.nq));
+    __syncthreads();
+    ntt::store_eval<C>(u, lds, dst + off, tid);
+}
+
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_ntt_inv_f(uint64_t* dst, const uint64_t* src, const double2* __restrict__ twf_all,
+                                                 const PrimeF* __restrict__ pfs, const PrimeConst* __restrict__ pcs,
+                                                 LimbMap map, int n_limbs, int reduce_in) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    double* ldsf = reinterpret_cast<double*>(lds);
+    const int tid = threadIdx.x;
+    const int p = map.idx[blockIdx.x % n_limbs];
+    const PrimeF pf = pfs[p];
+    const size_t off = (size_t)blockIdx.x * C::N;
+    uint64_t u[C::E];
+    ntt::load_eval<C>(u, lds, src + off, tid);
+    if (reduce_in) {
+        const PrimeConst pc = pcs[p];
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) u[e] = reduce64(u[e], pc);
+    }
+    double x[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = rmod(u2d(u[e]), pf.qinv, pf.nq);
+    nttf::inv_ntt<C>(x, ldsf, twf_all + ((size_t)p * 2 + 1) * C::N, pf, tid);
+    nttf::store_words<C>(x, dst + off, pf, tid);
 }
 
 template <typename K>
@@ -239,6 +322,14 @@ template <int LOGN, int T>
 hipError_t run_ntt(const DevTables& t, uint64_t* dst, const uint64_t* src, int rows, int n_limbs, const LimbMap& map, bool inv,
                    bool red, hipStream_t s) {
     using C = Cfg<LOGN, T>;
+    if (t.fp64) {
+        auto k = inv ? k_ntt_inv_f<LOGN, T> : k_ntt_fwd_f<LOGN, T>;
+        hipError_t e = set_lds(k, C::LDS_BYTES);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3(rows), dim3(T), C::LDS_BYTES, s, dst, src, t.twf, t.pf, t.pc, map, n_limbs,
+                           red ? 1 : 0);
+        return hipGetLastError();
+    }
     auto k = inv ? k_ntt_inv<LOGN, T> : k_ntt_fwd<LOGN, T>;
     hipError_t e = set_lds(k, C::LDS_BYTES);
     if (e != hipSuccess) return e;
@@ -254,6 +345,18 @@ hipError_t run_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b,
     hipError_t e = set_lds(k_polymul<LOGN, T>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_polymul<LOGN, T>), dim3(rows), dim3(T), lds, s, a, b, out, t.tw, t.pc, map,
+                       n_limbs);
+    return hipGetLastError();
+}
+
+template <int LOGN, int T>
+hipError_t run_polymul_f(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out, int rows,
+                         int n_limbs, const LimbMap& map, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    hipError_t e = set_lds(k_polymul_f<LOGN, T>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_polymul_f<LOGN, T>), dim3(rows), dim3(T), lds, s, a, b, out, t.twf, t.pf, map,
                        n_limbs);
     return hipGetLastError();
 }
@@ -293,6 +396,19 @@ hipError_t launch_polymul(const DevTables& t, const uint64_t* a, const uint64_t*
     const int rows = n_polys * n_limbs;
     if (rows == 0) return hipSuccess;
     if (!threads) threads = default_threads(t.logN);
+    if (t.fp64) {
+        switch (t.logN * 10000 + threads) {
+            case 120128: return run_polymul_f<12, 128>(t, a, b, out, rows, n_limbs, map, s);
+            case 120256: return run_polymul_f<12, 256>(t, a, b, out, rows, n_limbs, map, s);
+            case 120512: return run_polymul_f<12, 512>(t, a, b, out, rows, n_limbs, map, s);
+            case 130256: return run_polymul_f<13, 256>(t, a, b, out, rows, n_limbs, map, s);
+            case 130512: return run_polymul_f<13, 512>(t, a, b, out, rows, n_limbs, map, s);
+            case 131024: return run_polymul_f<13, 1024>(t, a, b, out, rows, n_limbs, map, s);
+            case 140512: return run_polymul_f<14, 512>(t, a, b, out, rows, n_limbs, map, s);
+            case 141024: return run_polymul_f<14, 1024>(t, a, b, out, rows, n_limbs, map, s);
+            default: break;
+        }
+    }
     switch (t.logN * 10000 + threads) {
         case 120128: return run_polymul<12, 128>(t, a, b, out, rows, n_limbs, map, s);
         case 120256: return run_polymul<12, 256>(t, a, b, out, rows, n_limbs, map, s);
@@ -328,11 +444,11 @@ hipError_t launch_pointwise(const DevTables& t, const uint64_t* a, const uint64_
     return hipGetLastError();
 }
 
-hipError_t launch_shoup_pairs(const DevTables& t, const uint64_t* in, ulonglong2* out, int n_polys, int n_limbs,
-                              const LimbMap& map, hipStream_t s) {
+hipError_t launch_fp64_pairs(const DevTables& t, const uint64_t* in, double2* out, int n_polys, int n_limbs,
+                             const LimbMap& map, hipStream_t s) {
     const size_t total = ((size_t)n_polys * n_limbs) << t.logN;
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_shoup_pairs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pc, map,
+    hipLaunchKernelGGL(k_fp64_pairs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pf, map,
                        n_limbs, t.logN, total);
     return hipGetLastError();
 }

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "modarith.h"
+#include "modf64.h"
 
 struct DevTables {
     const PrimeConst* pc;   // [K]
@@ -15,6 +16,13 @@ struct DevTables {
     // prime is then below 2q for any other q, which the lazy forward
     // transform accepts as input without a reduction.
     int narrow = 0;
+    // The FP64 pipeline's tables (modf64.h): per-prime constants and
+    // (w, RN(w / q)) twiddle pairs in the order of tw.
+    const PrimeF* pf = nullptr;   // [K]
+    const double2* twf = nullptr; // [K][2][N]
+    // Arithmetic of the transform kernels that have both forms: 0 = 64-bit
+    // integer words, 1 = FP64 (results identical).
+    int fp64 = 0;
 };
 
 // Cumulative table of |e| for the rounded Gaussian: cdf[k] = floor(2^64
@@ -34,8 +42,10 @@ hipError_t launch_ntt(const DevTables& t, uint64_t* dst, const uint64_t* src, in
                       bool inverse, bool reduce_input, int threads, hipStream_t s);
 hipError_t launch_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out,
                           int n_polys, int n_limbs, const LimbMap& map, int threads, hipStream_t s);
-hipError_t launch_shoup_pairs(const DevTables& t, const uint64_t* in, ulonglong2* out, int n_polys, int n_limbs,
-                              const LimbMap& map, hipStream_t s);
+// (value, RN(value / q)) FP64 pairs of reduced words: the prepared form of
+// keys and plaintexts read by the FP64 key switch and linear combination.
+hipError_t launch_fp64_pairs(const DevTables& t, const uint64_t* in, double2* out, int n_polys, int n_limbs,
+                             const LimbMap& map, hipStream_t s);
 hipError_t launch_bitrev(uint64_t* data, int n_rows, int logN, hipStream_t s);
 hipError_t launch_pointwise(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out,
                             int n_polys, int n_limbs, const LimbMap& map, int op, hipStream_t s);
@@ -52,6 +62,8 @@ struct ModDownConst {
     uint64_t qinv_s[CKKS_MAX_PRIMES];
     uint64_t p_mod[CKKS_MAX_PRIMES];   // p mod q_i, and Shoup companion
     uint64_t p_mod_s[CKKS_MAX_PRIMES];
+    double pmf[CKKS_MAX_PRIMES];       // p mod q_i as a double, and RN(pmf / q_i) (FP64 pipeline)
+    double pmf_q[CKKS_MAX_PRIMES];
 };
 
 hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint64_t* eb, uint64_t* d01,
@@ -74,7 +86,7 @@ struct KsTargets {
 // key_table (optional): ciphertext b uses the key at device address
 // key_table[b] instead of key.
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
-                            const uint64_t* d01, const ulonglong2* key, int key_limbs, uint64_t* acc,
+                            const uint64_t* d01, const double2* key, int key_limbs, uint64_t* acc,
                             int batch, int l, const LimbMap& map, int pidx_special, int threads,
                             hipStream_t s, const uint64_t* key_table = nullptr, const KsTargets& tg = KsTargets());
 hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
@@ -92,7 +104,7 @@ hipError_t launch_automorphism_groups(const DevTables& t, const uint64_t* in, ui
                                       hipStream_t s, const uint64_t* gal_table = nullptr, int gal_div = 1);
 hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
                              int ell, hipStream_t s);
-hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb, int ng,
+hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const double2* D, uint64_t* out, int nb, int ng,
                           int batch, int l, const LimbMap& map, int scalar, hipStream_t s);
 hipError_t launch_lift_small(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, int l_in, int l_out,
                              const LimbMap& map, hipStream_t s);
@@ -188,7 +200,7 @@ hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int
 // limb's in coefficient form as always.
 // order: workgroup placement (hoist_slot); btile 0 picks the measured
 // default for the output form, btile above batch means batch.
-hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const ulonglong2* keys, size_t key_stride,
+hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const double2* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
                              const LimbMap& map, int pidx_p, int R, int eval_out, HoistOrder order, int threads,
                              hipStream_t s);

@@ -235,20 +235,22 @@ struct Lay {
     static constexpr int SG = RAW_G + (RAW_G >> K);
 };
 
-template <class C, int P, int K = C::PS>
-__device__ __forceinline__ void to_lds(const uint64_t (&x)[C::E], uint64_t* lds, int tid) {
+// W is the 8-byte word of the registers and the LDS: uint64_t (integer
+// residues) or double (residues held as exact FP64 integers).
+template <class C, int P, int K = C::PS, class W = uint64_t>
+__device__ __forceinline__ void to_lds(const W (&x)[C::E], W* lds, int tid) {
     constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
-    uint64_t* base = lds + pad<C, K>(index_of<C, P>(0, 0, tid));
+    W* base = lds + pad<C, K>(index_of<C, P>(0, 0, tid));
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
         for (int e = 0; e < M; ++e) base[g * Lay<C, P, K>::SG + e * Lay<C, P, K>::SE] = x[g * M + e];
 }
 
-template <class C, int P, int K = C::PS>
-__device__ __forceinline__ void from_lds(uint64_t (&x)[C::E], const uint64_t* lds, int tid) {
+template <class C, int P, int K = C::PS, class W = uint64_t>
+__device__ __forceinline__ void from_lds(W (&x)[C::E], const W* lds, int tid) {
     constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
-    const uint64_t* base = lds + pad<C, K>(index_of<C, P>(0, 0, tid));
+    const W* base = lds + pad<C, K>(index_of<C, P>(0, 0, tid));
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll

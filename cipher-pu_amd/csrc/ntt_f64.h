@@ -1,0 +1,416 @@
+// LDS-resident negacyclic (or cyclic) transforms on the FP64 pipeline: the
+// register layouts, pass plan, exchanges and LDS padding of ntt_core.h, with
+// residues held as exact FP64 integers (modf64.h) instead of 64-bit words.
+//
+// Behavioural spec: the reference's radix-2 transform
+// (tb/ref/gen_poly_mult_vectors_n12.py:54-100; rtl/m1/ntt_unit.sv:154-382).
+// The outputs are those of the integer transforms of ntt_core.h, residue for
+// residue; only the representative of each residue differs (signed, lazy).
+//
+//  * Forward (Cooley-Tukey, natural order in, bit-reversed out): each stage
+//    adds at most 5q/8 to the bound, so from inputs bounded by B q the
+//    outputs stay below (B + 5 log2(N) / 8) q; every value entering a product
+//    must be below 2^51, so B + 5 log2(N) / 8 < 2^51 / q (at least 64).
+//  * Inverse (Gentleman-Sande, N^-1 folded into the last stage): the sums
+//    double the bound at each stage.  From inputs bounded by q, and again
+//    after each reduction to (q + 1) / 2, at most six stages keep every
+//    product input below 2^51; the plan below reduces all registers at the
+//    start of the passes where the next pass would exceed that.  Outputs
+//    are bounded by 5q/8.
+//  * Twiddles are (w, RN(w / q)) double pairs, one 16-byte load each, in the
+//    same [prime][forward | inverse][N] order as the integer table.
+#pragma once
+#include "modf64.h"
+#include "ntt_core.h"
+
+namespace nttf {
+
+using ntt::Cfg;
+
+__device__ __forceinline__ void opaque(const double2* __restrict__& tw, int& tid) {
+    int zero = 0;
+    asm volatile("" : "+v"(tid), "+s"(zero));
+    tw += zero;
+}
+
+template <class C, int P>
+struct FwdFirst {
+    double2 w[C::E >> C::rsize(P)];
+};
+
+template <class C, int P>
+__device__ __forceinline__ FwdFirst<C, P> fwd_first(const double2* __restrict__ tw, int tid) {
+    constexpr int G = C::E >> C::rsize(P);
+    constexpr int base = 1 << (C::LOGN - 1 - C::hi(P));
+    FwdFirst<C, P> f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) f.w[g] = tw[base + ntt::group_of<C, P>(g, tid)];
+    return f;
+}
+
+template <class C, int P, bool PRE = false>
+__device__ __forceinline__ void fwd_pass(double (&x)[C::E], const double2* __restrict__ tw, double nq, int tid,
+                                         const FwdFirst<C, P>& pre = FwdFirst<C, P>{}) {
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = ntt::group_of<C, P>(g, tid);
+#pragma unroll
+        for (int b = HI; b >= LO; --b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                const double2 w = (PRE && b == HI) ? pre.w[g] : tw[base + i];
+                ct_bfly_f(x[g * M + e], x[g * M + e + h], w.x, w.y, nq);
+            }
+        }
+    }
+}
+
+template <class C, int P, bool PRE = false>
+__device__ __forceinline__ void fwd_pass2(double (&x)[C::E], double (&y)[C::E], const double2* __restrict__ tw,
+                                          double nq, int tid, const FwdFirst<C, P>& pre = FwdFirst<C, P>{}) {
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = ntt::group_of<C, P>(g, tid);
+#pragma unroll
+        for (int b = HI; b >= LO; --b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                const double2 w = (PRE && b == HI) ? pre.w[g] : tw[base + i];
+                ct_bfly_f(x[g * M + e], x[g * M + e + h], w.x, w.y, nq);
+                ct_bfly_f(y[g * M + e], y[g * M + e + h], w.x, w.y, nq);
+            }
+        }
+    }
+}
+
+// Inverse passes run from NP - 1 down to 0.  True when every register is
+// reduced at the start of inverse pass P: greedy segments of at most six
+// stages, the first starting from inputs bounded by q.
+template <class C>
+constexpr bool inv_reduce_at(int P) {
+    int since = 0;
+    for (int p = C::NP - 1; p >= 0; --p) {
+        const bool red = since + C::rsize(p) > 6;
+        if (red) since = 0;
+        if (p == P) return red;
+        since += C::rsize(p);
+    }
+    return false;
+}
+
+template <class C, int P>
+struct InvFirst {
+    double2 w[C::E / 2];
+};
+
+template <class C, int P>
+__device__ __forceinline__ InvFirst<C, P> inv_first(const double2* __restrict__ itw, int tid) {
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+    constexpr int base = 1 << (C::LOGN - 1 - LO);
+    InvFirst<C, P> f;
+    if constexpr (LO != C::LOGN - 1) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int fh = ntt::group_of<C, P>(g, tid);
+#pragma unroll
+            for (int e = 0; e < M; e += 2) f.w[g * (M / 2) + e / 2] = itw[base + ((fh << (HI - LO)) | (e >> 1))];
+        }
+    }
+    return f;
+}
+
+template <class C, int P, bool PRE = false>
+__device__ __forceinline__ void inv_pass(double (&x)[C::E], const double2* __restrict__ itw, const PrimeF& pf,
+                                         int tid, const InvFirst<C, P>& pre = InvFirst<C, P>{}) {
+    static_assert(C::rsize(P) <= 6, "an inverse pass longer than one reduction segment");
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+    const double nq = pf.nq;
+    if constexpr (inv_reduce_at<C>(P)) {
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) x[e] = rmod(x[e], pf.qinv, nq);
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = ntt::group_of<C, P>(g, tid);
+#pragma unroll
+        for (int b = LO; b <= HI; ++b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                double& X = x[g * M + e];
+                double& Y = x[g * M + e + h];
+                if (b == C::LOGN - 1) {
+#pragma clang fp contract(off)
+                    const double u = X, v = Y;
+                    X = mulw(u + v, pf.ninv, pf.ninv_q, nq);
+                    Y = mulw(u - v, pf.ninv_w1, pf.ninv_w1_q, nq);
+                } else {
+                    const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                    const double2 w = (PRE && b == LO) ? pre.w[g * (M / 2) + e / 2] : itw[base + i];
+                    gs_bfly_f(X, Y, w.x, w.y, nq);
+                }
+            }
+        }
+    }
+}
+
+template <class C, int P, bool PRE = false>
+__device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], const double2* __restrict__ itw,
+                                          const PrimeF& pf, int tid, const InvFirst<C, P>& pre = InvFirst<C, P>{}) {
+    static_assert(C::rsize(P) <= 6, "an inverse pass longer than one reduction segment");
+    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
+    constexpr int M = 1 << RP, G = C::E >> RP;
+    const double nq = pf.nq;
+    if constexpr (inv_reduce_at<C>(P)) {
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) {
+            x[e] = rmod(x[e], pf.qinv, nq);
+            y[e] = rmod(y[e], pf.qinv, nq);
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int fh = ntt::group_of<C, P>(g, tid);
+#pragma unroll
+        for (int b = LO; b <= HI; ++b) {
+            const int h = 1 << (b - LO);
+            const int base = 1 << (C::LOGN - 1 - b);
+#pragma unroll
+            for (int e = 0; e < M; ++e) {
+                if (e & h) continue;
+                double& X = x[g * M + e];
+                double& Y = x[g * M + e + h];
+                double& U = y[g * M + e];
+                double& V = y[g * M + e + h];
+                if (b == C::LOGN - 1) {
+#pragma clang fp contract(off)
+                    const double x0 = X, x1 = Y, u0 = U, u1 = V;
+                    X = mulw(x0 + x1, pf.ninv, pf.ninv_q, nq);
+                    Y = mulw(x0 - x1, pf.ninv_w1, pf.ninv_w1_q, nq);
+                    U = mulw(u0 + u1, pf.ninv, pf.ninv_q, nq);
+                    V = mulw(u0 - u1, pf.ninv_w1, pf.ninv_w1_q, nq);
+                } else {
+                    const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
+                    const double2 w = (PRE && b == LO) ? pre.w[g * (M / 2) + e / 2] : itw[base + i];
+                    gs_bfly_f(X, Y, w.x, w.y, nq);
+                    gs_bfly_f(U, V, w.x, w.y, nq);
+                }
+            }
+        }
+    }
+}
+
+// ---- single transforms, one exchange buffer --------------------------------
+
+template <class C, int P>
+__device__ __forceinline__ void fwd_rest(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+                                         int tid) {
+    if constexpr (P < C::NP) {
+        constexpr bool W = ntt::wave_local<C, P>();
+        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
+        if constexpr (P == 1) __syncthreads();
+        else if constexpr (ntt::relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
+        ntt::to_lds<C, P - 1, ntt::xshift<C>(P)>(x, lds, tid);
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, ntt::xshift<C>(P)>(x, lds, tid);
+        ntt::exchange_sync<W>();
+        fwd_pass<C, P, true>(x, tw, nq, tid, pre);
+        fwd_rest<C, P + 1>(x, lds, tw, nq, tid);
+    }
+}
+
+// x holds pass-0 layout on entry (x[e] = a[e T + tid]), the last pass's
+// layout on exit (bit-reversed evaluation order, see ntt::eval_index).
+template <class C>
+__device__ __forceinline__ void fwd_ntt(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+                                        int tid) {
+    opaque(tw, tid);
+    fwd_pass<C, 0>(x, tw, nq, tid);
+    fwd_rest<C, 1>(x, lds, tw, nq, tid);
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_rest(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+                                         const PrimeF& pf, int tid) {
+    if constexpr (P >= 0) {
+        constexpr bool W = ntt::wave_local<C, P + 1>();
+        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
+        if constexpr (ntt::relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
+        ntt::to_lds<C, P + 1, ntt::xshift<C>(P + 1)>(x, lds, tid);
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, ntt::xshift<C>(P + 1)>(x, lds, tid);
+        ntt::exchange_sync<W>();
+        inv_pass<C, P, true>(x, itw, pf, tid, pre);
+        inv_rest<C, P - 1>(x, lds, itw, pf, tid);
+    }
+}
+
+// x holds the last forward pass's layout on entry (bounded by q), pass-0
+// (coalesced) layout on exit, bounded by 5q/8.
+template <class C>
+__device__ __forceinline__ void inv_ntt(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+                                        const PrimeF& pf, int tid) {
+    opaque(itw, tid);
+    inv_pass<C, C::NP - 1>(x, itw, pf, tid);
+    inv_rest<C, C::NP - 2>(x, lds, itw, pf, tid);
+}
+
+// ---- two alternating exchange buffers (2 LDS_WORDS of LDS) ------------------
+// The ordering argument of ntt::fwd_rest_db / inv_rest_db: an exchange needs
+// only its write -> read ordering; a workgroup barrier precedes a write into
+// a buffer whose previous exchange was wave-local.  On exit other waves may
+// still be reading the LDS: the caller puts a workgroup barrier before any
+// other LDS write.
+
+template <class C, int P>
+__device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+                                            int tid) {
+    if constexpr (P < C::NP) {
+        constexpr bool W = ntt::wave_local<C, P>();
+        if constexpr (P == 1) __syncthreads();
+        else if constexpr (P >= 3 && ntt::wave_local<C, P - 1>()) __syncthreads();
+        double* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
+        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
+        ntt::to_lds<C, P - 1, ntt::xshift<C>(P)>(x, buf, tid);
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, ntt::xshift<C>(P)>(x, buf, tid);
+        fwd_pass<C, P, true>(x, tw, nq, tid, pre);
+        fwd_rest_db<C, P + 1>(x, lds, tw, nq, tid);
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void fwd_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+                                           int tid) {
+    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
+    opaque(tw, tid);
+    fwd_pass<C, 0>(x, tw, nq, tid);
+    fwd_rest_db<C, 1>(x, lds, tw, nq, tid);
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_rest_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+                                            const PrimeF& pf, int tid) {
+    if constexpr (P >= 0) {
+        constexpr bool W = ntt::wave_local<C, P + 1>();
+        if constexpr (P == C::NP - 2) __syncthreads();
+        else if constexpr (P <= C::NP - 4 && ntt::wave_local<C, P + 2>()) __syncthreads();
+        double* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
+        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
+        ntt::to_lds<C, P + 1, ntt::xshift<C>(P + 1)>(x, buf, tid);
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, ntt::xshift<C>(P + 1)>(x, buf, tid);
+        inv_pass<C, P, true>(x, itw, pf, tid, pre);
+        inv_rest_db<C, P - 1>(x, lds, itw, pf, tid);
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+                                           const PrimeF& pf, int tid) {
+    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
+    opaque(itw, tid);
+    inv_pass<C, C::NP - 1>(x, itw, pf, tid);
+    inv_rest_db<C, C::NP - 2>(x, lds, itw, pf, tid);
+}
+
+// inv_ntt_db with the first pass's first-stage twiddles already loaded.
+template <class C>
+__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+                                           const PrimeF& pf, int tid, const InvFirst<C, C::NP - 1>& pre) {
+    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
+    opaque(itw, tid);
+    inv_pass<C, C::NP - 1, true>(x, itw, pf, tid, pre);
+    inv_rest_db<C, C::NP - 2>(x, lds, itw, pf, tid);
+}
+
+// ---- two transforms of one prime, interleaved ------------------------------
+
+template <class C, int A, int B, bool W>
+__device__ __forceinline__ void exchange2(double (&x)[C::E], double (&y)[C::E], double* lds, int tid) {
+    constexpr int K = ntt::xshift<C>(A > B ? A : B);
+    ntt::to_lds<C, A, K>(x, lds, tid);
+    ntt::to_lds<C, A, K>(y, lds + C::LDS_WORDS, tid);
+    ntt::exchange_sync<W>();
+    ntt::from_lds<C, B, K>(x, lds, tid);
+    ntt::from_lds<C, B, K>(y, lds + C::LDS_WORDS, tid);
+    ntt::exchange_sync<W>();
+}
+
+template <class C, int P>
+__device__ __forceinline__ void fwd_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
+                                          const double2* __restrict__ tw, double nq, int tid) {
+    if constexpr (P < C::NP) {
+        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
+        if constexpr (P == 1) __syncthreads();
+        else if constexpr (ntt::relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
+        exchange2<C, P - 1, P, ntt::wave_local<C, P>()>(x, y, lds, tid);
+        fwd_pass2<C, P, true>(x, y, tw, nq, tid, pre);
+        fwd_rest2<C, P + 1>(x, y, lds, tw, nq, tid);
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void fwd_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
+                                         const double2* __restrict__ tw, double nq, int tid) {
+    static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
+    opaque(tw, tid);
+    fwd_pass2<C, 0>(x, y, tw, nq, tid);
+    fwd_rest2<C, 1>(x, y, lds, tw, nq, tid);
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
+                                          const double2* __restrict__ itw, const PrimeF& pf, int tid) {
+    if constexpr (P >= 0) {
+        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
+        if constexpr (ntt::relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
+        exchange2<C, P + 1, P, ntt::wave_local<C, P + 1>()>(x, y, lds, tid);
+        inv_pass2<C, P, true>(x, y, itw, pf, tid, pre);
+        inv_rest2<C, P - 1>(x, y, lds, itw, pf, tid);
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
+                                         const double2* __restrict__ itw, const PrimeF& pf, int tid) {
+    static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
+    opaque(itw, tid);
+    inv_pass2<C, C::NP - 1>(x, y, itw, pf, tid);
+    inv_rest2<C, C::NP - 2>(x, y, lds, itw, pf, tid);
+}
+
+// ---- HBM <-> registers -----------------------------------------------------
+
+// uint64 words below 2^52 in pass-0 (coalesced) layout.
+template <class C>
+__device__ __forceinline__ void load_words(double (&x)[C::E], const uint64_t* __restrict__ src, int tid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = u2d(src[e * C::T + tid]);
+}
+
+// Canonical words (bounded by 16 q on entry) in pass-0 layout.
+template <class C>
+__device__ __forceinline__ void store_words(const double (&x)[C::E], uint64_t* __restrict__ dst, const PrimeF& pf,
+                                            int tid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) dst[e * C::T + tid] = d2u(canon(x[e], pf.qinv, pf.nq));
+}
+
+}  // namespace nttf

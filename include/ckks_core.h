@@ -9,10 +9,10 @@
  *
  * Conventions
  *  - All data pointers are caller-owned device pointers (hipMalloc or a
- *    torch tensor's data_ptr()), 8-byte aligned; buffers of (value, Shoup
- *    companion) pairs -- prepared keys and prepared plaintexts -- 16-byte
- *    aligned.  Nothing is allocated behind the caller's back except inside
- *    ckks_ctx_create.
+ *    torch tensor's data_ptr()), 8-byte aligned; buffers of prepared pairs
+ *    -- prepared keys and prepared plaintexts, each word w < q stored as
+ *    the two IEEE doubles (w, RN(w / q)) -- 16-byte aligned.  Nothing is
+ *    allocated behind the caller's back except inside ckks_ctx_create.
  *  - Polynomials are uint64 rows of N coefficients; RNS tensors are
  *    [poly][limb][N] with limb i reduced modulo prime (prime0 + i) of the
  *    context, or [batch][2][limb][N] for ciphertexts (c0, c1), decrypt =
@@ -64,7 +64,8 @@ typedef struct ckks_ctx ckks_ctx;
  * of order 2N (negacyclic) or N (cyclic); NULL derives g^((q-1)/order) from
  * the smallest generator g, the chain's construction
  * (scripts/gen_modulus_chain.py:128-130).  Precomputes the twiddle tables
- * (scripts/gen_omega_tables.py:96-106) with their Shoup companions.
+ * (scripts/gen_omega_tables.py:96-106) with their Shoup companions, and
+ * the FP64 pipeline's (w, RN(w / q)) twiddle pairs.
  * Replaces the module constants of tb/ref/gen_poly_mult_vectors_n12.py:36-44
  * and the chain loader verify_rescale.py:34-35. */
 int ckks_ctx_create(ckks_ctx **out, uint64_t N, int n_primes, const uint64_t *primes,
@@ -113,8 +114,8 @@ int ckks_rescale(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, int n_p
 uint64_t ckks_workspace_bytes(const ckks_ctx *ctx, int batch, int l);
 
 /* Key material for the hybrid key switch, in the layout the kernels read:
- * [dnum][2][key_limbs][N] pairs of (value, Shoup companion), evaluation
- * order.  Builds it from coefficient-form b_j, a_j ([dnum][2][key_limbs][N]
+ * [dnum][2][key_limbs][N] prepared pairs (the doubles w and RN(w / q)),
+ * evaluation order.  Builds it from coefficient-form b_j, a_j ([dnum][2][key_limbs][N]
  * uint64, limbs = primes 0..key_limbs-2 then the special prime).  Device
  * pointers; key_out holds 2x the input's words.  key_coeff is transformed
  * in place (it holds the evaluation form afterwards). */
@@ -266,8 +267,8 @@ int ckks_embed(const ckks_ctx *ctx, const double *coeffs, double *slots, int n_v
 int ckks_embed_inv(const ckks_ctx *ctx, const double *slots, double *coeffs, int n_vec, int order, void *stream);
 
 /* Plaintext rows for ckks_lincomb: coefficient-form pt [n_polys][l][N]
- * (transformed in place) -> out [n_polys][l][N] (value, Shoup companion)
- * pairs in evaluation order. */
+ * (transformed in place) -> out [n_polys][l][N] prepared pairs (the
+ * doubles w and RN(w / q)) in evaluation order. */
 int ckks_pt_prepare(const ckks_ctx *ctx, uint64_t *pt, uint64_t *out, int n_polys, int l, int prime0,
                     void *stream);
 
@@ -280,8 +281,8 @@ int ckks_pt_prepare(const ckks_ctx *ctx, uint64_t *pt, uint64_t *out, int n_poly
  * [ng][batch][2][l][N].  nb <= 32. */
 int ckks_lincomb(const ckks_ctx *ctx, const uint64_t *E, const uint64_t *D, uint64_t *out, int nb, int ng,
                  int batch, int l, void *stream);
-/* The same with one constant per (g, b, limb): D is [ng][nb][l] (value,
- * Shoup companion) pairs, the residues of an integer weight.  A constant
+/* The same with one constant per (g, b, limb): D is [ng][nb][l] prepared
+ * pairs (the doubles w and RN(w / q)) of the residues w of an integer weight.  A constant
  * multiplies every coefficient alike, so E may be in either domain.  Used by
  * the encrypted convolution (the reference's encrypted_conv,
  * scripts/exp_e14_cnn_e2e.py:90-127, as rotations and scalar weights). */
@@ -331,6 +332,17 @@ int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads, int ten
  * forward transforms, the tensor product and INTT(d2); 0 (default) =
  * separate transform launches then the tensor kernel.  Same results. */
 int ckks_ctx_set_front(ckks_ctx *ctx, int fused);
+/* Arithmetic of the LDS-resident transform kernels that exist in both
+ * forms (the standalone transforms and ckks_poly_mul): CKKS_ARITH_INT64
+ * (64-bit integer words, Shoup and Barrett products) or CKKS_ARITH_FP64
+ * (default: residues held as exact FP64 integers, products assembled from
+ * fused multiply-adds).  Results are identical word for word; the choice is
+ * a speed setting.  The HMult front, the key switches and the plaintext
+ * products read prepared FP64 pairs and run on the FP64 pipeline only.  No reference counterpart
+ * (the RTL multiplier rtl/m1/mult_43x43_horner4.sv is one fixed datapath). */
+#define CKKS_ARITH_INT64 0
+#define CKKS_ARITH_FP64 1
+int ckks_ctx_set_arith(ckks_ctx *ctx, int arith);
 /* Workgroup placement of the hoisted key switch (ckks_rotate_hoisted and
  * its evaluation-form variant): inside one target limb, tiles of btile
  * ciphertexts with every rotation of a tile before the next tile (btile =

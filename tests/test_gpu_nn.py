@@ -61,7 +61,7 @@ def test_lincomb_scalar_bit_exact(mlp_env):
     against Python integers."""
     import torch
     from cipherpu._lib import call
-    from cipherpu.ring import to_numpy
+    from cipherpu.ring import prepared_pairs, to_numpy
     import ctypes
     ck = mlp_env[0]
     rng = np.random.default_rng(9)
@@ -75,7 +75,7 @@ def test_lincomb_scalar_bit_exact(mlp_env):
             for i in range(l):
                 q = ck.primes[i]
                 r = int(W[g, b]) % q
-                D[g, b, i] = (r, (r << 63) // q)
+                D[g, b, i] = prepared_pairs(r, q)
     Et, Dt = ck.ring.tensor(E), ck.ring.tensor(D)
     out = torch.empty(ng, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)
     p = lambda t: ctypes.c_void_p(t.data_ptr())
@@ -99,7 +99,7 @@ def test_lincomb_shapes_bit_exact(mlp_env, nb, ng, B, l, scalar):
     import ctypes
     import torch
     from cipherpu._lib import call
-    from cipherpu.ring import to_numpy
+    from cipherpu.ring import prepared_pairs, to_numpy
     ck = mlp_env[0]
     rng = np.random.default_rng(nb * 100 + ng)
     N, K = ck.N, 64
@@ -108,8 +108,7 @@ def test_lincomb_shapes_bit_exact(mlp_env, nb, ng, B, l, scalar):
     W = np.stack([rng.integers(0, ck.primes[i], dshape[:2] + dshape[3:], dtype=np.uint64) for i in range(l)], axis=2)
     q = np.array([ck.primes[i] for i in range(l)], dtype=object)
     qb = q.reshape((1, 1, l) + ((1,) if not scalar else ()))
-    comp = (W.astype(object) << 63) // qb
-    D = np.stack([W, comp.astype(np.uint64)], axis=-1)
+    D = prepared_pairs(W, qb.astype(np.float64))
     Et, Dt = ck.ring.tensor(E), ck.ring.tensor(D)
     out = torch.empty(ng, B, 2, l, N, dtype=torch.uint64, device=ck.device)
     p = lambda t: ctypes.c_void_p(t.data_ptr())

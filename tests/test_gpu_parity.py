@@ -91,13 +91,16 @@ def test_ntt_edge_values(gpu):
     ring.ntt_(empty)
 
 
+@pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("threads", [0, 128, 512])
-def test_single_prime_product_bit_exact(gpu, threads):
+def test_single_prime_product_bit_exact(gpu, threads, fp64):
     """Single-prime product: N=4096, q = largest 45-bit prime = 1 mod 8192, psi =
-    smallest primitive 8192-th root; forward NTT -> point-wise -> INTT."""
+    smallest primitive 8192-th root; forward NTT -> point-wise -> INTT, on
+    64-bit integer words and on the FP64 pipeline."""
     from cipherpu.ring import RingContext, to_numpy
     ring = RingContext(4096, [Q1], roots=[PSI1])
     ring.set_threads(threads, 0)
+    ring.set_arith(fp64)
     assert ring.roots == [PSI1]
     rng = np.random.default_rng(123)
     a = rng.integers(0, Q1, (6, 1, 4096), dtype=np.uint64)
@@ -762,9 +765,10 @@ def test_coefficient_encoding_drop_ins(gpu):
         assert G.coef_sum(ct, 64, s, q, scale) == O.coef_sum(ct, 64, s, q, scale)
 
 
+@pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("N,threads", [(4096, 256), (4096, 512), (8192, 256), (8192, 512), (8192, 1024),
                                        (16384, 512), (16384, 1024)])
-def test_transforms_and_products_over_many_rows(gpu, N, threads):
+def test_transforms_and_products_over_many_rows(gpu, N, threads, fp64):
     """Every built transform shape over 512 rows at once (enough workgroups
     that every CU runs several in turn): the forward-inverse round trip
     returns every row, sampled forward rows and products equal the C oracle.
@@ -774,6 +778,7 @@ def test_transforms_and_products_over_many_rows(gpu, N, threads):
     q = CHAIN4[0]
     ring = RingContext(N, [q])
     ring.set_threads(threads, 0, 0)
+    ring.set_arith(fp64)
     rng = np.random.default_rng(N + threads)
     rows = 512
     a = rng.integers(0, q, (rows, 1, N), dtype=np.uint64)
@@ -788,3 +793,64 @@ def test_transforms_and_products_over_many_rows(gpu, N, threads):
     P = to_numpy(ring.poly_mul(A, ring.tensor(b)))
     for r in (0, 137, rows - 1):
         assert np.array_equal(P[r, 0], OC.polymul_ntt(a[r, 0], b[r, 0], q, psi)), r
+
+
+@pytest.mark.parametrize("N,threads", [(4096, 128), (4096, 512), (8192, 512), (8192, 1024), (16384, 1024)])
+def test_fp64_products_at_extreme_operands(gpu, N, threads):
+    """The FP64 pipeline's products at the operands that push its bounds
+    hardest, for the 45-bit prime just below 2^45 and a chain prime: all
+    coefficients q - 1, alternating 0 / q - 1, a single q - 1, and random
+    rows; every product equals the integer pipeline's and sampled ones the
+    C oracle's."""
+    from cipherpu.ring import RingContext, to_numpy
+    for q, psi in ((Q1, None), (CHAIN4[3], None)):
+        if N != 4096 and q == Q1:
+            continue
+        ring = RingContext(N, [q], roots=[PSI1] if q == Q1 else None)
+        ring.set_threads(threads, 0, 0)
+        rng = np.random.default_rng(N * 7 + threads)
+        rows = 8
+        a = rng.integers(0, q, (rows, 1, N), dtype=np.uint64)
+        b = rng.integers(0, q, (rows, 1, N), dtype=np.uint64)
+        a[0], b[0] = q - 1, q - 1
+        a[1, 0, ::2], a[1, 0, 1::2] = 0, q - 1
+        b[1] = q - 1
+        a[2] = 0
+        a[2, 0, N - 1] = q - 1
+        b[3] = 0
+        A, B = ring.tensor(a), ring.tensor(b)
+        ring.set_arith(False)
+        want = to_numpy(ring.poly_mul(A, B))
+        ring.set_arith(True)
+        got = to_numpy(ring.poly_mul(A, B))
+        assert np.array_equal(got, want)
+        for r in (0, 1, 2, 5):
+            assert np.array_equal(got[r, 0], OC.polymul_ntt(a[r, 0], b[r, 0], q, ring.roots[0])), (q, r)
+
+
+def test_prepared_pairs_are_correctly_rounded(ck):
+    """The device's prepared pairs (ckks_pt_prepare, ckks_key_prepare) hold
+    w and RN(w / q) bit for bit as IEEE double division on the host gives
+    them, for random rows and the words 0, 1, q - 2 and q - 1."""
+    import ctypes
+    import torch
+    from cipherpu._lib import call
+    from cipherpu.ring import prepared_pairs, to_numpy
+    c = ck[0]
+    l = c.L
+    rng = np.random.default_rng(77)
+    pt = np.stack([rng.integers(0, q, (3, c.N), dtype=np.uint64) for q in c.primes[:l]], axis=1)
+    for i, q in enumerate(c.primes[:l]):
+        pt[0, i, :4] = [0, 1, q - 2, q - 1]
+    t = c.ring.tensor(pt)
+    prep = torch.empty(*t.shape, 2, dtype=torch.uint64, device=c.device)
+    p = lambda x: ctypes.c_void_p(x.data_ptr())
+    call("ckks_pt_prepare", c.ring.handle, p(t), p(prep), 3, l, 0, c.ring.stream())
+    ev = to_numpy(t)        # transformed in place: the evaluation-form words
+    got = to_numpy(prep)
+    q = np.array(c.primes[:l], dtype=np.float64).reshape(1, l, 1)
+    assert np.array_equal(got, prepared_pairs(ev, q))
+    # the words themselves, placed directly
+    raw = c.ring.tensor(pt)
+    call("ckks_pt_prepare", c.ring.handle, p(raw), p(prep), 3, l, 0, c.ring.stream())
+    assert np.array_equal(to_numpy(prep)[..., 0].view(np.float64), to_numpy(raw).astype(np.float64))

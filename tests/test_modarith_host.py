@@ -43,3 +43,18 @@ def test_hoist_placement_is_a_bijection(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith(", 0 failures"), r.stdout
+
+
+def test_fp64_residue_arithmetic_against_128_bit_arithmetic(tmp_path):
+    """The FP64 residue arithmetic (modf64.h: the fixed-operand and data
+    products, the two reductions, the word conversions and both butterflies)
+    on its documented input bounds, against exact 128-bit integer arithmetic
+    (tests/modf64_check.cpp).  The checker was seen to fail when the product's
+    low word was dropped, when canon's offset was raised to 2^-42, and on
+    fixed-operand inputs beyond 2^51."""
+    exe = str(tmp_path / "modf64_check")
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(HERE, "modf64_check.cpp"), "-o", exe],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([exe, "200000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith(", 0 failures"), r.stdout
