@@ -45,6 +45,7 @@ _SIGS = {
     "ckks_ctx_set_threads": (_I, [_P, _I, _I, _I]),
     "ckks_ctx_set_front": (_I, [_P, _I]),
     "ckks_ctx_set_arith": (_I, [_P, _I]),
+    "ckks_ctx_set_ks_order": (_I, [_P, _I]),
     "ckks_ctx_set_hoist_order": (_I, [_P, _I, _I]),
     "ckks_version": (ctypes.c_char_p, []),
     "ckks_ntt_fwd": (_I, [_P, _P, _I, _I, _I, _I, _P]),

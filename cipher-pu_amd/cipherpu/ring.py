@@ -79,6 +79,11 @@ class RingContext:
     def set_front(self, fused: bool = True):
         call("ckks_ctx_set_front", self._h, 1 if fused else 0)
 
+    def set_ks_order(self, pair_major: bool = True):
+        """HMult key-switch placement: ciphertext-major per XCD (True) or
+        target-major (False); the results are the same words."""
+        call("ckks_ctx_set_ks_order", self._h, 1 if pair_major else 0)
+
     def set_arith(self, fp64: bool = True):
         """Transform kernels on the FP64 pipeline (True) or on 64-bit integer
         words (False); the results are the same words."""

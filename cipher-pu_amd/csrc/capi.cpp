@@ -105,6 +105,7 @@ struct ckks_ctx {
     int ntt_threads = 0, ks_threads = 0, tensor_threads = 0;
     int front_fused = 0;   // HMult front: one fused kernel (1) or 2 transforms + tensor (0)
     HoistOrder hoist = {0, 0};   // hoisted key-switch placement; btile 0 = default for the output form
+    int ks_pair_major = 1;       // HMult key-switch placement (ckks_ctx_set_ks_order)
     // Optional per-launch timing (ckks_timing_start/stop): an event pair
     // around every kernel the entry points launch, tagged by kernel kind.
     struct Timer {
@@ -206,11 +207,17 @@ ModDownConst moddown_const(const ckks_ctx* c, int l, int sp) {
             m.p_mod_s[i] = shoup(m.p_mod[i], qi);
             m.pmf[i] = (double)m.p_mod[i];
             m.pmf_q[i] = (double)m.p_mod[i] / (double)qi;
+            m.hpf[i] = (double)m.hp[i];
+            m.pinvf[i] = (double)m.pinv[i];
+            m.pinvf_q[i] = (double)m.pinv[i] / (double)qi;
         }
         if (i < l - 1) {
             m.hq[i] = (qL >> 1) % qi;
             m.qinv[i] = invmod(qL, qi);
             m.qinv_s[i] = shoup(m.qinv[i], qi);
+            m.hqf[i] = (double)m.hq[i];
+            m.qinvf[i] = (double)m.qinv[i];
+            m.qinvf_q[i] = (double)m.qinv[i] / (double)qi;
         }
     }
     return m;
@@ -424,6 +431,12 @@ int ckks_ctx_set_arith(ckks_ctx* c, int arith) {
 int ckks_ctx_set_front(ckks_ctx* c, int fused) {
     if (!c) return CKKS_ERR_NULL;
     c->front_fused = fused ? 1 : 0;
+    return CKKS_OK;
+}
+
+int ckks_ctx_set_ks_order(ckks_ctx* c, int pair_major) {
+    if (!c) return CKKS_ERR_NULL;
+    c->ks_pair_major = pair_major ? 1 : 0;
     return CKKS_OK;
 }
 
@@ -679,10 +692,12 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
     KsTargets tail, rest;
     tail.first = l - 1;
     tail.count = 2;
+    tail.pair_major = c->ks_pair_major;
     rest.first = 0;
     rest.count = l - 1;
     rest.out = out;
     rest.mc = &mc;
+    rest.pair_major = c->ks_pair_major;
     TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2c, LN, d2e, d01, (const double2*)key, key_limbs, acc, batch,
                                                 l, m, special, c->ks_threads, S(stream), nullptr, tail));
     TIMED(CKKS_KIND_KEYSWITCH_FUSED, launch_keyswitch(t, d2c, LN, d2e, d01, (const double2*)key, key_limbs, acc,

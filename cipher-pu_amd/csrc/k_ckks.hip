@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(T) k_tensor_intt(const uint64_t* __restrict__ 
     extern __shared__ uint64_t lds_u[];
     double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
-    const int b = blockIdx.x / l, i = blockIdx.x % l;
+    const int b = wave_uniform(blockIdx.x / l), i = wave_uniform(blockIdx.x % l);
     const int p = map.idx[i];
     const PrimeF pf = pfs[p];
     const double qi = pf.qinv, nq = pf.nq;
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
     extern __shared__ uint64_t lds_u[];
     double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
-    const int i = blockIdx.x / batch, b = blockIdx.x % batch;
+    const int i = wave_uniform(blockIdx.x / batch), b = wave_uniform(blockIdx.x % batch);
     const int p = map.idx[i];
     const PrimeF pf = pfs[p];
     const double2* tw = twf_all + (size_t)p * 2 * C::N;
@@ -113,14 +113,19 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
     if constexpr (ntt::pair_pays<C>()) {
         // a0 with a1 and b0 with b1 as transform pairs
         double x0[C::E], x1[C::E], y0[C::E], y1[C::E];
+        uint64_t nb0[C::E];
         nttf::load_words<C>(x0, A + c0, tid);
         nttf::load_words<C>(x1, A + c0 + LN, tid);
+        // b0 is loaded ahead, in flight while the first passes (scalar
+        // twiddle loads only) transform a0 and a1
+        nttf::issue_row<C>(nb0, B + c0, tid);
         nttf::fwd_ntt2<C>(x0, x1, lds, tw, pf.nq, tid);
-        nttf::load_words<C>(y0, B + c0, tid);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) y0[k] = u2d(nb0[k]);
         nttf::load_words<C>(y1, B + c0 + LN, tid);
         nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf.nq, tid);
         // the inverse transform's first twiddles, loaded before the stores
-        const auto pre = nttf::inv_first<C, C::NP - 1>(tw + C::N, tid);
+        const auto pre = nttf::inv_tw<C, C::NP - 1>(tw + C::N, tid);
 #pragma unroll
         for (int k = 0; k < C::E; ++k)
             x0[k] = tensor_one<C>(x0[k], x1[k], y0[k], y1[k], o0, o2, LN, ntt::eval_index<C>(k, tid), pf, pm, pmq);
@@ -182,38 +187,42 @@ __device__ __forceinline__ void key_mac(double (&a0)[C::E], double (&a1)[C::E], 
 }
 
 // ModDown by p and rescale by q_(l-1) of one coefficient of chain limb t <
-// l - 1, from the coefficient-form accumulators of limb t (xt), of limb
-// l - 1 (xl) and of the special prime (xp), all reduced: the per-limb steps
-// of k_moddown_rescale_l, word for word.
-__device__ __forceinline__ uint64_t moddown_rescale_one(uint64_t xt, uint64_t xl, uint64_t xp, int t, int lm1,
-                                                        const PrimeConst& pc, const PrimeConst& pl,
-                                                        const PrimeConst& pp, const ModDownConst& mc, int narrow) {
-    const uint64_t tp = csub(xp + (pp.q >> 1), pp.q);
-    const uint64_t ql = pl.q, q = pc.q;
-    const uint64_t tpl = narrow ? csub(tp, ql) : reduce64(tp, pl);
-    const uint64_t yl = csub(mul_shoup(xl + mc.hp[lm1] + ql - tpl, mc.pinv[lm1], mc.pinv_s[lm1], ql), ql);
-    const uint64_t tl = csub(yl + (ql >> 1), ql);
-    const uint64_t tpt = narrow ? csub(tp, q) : reduce64(tp, pc);
-    const uint64_t yt = csub(mul_shoup(xt + mc.hp[t] + q - tpt, mc.pinv[t], mc.pinv_s[t], q), q);
-    const uint64_t tlt = narrow ? csub(tl, q) : reduce64(tl, pc);
-    return csub(mul_shoup(yt + mc.hq[t] + q - tlt, mc.qinv[t], mc.qinv_s[t], q), q);
+// l - 1 on the FP64 pipeline: the per-limb steps of k_moddown_rescale_l,
+//   y_i = ((x_i + h_p) - [(x_p + h_p) mod p]) p^-1 mod q_i,  i = t and l - 1,
+//   out = ((y_t + h_q) - [(y_(l-1) + h_q) mod q_(l-1)]) q_(l-1)^-1 mod q_t,
+// with h_p = (p - 1) / 2, h_q = (q_(l-1) - 1) / 2 (and their residues).  The
+// two bracketed lifts are exact canonical integers; every other value is a
+// signed residue, so no reduction is needed before the products (each
+// product input is below 3 * 2^45).  xt: the target's accumulator (bounded
+// by 5q/8); xl, xp: the canonical accumulators of limb l - 1 and of p.
+__device__ __forceinline__ uint64_t moddown_rescale_one(double xt, double xl, double xp, int t, int lm1,
+                                                        const PrimeF& pc, const PrimeF& pl, const PrimeF& pp,
+                                                        const ModDownConst& mc) {
+#pragma clang fp contract(off)
+    double tp = xp + __builtin_floor(0.5 * pp.q);       // (x_p + h_p) mod p in [0, p)
+    tp = tp >= pp.q ? tp - pp.q : tp;
+    const double yl = mulw(xl + mc.hpf[lm1] - tp, mc.pinvf[lm1], mc.pinvf_q[lm1], pl.nq);
+    const double tl = canon(yl + __builtin_floor(0.5 * pl.q), pl.qinv, pl.nq);   // (y_l + h_q) mod q_l
+    const double yt = mulw(xt + mc.hpf[t] - tp, mc.pinvf[t], mc.pinvf_q[t], pc.nq);
+    return d2u(canon(mulw(yt + mc.hqf[t] - tl, mc.qinvf[t], mc.qinvf_q[t], pc.nq), pc.qinv, pc.nq));
 }
 
 // The fused epilogue for poly c of target t: x holds the accumulator in
-// coefficient form (coalesced layout, reduced); the special-prime and limb
-// l - 1 accumulators come from an earlier launch through acc_c ([l+1][N]).
+// coefficient form (coalesced layout, bounded by 5q/8); the special-prime
+// and limb l - 1 accumulators come from an earlier launch through acc_c
+// ([l+1][N], canonical words).
 template <class C>
-__device__ __forceinline__ void fused_out(const uint64_t (&x)[C::E], const uint64_t* __restrict__ acc_c,
+__device__ __forceinline__ void fused_out(const double (&x)[C::E], const uint64_t* __restrict__ acc_c,
                                           uint64_t* __restrict__ out_c, int t, int l,
-                                          const PrimeConst* __restrict__ pcs, const LimbMap& map, int pidx_p,
-                                          const ModDownConst& mc, int narrow, int tid) {
-    const PrimeConst pc = pcs[map.idx[t]], pl = pcs[map.idx[l - 1]], pp = pcs[pidx_p];
+                                          const PrimeF* __restrict__ pfs, const LimbMap& map, int pidx_p,
+                                          const ModDownConst& mc, int tid) {
+    const PrimeF pc = pfs[map.idx[t]], pl = pfs[map.idx[l - 1]], pp = pfs[pidx_p];
     const uint64_t* sp = acc_c + (size_t)l * C::N;
     const uint64_t* sl = acc_c + (size_t)(l - 1) * C::N;
 #pragma unroll
     for (int e = 0; e < C::E; ++e) {
         const int j = e * C::T + tid;
-        out_c[j] = moddown_rescale_one(x[e], sl[j], sp[j], t, l - 1, pc, pl, pp, mc, narrow);
+        out_c[j] = moddown_rescale_one(x[e], u2d(sl[j]), u2d(sp[j]), t, l - 1, pc, pl, pp, mc);
     }
 }
 
@@ -243,17 +252,30 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
                                                  const uint64_t* __restrict__ d01, const double2* __restrict__ key,
                                                  int key_limbs, uint64_t* __restrict__ acc,
                                                  const double2* __restrict__ twf_all, const PrimeF* __restrict__ pfs,
-                                                 const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
-                                                 int batch, int reduce_digits, const uint64_t* __restrict__ key_table,
-                                                 int t0, uint64_t* __restrict__ out_ct, ModDownConst mc, int narrow) {
+                                                 LimbMap map, int l, int pidx_p, int batch, int reduce_digits,
+                                                 const uint64_t* __restrict__ key_table, int t0,
+                                                 uint64_t* __restrict__ out_ct, ModDownConst mc, int pair_major) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds_u[];
     double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
     // Target-major order: workgroups in flight together share one target
     // limb, so each XCD's L2 holds one limb's key slice (2 dnum N 16 B) and
-    // one prime's twiddles rather than the whole key.
-    const int t = t0 + (int)(blockIdx.x / batch), b = blockIdx.x % batch;
+    // one prime's twiddles rather than the whole key.  Ciphertext-major
+    // (pair_major): workgroup i runs on XCD i mod 8, which takes logical
+    // index (i mod 8) G/8 + i/8, so each XCD walks one contiguous range of
+    // ciphertexts, all targets of one ciphertext back to back.
+    const int G = (int)gridDim.x, nt = G / batch;
+    int t, b;
+    if (pair_major) {
+        const int i = (int)blockIdx.x;
+        const int L = (G & 7) == 0 ? (i & 7) * (G >> 3) + (i >> 3) : i;
+        t = wave_uniform(t0 + L % nt);
+        b = wave_uniform(L / nt);
+    } else {
+        t = wave_uniform(t0 + (int)(blockIdx.x / batch));
+        b = wave_uniform(blockIdx.x % batch);
+    }
     if constexpr (KEYTAB) key = reinterpret_cast<const double2*>(key_table[b]);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
@@ -274,20 +296,35 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
 #pragma unroll
         for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
     }
+    // The coefficient-form digits are loaded one ahead: the next digit's
+    // loads are issued as the current digit's transform starts, whose first
+    // passes take their twiddles through scalar loads, so the loads are in
+    // flight for those passes before any vector load must wait for them.
+    const bool own = d2e != nullptr;   // digit t comes in evaluation form
+    const uint64_t* dig = d2c + (size_t)b * d2c_bstride;
+    uint64_t nx[C::E];
+    int jn = (own && t == 0) ? 1 : 0;
+    // (16 coefficients per thread: the extra row would spill)
+    constexpr bool PF = C::E <= 8;
+    if (PF && jn < l) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);
     for (int j = 0; j < l; ++j) {
-        if (j == t && d2e) {
+        if (j == t && own) {
             // the digit of the target's own prime is already in evaluation form
             const uint64_t* s = d2e + (size_t)b * LN + (size_t)j * C::N;
 #pragma unroll
             for (int k = 0; k < C::E; ++k) x[k] = u2d(s[ntt::eval_index<C>(k, tid)]);
         } else {
-            nttf::load_words<C>(x, d2c + (size_t)b * d2c_bstride + (size_t)j * C::N, tid);
+            if (!PF) nttf::issue_row<C>(nx, dig + (size_t)j * C::N, tid);
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) x[k] = u2d(nx[k]);
             // the digit's residues mod q_j enter the transform mod q_t as they
             // are when q_j < 2 q_t (reduce_digits = 0); else reduce them first
             if (reduce_digits) {
 #pragma unroll
                 for (int k = 0; k < C::E; ++k) x[k] = rmod(x[k], pf.qinv, nq);
             }
+            jn = j + 1 + ((own && j + 1 == t) ? 1 : 0);
+            if (PF && jn < l) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);
             // left unreduced (below 11 q): the key product is exact for any such input
             if constexpr (ntt::pair_pays<C>()) nttf::fwd_ntt_db<C>(x, lds, tw, nq, tid);
             else nttf::fwd_ntt<C>(x, lds, tw, nq, tid);
@@ -305,35 +342,28 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     const uint64_t* acc_b = acc + (size_t)b * 2 * (l + 1) * C::N;
     uint64_t* out_b = FUSED ? out_ct + ((size_t)b * 2 * (l - 1) + t) * C::N : nullptr;
     const double2* itw = tw + C::N;
-    uint64_t u[C::E];
     if constexpr (ntt::pair_pays<C>()) {
         // both accumulators as one transform pair
         nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid);
-#pragma unroll
-        for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a0[k], pf.qinv, nq));
-        if constexpr (FUSED) fused_out<C>(u, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
-        else ntt::store_coalesced<C>(u, out, tid);
-#pragma unroll
-        for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a1[k], pf.qinv, nq));
-        if constexpr (FUSED)
-            fused_out<C>(u, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map, pidx_p,
-                         mc, narrow, tid);
-        else ntt::store_coalesced<C>(u, out + (size_t)(l + 1) * C::N, tid);
+        if constexpr (FUSED) {
+            fused_out<C>(a0, acc_b, out_b, t, l, pfs, map, pidx_p, mc, tid);
+            fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p,
+                         mc, tid);
+        } else {
+            nttf::store_words<C>(a0, out, pf, tid);
+            nttf::store_words<C>(a1, out + (size_t)(l + 1) * C::N, pf, tid);
+        }
         return;
     }
     nttf::inv_ntt<C>(a0, lds, itw, pf, tid);
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a0[k], pf.qinv, nq));
-    if constexpr (FUSED) fused_out<C>(u, acc_b, out_b, t, l, pcs, map, pidx_p, mc, narrow, tid);
-    else ntt::store_coalesced<C>(u, out, tid);
+    if constexpr (FUSED) fused_out<C>(a0, acc_b, out_b, t, l, pfs, map, pidx_p, mc, tid);
+    else nttf::store_words<C>(a0, out, pf, tid);
     __syncthreads();
     nttf::inv_ntt<C>(a1, lds, itw, pf, tid);
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) u[k] = d2u(canon(a1[k], pf.qinv, nq));
     if constexpr (FUSED)
-        fused_out<C>(u, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pcs, map, pidx_p, mc,
-                     narrow, tid);
-    else ntt::store_coalesced<C>(u, out + (size_t)(l + 1) * C::N, tid);
+        fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p, mc,
+                     tid);
+    else nttf::store_words<C>(a1, out + (size_t)(l + 1) * C::N, pf, tid);
 }
 
 // ModDown by p (round-half-up of X/p), optional addition of coefficient-form
@@ -818,7 +848,8 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
     if (tg.mc) mc = *tg.mc;
     else memset(&mc, 0, sizeof(mc));
     hipLaunchKernelGGL(k, dim3(batch * nt), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.twf,
-                       t.pf, t.pc, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table, tg.first, tg.out, mc, t.narrow);
+                       t.pf, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table, tg.first, tg.out, mc,
+                       tg.pair_major);
     return hipGetLastError();
 }
 

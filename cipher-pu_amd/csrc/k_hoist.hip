@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(T) k_modup(const uint64_t* __restrict__ ct, ui
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
     const int per_b = l * (l + 1);
-    const int b = blockIdx.x / per_b, rem = blockIdx.x % per_b;
+    const int b = wave_uniform(blockIdx.x / per_b), rem = wave_uniform(blockIdx.x % per_b);
     const int j = rem / (l + 1), t = rem % (l + 1);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const PrimeConst pc = pcs[pt];
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
     double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
     const TRB s = hoist_slot(blockIdx.x, gridDim.x, R, batch, order);
-    const int b = s.b, r = s.r, t = s.t;
+    const int b = wave_uniform(s.b), r = wave_uniform(s.r), t = wave_uniform(s.t);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeF pf = pfs[pt];
@@ -194,8 +194,8 @@ __global__ void __launch_bounds__(T) k_moddown_eval(const uint64_t* __restrict__
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
-    const int i = blockIdx.x % l;
-    const int row = blockIdx.x / l;                 // ((r batch + b) 2 + poly)
+    const int i = wave_uniform(blockIdx.x % l);
+    const int row = wave_uniform(blockIdx.x / l);                 // ((r batch + b) 2 + poly)
     const int poly = row & 1, rb = row >> 1;
     const int b = rb % batch, r = rb / batch;
     const int p = map.idx[i];

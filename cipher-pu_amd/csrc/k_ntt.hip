@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(T) k_ntt_fwd(uint64_t* dst, const uint64_t* sr
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
-    const int limb = blockIdx.x % n_limbs;
+    const int limb = wave_uniform(blockIdx.x % n_limbs);
     const int p = map.idx[limb];
     const PrimeConst pc = pcs[p];
     const size_t off = (size_t)blockIdx.x * C::N;
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(T) k_ntt_inv(uint64_t* dst, const uint64_t* sr
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
-    const int limb = blockIdx.x % n_limbs;
+    const int limb = wave_uniform(blockIdx.x % n_limbs);
     const int p = map.idx[limb];
     const PrimeConst pc = pcs[p];
     const size_t off = (size_t)blockIdx.x * C::N;
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, c
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
     const int tid = threadIdx.x;
-    const int limb = blockIdx.x % n_limbs;
+    const int limb = wave_uniform(blockIdx.x % n_limbs);
     const int p = map.idx[limb];
     const PrimeConst pc = pcs[p];
     const size_t off = (size_t)blockIdx.x * C::N;
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(T) k_polymul_f(const uint64_t* __restrict__ A,
     extern __shared__ uint64_t lds[];
     double* ldsf = reinterpret_cast<double*>(lds);
     const int tid = threadIdx.x;
-    const int limb = blockIdx.x % n_limbs;
+    const int limb = wave_uniform(blockIdx.x % n_limbs);
     const int p = map.idx[limb];
     const PrimeF pf = pfs[p];
     const size_t off = (size_t)blockIdx.x * C::N;
@@ -173,7 +173,7 @@ __global__ void k_ntt_small(uint64_t* dst, const uint64_t* src, const ulonglong2
                             int reduce_in) {
     extern __shared__ uint64_t s[];
     const int N = 1 << logN;
-    const int p = map.idx[blockIdx.x % n_limbs];
+    const int p = map.idx[wave_uniform(blockIdx.x % n_limbs)];
     const PrimeConst pc = pcs[p];
     const size_t off = (size_t)blockIdx.x * N;
     for (int k = threadIdx.x; k < N; k += blockDim.x) {
@@ -193,7 +193,7 @@ __global__ void k_polymul_small(const uint64_t* __restrict__ A, const uint64_t* 
                                 const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, int logN) {
     extern __shared__ uint64_t s[];
     const int N = 1 << logN;
-    const int p = map.idx[blockIdx.x % n_limbs];
+    const int p = map.idx[wave_uniform(blockIdx.x % n_limbs)];
     const PrimeConst pc = pcs[p];
     const size_t off = (size_t)blockIdx.x * N;
     const ulonglong2* tw = tw_all + (size_t)p * 2 * N;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(T) k_ntt_fwd_f(uint64_t* dst, const uint64_t* 
     extern __shared__ uint64_t lds[];
     double* ldsf = reinterpret_cast<double*>(lds);
     const int tid = threadIdx.x;
-    const int p = map.idx[blockIdx.x % n_limbs];
+    const int p = map.idx[wave_uniform(blockIdx.x % n_limbs)];
     const PrimeF pf = pfs[p];
     const size_t off = (size_t)blockIdx.x * C::N;
     uint64_t u[C::E];
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(T) k_ntt_inv_f(uint64_t* dst, const uint64_t* 
     extern __shared__ uint64_t lds[];
     double* ldsf = reinterpret_cast<double*>(lds);
     const int tid = threadIdx.x;
-    const int p = map.idx[blockIdx.x % n_limbs];
+    const int p = map.idx[wave_uniform(blockIdx.x % n_limbs)];
     const PrimeF pf = pfs[p];
     const size_t off = (size_t)blockIdx.x * C::N;
     uint64_t u[C::E];

@@ -7,6 +7,12 @@
 #include "modarith.h"
 #include "modf64.h"
 
+// A value the compiler cannot prove wave-uniform (a runtime division of the
+// block index, say), marked uniform: it then lives in an SGPR, and the
+// table entries it selects come through scalar loads instead of a chain of
+// vector loads at the start of every workgroup.
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 struct DevTables {
     const PrimeConst* pc;   // [K]
     const ulonglong2* tw;   // [K][2][N]: forward table, inverse table
@@ -62,8 +68,12 @@ struct ModDownConst {
     uint64_t qinv_s[CKKS_MAX_PRIMES];
     uint64_t p_mod[CKKS_MAX_PRIMES];   // p mod q_i, and Shoup companion
     uint64_t p_mod_s[CKKS_MAX_PRIMES];
-    double pmf[CKKS_MAX_PRIMES];       // p mod q_i as a double, and RN(pmf / q_i) (FP64 pipeline)
-    double pmf_q[CKKS_MAX_PRIMES];
+    // The same constants for the FP64 pipeline (modf64.h): values as doubles,
+    // multipliers with their RN(w / q_i) companions.
+    double pmf[CKKS_MAX_PRIMES], pmf_q[CKKS_MAX_PRIMES];         // p mod q_i
+    double hpf[CKKS_MAX_PRIMES], hqf[CKKS_MAX_PRIMES];           // hp, hq
+    double pinvf[CKKS_MAX_PRIMES], pinvf_q[CKKS_MAX_PRIMES];     // p^-1 mod q_i
+    double qinvf[CKKS_MAX_PRIMES], qinvf_q[CKKS_MAX_PRIMES];     // q_(l-1)^-1 mod q_i
 };
 
 hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint64_t* eb, uint64_t* d01,
@@ -77,10 +87,16 @@ hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b
 // the workgroups finish ModDown and the rescale by the last chain prime
 // themselves, from the accumulators of limb l - 1 and of the special prime
 // that an earlier launch left in acc, and write [batch][2][l-1][N] to out.
+// pair_major: workgroup placement -- 0 = target-major (all ciphertexts of
+// one target limb in flight together: that limb's key slice stays in L2);
+// 1 = ciphertext-major with each XCD given one contiguous range of
+// ciphertexts, so a ciphertext's targets run back to back on one XCD and
+// its digits (and the fused ModDown's inputs) are read from that XCD's L2.
 struct KsTargets {
     int first = 0, count = 0;
     uint64_t* out = nullptr;
     const ModDownConst* mc = nullptr;
+    int pair_major = 0;
 };
 
 // key_table (optional): ciphertext b uses the key at device address

@@ -33,61 +33,87 @@ __device__ __forceinline__ void opaque(const double2* __restrict__& tw, int& tid
     tw += zero;
 }
 
+// The twiddles of forward pass P, loaded in one go: for register group g
+// and stage b (HI down to LO), the 2^(HI-b) entries 2^(LOGN-1-b) + (fh(g)
+// 2^(HI-b) | k), at slot g (M-1) + 2^(HI-b) - 1 + k.  The loads are issued
+// between an exchange's LDS writes and its barrier -- the register data is
+// dead there -- so their latency overlaps the barrier instead of stalling
+// each stage (a wait for one vector load waits for all earlier ones).  A
+// pass that needs more than TW_SET pairs per thread (16 coefficients per
+// thread) keeps only its first stage's here and loads the rest as it goes,
+// so that the set does not crowd out the kernel's other registers.
+constexpr int TW_SET = 8;
 template <class C, int P>
-struct FwdFirst {
-    double2 w[C::E >> C::rsize(P)];
+struct FwdTw {
+    static constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
+    static constexpr bool FULL = G * (M - 1) <= TW_SET;
+    // slot of stage b's k-th twiddle of group g (first stage only: g)
+    static constexpr int slot(int g, int b, int k) { return FULL ? g * (M - 1) + (1 << (C::hi(P) - b)) - 1 + k : g; }
+    double2 w[FULL ? G * (M - 1) : G];
 };
 
 template <class C, int P>
-__device__ __forceinline__ FwdFirst<C, P> fwd_first(const double2* __restrict__ tw, int tid) {
-    constexpr int G = C::E >> C::rsize(P);
-    constexpr int base = 1 << (C::LOGN - 1 - C::hi(P));
-    FwdFirst<C, P> f;
+__device__ __forceinline__ FwdTw<C, P> fwd_tw(const double2* __restrict__ tw, int tid) {
+    constexpr int LO = C::lo(P), HI = C::hi(P);
+    using S = FwdTw<C, P>;
+    S f;
 #pragma unroll
-    for (int g = 0; g < G; ++g) f.w[g] = tw[base + ntt::group_of<C, P>(g, tid)];
+    for (int g = 0; g < S::G; ++g) {
+        const int fh = ntt::group_of<C, P>(g, tid);
+#pragma unroll
+        for (int b = HI; b >= (S::FULL ? LO : HI); --b)
+#pragma unroll
+            for (int k = 0; k < (1 << (HI - b)); ++k)
+                f.w[S::slot(g, b, k)] = tw[(1 << (C::LOGN - 1 - b)) + ((fh << (HI - b)) | k)];
+    }
     return f;
 }
 
-template <class C, int P, bool PRE = false>
-__device__ __forceinline__ void fwd_pass(double (&x)[C::E], const double2* __restrict__ tw, double nq, int tid,
-                                         const FwdFirst<C, P>& pre = FwdFirst<C, P>{}) {
+// Twiddle of stage b for butterfly k = e >> (b - LO + 1) of group g: from the
+// pass's set, or loaded here when the set holds the first stage only.
+template <class C, int P>
+__device__ __forceinline__ double2 fwd_w(const FwdTw<C, P>& set, const double2* __restrict__ tw, int g, int b, int k,
+                                         int tid) {
+    constexpr int HI = C::hi(P);
+    if (FwdTw<C, P>::FULL || b == HI) return set.w[FwdTw<C, P>::slot(g, b, k)];
+    const int fh = ntt::group_of<C, P>(g, tid);
+    return tw[(1 << (C::LOGN - 1 - b)) + ((fh << (HI - b)) | k)];
+}
+
+template <class C, int P>
+__device__ __forceinline__ void fwd_pass(double (&x)[C::E], const FwdTw<C, P>& set, const double2* __restrict__ tw,
+                                         double nq, int tid) {
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = ntt::group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = HI; b >= LO; --b) {
             const int h = 1 << (b - LO);
-            const int base = 1 << (C::LOGN - 1 - b);
 #pragma unroll
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
-                const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                const double2 w = (PRE && b == HI) ? pre.w[g] : tw[base + i];
+                const double2 w = fwd_w<C, P>(set, tw, g, b, e >> (b - LO + 1), tid);
                 ct_bfly_f(x[g * M + e], x[g * M + e + h], w.x, w.y, nq);
             }
         }
     }
 }
 
-template <class C, int P, bool PRE = false>
-__device__ __forceinline__ void fwd_pass2(double (&x)[C::E], double (&y)[C::E], const double2* __restrict__ tw,
-                                          double nq, int tid, const FwdFirst<C, P>& pre = FwdFirst<C, P>{}) {
+template <class C, int P>
+__device__ __forceinline__ void fwd_pass2(double (&x)[C::E], double (&y)[C::E], const FwdTw<C, P>& set,
+                                          const double2* __restrict__ tw, double nq, int tid) {
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = ntt::group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = HI; b >= LO; --b) {
             const int h = 1 << (b - LO);
-            const int base = 1 << (C::LOGN - 1 - b);
 #pragma unroll
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
-                const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                const double2 w = (PRE && b == HI) ? pre.w[g] : tw[base + i];
+                const double2 w = fwd_w<C, P>(set, tw, g, b, e >> (b - LO + 1), tid);
                 ct_bfly_f(x[g * M + e], x[g * M + e + h], w.x, w.y, nq);
                 ct_bfly_f(y[g * M + e], y[g * M + e + h], w.x, w.y, nq);
             }
@@ -110,31 +136,52 @@ constexpr bool inv_reduce_at(int P) {
     return false;
 }
 
+// The twiddles of inverse pass P (as FwdTw): for group g and stage b (LO up
+// to HI, the last stage of the transform excepted: it uses N^-1 constants),
+// the 2^(HI-b) entries 2^(LOGN-1-b) + (fh(g) 2^(HI-b) | k) of the inverse
+// table; a set of the first stage (b = LO) only when the pass needs more
+// than TW_SET.
 template <class C, int P>
-struct InvFirst {
-    double2 w[C::E / 2];
+struct InvTw {
+    static constexpr int M = 1 << C::rsize(P), G = C::E >> C::rsize(P);
+    static constexpr bool FULL = G * (M - 1) <= TW_SET;
+    static constexpr int slot(int g, int b, int k) {
+        return FULL ? g * (M - 1) + (1 << (C::hi(P) - b)) - 1 + k : g * (M / 2) + k;
+    }
+    double2 w[FULL ? G * (M - 1) : G * (M / 2)];
 };
 
 template <class C, int P>
-__device__ __forceinline__ InvFirst<C, P> inv_first(const double2* __restrict__ itw, int tid) {
-    constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
-    constexpr int M = 1 << RP, G = C::E >> RP;
-    constexpr int base = 1 << (C::LOGN - 1 - LO);
-    InvFirst<C, P> f;
-    if constexpr (LO != C::LOGN - 1) {
+__device__ __forceinline__ InvTw<C, P> inv_tw(const double2* __restrict__ itw, int tid) {
+    constexpr int LO = C::lo(P), HI = C::hi(P);
+    using S = InvTw<C, P>;
+    S f;
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int fh = ntt::group_of<C, P>(g, tid);
+    for (int g = 0; g < S::G; ++g) {
+        const int fh = ntt::group_of<C, P>(g, tid);
 #pragma unroll
-            for (int e = 0; e < M; e += 2) f.w[g * (M / 2) + e / 2] = itw[base + ((fh << (HI - LO)) | (e >> 1))];
+        for (int b = LO; b <= (S::FULL ? HI : LO); ++b) {
+            if (b == C::LOGN - 1) continue;
+#pragma unroll
+            for (int k = 0; k < (1 << (HI - b)); ++k)
+                f.w[S::slot(g, b, k)] = itw[(1 << (C::LOGN - 1 - b)) + ((fh << (HI - b)) | k)];
         }
     }
     return f;
 }
 
-template <class C, int P, bool PRE = false>
-__device__ __forceinline__ void inv_pass(double (&x)[C::E], const double2* __restrict__ itw, const PrimeF& pf,
-                                         int tid, const InvFirst<C, P>& pre = InvFirst<C, P>{}) {
+template <class C, int P>
+__device__ __forceinline__ double2 inv_w(const InvTw<C, P>& set, const double2* __restrict__ itw, int g, int b, int k,
+                                         int tid) {
+    constexpr int LO = C::lo(P), HI = C::hi(P);
+    if (InvTw<C, P>::FULL || b == LO) return set.w[InvTw<C, P>::slot(g, b, k)];
+    const int fh = ntt::group_of<C, P>(g, tid);
+    return itw[(1 << (C::LOGN - 1 - b)) + ((fh << (HI - b)) | k)];
+}
+
+template <class C, int P>
+__device__ __forceinline__ void inv_pass(double (&x)[C::E], const InvTw<C, P>& set, const double2* __restrict__ itw,
+                                         const PrimeF& pf, int tid) {
     static_assert(C::rsize(P) <= 6, "an inverse pass longer than one reduction segment");
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
@@ -145,11 +192,9 @@ __device__ __forceinline__ void inv_pass(double (&x)[C::E], const double2* __res
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = ntt::group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = LO; b <= HI; ++b) {
             const int h = 1 << (b - LO);
-            const int base = 1 << (C::LOGN - 1 - b);
 #pragma unroll
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
@@ -161,8 +206,7 @@ __device__ __forceinline__ void inv_pass(double (&x)[C::E], const double2* __res
                     X = mulw(u + v, pf.ninv, pf.ninv_q, nq);
                     Y = mulw(u - v, pf.ninv_w1, pf.ninv_w1_q, nq);
                 } else {
-                    const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                    const double2 w = (PRE && b == LO) ? pre.w[g * (M / 2) + e / 2] : itw[base + i];
+                    const double2 w = inv_w<C, P>(set, itw, g, b, e >> (b - LO + 1), tid);
                     gs_bfly_f(X, Y, w.x, w.y, nq);
                 }
             }
@@ -170,9 +214,9 @@ __device__ __forceinline__ void inv_pass(double (&x)[C::E], const double2* __res
     }
 }
 
-template <class C, int P, bool PRE = false>
-__device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], const double2* __restrict__ itw,
-                                          const PrimeF& pf, int tid, const InvFirst<C, P>& pre = InvFirst<C, P>{}) {
+template <class C, int P>
+__device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], const InvTw<C, P>& set,
+                                          const double2* __restrict__ itw, const PrimeF& pf, int tid) {
     static_assert(C::rsize(P) <= 6, "an inverse pass longer than one reduction segment");
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
@@ -186,11 +230,9 @@ __device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], 
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int fh = ntt::group_of<C, P>(g, tid);
 #pragma unroll
         for (int b = LO; b <= HI; ++b) {
             const int h = 1 << (b - LO);
-            const int base = 1 << (C::LOGN - 1 - b);
 #pragma unroll
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
@@ -206,8 +248,7 @@ __device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], 
                     U = mulw(u0 + u1, pf.ninv, pf.ninv_q, nq);
                     V = mulw(u0 - u1, pf.ninv_w1, pf.ninv_w1_q, nq);
                 } else {
-                    const int i = (fh << (HI - b)) | (e >> (b - LO + 1));
-                    const double2 w = (PRE && b == LO) ? pre.w[g * (M / 2) + e / 2] : itw[base + i];
+                    const double2 w = inv_w<C, P>(set, itw, g, b, e >> (b - LO + 1), tid);
                     gs_bfly_f(X, Y, w.x, w.y, nq);
                     gs_bfly_f(U, V, w.x, w.y, nq);
                 }
@@ -223,14 +264,14 @@ __device__ __forceinline__ void fwd_rest(double (&x)[C::E], double* lds, const d
                                          int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = ntt::wave_local<C, P>();
-        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
         if constexpr (P == 1) __syncthreads();
         else if constexpr (ntt::relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
         ntt::to_lds<C, P - 1, ntt::xshift<C>(P)>(x, lds, tid);
+        const FwdTw<C, P> w = fwd_tw<C, P>(tw, tid);
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, ntt::xshift<C>(P)>(x, lds, tid);
         ntt::exchange_sync<W>();
-        fwd_pass<C, P, true>(x, tw, nq, tid, pre);
+        fwd_pass<C, P>(x, w, tw, nq, tid);
         fwd_rest<C, P + 1>(x, lds, tw, nq, tid);
     }
 }
@@ -241,7 +282,7 @@ template <class C>
 __device__ __forceinline__ void fwd_ntt(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
                                         int tid) {
     opaque(tw, tid);
-    fwd_pass<C, 0>(x, tw, nq, tid);
+    fwd_pass<C, 0>(x, fwd_tw<C, 0>(tw, tid), tw, nq, tid);
     fwd_rest<C, 1>(x, lds, tw, nq, tid);
 }
 
@@ -250,13 +291,13 @@ __device__ __forceinline__ void inv_rest(double (&x)[C::E], double* lds, const d
                                          const PrimeF& pf, int tid) {
     if constexpr (P >= 0) {
         constexpr bool W = ntt::wave_local<C, P + 1>();
-        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
         if constexpr (ntt::relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
         ntt::to_lds<C, P + 1, ntt::xshift<C>(P + 1)>(x, lds, tid);
+        const InvTw<C, P> w = inv_tw<C, P>(itw, tid);
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, ntt::xshift<C>(P + 1)>(x, lds, tid);
         ntt::exchange_sync<W>();
-        inv_pass<C, P, true>(x, itw, pf, tid, pre);
+        inv_pass<C, P>(x, w, itw, pf, tid);
         inv_rest<C, P - 1>(x, lds, itw, pf, tid);
     }
 }
@@ -267,7 +308,7 @@ template <class C>
 __device__ __forceinline__ void inv_ntt(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
                                         const PrimeF& pf, int tid) {
     opaque(itw, tid);
-    inv_pass<C, C::NP - 1>(x, itw, pf, tid);
+    inv_pass<C, C::NP - 1>(x, inv_tw<C, C::NP - 1>(itw, tid), itw, pf, tid);
     inv_rest<C, C::NP - 2>(x, lds, itw, pf, tid);
 }
 
@@ -286,11 +327,11 @@ __device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, cons
         if constexpr (P == 1) __syncthreads();
         else if constexpr (P >= 3 && ntt::wave_local<C, P - 1>()) __syncthreads();
         double* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
-        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
         ntt::to_lds<C, P - 1, ntt::xshift<C>(P)>(x, buf, tid);
+        const FwdTw<C, P> w = fwd_tw<C, P>(tw, tid);
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, ntt::xshift<C>(P)>(x, buf, tid);
-        fwd_pass<C, P, true>(x, tw, nq, tid, pre);
+        fwd_pass<C, P>(x, w, tw, nq, tid);
         fwd_rest_db<C, P + 1>(x, lds, tw, nq, tid);
     }
 }
@@ -300,7 +341,7 @@ __device__ __forceinline__ void fwd_ntt_db(double (&x)[C::E], double* lds, const
                                            int tid) {
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     opaque(tw, tid);
-    fwd_pass<C, 0>(x, tw, nq, tid);
+    fwd_pass<C, 0>(x, fwd_tw<C, 0>(tw, tid), tw, nq, tid);
     fwd_rest_db<C, 1>(x, lds, tw, nq, tid);
 }
 
@@ -312,11 +353,11 @@ __device__ __forceinline__ void inv_rest_db(double (&x)[C::E], double* lds, cons
         if constexpr (P == C::NP - 2) __syncthreads();
         else if constexpr (P <= C::NP - 4 && ntt::wave_local<C, P + 2>()) __syncthreads();
         double* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
-        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
         ntt::to_lds<C, P + 1, ntt::xshift<C>(P + 1)>(x, buf, tid);
+        const InvTw<C, P> w = inv_tw<C, P>(itw, tid);
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, ntt::xshift<C>(P + 1)>(x, buf, tid);
-        inv_pass<C, P, true>(x, itw, pf, tid, pre);
+        inv_pass<C, P>(x, w, itw, pf, tid);
         inv_rest_db<C, P - 1>(x, lds, itw, pf, tid);
     }
 }
@@ -326,42 +367,43 @@ __device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const
                                            const PrimeF& pf, int tid) {
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     opaque(itw, tid);
-    inv_pass<C, C::NP - 1>(x, itw, pf, tid);
+    inv_pass<C, C::NP - 1>(x, inv_tw<C, C::NP - 1>(itw, tid), itw, pf, tid);
     inv_rest_db<C, C::NP - 2>(x, lds, itw, pf, tid);
 }
 
-// inv_ntt_db with the first pass's first-stage twiddles already loaded.
+// inv_ntt_db with the first pass's twiddles already loaded: a caller that
+// stores results just before the transform loads them first, since a wait
+// for a load also waits for every memory operation issued before it.
 template <class C>
 __device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
-                                           const PrimeF& pf, int tid, const InvFirst<C, C::NP - 1>& pre) {
+                                           const PrimeF& pf, int tid, const InvTw<C, C::NP - 1>& first) {
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     opaque(itw, tid);
-    inv_pass<C, C::NP - 1, true>(x, itw, pf, tid, pre);
+    inv_pass<C, C::NP - 1>(x, first, itw, pf, tid);
     inv_rest_db<C, C::NP - 2>(x, lds, itw, pf, tid);
 }
 
 // ---- two transforms of one prime, interleaved ------------------------------
-
-template <class C, int A, int B, bool W>
-__device__ __forceinline__ void exchange2(double (&x)[C::E], double (&y)[C::E], double* lds, int tid) {
-    constexpr int K = ntt::xshift<C>(A > B ? A : B);
-    ntt::to_lds<C, A, K>(x, lds, tid);
-    ntt::to_lds<C, A, K>(y, lds + C::LDS_WORDS, tid);
-    ntt::exchange_sync<W>();
-    ntt::from_lds<C, B, K>(x, lds, tid);
-    ntt::from_lds<C, B, K>(y, lds + C::LDS_WORDS, tid);
-    ntt::exchange_sync<W>();
-}
+// Both polynomials cross each exchange between one pair of barriers, in two
+// exchange buffers; the pass's twiddles (shared by both) are loaded between
+// the LDS writes and the first barrier.
 
 template <class C, int P>
 __device__ __forceinline__ void fwd_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
                                           const double2* __restrict__ tw, double nq, int tid) {
     if constexpr (P < C::NP) {
-        const FwdFirst<C, P> pre = fwd_first<C, P>(tw, tid);
+        constexpr bool W = ntt::wave_local<C, P>();
+        constexpr int K = ntt::xshift<C>(P);
         if constexpr (P == 1) __syncthreads();
         else if constexpr (ntt::relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
-        exchange2<C, P - 1, P, ntt::wave_local<C, P>()>(x, y, lds, tid);
-        fwd_pass2<C, P, true>(x, y, tw, nq, tid, pre);
+        ntt::to_lds<C, P - 1, K>(x, lds, tid);
+        ntt::to_lds<C, P - 1, K>(y, lds + C::LDS_WORDS, tid);
+        const FwdTw<C, P> w = fwd_tw<C, P>(tw, tid);
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, K>(x, lds, tid);
+        ntt::from_lds<C, P, K>(y, lds + C::LDS_WORDS, tid);
+        ntt::exchange_sync<W>();
+        fwd_pass2<C, P>(x, y, w, tw, nq, tid);
         fwd_rest2<C, P + 1>(x, y, lds, tw, nq, tid);
     }
 }
@@ -371,7 +413,7 @@ __device__ __forceinline__ void fwd_ntt2(double (&x)[C::E], double (&y)[C::E], d
                                          const double2* __restrict__ tw, double nq, int tid) {
     static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
     opaque(tw, tid);
-    fwd_pass2<C, 0>(x, y, tw, nq, tid);
+    fwd_pass2<C, 0>(x, y, fwd_tw<C, 0>(tw, tid), tw, nq, tid);
     fwd_rest2<C, 1>(x, y, lds, tw, nq, tid);
 }
 
@@ -379,10 +421,17 @@ template <class C, int P>
 __device__ __forceinline__ void inv_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
                                           const double2* __restrict__ itw, const PrimeF& pf, int tid) {
     if constexpr (P >= 0) {
-        const InvFirst<C, P> pre = inv_first<C, P>(itw, tid);
+        constexpr bool W = ntt::wave_local<C, P + 1>();
+        constexpr int K = ntt::xshift<C>(P + 1);
         if constexpr (ntt::relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
-        exchange2<C, P + 1, P, ntt::wave_local<C, P + 1>()>(x, y, lds, tid);
-        inv_pass2<C, P, true>(x, y, itw, pf, tid, pre);
+        ntt::to_lds<C, P + 1, K>(x, lds, tid);
+        ntt::to_lds<C, P + 1, K>(y, lds + C::LDS_WORDS, tid);
+        const InvTw<C, P> w = inv_tw<C, P>(itw, tid);
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, K>(x, lds, tid);
+        ntt::from_lds<C, P, K>(y, lds + C::LDS_WORDS, tid);
+        ntt::exchange_sync<W>();
+        inv_pass2<C, P>(x, y, w, itw, pf, tid);
         inv_rest2<C, P - 1>(x, y, lds, itw, pf, tid);
     }
 }
@@ -392,17 +441,29 @@ __device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], d
                                          const double2* __restrict__ itw, const PrimeF& pf, int tid) {
     static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
     opaque(itw, tid);
-    inv_pass2<C, C::NP - 1>(x, y, itw, pf, tid);
+    inv_pass2<C, C::NP - 1>(x, y, inv_tw<C, C::NP - 1>(itw, tid), itw, pf, tid);
     inv_rest2<C, C::NP - 2>(x, y, lds, itw, pf, tid);
 }
 
 // ---- HBM <-> registers -----------------------------------------------------
 
+// Issue every load of a row before any of its words is used: the empty
+// asm (a compiler memory barrier) keeps the loads above it, so the waits for
+// them come after the last one is issued instead of between small groups.
+template <class C>
+__device__ __forceinline__ void issue_row(uint64_t (&u)[C::E], const uint64_t* __restrict__ src, int tid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) u[e] = src[e * C::T + tid];
+    asm volatile("" ::: "memory");
+}
+
 // uint64 words below 2^52 in pass-0 (coalesced) layout.
 template <class C>
 __device__ __forceinline__ void load_words(double (&x)[C::E], const uint64_t* __restrict__ src, int tid) {
+    uint64_t u[C::E];
+    issue_row<C>(u, src, tid);
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) x[e] = u2d(src[e * C::T + tid]);
+    for (int e = 0; e < C::E; ++e) x[e] = u2d(u[e]);
 }
 
 // Canonical words (bounded by 16 q on entry) in pass-0 layout.

@@ -536,9 +536,10 @@ def test_rotate_hoisted_bit_exact_and_slots_move(ck):
 def test_every_launch_shape_gives_the_same_words(gpu, N):
     """HMult (fused and separate front), rotation and hoisted rotation under
     every built thread count of the transform, tensor/front and key-switch
-    kernels give the same words as the default shape, whose HMult equals the
-    C oracle: the launch shape (single or paired transforms, 8 to 32
-    coefficients per thread) never changes a result."""
+    kernels, and both key-switch placements, give the same words as the
+    default shape, whose HMult equals the C oracle: the launch shape (single
+    or paired transforms, 8 to 32 coefficients per thread) never changes a
+    result."""
     import itertools
     from cipherpu.ckks import CKKS, CKKSParams
     from cipherpu.ring import to_numpy
@@ -561,6 +562,7 @@ def test_every_launch_shape_gives_the_same_words(gpu, N):
     for front, ntt, tensor, ks in itertools.product((0, 1), BUILT[logN], BUILT_FUSED[logN], BUILT_FUSED[logN]):
         c.ring.set_front(bool(front))
         c.ring.set_threads(ntt, ks, tensor)
+        c.ring.set_ks_order(bool((ntt + ks) & 256))   # both key-switch placements across the grid
         got = run()
         shape = (front, ntt, tensor, ks)
         assert np.array_equal(got[0], ref[0]), ("hmult",) + shape
