@@ -53,7 +53,7 @@ struct Cfg {
 };
 
 template <class C, int K = C::PS>
-__device__ __forceinline__ int pad(int j) { return j + (j >> K); }
+__host__ __device__ __forceinline__ int pad(int j) { return j + (j >> K); }
 
 // Padding shift of the exchange between the layouts of passes P - 1 and P
 // (either direction).  Every exchange may pad differently: its two sides
@@ -87,7 +87,7 @@ __device__ __forceinline__ void opaque(const ulonglong2* __restrict__& tw, int& 
 
 // Global index of register (g, e) in the layout of pass P.
 template <class C, int P>
-__device__ __forceinline__ int index_of(int g, int e, int tid) {
+__host__ __device__ __forceinline__ int index_of(int g, int e, int tid) {
     constexpr int LO = C::lo(P), HI = C::hi(P);
     const int f = g * C::T + tid;
     return ((f >> LO) << (HI + 1)) | (e << LO) | (f & ((1 << LO) - 1));

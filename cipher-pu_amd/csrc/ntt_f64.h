@@ -383,67 +383,122 @@ __device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const
     inv_rest_db<C, C::NP - 2>(x, lds, itw, pf, tid);
 }
 
-// ---- two transforms of one prime, interleaved ------------------------------
-// Both polynomials cross each exchange between one pair of barriers, in two
-// exchange buffers; the pass's twiddles (shared by both) are loaded between
-// the LDS writes and the first barrier.
+// ---- two transforms of one prime, software-pipelined -----------------------
+// y runs half a pass behind x, each polynomial in its own exchange buffer
+// with its own barrier, so the LDS traffic of one overlaps the butterflies
+// of the other:
+//   for P = 1 .. NP-1:  write x (layout P-1); y's pass P-1; load pass P's
+//                       twiddles; sync x; read x (layout P); write y (layout
+//                       P-1); x's pass P; sync y; read y (layout P)
+//   then y's last pass.
+// One twiddle set is live at a time: pass P's serves x's pass P and then y's
+// (in the next round), and is replaced only after that.  Buffer reuse: a
+// wave writes a buffer in round P into the slots of the layout it read from
+// that buffer in round P-1, under round P's padding.  With the same padding
+// those are the very slots it read itself; with a different one they may be
+// slots another wave has still to read, so a workgroup sync must separate
+// the two.  For x the sync in between is round P-1's y sync (a workgroup
+// barrier unless exchange P-1 is wave-local); for y it is round P's x sync
+// (unless exchange P is wave-local).  Where the sync in between is
+// wave-level and the padding changes, a workgroup barrier comes first.
+template <class C, int X>
+constexpr bool y_relaid() {   // forward round X: y's write needs a workgroup barrier first
+    if constexpr (X < 2 || X >= C::NP) return false;
+    else return ntt::wave_local<C, X>() && ntt::xshift<C>(X) != ntt::xshift<C>(X - 1);
+}
+template <class C, int X>
+constexpr bool y_relaid_inv() {   // inverse round through exchange X
+    if constexpr (X < 1 || X + 1 >= C::NP) return false;
+    else return ntt::wave_local<C, X>() && ntt::xshift<C>(X) != ntt::xshift<C>(X + 1);
+}
 
 template <class C, int P>
 __device__ __forceinline__ void fwd_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                          const double2* __restrict__ tw, double nq, int tid) {
+                                          const double2* __restrict__ tw, const FwdTw<C, P - 1>& prev, double nq,
+                                          int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = ntt::wave_local<C, P>();
         constexpr int K = ntt::xshift<C>(P);
+        double* bx = lds;
+        double* by = lds + C::LDS_WORDS;
         if constexpr (P == 1) __syncthreads();
         else if constexpr (ntt::relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
-        ntt::to_lds<C, P - 1, K>(x, lds, tid);
-        ntt::to_lds<C, P - 1, K>(y, lds + C::LDS_WORDS, tid);
+        ntt::to_lds<C, P - 1, K>(x, bx, tid);
+        fwd_pass<C, P - 1>(y, prev, tw, nq, tid);
         const FwdTw<C, P> w = fwd_tw<C, P>(tw, tid);
         ntt::exchange_sync<W>();
-        ntt::from_lds<C, P, K>(x, lds, tid);
-        ntt::from_lds<C, P, K>(y, lds + C::LDS_WORDS, tid);
+        ntt::from_lds<C, P, K>(x, bx, tid);
+        if constexpr (y_relaid<C, P>()) __syncthreads();
+        ntt::to_lds<C, P - 1, K>(y, by, tid);
+        fwd_pass<C, P>(x, w, tw, nq, tid);
         ntt::exchange_sync<W>();
-        fwd_pass2<C, P>(x, y, w, tw, nq, tid);
-        fwd_rest2<C, P + 1>(x, y, lds, tw, nq, tid);
+        ntt::from_lds<C, P, K>(y, by, tid);
+        fwd_rest2<C, P + 1>(x, y, lds, tw, w, nq, tid);
+    } else {
+        fwd_pass<C, C::NP - 1>(y, prev, tw, nq, tid);
     }
 }
 
+// fwd_ntt on x and y at once (same entry and exit conditions).  On exit
+// other waves may still be reading y's buffer: a workgroup barrier precedes
+// any further LDS write (the transforms here start with one).
 template <class C>
 __device__ __forceinline__ void fwd_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
                                          const double2* __restrict__ tw, double nq, int tid) {
     static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
     opaque(tw, tid);
-    fwd_pass2<C, 0>(x, y, fwd_tw<C, 0>(tw, tid), tw, nq, tid);
-    fwd_rest2<C, 1>(x, y, lds, tw, nq, tid);
+    const FwdTw<C, 0> w0 = fwd_tw<C, 0>(tw, tid);
+    fwd_pass<C, 0>(x, w0, tw, nq, tid);
+    fwd_rest2<C, 1>(x, y, lds, tw, w0, nq, tid);
 }
 
 template <class C, int P>
 __device__ __forceinline__ void inv_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                          const double2* __restrict__ itw, const PrimeF& pf, int tid) {
+                                          const double2* __restrict__ itw, const InvTw<C, P + 1>& prev,
+                                          const PrimeF& pf, int tid) {
     if constexpr (P >= 0) {
         constexpr bool W = ntt::wave_local<C, P + 1>();
         constexpr int K = ntt::xshift<C>(P + 1);
-        if constexpr (ntt::relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
-        ntt::to_lds<C, P + 1, K>(x, lds, tid);
-        ntt::to_lds<C, P + 1, K>(y, lds + C::LDS_WORDS, tid);
+        double* bx = lds;
+        double* by = lds + C::LDS_WORDS;
+        if constexpr (P == C::NP - 2) __syncthreads();
+        else if constexpr (ntt::relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
+        ntt::to_lds<C, P + 1, K>(x, bx, tid);
+        inv_pass<C, P + 1>(y, prev, itw, pf, tid);
         const InvTw<C, P> w = inv_tw<C, P>(itw, tid);
         ntt::exchange_sync<W>();
-        ntt::from_lds<C, P, K>(x, lds, tid);
-        ntt::from_lds<C, P, K>(y, lds + C::LDS_WORDS, tid);
+        ntt::from_lds<C, P, K>(x, bx, tid);
+        if constexpr (y_relaid_inv<C, P + 1>()) __syncthreads();
+        ntt::to_lds<C, P + 1, K>(y, by, tid);
+        inv_pass<C, P>(x, w, itw, pf, tid);
         ntt::exchange_sync<W>();
-        inv_pass2<C, P>(x, y, w, itw, pf, tid);
-        inv_rest2<C, P - 1>(x, y, lds, itw, pf, tid);
+        ntt::from_lds<C, P, K>(y, by, tid);
+        inv_rest2<C, P - 1>(x, y, lds, itw, w, pf, tid);
+    } else {
+        inv_pass<C, 0>(y, prev, itw, pf, tid);
     }
 }
 
+// inv_ntt on x and y at once (same entry and exit conditions; the first
+// exchange starts with a workgroup barrier).  On exit other waves may still
+// be reading y's buffer: a workgroup barrier precedes any further LDS write.
 template <class C>
 __device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
                                          const double2* __restrict__ itw, const PrimeF& pf, int tid) {
     static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
     opaque(itw, tid);
-    inv_pass2<C, C::NP - 1>(x, y, inv_tw<C, C::NP - 1>(itw, tid), itw, pf, tid);
-    inv_rest2<C, C::NP - 2>(x, y, lds, itw, pf, tid);
+    const InvTw<C, C::NP - 1> w = inv_tw<C, C::NP - 1>(itw, tid);
+    inv_pass<C, C::NP - 1>(x, w, itw, pf, tid);
+    inv_rest2<C, C::NP - 2>(x, y, lds, itw, w, pf, tid);
 }
+
+// The built paired shapes: at N = 4096 with 512 threads exchanges 2 and 3
+// are wave-local and exchange 2 pads differently from exchange 1 (forward
+// round 2 needs the barrier); at N = 8192 with 1024 threads the one
+// wave-local exchange (4) keeps exchange 3's padding.
+static_assert(y_relaid<Cfg<12, 512>, 2>() && !y_relaid<Cfg<12, 512>, 3>());
+static_assert(!y_relaid_inv<Cfg<12, 512>, 2>());
+static_assert(!y_relaid<Cfg<13, 1024>, 4>() && !y_relaid<Cfg<13, 1024>, 3>());
 
 // ---- HBM <-> registers -----------------------------------------------------
 

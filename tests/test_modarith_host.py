@@ -58,3 +58,21 @@ def test_fp64_residue_arithmetic_against_128_bit_arithmetic(tmp_path):
     r = subprocess.run([exe, "200000"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith(", 0 failures"), r.stdout
+
+
+def test_lds_exchange_schedules_race_free(tmp_path):
+    """Every LDS exchange schedule of the FP64 transforms (single, paired and
+    software-pipelined, double-buffered, as the kernels chain them) for every
+    built launch shape: between two workgroup barriers no LDS slot is touched
+    by two waves with a write among them (tests/lds_schedule_check.cpp, with
+    the headers' own layouts, paddings and barrier predicates).  The checker
+    was seen to report the race of a pipelined pair without the barrier
+    before the second polynomial's write at N = 4096 with 512 threads (a
+    wave-local exchange that changes the padding), and a double-buffered
+    transform without its barrier after a wave-local exchange."""
+    exe = str(tmp_path / "lds_schedule_check")
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(HERE, "lds_schedule_check.cpp"), "-o", exe],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith(", 0 failures"), r.stdout
