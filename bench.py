@@ -96,10 +96,10 @@ def kernel_model_bytes(kind: str, batch: int, l: int = L):
 # copy, and wave-level VALU instructions (SQ_INSTS_VALU).  Keyed by (batch,
 # key-switch threads); the bench only runs shapes listed here.
 PROFILE_SOURCE = "profiles/hmult_kernels_n8192_l4_b256.md"
-MEASURED_TRAFFIC = {("front", 256, 1024): 558112345, ("keyswitch_fused", 256, 1024): 636649116,
-                    ("keyswitch", 256, 1024): 256452104}
-MEASURED_VALU_INSTR = {("front", 256, 1024): 134250496, ("keyswitch_fused", 256, 1024): 115593216,
-                       ("keyswitch", 256, 1024): 65171456}
+MEASURED_TRAFFIC = {("front", 256, 1024): 547846604, ("keyswitch_fused", 256, 1024): 504481086,
+                    ("keyswitch", 256, 1024): 235447492}
+MEASURED_VALU_INSTR = {("front", 256, 1024): 52002816, ("keyswitch_fused", 256, 1024): 50368512,
+                       ("keyswitch", 256, 1024): 30908416}
 PROFILED_THREADS = (1024,)   # key-switch and front/tensor threads of the profiled shapes
 # The issue peak counts one wave64 VALU instruction per SIMD every 4 cycles:
 # 1024 SIMDs x 2.4 GHz / 4.
@@ -520,12 +520,6 @@ def main():
         prof = B in {b for _, b, _ in MEASURED_TRAFFIC}
         tuned = tune_hmult(ck, cta, ctb, rlk, reps=10, rounds=4, ks=PROFILED_THREADS if prof else None,
                            tensor=PROFILED_THREADS if prof else None)
-    out = ck.hmult(cta, ctb, rlk)
-    torch.cuda.synchronize()
-    # decrypted product vs the FP64 slot-wise product (all pairs)
-    dec = to_numpy(ck.decode(ck.decrypt(out, sk), out.scale)).real
-    max_err = float(np.max(np.abs(dec - za * zb)))
-
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         ck.hmult(cta, ctb, rlk)
@@ -550,6 +544,12 @@ def main():
     torch.cuda.synchronize()
     kern = ck.ring.timing_stop()
     ms_kernel_timed = ev0.elapsed_time(ev1) / args.steps
+    # decrypted product vs the FP64 slot-wise product (all pairs), after the
+    # timed steps so that the host-side check leaves no idle gap before them
+    out = ck.hmult(cta, ctb, rlk)
+    torch.cuda.synchronize()
+    dec = to_numpy(ck.decode(ck.decrypt(out, sk), out.scale)).real
+    max_err = float(np.max(np.abs(dec - za * zb)))
     value = B / (ms / 1e3)
     nbytes = algorithmic_bytes(B)
     shape = ({k: tuned[k] for k in ("front", "ntt", "tensor", "ks")} if tuned else
