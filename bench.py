@@ -315,6 +315,38 @@ def product_leg(count: int = 4096, reps: int = 20) -> dict:
                          "algorithmic_bytes_per_product": PRODUCT_BYTES}}
 
 
+def arith_leg(ck, cta, ctb, rlk, want, fp64: bool, steps: int, warmup: int) -> dict:
+    """The same HMult batch on the other residue arithmetic: the key prepared
+    again from its coefficients, the same launch shapes, K timed steps, and
+    the output words checked equal to the headline run's."""
+    import torch
+    ck.ring.set_arith(fp64)
+    key = ck.load_switch_key(rlk.coeff)
+    for _ in range(warmup):
+        ck.hmult(cta, ctb, key)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(steps):
+        ck.hmult(cta, ctb, key)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    ck.ring.timing_start(8 * steps)
+    for _ in range(steps):
+        ck.hmult(cta, ctb, key)
+    torch.cuda.synchronize()
+    kern = ck.ring.timing_stop()
+    same = bool(torch.equal(ck.hmult(cta, ctb, key).data, want.data))
+    ck.ring.set_arith(not fp64)
+    if not same:
+        raise RuntimeError("the two residue arithmetics gave different HMult words")
+    return {"arith": "fp64" if fp64 else "int64", "value": round(cta.batch / (ms / 1e3), 1), "unit": "HMult/s",
+            "ms_per_step": round(ms, 4), "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
+            "same_words": same}
+
+
 def cpu_product_baseline(products: int = 2048) -> dict:
     """The C oracle's transform product (N = 4096, the single 45-bit prime)
     on one host core, for the plain-product leg."""
@@ -479,6 +511,10 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the uneven-batch graph leg")
     ap.add_argument("--no-product", action="store_true", help="skip the N=4096 single-prime product leg")
     ap.add_argument("--graph-ops", type=int, default=2048)
+    ap.add_argument("--arith", choices=("fp64", "int64"), default="fp64",
+                    help="residue arithmetic of the timed HMult (ckks_ctx_set_arith)")
+    ap.add_argument("--no-arith-leg", action="store_true",
+                    help="skip timing the other residue arithmetic on the same batch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the library runs on a single MI355X and has no "
@@ -502,6 +538,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     B = args.batch
     ck = CKKS(CKKSParams(), seed=2024)
+    ck.ring.set_arith(args.arith == "fp64")
     sk = ck.keygen()
     rlk = ck.relin_key(sk)
     rng = np.random.default_rng(7)
@@ -570,16 +607,17 @@ def main():
     kbytes = kernel_model_bytes(dom, B)
     k_achieved = nbytes / (dom_ms / 1e3) / 1e9
     prof_key = (dom, B, threads)
-    if prof_key in MEASURED_TRAFFIC:
+    if prof_key in MEASURED_TRAFFIC and args.arith == "fp64":
         traffic = MEASURED_TRAFFIC[prof_key]
         rate = MEASURED_VALU_INSTR[prof_key] / (dom_ms / 1e3) / 1e9
         valu = {"wave_instructions": MEASURED_VALU_INSTR[prof_key], "achieved": round(rate, 1),
                 "peak": VALU_ISSUE_PEAK_G, "unit": "G wave-instr/s", "frac": round(rate / VALU_ISSUE_PEAK_G, 4),
                 "source": PROFILE_SOURCE}
-    elif B == 256 and not (args.ntt_threads or args.ks_threads or args.tensor_threads or args.no_tune):
+    elif B == 256 and args.arith == "fp64" and not (args.ntt_threads or args.ks_threads or args.tensor_threads
+                                                    or args.no_tune):
         raise RuntimeError(f"no counter profile for {dom} at batch {B}, {threads} threads")
     else:   # an explicit shape or batch outside the profiled ones: say so
-        traffic, valu = None, {"source": f"not profiled: {dom} at batch {B}, {threads} threads"}
+        traffic, valu = None, {"source": f"not profiled: {dom} at batch {B}, {threads} threads, {args.arith}"}
 
     res = {
         "metric": "HMult+relin+rescale/s at N=2^13 L=4",
@@ -593,10 +631,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64",
+        "dtype": "f64" if args.arith == "fp64" else "u64",
         "data": "synthetic",
         "config": {"workload": "ckks_hmult_relin_rescale", "N": N, "L": L, "dnum": L, "special_primes": 1,
                    "batch": B, "scale_bits": 40, "prime_bits": 45, "domain": "coefficient in/out",
+                   "residue_arith": args.arith,
                    "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": round(k_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k_achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -618,6 +657,8 @@ def main():
         "wall_s_timed": round(wall, 4),
         "device": torch.cuda.get_device_name(0),
     }
+    if not args.no_arith_leg:
+        res["other_arith"] = arith_leg(ck, cta, ctb, rlk, out, args.arith != "fp64", args.steps, args.warmup)
     if not args.no_rotation_probe:
         res["rotation_probe"] = rotation_probe(ck, sk)
     if not args.no_product:

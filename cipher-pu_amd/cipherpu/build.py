@@ -17,8 +17,8 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 INCLUDE = os.path.join(os.path.dirname(PKG_ROOT), "include")
 BUILD = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(HERE, "libcipherpu_hip.so")
-SOURCES = ["k_ntt.hip", "k_ckks.hip", "k_fft.hip", "k_gear.hip", "k_sample.hip", "k_hoist.hip", "k_mfma_probe.hip",
-           "capi.cpp"]
+SOURCES = ["k_ntt.hip", "k_ckks.hip", "k_ckks_int.hip", "k_fft.hip", "k_gear.hip", "k_sample.hip", "k_hoist.hip",
+           "k_mfma_probe.hip", "capi.cpp"]
 ARCH = os.environ.get("CIPHERPU_ARCH", "gfx950")
 
 

@@ -78,8 +78,9 @@ class SecretKey:
 @dataclass
 class SwitchKey:
     coeff: torch.Tensor         # [L, 2, L+1, N] uint64, coefficient form (b_j, a_j)
-    prepared: torch.Tensor      # [L, 2, L+1, N, 2] uint64 (value, Shoup), evaluation order
+    prepared: torch.Tensor      # [L, 2, L+1, N, 2] uint64 prepared pairs, evaluation order
     galois: int = 0             # 0 for the relinearisation key
+    fp64: bool = True           # the arithmetic the pairs were prepared for (RingContext.set_arith)
     hoisted: Optional[torch.Tensor] = None   # prepared, permuted for ckks_rotate_hoisted
 
 
@@ -177,7 +178,7 @@ class CKKS:
         prepared = torch.empty(L, 2, K, self.N, 2, dtype=torch.uint64, device=self.device)
         work = coeff.clone()
         call("ckks_key_prepare", self.ring.handle, _ptr(work), _ptr(prepared), L, K, self.sp, self.ring.stream())
-        return SwitchKey(coeff=coeff, prepared=prepared, galois=galois)
+        return SwitchKey(coeff=coeff, prepared=prepared, galois=galois, fp64=self.ring.fp64)
 
     def load_switch_key(self, coeff, galois: int = 0) -> SwitchKey:
         """Key from its coefficient form [L, 2, L+1, N] (b_j, a_j) -- e.g. read
@@ -189,7 +190,7 @@ class CKKS:
         prepared = torch.empty(L, 2, K, self.N, 2, dtype=torch.uint64, device=self.device)
         work = coeff.clone()
         call("ckks_key_prepare", self.ring.handle, _ptr(work), _ptr(prepared), L, K, self.sp, self.ring.stream())
-        return SwitchKey(coeff=coeff, prepared=prepared, galois=galois)
+        return SwitchKey(coeff=coeff, prepared=prepared, galois=galois, fp64=self.ring.fp64)
 
     def relin_key(self, sk: SecretKey, **kw) -> SwitchKey:
         s2 = self.ring.poly_mul(sk.rns, sk.rns)
@@ -252,8 +253,16 @@ class CKKS:
             self._ws[key] = ws
         return ws
 
+    def check_keys(self, *keys: SwitchKey):
+        """Prepared keys are read in the arithmetic they were prepared for."""
+        for k in keys:
+            if k.fp64 != self.ring.fp64:
+                raise ValueError("key prepared for the %s arithmetic, context set to %s: prepare it again"
+                                 % ("FP64" if k.fp64 else "INT64", "FP64" if self.ring.fp64 else "INT64"))
+
     def hmult(self, a: Ciphertext, b: Ciphertext, rlk: SwitchKey) -> Ciphertext:
         """HMult + relinearise + rescale (fused; verify_e2e_mult.py:78-102)."""
+        self.check_keys(rlk)
         if a.data.shape != b.data.shape:
             raise ValueError("ciphertext shapes differ")
         B, l = a.batch, a.level
@@ -266,6 +275,7 @@ class CKKS:
     def relin(self, d0: torch.Tensor, d1: torch.Tensor, d2: torch.Tensor, rlk: SwitchKey) -> torch.Tensor:
         """(d0, d1, d2) [B, l, N] each -> [B, 2, l, N] = (d0, d1) + keyswitch(d2)
         (verify_keyswitch.py:62-68 generalised to the hybrid key)."""
+        self.check_keys(rlk)
         B, l = d2.shape[0], d2.shape[1]
         d01 = torch.stack([d0, d1], dim=1).contiguous()
         out = torch.empty(B, 2, l, self.N, dtype=torch.uint64, device=self.device)
@@ -275,6 +285,7 @@ class CKKS:
         return out
 
     def keyswitch(self, d2: torch.Tensor, key: SwitchKey) -> torch.Tensor:
+        self.check_keys(key)
         B, l = d2.shape[0], d2.shape[1]
         out = torch.empty(B, 2, l, self.N, dtype=torch.uint64, device=self.device)
         ws = self.workspace(B, l)
@@ -295,6 +306,7 @@ class CKKS:
 
     def rotate(self, ct: Ciphertext, gk: SwitchKey) -> Ciphertext:
         """Slot rotation by the step gk was made for."""
+        self.check_keys(gk)
         B, l = ct.batch, ct.level
         out = torch.empty_like(ct.data)
         ws = self.workspace(B, l)
@@ -319,6 +331,7 @@ class CKKS:
         B, l = ct.batch, ct.level
         if len(gks) != B:
             raise ValueError("one Galois key per ciphertext")
+        self.check_keys(*gks)
         keys, gal = tables if tables is not None else self.rotation_tables(gks, self.device)
         out = torch.empty_like(ct.data)
         ws = self.workspace(B, l)
@@ -348,6 +361,7 @@ class CKKS:
         B, l, R = ct.batch, ct.level, len(gks)
         if R == 0:
             return []
+        self.check_keys(*gks)
         if keys is None:
             keys = torch.stack([self.hoist_key(g) for g in gks]).contiguous()
         if out is None:

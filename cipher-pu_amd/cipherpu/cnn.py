@@ -37,7 +37,7 @@ import torch
 from ._lib import call
 from .ckks import CKKS, Ciphertext, SecretKey
 from .nn import GEAR_FRACTIONS, SLOTS_PER_SAMPLE, EncryptedNet, block_keep_mask
-from .ring import prepared_pairs, to_numpy
+from .ring import to_numpy
 
 ROW = 16          # row stride of the polyphase grids and of the output grid
 PHASE = 256       # slots per polyphase component
@@ -146,9 +146,9 @@ class EncryptedCNN(EncryptedNet):
 
     # -- plaintext material -------------------------------------------------
     def _conv_weights(self, level: int, scale: float) -> torch.Tensor:
-        """[8][taps][level] prepared pairs (the doubles r and RN(r / q)) of the
+        """[8][taps][level] prepared pairs (ring.prepared_pairs) of the
         residues r of round(K * scale)."""
-        key = (self.gears[0], level, scale)
+        key = (self.gears[0], level, scale, self.ck.ring.fp64)
         if key not in self._conv:
             ck = self.ck
             D = np.zeros((N_CH, len(self.taps), level, 2), dtype=np.uint64)
@@ -158,7 +158,7 @@ class EncryptedCNN(EncryptedNet):
                     for i in range(level):
                         q = ck.primes[i]
                         r = w % q
-                        D[c, b, i] = prepared_pairs(r, q)
+                        D[c, b, i] = ck.ring.prepared_pairs(r, q)
             self._conv[key] = ck.ring.tensor(D)
         return self._conv[key]
 
@@ -237,6 +237,7 @@ class EncryptedCNN(EncryptedNet):
         return [Ciphertext(zc.data[i * B:(i + 1) * B], zc.scale) for i in range(len(groups))]
 
     def forward(self, ct: Ciphertext) -> List[Ciphertext]:
+        self.check_arith()
         B = ct.batch
         y = self.conv(ct)
         y = self.dare(y)

@@ -223,6 +223,12 @@ class EncryptedNet:
         self._rtab: Dict[tuple, tuple] = {}
         self.ops = {"rot": 0, "ptmul_rows": 0, "hmult": 0, "dare": 0}
         self.gears = (0, 0)
+        self.fp64 = ck.ring.fp64    # prepared plaintexts and keys are in this arithmetic
+
+    def check_arith(self):
+        if self.ck.ring.fp64 != self.fp64:
+            raise ValueError("the context's arithmetic changed since this network prepared its keys and "
+                             "plaintexts: build the network again")
 
     # -- keys -------------------------------------------------------------
     def gk(self, step: int) -> SwitchKey:
@@ -332,8 +338,8 @@ class EncryptedNet:
         return ck.encrypt(top, self.sk, scale=ct.scale)
 
     def prepared_pt(self, z: np.ndarray, level: int, scale: float) -> torch.Tensor:
-        """Slot vectors [rows, N/2] -> evaluation-form prepared pairs (the
-        doubles w and RN(w / q)) [rows, level, N, 2] for ckks_lincomb."""
+        """Slot vectors [rows, N/2] -> evaluation-form prepared pairs
+        (ring.prepared_pairs) [rows, level, N, 2] for ckks_lincomb."""
         ck = self.ck
         pt = ck.encode(np.asarray(z, dtype=np.complex128), level=level, scale=scale)
         prep = torch.empty(*pt.shape, 2, dtype=torch.uint64, device=ck.device)
@@ -492,6 +498,7 @@ class EncryptedMLP(EncryptedNet):
         return self._linear(ct, self.p2)
 
     def forward(self, ct: Ciphertext) -> Ciphertext:
+        self.check_arith()
         h = self.dare(self.layer1(ct))
         return self.layer2(self.activation(h))
 

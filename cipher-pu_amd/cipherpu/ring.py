@@ -55,6 +55,7 @@ class RingContext:
             call("ckks_ctx_create", ctypes.byref(self._h), self.N, len(self.primes), pr, rt,
                  _lib.CKKS_CYCLIC if cyclic else 0, device)
         r = ctypes.c_uint64()
+        self.fp64 = True      # the context's arithmetic (set_arith)
         self.roots = []
         for i in range(len(self.primes)):
             call("ckks_ctx_root", self._h, i, ctypes.byref(r))
@@ -85,9 +86,17 @@ class RingContext:
         call("ckks_ctx_set_ks_order", self._h, 1 if pair_major else 0)
 
     def set_arith(self, fp64: bool = True):
-        """Transform kernels on the FP64 pipeline (True) or on 64-bit integer
-        words (False); the results are the same words."""
+        """Kernels on the FP64 residue pipeline (True) or on 64-bit integer
+        words with Shoup and Barrett products (False); the results are the
+        same words.  Prepared keys and plaintexts are in the form of the
+        arithmetic in force when they were prepared (``fp64``) and must be
+        prepared again after a change."""
         call("ckks_ctx_set_arith", self._h, 1 if fp64 else 0)
+        self.fp64 = bool(fp64)
+
+    def prepared_pairs(self, w, q) -> np.ndarray:
+        """prepared_pairs in this context's arithmetic."""
+        return prepared_pairs(w, q, self.fp64)
 
     def set_hoist_order(self, btile: int = 0, xcd: bool = False):
         """Workgroup placement of the hoisted key switch (ckks_ctx_set_hoist_order)."""
@@ -266,13 +275,37 @@ class RingContext:
         return out
 
 
-def prepared_pairs(w, q) -> np.ndarray:
+def prepared_pairs(w, q, fp64: bool = True) -> np.ndarray:
     """Reduced words w < q -> the prepared-pair form that the key switches
-    and the plaintext products read: the IEEE doubles w and RN(w / q), as
-    uint64 bit patterns, shape w.shape + (2,).  q broadcasts against w."""
-    wd = np.asarray(w, dtype=np.uint64).astype(np.float64)   # exact below 2^53
-    qd = np.asarray(q, dtype=np.float64)
-    return np.ascontiguousarray(np.stack([wd, wd / qd], axis=-1)).view(np.uint64)
+    and the plaintext products read, shape w.shape + (2,) uint64; q
+    broadcasts against w.  FP64 arithmetic: the IEEE doubles w and
+    RN(w / q), as bit patterns.  Integer arithmetic: w and its Shoup
+    companion floor(w 2^63 / q)."""
+    w = np.asarray(w, dtype=np.uint64)
+    if fp64:
+        wd = w.astype(np.float64)                             # exact below 2^53
+        qd = np.asarray(q, dtype=np.float64)
+        return np.ascontiguousarray(np.stack([wd, wd / qd], axis=-1)).view(np.uint64)
+    wb, qb = np.broadcast_arrays(w, np.asarray(q, dtype=np.uint64))
+    return np.ascontiguousarray(np.stack([wb, shoup_companion(wb, qb)], axis=-1))
+
+
+def shoup_companion(w, q) -> np.ndarray:
+    """floor(w 2^63 / q) for words w < q < 2^63, by long division in chunks
+    of s bits with q 2^s < 2^64, so that no step overflows."""
+    w, q = np.broadcast_arrays(np.asarray(w, dtype=np.uint64), np.asarray(q, dtype=np.uint64))
+    if w.size == 0:
+        return np.zeros(w.shape, dtype=np.uint64)
+    step = 64 - int(q.max()).bit_length()
+    r, quo, left = w.copy(), np.zeros(w.shape, dtype=np.uint64), 63
+    while left:
+        k = np.uint64(min(step, left))
+        r = r << k
+        d = r // q
+        r = r - d * q
+        quo = (quo << k) | d
+        left -= int(k)
+    return quo
 
 
 def bitrev_(x: torch.Tensor) -> torch.Tensor:

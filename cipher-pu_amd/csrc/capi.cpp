@@ -626,7 +626,7 @@ int ckks_key_prepare(const ckks_ctx* c, const uint64_t* key_coeff, uint64_t* key
     const DevTables t = c->tables();
     uint64_t* tmp = const_cast<uint64_t*>(key_coeff);   // transformed in place (documented)
     HIPCHK(launch_ntt(t, tmp, tmp, dnum * 2, key_limbs, m, false, true, c->ntt_threads, S(stream)));
-    HIPCHK(launch_fp64_pairs(t, tmp, (double2*)key_out, dnum * 2, key_limbs, m, S(stream)));
+    HIPCHK(launch_prepare_pairs(t, tmp, key_out, dnum * 2, key_limbs, m, S(stream)));
     return CKKS_OK;
 }
 
@@ -655,7 +655,7 @@ int ckks_relin(const ckks_ctx* c, const uint64_t* d01, const uint64_t* d2, const
     const ModDownConst mc = moddown_const(c, l, special);
     uint64_t* acc = (uint64_t*)workspace;
     const size_t LN = (size_t)l * c->N;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2, LN, nullptr, nullptr, (const double2*)key, key_limbs, acc,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2, LN, nullptr, nullptr, key, key_limbs, acc,
                                                 batch, l, m, special, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, d01, 3, out, batch, l, m, special, mc, false, S(stream)));
     return CKKS_OK;
@@ -698,9 +698,9 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
     rest.out = out;
     rest.mc = &mc;
     rest.pair_major = c->ks_pair_major;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2c, LN, d2e, d01, (const double2*)key, key_limbs, acc, batch,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, d2c, LN, d2e, d01, key, key_limbs, acc, batch,
                                                 l, m, special, c->ks_threads, S(stream), nullptr, tail));
-    TIMED(CKKS_KIND_KEYSWITCH_FUSED, launch_keyswitch(t, d2c, LN, d2e, d01, (const double2*)key, key_limbs, acc,
+    TIMED(CKKS_KIND_KEYSWITCH_FUSED, launch_keyswitch(t, d2c, LN, d2e, d01, key, key_limbs, acc,
                                                       batch, l, m, special, c->ks_threads, S(stream), nullptr, rest));
     return CKKS_OK;
 }
@@ -747,7 +747,7 @@ int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int
     uint64_t* sct = (uint64_t*)workspace;   // sigma(c0), sigma(c1)
     uint64_t* acc = sct + B * 2 * LN;
     TIMED(CKKS_KIND_AUTOMORPHISM, launch_automorphism(t, ct, sct, batch * 2, l, m, galois, S(stream)));
-    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, (const double2*)gkey,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, gkey,
                                                 key_limbs, acc, batch, l, m, special, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream)));
     return CKKS_OK;
@@ -823,7 +823,7 @@ int ckks_rotate_hoisted(const ckks_ctx* c, const uint64_t* ct, const uint64_t* k
                                                                  2 * LN, m, galois[r], S(stream)));
     // one prepared key per rotation: [dnum = key_limbs - 1][2][key_limbs][N] pairs
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const double2*)keys, key_stride, key_limbs, gl, acc, batch,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, keys, key_stride, key_limbs, gl, acc, batch,
                                                  l, m, special, R, 0, c->hoist, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch * R, l, m, special, mc, false,
                                                     S(stream)));
@@ -854,7 +854,7 @@ int ckks_rotate_hoisted_eval(const ckks_ctx* c, const uint64_t* ct, const uint64
     TIMED(CKKS_KIND_KEYSWITCH, launch_modup(t, ct, D, batch, l, m, special, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_NTT, launch_ntt(t, cn, ct, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
     const size_t key_stride = (size_t)(key_limbs - 1) * 2 * key_limbs * c->N;
-    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, (const double2*)keys, key_stride, key_limbs, gl, acc, batch,
+    TIMED(CKKS_KIND_KEYSWITCH, launch_ks_hoisted(t, D, keys, key_stride, key_limbs, gl, acc, batch,
                                                  l, m, special, R, 1, c->hoist, c->ks_threads, S(stream)));
     TIMED(CKKS_KIND_MODDOWN, launch_moddown_eval(t, acc, cn, out, batch, l, m, special, mc, gl, R, c->ks_threads,
                                                  S(stream)));
@@ -1022,7 +1022,7 @@ int ckks_pt_prepare(const ckks_ctx* c, uint64_t* pt, uint64_t* out, int n_polys,
     const DevTables t = c->tables();
     const LimbMap m = limb_map(prime0, l);
     HIPCHK(launch_ntt(t, pt, pt, n_polys, l, m, false, true, c->ntt_threads, S(stream)));
-    HIPCHK(launch_fp64_pairs(t, pt, (double2*)out, n_polys, l, m, S(stream)));
+    HIPCHK(launch_prepare_pairs(t, pt, out, n_polys, l, m, S(stream)));
     return CKKS_OK;
 }
 
@@ -1034,7 +1034,7 @@ int ckks_lincomb(const ckks_ctx* c, const uint64_t* E, const uint64_t* D, uint64
     if (!E || !D || !out) return CKKS_ERR_NULL;
     if (misaligned(E) || misaligned_pairs(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
-    HIPCHK(launch_lincomb(c->tables(), E, (const double2*)D, out, nb, ng, batch, l, limb_map(0, l), 0, S(stream)));
+    HIPCHK(launch_lincomb(c->tables(), E, D, out, nb, ng, batch, l, limb_map(0, l), 0, S(stream)));
     return CKKS_OK;
 }
 
@@ -1046,7 +1046,7 @@ int ckks_lincomb_scalar(const ckks_ctx* c, const uint64_t* E, const uint64_t* D,
     if (!E || !D || !out) return CKKS_ERR_NULL;
     if (misaligned(E) || misaligned_pairs(D) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
-    HIPCHK(launch_lincomb(c->tables(), E, (const double2*)D, out, nb, ng, batch, l, limb_map(0, l), 1, S(stream)));
+    HIPCHK(launch_lincomb(c->tables(), E, D, out, nb, ng, batch, l, limb_map(0, l), 1, S(stream)));
     return CKKS_OK;
 }
 

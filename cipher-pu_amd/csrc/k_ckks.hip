@@ -864,6 +864,7 @@ hipError_t launch_tensor_intt(const DevTables& t, const uint64_t* ea, const uint
                               const ModDownConst& mc, int threads, hipStream_t s) {
     if (batch * l == 0) return hipSuccess;
     if (!threads) threads = default_threads(t.logN);
+    if (!t.fp64) return launch_tensor_intt_int(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, threads, s);
     switch (t.logN * 10000 + threads) {
         case 120256: return run_tensor<12, 256>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
         case 120512: return run_tensor<12, 512>(t, ea, eb, d01, d2e, d2c, batch, l, map, mc, s);
@@ -881,6 +882,7 @@ hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b
                         hipStream_t s) {
     if (batch * l == 0) return hipSuccess;
     if (!threads) threads = default_threads(t.logN);
+    if (!t.fp64) return launch_front_int(t, a, b, d01, d2e, d2c, batch, l, map, mc, threads, s);
     switch (t.logN * 10000 + threads) {
         case 120256: return run_front<12, 256>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
         case 120512: return run_front<12, 512>(t, a, b, d01, d2e, d2c, batch, l, map, mc, s);
@@ -895,13 +897,17 @@ hipError_t launch_front(const DevTables& t, const uint64_t* a, const uint64_t* b
 
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01,
-                            const double2* key, int key_limbs, uint64_t* acc, int batch, int l,
+                            const void* key_pairs, int key_limbs, uint64_t* acc, int batch, int l,
                             const LimbMap& map, int pidx_p, int threads, hipStream_t s, const uint64_t* key_table,
                             const KsTargets& tg) {
     if (batch * l == 0) return hipSuccess;
     if (tg.out && (key_table || tg.first < 0 || tg.first + tg.count > l - 1 || !tg.mc)) return hipErrorInvalidValue;
     if (tg.count < 0 || tg.first < 0 || tg.first + (tg.count ? tg.count : l + 1) > l + 1) return hipErrorInvalidValue;
     if (!threads) threads = default_threads(t.logN);
+    if (!t.fp64)
+        return launch_keyswitch_int(t, d2c, d2c_bstride, d2e, d01, (const ulonglong2*)key_pairs, key_limbs, acc,
+                                    batch, l, map, pidx_p, threads, s, key_table, tg);
+    const double2* key = (const double2*)key_pairs;
     switch (t.logN * 10000 + threads) {
         case 120256: return run_ks<12, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
         case 120512: return run_ks<12, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
@@ -1057,10 +1063,12 @@ hipError_t run_lincomb_blk(const DevTables& t, const uint64_t* E, const double2*
 
 }  // namespace
 
-hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const double2* D, uint64_t* out, int nb, int ng,
+hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const void* D_pairs, uint64_t* out, int nb, int ng,
                           int batch, int l, const LimbMap& map, int scalar, hipStream_t s) {
     const int rows = batch * 2;
     if (!rows || !ng || !l) return hipSuccess;
+    if (!t.fp64) return launch_lincomb_int(t, E, (const ulonglong2*)D_pairs, out, nb, ng, batch, l, map, scalar, s);
+    const double2* D = (const double2*)D_pairs;
     const size_t dn = scalar ? 1 : ((size_t)1 << t.logN);
     const size_t LN = (size_t)l << t.logN;
     // giant steps in chunks of at most 16

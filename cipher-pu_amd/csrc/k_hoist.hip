@@ -177,6 +177,136 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
     }
 }
 
+// Reduce (and negate) an accumulator whose register k holds evaluation slot
+// k T + tid (the coalesced order the key products are summed in), apply
+// pi_g through LDS, inverse-transform and store it in coefficient form.
+template <class C>
+__device__ __forceinline__ void permute_inverse_i(uint64_t (&a)[C::E], bool negate, uint32_t g, uint64_t* lds,
+                                                const ulonglong2* __restrict__ itw, const PrimeConst& pc,
+                                                uint64_t* __restrict__ out, int tid) {
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const uint64_t v = reduce64(a[k], pc);
+        lds[ntt::pad<C>(k * C::T + tid)] = (negate && v) ? pc.q - v : v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a[k] = lds[ntt::pad<C>(eval_perm(ntt::eval_index<C>(k, tid), g, C::LOGN))];
+    __syncthreads();
+    ntt::inv_ntt<C>(a, lds, itw, pc, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a[k] = csub(a[k], pc.q);
+    ntt::store_coalesced<C>(a, out, tid);
+    __syncthreads();
+}
+
+// Reduce (and negate) an accumulator whose register k holds evaluation slot
+// k T + tid and
+// store pi_g of it in evaluation order: one LDS round trip, coalesced stores.
+template <class C>
+__device__ __forceinline__ void permute_store_eval_i(uint64_t (&a)[C::E], bool negate, uint32_t g, uint64_t* lds,
+                                                   const PrimeConst& pc, uint64_t* __restrict__ out, int tid) {
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const uint64_t v = reduce64(a[k], pc);
+        lds[ntt::pad<C>(k * C::T + tid)] = (negate && v) ? pc.q - v : v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+        const int j = e * C::T + tid;
+        out[j] = lds[ntt::pad<C>(eval_perm(j, g, C::LOGN))];
+    }
+    __syncthreads();
+}
+
+// permute_inverse_i for both accumulators at once (the second negated): two
+// LDS buffers, one gather each, one transform pair.
+template <class C>
+__device__ __forceinline__ void permute_inverse2_i(uint64_t (&a0)[C::E], uint64_t (&a1)[C::E], uint32_t g,
+                                                 uint64_t* lds, const ulonglong2* __restrict__ itw,
+                                                 const PrimeConst& pc, uint64_t* __restrict__ out0,
+                                                 uint64_t* __restrict__ out1, int tid) {
+    uint64_t* lds1 = lds + C::LDS_WORDS;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int slot = ntt::pad<C>(k * C::T + tid);
+        lds[slot] = reduce64(a0[k], pc);
+        const uint64_t v = reduce64(a1[k], pc);
+        lds1[slot] = v ? pc.q - v : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int src = ntt::pad<C>(eval_perm(ntt::eval_index<C>(k, tid), g, C::LOGN));
+        a0[k] = lds[src];
+        a1[k] = lds1[src];
+    }
+    __syncthreads();
+    ntt::inv_ntt2<C>(a0, a1, lds, itw, pc, tid);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        a0[k] = csub(a0[k], pc.q);
+        a1[k] = csub(a1[k], pc.q);
+    }
+    ntt::store_coalesced<C>(a0, out0, tid);
+    ntt::store_coalesced<C>(a1, out1, tid);
+}
+
+// k_ks_hoisted on 64-bit integer words (keys of (value, Shoup companion)
+// pairs), for contexts set to CKKS_ARITH_INT64.  One workgroup per (target
+// t, ciphertext b, rotation r), placed by hoist_slot.  keys: [R][dnum][2][key_limbs][N]
+// (value, companion) pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N]
+// coefficient form, the layout ckks_relin's ModDown reads.
+template <int LOGN, int T, bool EVAL>
+__global__ void __launch_bounds__(T) k_ks_hoisted_i(const uint64_t* __restrict__ D, const ulonglong2* __restrict__ keys,
+                                                  size_t key_stride, int key_limbs, GaloisList galois,
+                                                  uint64_t* __restrict__ acc, const ulonglong2* __restrict__ tw_all,
+                                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
+                                                  int batch, int R, HoistOrder order) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    const int tid = threadIdx.x;
+    const TRB s = hoist_slot(blockIdx.x, gridDim.x, R, batch, order);
+    const int b = wave_uniform(s.b), r = wave_uniform(s.r), t = wave_uniform(s.t);
+    const int pt = t < l ? map.idx[t] : pidx_p;
+    const int kt = t < l ? t : key_limbs - 1;
+    const PrimeConst pc = pcs[pt];
+    const uint64_t nq = (uint64_t)0 - pc.q;
+    const ulonglong2* key = keys + (size_t)r * key_stride;
+    uint64_t a0[C::E], a1[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0;
+    for (int j = 0; j < l; ++j) {
+        const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
+        const ulonglong2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+        const ulonglong2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const int idx = k * C::T + tid;          // coalesced: the sum is slot by slot
+            const uint64_t x = d[idx];
+            const ulonglong2 wb = kb[idx], wa = ka[idx];
+            a0[k] += mul_shoup4_nq(x, wb.x, wb.y, nq);   // < 4q each, l <= 15 digits
+            a1[k] += mul_shoup4_nq(x, wa.x, wa.y, nq);
+        }
+    }
+    const uint32_t g = galois.g[r];
+    const ulonglong2* itw = tw_all + ((size_t)pt * 2 + 1) * C::N;
+    uint64_t* out = acc + (((size_t)r * batch + b) * 2 * (l + 1) + t) * C::N;
+    if (EVAL && t < l) {
+        // chain limbs stay in evaluation form; ModDown finishes there
+        permute_store_eval_i<C>(a0, false, g, lds, pc, out, tid);
+        permute_store_eval_i<C>(a1, true, g, lds, pc, out + (size_t)(l + 1) * C::N, tid);
+        return;
+    }
+    if constexpr (ntt::pair_pays<C>()) {
+        permute_inverse2_i<C>(a0, a1, g, lds, itw, pc, out, out + (size_t)(l + 1) * C::N, tid);
+    } else {
+        permute_inverse_i<C>(a0, false, g, lds, itw, pc, out, tid);
+        permute_inverse_i<C>(a1, true, g, lds, itw, pc, out + (size_t)(l + 1) * C::N, tid);
+    }
+}
+
 // ModDown in the evaluation domain for hoisted rotations, one workgroup per
 // (rotation r, ciphertext b, poly, limb i):
 //   out_i = (X_i - NTT_i(w)) p^-1 (+ pi_g(NTT_i(c0)) for poly 0),
@@ -256,16 +386,24 @@ hipError_t run_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int ba
 }
 
 template <int LOGN, int T>
-hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const double2* keys, size_t key_stride,
+hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const void* keys, size_t key_stride,
                           int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l, const LimbMap& map,
                           int pidx_p, int R, int eval_out, HoistOrder order, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
+    if (!t.fp64) {
+        auto k = eval_out ? k_ks_hoisted_i<LOGN, T, true> : k_ks_hoisted_i<LOGN, T, false>;
+        hipError_t e = set_lds(k, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, (const ulonglong2*)keys, key_stride,
+                           key_limbs, galois, acc, t.tw, t.pc, map, l, pidx_p, batch, R, order);
+        return hipGetLastError();
+    }
     auto k = eval_out ? k_ks_hoisted<LOGN, T, true> : k_ks_hoisted<LOGN, T, false>;
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, keys, key_stride, key_limbs, galois, acc,
-                       t.twf, t.pf, map, l, pidx_p, batch, R, order);
+    hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, (const double2*)keys, key_stride, key_limbs,
+                       galois, acc, t.twf, t.pf, map, l, pidx_p, batch, R, order);
     return hipGetLastError();
 }
 
@@ -298,7 +436,7 @@ hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int
     }
 }
 
-hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const double2* keys, size_t key_stride,
+hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const void* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
                              const LimbMap& map, int pidx_p, int R, int eval_out, HoistOrder order, int threads,
                              hipStream_t s) {

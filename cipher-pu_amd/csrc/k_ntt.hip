@@ -244,6 +244,26 @@ __global__ void k_pointwise(const uint64_t* __restrict__ a, const uint64_t* __re
     out[g] = r;
 }
 
+// (w, floor(w 2^63 / q)) pairs for key material; w < q < 2^45, the quotient
+// by restoring division.
+__global__ void k_shoup_pairs(const uint64_t* __restrict__ in, ulonglong2* __restrict__ out,
+                              const PrimeConst* __restrict__ pcs, LimbMap map, int n_limbs, int logN, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint64_t q = pcs[map.idx[(g >> logN) % n_limbs]].q;
+    const uint64_t w = in[g];
+    uint64_t r = w, quo = 0;
+    for (int k = 0; k < 63; ++k) {
+        r <<= 1;
+        quo <<= 1;
+        if (r >= q) {
+            r -= q;
+            quo |= 1;
+        }
+    }
+    out[g] = make_ulonglong2(w, quo);
+}
+
 // (w, RN(w / q)) FP64 pairs of reduced words w < q: the prepared form of
 // key and plaintext rows.  The double division is correctly rounded (IEEE
 // 754, as the gfx950 division sequence computes it).
@@ -444,11 +464,15 @@ hipError_t launch_pointwise(const DevTables& t, const uint64_t* a, const uint64_
     return hipGetLastError();
 }
 
-hipError_t launch_fp64_pairs(const DevTables& t, const uint64_t* in, double2* out, int n_polys, int n_limbs,
-                             const LimbMap& map, hipStream_t s) {
+hipError_t launch_prepare_pairs(const DevTables& t, const uint64_t* in, void* out, int n_polys, int n_limbs,
+                                const LimbMap& map, hipStream_t s) {
     const size_t total = ((size_t)n_polys * n_limbs) << t.logN;
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_fp64_pairs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, t.pf, map,
-                       n_limbs, t.logN, total);
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (t.fp64)
+        hipLaunchKernelGGL(k_fp64_pairs, grid, dim3(256), 0, s, in, (double2*)out, t.pf, map, n_limbs, t.logN, total);
+    else
+        hipLaunchKernelGGL(k_shoup_pairs, grid, dim3(256), 0, s, in, (ulonglong2*)out, t.pc, map, n_limbs, t.logN,
+                           total);
     return hipGetLastError();
 }

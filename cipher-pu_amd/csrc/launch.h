@@ -48,10 +48,11 @@ hipError_t launch_ntt(const DevTables& t, uint64_t* dst, const uint64_t* src, in
                       bool inverse, bool reduce_input, int threads, hipStream_t s);
 hipError_t launch_polymul(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out,
                           int n_polys, int n_limbs, const LimbMap& map, int threads, hipStream_t s);
-// (value, RN(value / q)) FP64 pairs of reduced words: the prepared form of
-// keys and plaintexts read by the FP64 key switch and linear combination.
-hipError_t launch_fp64_pairs(const DevTables& t, const uint64_t* in, double2* out, int n_polys, int n_limbs,
-                             const LimbMap& map, hipStream_t s);
+// Prepared pairs of reduced words, in the context's arithmetic: (value,
+// RN(value / q)) doubles for FP64, (value, floor(value 2^63 / q)) for INT64;
+// the form of keys and plaintexts the key switches and combinations read.
+hipError_t launch_prepare_pairs(const DevTables& t, const uint64_t* in, void* out, int n_polys, int n_limbs,
+                                const LimbMap& map, hipStream_t s);
 hipError_t launch_bitrev(uint64_t* data, int n_rows, int logN, hipStream_t s);
 hipError_t launch_pointwise(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out,
                             int n_polys, int n_limbs, const LimbMap& map, int op, hipStream_t s);
@@ -101,10 +102,26 @@ struct KsTargets {
 
 // key_table (optional): ciphertext b uses the key at device address
 // key_table[b] instead of key.
+// key_pairs: prepared pairs in the context's arithmetic (double2 (w, RN(w/q))
+// for FP64, ulonglong2 (w, Shoup companion) for INT64).
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
-                            const uint64_t* d01, const double2* key, int key_limbs, uint64_t* acc,
+                            const uint64_t* d01, const void* key_pairs, int key_limbs, uint64_t* acc,
                             int batch, int l, const LimbMap& map, int pidx_special, int threads,
                             hipStream_t s, const uint64_t* key_table = nullptr, const KsTargets& tg = KsTargets());
+// The integer-word versions (k_ckks_int.hip) the launchers above dispatch to
+// when the context's arithmetic is INT64.
+hipError_t launch_tensor_intt_int(const DevTables& t, const uint64_t* ea, const uint64_t* eb, uint64_t* d01,
+                                  uint64_t* d2e, uint64_t* d2c, int batch, int l, const LimbMap& map,
+                                  const ModDownConst& mc, int threads, hipStream_t s);
+hipError_t launch_front_int(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* d01, uint64_t* d2e,
+                            uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, int threads,
+                            hipStream_t s);
+hipError_t launch_keyswitch_int(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
+                                const uint64_t* d01, const ulonglong2* key, int key_limbs, uint64_t* acc, int batch,
+                                int l, const LimbMap& map, int pidx_p, int threads, hipStream_t s,
+                                const uint64_t* key_table, const KsTargets& tg);
+hipError_t launch_lincomb_int(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb,
+                              int ng, int batch, int l, const LimbMap& map, int scalar, hipStream_t s);
 hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
                                   uint64_t* out, int batch, int l, const LimbMap& map, int pidx_special,
                                   const ModDownConst& mc, bool rescale, hipStream_t s);
@@ -120,7 +137,7 @@ hipError_t launch_automorphism_groups(const DevTables& t, const uint64_t* in, ui
                                       hipStream_t s, const uint64_t* gal_table = nullptr, int gal_div = 1);
 hipError_t launch_mod_switch(const uint64_t* in, uint64_t* out, size_t n, uint64_t q_in, uint64_t q_out, uint64_t mu,
                              int ell, hipStream_t s);
-hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const double2* D, uint64_t* out, int nb, int ng,
+hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const void* D_pairs, uint64_t* out, int nb, int ng,
                           int batch, int l, const LimbMap& map, int scalar, hipStream_t s);
 hipError_t launch_lift_small(const DevTables& t, const uint64_t* in, uint64_t* out, int rows, int l_in, int l_out,
                              const LimbMap& map, hipStream_t s);
@@ -216,7 +233,7 @@ hipError_t launch_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int
 // limb's in coefficient form as always.
 // order: workgroup placement (hoist_slot); btile 0 picks the measured
 // default for the output form, btile above batch means batch.
-hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const double2* keys, size_t key_stride,
+hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const void* keys, size_t key_stride,
                              int key_limbs, const GaloisList& galois, uint64_t* acc, int batch, int l,
                              const LimbMap& map, int pidx_p, int R, int eval_out, HoistOrder order, int threads,
                              hipStream_t s);

@@ -56,12 +56,21 @@ def test_gears_prune_and_match_pruned_network(mlp_env, gear):
     assert nn.output_snr_db(dense, out) < 60
 
 
-def test_lincomb_scalar_bit_exact(mlp_env):
+@pytest.fixture(params=[True, False], ids=["fp64", "int64"])
+def arith(request, mlp_env):
+    """Runs a test under each arithmetic of the shared context, restoring FP64."""
+    ring = mlp_env[0].ring
+    ring.set_arith(request.param)
+    yield request.param
+    ring.set_arith(True)
+
+
+def test_lincomb_scalar_bit_exact(mlp_env, arith):
     """out[g] = sum_b D[g][b] E[b] mod q_i with one integer per (g, b, limb),
     against Python integers."""
     import torch
     from cipherpu._lib import call
-    from cipherpu.ring import prepared_pairs, to_numpy
+    from cipherpu.ring import to_numpy
     import ctypes
     ck = mlp_env[0]
     rng = np.random.default_rng(9)
@@ -75,7 +84,7 @@ def test_lincomb_scalar_bit_exact(mlp_env):
             for i in range(l):
                 q = ck.primes[i]
                 r = int(W[g, b]) % q
-                D[g, b, i] = prepared_pairs(r, q)
+                D[g, b, i] = ck.ring.prepared_pairs(r, q)
     Et, Dt = ck.ring.tensor(E), ck.ring.tensor(D)
     out = torch.empty(ng, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)
     p = lambda t: ctypes.c_void_p(t.data_ptr())
@@ -90,7 +99,7 @@ def test_lincomb_scalar_bit_exact(mlp_env):
 
 @pytest.mark.parametrize("nb,ng,B,l,scalar", [(16, 8, 2, 4, 0), (17, 10, 3, 3, 0), (32, 5, 3, 1, 0),
                                               (4, 20, 1, 2, 0), (24, 8, 3, 2, 1), (5, 17, 2, 1, 1)])
-def test_lincomb_shapes_bit_exact(mlp_env, nb, ng, B, l, scalar):
+def test_lincomb_shapes_bit_exact(mlp_env, arith, nb, ng, B, l, scalar):
     """out[g][row] = sum_b D[g][b] E[b][row] mod q_i, per coefficient, for
     giant-step counts across the kernel's blocks (<= 4, <= 8, <= 16 and
     chunks of 16), row counts that leave a partial row block, and up to 32
@@ -99,7 +108,7 @@ def test_lincomb_shapes_bit_exact(mlp_env, nb, ng, B, l, scalar):
     import ctypes
     import torch
     from cipherpu._lib import call
-    from cipherpu.ring import prepared_pairs, to_numpy
+    from cipherpu.ring import to_numpy
     ck = mlp_env[0]
     rng = np.random.default_rng(nb * 100 + ng)
     N, K = ck.N, 64
@@ -108,7 +117,7 @@ def test_lincomb_shapes_bit_exact(mlp_env, nb, ng, B, l, scalar):
     W = np.stack([rng.integers(0, ck.primes[i], dshape[:2] + dshape[3:], dtype=np.uint64) for i in range(l)], axis=2)
     q = np.array([ck.primes[i] for i in range(l)], dtype=object)
     qb = q.reshape((1, 1, l) + ((1,) if not scalar else ()))
-    D = prepared_pairs(W, qb.astype(np.float64))
+    D = ck.ring.prepared_pairs(W, qb.astype(np.uint64))
     Et, Dt = ck.ring.tensor(E), ck.ring.tensor(D)
     out = torch.empty(ng, B, 2, l, N, dtype=torch.uint64, device=ck.device)
     p = lambda t: ctypes.c_void_p(t.data_ptr())

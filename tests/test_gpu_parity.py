@@ -570,6 +570,58 @@ def test_every_launch_shape_gives_the_same_words(gpu, N):
         assert all(np.array_equal(g, r) for g, r in zip(got[2], ref[2])), ("hoisted",) + shape
 
 
+@pytest.mark.parametrize("N", [4096, 8192, 16384])
+def test_integer_arithmetic_gives_the_same_words(gpu, N):
+    """The 64-bit integer kernels (Shoup and Barrett products on
+    __umul64hi, CKKS_ARITH_INT64) against the FP64 residue pipeline, which
+    the oracle pins: HMult (fused and separate front) under every built
+    tensor/front and key-switch shape, relinearisation, rotation, mixed-step
+    rotation and hoisted rotations (both output forms) give the same words.
+    Keys are prepared again for the integer arithmetic from the same
+    coefficients, and a key of the other form is refused."""
+    import itertools
+    from cipherpu.ckks import CKKS, CKKSParams
+    from cipherpu.ring import prepared_pairs, to_numpy
+    from cipherpu.tune import BUILT_FUSED
+    c = CKKS(CKKSParams(N=N), seed=23)
+    sk = c.keygen()
+    rng = np.random.default_rng(N + 1)
+    z = rng.uniform(-1, 1, (3, N // 2))
+    a, b = (c.encrypt(c.encode(v), sk) for v in (z, z[::-1].copy()))
+    d = [c.ring.tensor(np.stack([rng.integers(0, q, (3, N), dtype=np.uint64) for q in c.primes[:3]], axis=1))
+         for _ in range(3)]
+
+    def run(rlk, gks):
+        return ([to_numpy(c.hmult(a, b, rlk).data), to_numpy(c.relin(*d, rlk)), to_numpy(c.rotate(a, gks[0]).data),
+                 to_numpy(c.rotate_many(a, [gks[0], gks[1], gks[0]]).data)] +
+                [to_numpy(r.data) for r in c.rotate_hoisted(a, gks)] +
+                [to_numpy(r.data) for r in c.rotate_hoisted(a, gks, eval_form=True)])
+
+    rlk = c.relin_key(sk)
+    gks = [c.galois_key(sk, s) for s in (1, 3)]
+    ref = run(rlk, gks)
+    pr, p, ps, pp = _oracle_args(c)
+    assert np.array_equal(ref[0][2], OC.hmult(to_numpy(a.data[2]), to_numpy(b.data[2]), to_numpy(rlk.coeff),
+                                              pr, p, ps, pp))
+    c.ring.set_arith(False)
+    with pytest.raises(ValueError):
+        c.hmult(a, b, rlk)
+    irlk = c.load_switch_key(rlk.coeff)
+    igks = [c.load_switch_key(g.coeff, g.galois) for g in gks]
+    # the same words in the integer form: (w, floor(w 2^63 / q))
+    words = to_numpy(rlk.prepared)[..., 0].view(np.float64).astype(np.uint64)
+    q = np.array(c.primes, dtype=np.uint64).reshape(1, 1, -1, 1)
+    assert np.array_equal(to_numpy(irlk.prepared), prepared_pairs(words, q, fp64=False))
+    logN = N.bit_length() - 1
+    for front, tensor, ks in itertools.product((0, 1), BUILT_FUSED[logN], BUILT_FUSED[logN]):
+        c.ring.set_front(bool(front))
+        c.ring.set_threads(0, ks, tensor)
+        c.ring.set_ks_order(bool(ks & 512))
+        got = run(irlk, igks)
+        for k, (g, r) in enumerate(zip(got, ref)):
+            assert np.array_equal(g, r), (k, front, tensor, ks)
+
+
 WIDE = [CHAIN4[0], 2199023288321, CHAIN4[2], 4398047232001]   # 2^44.., 2^41.., 2^44.., 2^42.. (= 1 mod 2^15)
 
 

@@ -168,3 +168,24 @@ def test_reference_block_sparsity_rules():
             assert np.array_equal(got[0], want[0]) and got[1:] == want[1:]
             got, want = nn.block_sparse_mask(W, t, B), O.block_sparse_mask(W, t, B)
             assert np.array_equal(got[0], want[0]) and got[1:] == want[1:]
+
+
+def test_prepared_pair_forms():
+    """The host's prepared pairs: FP64 form (w, RN(w / q)) as IEEE division
+    gives it; integer form (w, floor(w 2^63 / q)) against Python integers,
+    for 45-bit primes and the widest q < 2^63, at 0, 1 and q - 1."""
+    from cipherpu.ring import prepared_pairs, shoup_companion
+    rng = np.random.default_rng(5)
+    q45 = np.array([35184372744193, 35184371138561, 17592186028033], dtype=np.uint64)
+    w = rng.integers(0, 2 ** 45, (50, 3)).astype(np.uint64) % q45
+    w[0], w[1], w[2] = 0, 1, q45 - np.uint64(1)
+    f = prepared_pairs(w, q45).view(np.float64)
+    assert np.array_equal(f[..., 0], w.astype(np.float64))
+    assert np.array_equal(f[..., 1], w.astype(np.float64) / q45.astype(np.float64))
+    i = prepared_pairs(w, q45, fp64=False)
+    assert np.array_equal(i[..., 0], w)
+    want = [[(int(a) << 63) // int(b) for a, b in zip(row, q45)] for row in w]
+    assert np.array_equal(i[..., 1].astype(object), np.array(want, dtype=object))
+    big = np.array([2 ** 63 - 25], dtype=np.uint64)
+    wb = np.array([0, 1, 2 ** 62 + 7, 2 ** 63 - 26], dtype=np.uint64)
+    assert [int(x) for x in shoup_companion(wb, big)] == [(int(a) << 63) // int(big[0]) for a in wb]
