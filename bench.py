@@ -68,10 +68,11 @@ KERNEL_NAMES = {"front": "k_front<13,{}>", "tensor": "k_tensor_intt<13,{}>", "nt
                 "moddown": "k_moddown_rescale_l<4,true>"}
 
 
-def kernel_model_bytes(kind: str, batch: int, l: int = L):
+def kernel_model_bytes(kind: str, batch: int, l: int = L, fp64: bool = True):
     """Bytes one launch of an HMult kernel must move, from its inputs and
-    outputs (limb-polys of N 8-byte words; key rows are 16-byte value and
-    companion pairs, read once per launch):
+    outputs (limb-polys of N 8-byte words; key rows, read once per launch,
+    are the 8-byte words as doubles on the FP64 arithmetic and 16-byte value
+    and companion pairs on the integer one):
       front            per (pair, limb): a0, a1, b0, b1 in; p d0, p d1, d2
                        (evaluation form) and d2 (coefficient form) out;
       keyswitch        targets l-1 and p, per pair: the other limbs' digits
@@ -81,12 +82,13 @@ def kernel_model_bytes(kind: str, batch: int, l: int = L):
                        own digit, p d0, p d1, the accumulators of limb l-1
                        and of p (4 rows), 2 output limbs; their key rows."""
     limb = N * 8
+    key_row = limb * (1 if fp64 else 2)
     if kind == "front":
         return batch * l * 8 * limb
     if kind == "keyswitch":
-        return batch * ((l - 1 + 1 + 2 + 2) + (l + 2)) * limb + 2 * l * 2 * 2 * limb
+        return batch * ((l - 1 + 1 + 2 + 2) + (l + 2)) * limb + 2 * l * 2 * key_row
     if kind == "keyswitch_fused":
-        return batch * (l - 1) * ((l - 1) + 1 + 2 + 4 + 2) * limb + (l - 1) * l * 2 * 2 * limb
+        return batch * (l - 1) * ((l - 1) + 1 + 2 + 4 + 2) * limb + (l - 1) * l * 2 * key_row
     return None
 
 
@@ -604,7 +606,7 @@ def main():
     dom_ms = per_launch[dom]
     threads = {"front": shape.get("tensor"), "tensor": shape.get("tensor"), "keyswitch": shape.get("ks"),
                "keyswitch_fused": shape.get("ks"), "ntt": shape.get("ntt"), "moddown": 256}.get(dom)
-    kbytes = kernel_model_bytes(dom, B)
+    kbytes = kernel_model_bytes(dom, B, fp64=args.arith == "fp64")
     k_achieved = nbytes / (dom_ms / 1e3) / 1e9
     prof_key = (dom, B, threads)
     if prof_key in MEASURED_TRAFFIC and args.arith == "fp64":

@@ -78,7 +78,8 @@ class SecretKey:
 @dataclass
 class SwitchKey:
     coeff: torch.Tensor         # [L, 2, L+1, N] uint64, coefficient form (b_j, a_j)
-    prepared: torch.Tensor      # [L, 2, L+1, N, 2] uint64 prepared pairs, evaluation order
+    prepared: torch.Tensor      # evaluation order: FP64 [L, 2, L+1, N] (the words as doubles), INT64
+                                # [L, 2, L+1, N, 2] (word, Shoup companion) pairs
     galois: int = 0             # 0 for the relinearisation key
     fp64: bool = True           # the arithmetic the pairs were prepared for (RingContext.set_arith)
     hoisted: Optional[torch.Tensor] = None   # prepared, permuted for ckks_rotate_hoisted
@@ -175,10 +176,15 @@ class CKKS:
             g[j, j, :] = self.p.special % self.primes[j]
         b = self.ring.add(b, self.ring.mul(gs, self.ring.tensor(g)))
         coeff = torch.stack([b, a], dim=1).contiguous()          # [L, 2, K, N]
-        prepared = torch.empty(L, 2, K, self.N, 2, dtype=torch.uint64, device=self.device)
+        prepared = self._key_buffer()
         work = coeff.clone()
         call("ckks_key_prepare", self.ring.handle, _ptr(work), _ptr(prepared), L, K, self.sp, self.ring.stream())
         return SwitchKey(coeff=coeff, prepared=prepared, galois=galois, fp64=self.ring.fp64)
+
+    def _key_buffer(self) -> torch.Tensor:
+        """A prepared key in the context's arithmetic (ckks_key_prepare)."""
+        shape = (self.L, 2, self.L + 1, self.N) + (() if self.ring.fp64 else (2,))
+        return torch.empty(*shape, dtype=torch.uint64, device=self.device)
 
     def load_switch_key(self, coeff, galois: int = 0) -> SwitchKey:
         """Key from its coefficient form [L, 2, L+1, N] (b_j, a_j) -- e.g. read
@@ -187,7 +193,7 @@ class CKKS:
         L, K = self.L, self.L + 1
         if tuple(coeff.shape) != (L, 2, K, self.N):
             raise ValueError(f"key must be [{L}, 2, {K}, {self.N}]")
-        prepared = torch.empty(L, 2, K, self.N, 2, dtype=torch.uint64, device=self.device)
+        prepared = self._key_buffer()
         work = coeff.clone()
         call("ckks_key_prepare", self.ring.handle, _ptr(work), _ptr(prepared), L, K, self.sp, self.ring.stream())
         return SwitchKey(coeff=coeff, prepared=prepared, galois=galois, fp64=self.ring.fp64)
@@ -344,7 +350,7 @@ class CKKS:
         automorphism, as ckks_rotate_hoisted reads it (cached on the key)."""
         if gk.hoisted is None:
             out = torch.empty_like(gk.prepared)
-            rows = gk.prepared.numel() // (2 * self.N)
+            rows = self.L * 2 * (self.L + 1)
             call("ckks_key_permute", self.ring.handle, _ptr(gk.prepared), _ptr(out), rows, gk.galois,
                  self.ring.stream())
             gk.hoisted = out

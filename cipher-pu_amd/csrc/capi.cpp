@@ -626,7 +626,7 @@ int ckks_key_prepare(const ckks_ctx* c, const uint64_t* key_coeff, uint64_t* key
     const DevTables t = c->tables();
     uint64_t* tmp = const_cast<uint64_t*>(key_coeff);   // transformed in place (documented)
     HIPCHK(launch_ntt(t, tmp, tmp, dnum * 2, key_limbs, m, false, true, c->ntt_threads, S(stream)));
-    HIPCHK(launch_prepare_pairs(t, tmp, key_out, dnum * 2, key_limbs, m, S(stream)));
+    HIPCHK(launch_prepare_key(t, tmp, key_out, dnum * 2, key_limbs, m, S(stream)));
     return CKKS_OK;
 }
 
@@ -790,8 +790,7 @@ int ckks_key_permute(const ckks_ctx* c, const uint64_t* key_in, uint64_t* key_ou
     if (key_in == key_out) return CKKS_ERR_BAD_ARG;
     if (misaligned_pairs(key_in) || misaligned_pairs(key_out)) return CKKS_ERR_BAD_ALIGNMENT;
     if (set_device(c)) return CKKS_ERR_HIP;
-    HIPCHK(launch_key_permute(c->tables(), (const ulonglong2*)key_in, (ulonglong2*)key_out, rows, (uint32_t)galois,
-                              S(stream)));
+    HIPCHK(launch_key_permute(c->tables(), key_in, key_out, rows, (uint32_t)galois, S(stream)));
     return CKKS_OK;
 }
 

@@ -167,22 +167,21 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
 }
 
 // a0 += v * b_j,t and a1 -= v * a_j,t for digit j in evaluation order (key
-// rows of (value, RN(value / q)) FP64 pairs).  v below 2^51; each product is
-// bounded by 5q/8, so with at most 15 digits and a start below q the sums
-// stay below 11 q.
+// rows of the words as doubles, 8 bytes a word: the products form their
+// quotients, mulk).  v below 2^51; each product is bounded by 7q/8, so with
+// at most 15 digits and a start below q the sums stay below 15 q.
 template <class C>
 __device__ __forceinline__ void key_mac(double (&a0)[C::E], double (&a1)[C::E], const double (&v)[C::E],
-                                        const double2* __restrict__ key, int j, int key_limbs, int kt, double nq,
-                                        int tid) {
-    const double2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
-    const double2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+                                        const double* __restrict__ key, int j, int key_limbs, int kt, double qinv,
+                                        double nq, int tid) {
+    const double* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+    const double* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
 #pragma clang fp contract(off)
         const int idx = ntt::eval_index<C>(k, tid);
-        const double2 wb = kb[idx], wa = ka[idx];
-        a0[k] = a0[k] + mulw(v[k], wb.x, wb.y, nq);
-        a1[k] = a1[k] - mulw(v[k], wa.x, wa.y, nq);
+        a0[k] = a0[k] + mulk(v[k], kb[idx], qinv, nq);
+        a1[k] = a1[k] - mulk(v[k], ka[idx], qinv, nq);
     }
 }
 
@@ -232,7 +231,7 @@ __device__ __forceinline__ void fused_out(const double (&x)[C::E], const uint64_
 // rtl/m3/keyswitch_pipe.sv:90-100) to one digit per RNS prime:
 //   acc0_t = [p d0]_t + sum_j NTT_t([d2]_(q_j)) * b_j,t
 //   acc1_t = [p d1]_t - sum_j NTT_t([d2]_(q_j)) * a_j,t
-// (FP64 pipeline: signed residues, key rows of (value, RN(value / q)) pairs)
+// (FP64 pipeline: signed residues, key rows of the words as doubles)
 // then both accumulators go back to coefficient form.  The digit lift
 // ("ModUp") is the reduction of the digit's coefficients mod the target
 // prime on load; the digit NTTs, the key products and the inverse
@@ -249,7 +248,7 @@ __device__ __forceinline__ void fused_out(const double (&x)[C::E], const uint64_
 template <int LOGN, int T, bool KEYTAB = false, bool FUSED = false>
 __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2c, size_t d2c_bstride,
                                                  const uint64_t* __restrict__ d2e,
-                                                 const uint64_t* __restrict__ d01, const double2* __restrict__ key,
+                                                 const uint64_t* __restrict__ d01, const double* __restrict__ key,
                                                  int key_limbs, uint64_t* __restrict__ acc,
                                                  const double2* __restrict__ twf_all, const PrimeF* __restrict__ pfs,
                                                  LimbMap map, int l, int pidx_p, int batch, int reduce_digits,
@@ -276,7 +275,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
         t = wave_uniform(t0 + (int)(blockIdx.x / batch));
         b = wave_uniform(blockIdx.x % batch);
     }
-    if constexpr (KEYTAB) key = reinterpret_cast<const double2*>(key_table[b]);
+    if constexpr (KEYTAB) key = reinterpret_cast<const double*>(key_table[b]);
     const int pt = t < l ? map.idx[t] : pidx_p;
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeF pf = pfs[pt];
@@ -329,7 +328,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
             if constexpr (ntt::pair_pays<C>()) nttf::fwd_ntt_db<C>(x, lds, tw, nq, tid);
             else nttf::fwd_ntt<C>(x, lds, tw, nq, tid);
         }
-        key_mac<C>(a0, a1, x, key, j, key_limbs, kt, nq, tid);
+        key_mac<C>(a0, a1, x, key, j, key_limbs, kt, pf.qinv, nq, tid);
     }
     // the double-buffered digit transforms leave reads of the LDS in flight
     if constexpr (ntt::pair_pays<C>()) __syncthreads();
@@ -835,7 +834,7 @@ hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, u
 
 template <int LOGN, int T>
 hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uint64_t* d2e, const uint64_t* d01,
-                  const double2* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
+                  const double* key, int key_limbs, uint64_t* acc, int batch, int l, const LimbMap& map,
                   int pidx_p, const uint64_t* key_table, const KsTargets& tg, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
@@ -907,7 +906,7 @@ hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_
     if (!t.fp64)
         return launch_keyswitch_int(t, d2c, d2c_bstride, d2e, d01, (const ulonglong2*)key_pairs, key_limbs, acc,
                                     batch, l, map, pidx_p, threads, s, key_table, tg);
-    const double2* key = (const double2*)key_pairs;
+    const double* key = (const double*)key_pairs;
     switch (t.logN * 10000 + threads) {
         case 120256: return run_ks<12, 256>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);
         case 120512: return run_ks<12, 512>(t, d2c, d2c_bstride, d2e, d01, key, key_limbs, acc, batch, l, map, pidx_p, key_table, tg, s);

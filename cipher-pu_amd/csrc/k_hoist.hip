@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(T) k_modup(const uint64_t* __restrict__ ct, ui
 }
 
 // Reduce an accumulator whose register k holds evaluation slot k T + tid
-// (the coalesced order the key products are summed in; FP64, below 10 q),
+// (the coalesced order the key products are summed in; FP64, below 14 q),
 // apply pi_g through LDS, inverse-transform and store it in coefficient form.
 template <class C>
 __device__ __forceinline__ void permute_inverse(double (&a)[C::E], uint32_t g, double* lds,
@@ -123,11 +123,11 @@ __device__ __forceinline__ void permute_inverse2(double (&a0)[C::E], double (&a1
 }
 
 // One workgroup per (target t, ciphertext b, rotation r), placed by
-// hoist_slot.  keys: [R][dnum][2][key_limbs][N] (value, RN(value / q)) FP64
-// pairs permuted by pi_g^-1; acc: [R][B][2][l+1][N] coefficient form, the
+// hoist_slot.  keys: [R][dnum][2][key_limbs][N] key words as doubles,
+// permuted by pi_g^-1; acc: [R][B][2][l+1][N] coefficient form, the
 // layout ckks_relin's ModDown reads.  acc1 = -(sum D a) directly.
 template <int LOGN, int T, bool EVAL>
-__global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D, const double2* __restrict__ keys,
+__global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D, const double* __restrict__ keys,
                                                   size_t key_stride, int key_limbs, GaloisList galois,
                                                   uint64_t* __restrict__ acc, const double2* __restrict__ twf_all,
                                                   const PrimeF* __restrict__ pfs, LimbMap map, int l, int pidx_p,
@@ -142,22 +142,21 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeF pf = pfs[pt];
     const double nq = pf.nq;
-    const double2* key = keys + (size_t)r * key_stride;
+    const double* key = keys + (size_t)r * key_stride;
     double a0[C::E], a1[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
     for (int j = 0; j < l; ++j) {
         const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
-        const double2* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
-        const double2* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+        const double* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+        const double* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
 #pragma clang fp contract(off)
             const int idx = k * C::T + tid;          // coalesced: the sum is slot by slot
             const double x = u2d(d[idx]);
-            const double2 wb = kb[idx], wa = ka[idx];
-            a0[k] = a0[k] + mulw(x, wb.x, wb.y, nq);   // 5q/8 each, l <= 15 digits
-            a1[k] = a1[k] - mulw(x, wa.x, wa.y, nq);
+            a0[k] = a0[k] + mulk(x, kb[idx], pf.qinv, nq);   // 7q/8 each, l <= 15 digits
+            a1[k] = a1[k] - mulk(x, ka[idx], pf.qinv, nq);
         }
     }
     const uint32_t g = galois.g[r];
@@ -360,8 +359,8 @@ __global__ void __launch_bounds__(T) k_moddown_eval(const uint64_t* __restrict__
 }
 
 // out[row][j(i)] = in[row][i]: the key rows permuted by pi_g^-1.
-__global__ void k_key_permute(const ulonglong2* __restrict__ in, ulonglong2* __restrict__ out, uint32_t g, int logN,
-                              size_t total) {
+template <typename W>
+__global__ void k_key_permute(const W* __restrict__ in, W* __restrict__ out, uint32_t g, int logN, size_t total) {
     const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= total) return;
     const size_t N = (size_t)1 << logN;
@@ -402,7 +401,7 @@ hipError_t run_ks_hoisted(const DevTables& t, const uint64_t* D, const void* key
     auto k = eval_out ? k_ks_hoisted<LOGN, T, true> : k_ks_hoisted<LOGN, T, false>;
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, (const double2*)keys, key_stride, key_limbs,
+    hipLaunchKernelGGL(k, dim3((l + 1) * batch * R), dim3(T), lds, s, D, (const double*)keys, key_stride, key_limbs,
                        galois, acc, t.twf, t.pf, map, l, pidx_p, batch, R, order);
     return hipGetLastError();
 }
@@ -478,11 +477,15 @@ hipError_t launch_moddown_eval(const DevTables& t, const uint64_t* acc, const ui
     }
 }
 
-hipError_t launch_key_permute(const DevTables& t, const ulonglong2* in, ulonglong2* out, int rows, uint32_t g,
-                              hipStream_t s) {
+hipError_t launch_key_permute(const DevTables& t, const void* in, void* out, int rows, uint32_t g, hipStream_t s) {
     const size_t total = (size_t)rows << t.logN;
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_key_permute, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, g, t.logN,
-                       total);
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (t.fp64)
+        hipLaunchKernelGGL(k_key_permute<double>, grid, dim3(256), 0, s, (const double*)in, (double*)out, g, t.logN,
+                           total);
+    else
+        hipLaunchKernelGGL(k_key_permute<ulonglong2>, grid, dim3(256), 0, s, (const ulonglong2*)in, (ulonglong2*)out,
+                           g, t.logN, total);
     return hipGetLastError();
 }

@@ -276,6 +276,14 @@ __global__ void k_fp64_pairs(const uint64_t* __restrict__ in, double2* __restric
     out[g] = make_double2(w, w / q);
 }
 
+// Key rows on the FP64 pipeline: the doubles of reduced words w < q (the
+// key products form RN(w / q) themselves, modf64.h:mulk).
+__global__ void k_fp64_words(const uint64_t* __restrict__ in, double* __restrict__ out, size_t total) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    out[g] = u2d(in[g]);
+}
+
 // Batched transforms on the FP64 pipeline, the same words as k_ntt_fwd /
 // k_ntt_inv.  reduce_in: inputs are any 64-bit words, reduced exactly by
 // the integer reduce64 first; otherwise below 2q, the integer kernels'
@@ -474,5 +482,14 @@ hipError_t launch_prepare_pairs(const DevTables& t, const uint64_t* in, void* ou
     else
         hipLaunchKernelGGL(k_shoup_pairs, grid, dim3(256), 0, s, in, (ulonglong2*)out, t.pc, map, n_limbs, t.logN,
                            total);
+    return hipGetLastError();
+}
+
+hipError_t launch_prepare_key(const DevTables& t, const uint64_t* in, void* out, int n_polys, int n_limbs,
+                              const LimbMap& map, hipStream_t s) {
+    if (!t.fp64) return launch_prepare_pairs(t, in, out, n_polys, n_limbs, map, s);
+    const size_t total = ((size_t)n_polys * n_limbs) << t.logN;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_fp64_words, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, (double*)out, total);
     return hipGetLastError();
 }

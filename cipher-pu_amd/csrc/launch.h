@@ -53,6 +53,10 @@ hipError_t launch_polymul(const DevTables& t, const uint64_t* a, const uint64_t*
 // the form of keys and plaintexts the key switches and combinations read.
 hipError_t launch_prepare_pairs(const DevTables& t, const uint64_t* in, void* out, int n_polys, int n_limbs,
                                 const LimbMap& map, hipStream_t s);
+// Prepared key rows: the words as doubles for FP64 (8 bytes a word), the
+// prepared pairs above for INT64.
+hipError_t launch_prepare_key(const DevTables& t, const uint64_t* in, void* out, int n_polys, int n_limbs,
+                              const LimbMap& map, hipStream_t s);
 hipError_t launch_bitrev(uint64_t* data, int n_rows, int logN, hipStream_t s);
 hipError_t launch_pointwise(const DevTables& t, const uint64_t* a, const uint64_t* b, uint64_t* out,
                             int n_polys, int n_limbs, const LimbMap& map, int op, hipStream_t s);
@@ -102,8 +106,8 @@ struct KsTargets {
 
 // key_table (optional): ciphertext b uses the key at device address
 // key_table[b] instead of key.
-// key_pairs: prepared pairs in the context's arithmetic (double2 (w, RN(w/q))
-// for FP64, ulonglong2 (w, Shoup companion) for INT64).
+// key_pairs: prepared key rows in the context's arithmetic (doubles w for
+// FP64, ulonglong2 (w, Shoup companion) pairs for INT64).
 hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_bstride, const uint64_t* d2e,
                             const uint64_t* d01, const void* key_pairs, int key_limbs, uint64_t* acc,
                             int batch, int l, const LimbMap& map, int pidx_special, int threads,
@@ -240,8 +244,9 @@ hipError_t launch_ks_hoisted(const DevTables& t, const uint64_t* D, const void* 
 hipError_t launch_moddown_eval(const DevTables& t, const uint64_t* acc, const uint64_t* Cn, uint64_t* out, int batch,
                                int l, const LimbMap& map, int pidx_p, const ModDownConst& mc,
                                const GaloisList& galois, int R, int threads, hipStream_t s);
-hipError_t launch_key_permute(const DevTables& t, const ulonglong2* in, ulonglong2* out, int rows, uint32_t g,
-                              hipStream_t s);
+// rows of N prepared key entries (8-byte doubles for FP64, 16-byte pairs for
+// INT64) permuted by pi_g^-1.
+hipError_t launch_key_permute(const DevTables& t, const void* in, void* out, int rows, uint32_t g, hipStream_t s);
 
 
 // Matrix-core probe (k_mfma_probe.hip): y = G^reps x for V length-64 vectors

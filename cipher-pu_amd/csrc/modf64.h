@@ -14,6 +14,7 @@
 //
 // Bounds (q < 2^45 throughout; "|x| <= B q" is written "bound B"):
 //  * mulw(a, w):  |a| < 2^51                  -> |r| <= 5q/8
+//  * mulk(a, w):  |a| < 2^51                  -> |r| <= 7q/8
 //  * muld(a, b):  |a b| <= 2^50 q             -> |r| <= 3q/4
 //  * rmod(x):     |x| < 2^52                  -> |r| <= (q + 1)/2
 //  * canon(x):    |x| <= 16 q                 -> r in [0, q), r = x mod q
@@ -49,6 +50,18 @@ MODF_FN double mulw(double a, double w, double wq, double nq) {
     const double l = __builtin_fma(a, w, -h);
     const double k = __builtin_fma(a, wq, RNDC) - RNDC;
     return __builtin_fma(k, nq, h) + l;
+}
+
+// a w mod q for a fixed operand w in [0, q) given without its quotient,
+// for |a| < 2^51: wq = RN(w qinv) is within (w / q) 2^-53 + 2^-54 < 1.5 2^-53
+// of w / q (qinv = RN(1/q) has relative error 2^-53; w qinv < 1 rounds
+// within 2^-54), so k = round(a wq) is within 1/2 + 3/8 of a w / q and
+// |r| <= 7q/8; r = a w - k q exactly, as in mulw.  Key rows stored as their
+// words alone halve the bytes a key product reads, for one more multiply.
+MODF_FN double mulk(double a, double w, double qinv, double nq) {
+#pragma clang fp contract(off)
+    const double wq = w * qinv;
+    return mulw(a, w, wq, nq);
 }
 
 // a b mod q for data operands with |a b| <= 2^50 q (e.g. both reduced by

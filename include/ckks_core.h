@@ -113,12 +113,17 @@ int ckks_rescale(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, int n_p
  * ckks_hmult / ckks_rotate for a batch at level l. */
 uint64_t ckks_workspace_bytes(const ckks_ctx *ctx, int batch, int l);
 
-/* Key material for the hybrid key switch, in the layout the kernels read:
- * [dnum][2][key_limbs][N] prepared pairs (the doubles w and RN(w / q)),
- * evaluation order.  Builds it from coefficient-form b_j, a_j ([dnum][2][key_limbs][N]
- * uint64, limbs = primes 0..key_limbs-2 then the special prime).  Device
- * pointers; key_out holds 2x the input's words.  key_coeff is transformed
- * in place (it holds the evaluation form afterwards). */
+/* Key material for the hybrid key switch, in the layout the kernels read,
+ * evaluation order, in the context's arithmetic (ckks_ctx_set_arith):
+ * CKKS_ARITH_FP64, [dnum][2][key_limbs][N] doubles (the words w; the key
+ * products form RN(w / q) themselves), as many 8-byte words as the input;
+ * CKKS_ARITH_INT64, [dnum][2][key_limbs][N] (w, floor(w 2^63 / q)) pairs,
+ * twice the input's words.  A buffer of the pair size serves both.  Builds
+ * it from coefficient-form b_j, a_j ([dnum][2][key_limbs][N] uint64, limbs
+ * = primes 0..key_limbs-2 then the special prime).  Device pointers, key_out
+ * 16-byte aligned.  key_coeff is transformed in place (it holds the
+ * evaluation form afterwards).  A key is read in the arithmetic it was
+ * prepared in. */
 int ckks_key_prepare(const ckks_ctx *ctx, const uint64_t *key_coeff, uint64_t *key_out, int dnum,
                      int key_limbs, int special, void *stream);
 
@@ -201,8 +206,9 @@ int ckks_rotate_hoisted(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t 
 int ckks_rotate_hoisted_eval(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *keys, int key_limbs,
                              const uint64_t *galois, int R, uint64_t *out, int batch, int l, int special,
                              void *workspace, void *stream);
-/* key_out = key_in (prepared pairs, rows of N) with every row permuted by
- * the inverse evaluation-domain automorphism of galois, for ckks_rotate_hoisted. */
+/* key_out = key_in (prepared key rows of N entries, in the context's
+ * arithmetic as ckks_key_prepare wrote them) with every row permuted by the
+ * inverse evaluation-domain automorphism of galois, for ckks_rotate_hoisted. */
 int ckks_key_permute(const ckks_ctx *ctx, const uint64_t *key_in, uint64_t *key_out, int rows, uint64_t galois,
                      void *stream);
 
@@ -267,8 +273,9 @@ int ckks_embed(const ckks_ctx *ctx, const double *coeffs, double *slots, int n_v
 int ckks_embed_inv(const ckks_ctx *ctx, const double *slots, double *coeffs, int n_vec, int order, void *stream);
 
 /* Plaintext rows for ckks_lincomb: coefficient-form pt [n_polys][l][N]
- * (transformed in place) -> out [n_polys][l][N] prepared pairs (the
- * doubles w and RN(w / q)) in evaluation order. */
+ * (transformed in place) -> out [n_polys][l][N] prepared pairs in
+ * evaluation order, in the context's arithmetic: the doubles w and
+ * RN(w / q) (FP64), or w and floor(w 2^63 / q) (INT64). */
 int ckks_pt_prepare(const ckks_ctx *ctx, uint64_t *pt, uint64_t *out, int n_polys, int l, int prime0,
                     void *stream);
 
@@ -332,13 +339,15 @@ int ckks_ctx_set_threads(ckks_ctx *ctx, int ntt_threads, int ks_threads, int ten
  * forward transforms, the tensor product and INTT(d2); 0 (default) =
  * separate transform launches then the tensor kernel.  Same results. */
 int ckks_ctx_set_front(ckks_ctx *ctx, int fused);
-/* Arithmetic of the LDS-resident transform kernels that exist in both
- * forms (the standalone transforms and ckks_poly_mul): CKKS_ARITH_INT64
- * (64-bit integer words, Shoup and Barrett products) or CKKS_ARITH_FP64
- * (default: residues held as exact FP64 integers, products assembled from
- * fused multiply-adds).  Results are identical word for word; the choice is
- * a speed setting.  The HMult front, the key switches and the plaintext
- * products read prepared FP64 pairs and run on the FP64 pipeline only.  No reference counterpart
+/* Residue arithmetic of every kernel on the path (transforms, products,
+ * the HMult front, key switches, hoisted key switch, plaintext
+ * combinations): CKKS_ARITH_INT64 (64-bit integer words, Shoup and Barrett
+ * products on __umul64hi) or CKKS_ARITH_FP64 (default: residues held as
+ * exact FP64 integers, products assembled from fused multiply-adds).
+ * Results are identical word for word; words at rest are canonical uint64
+ * residues in both.  Prepared keys and plaintexts (ckks_key_prepare,
+ * ckks_pt_prepare) are in the form of the arithmetic in force when they
+ * were prepared: prepare them after setting it.  No reference counterpart
  * (the RTL multiplier rtl/m1/mult_43x43_horner4.sv is one fixed datapath). */
 #define CKKS_ARITH_INT64 0
 #define CKKS_ARITH_FP64 1

@@ -82,6 +82,18 @@ int main(int argc, char** argv) {
                 CHECK(8 * (ri < 0 ? -ri : ri) <= 5 * (i128)q, "mulw bound q=%lld a=%lld w=%lld r=%.0f", (long long)q,
                       (long long)a, (long long)w, r);
             }
+            // mulk: |a| < 2^51, w in [0, q), quotient formed from qinv
+            {
+                const int64_t a = signed_operand((1ll << 51) - 1, it + 1);
+                const int64_t w = (it % 5 == 0) ? q - 1 : (it % 5 == 1 ? 1 : (int64_t)below(q));
+                const double r = mulk((double)a, (double)w, qinv, nq);
+                CHECK(is_int(r), "mulk not an integer q=%lld a=%lld w=%lld", (long long)q, (long long)a, (long long)w);
+                const i128 ri = (i128)r;
+                CHECK(mod(ri - (i128)a * w, q) == 0, "mulk residue q=%lld a=%lld w=%lld r=%.0f", (long long)q,
+                      (long long)a, (long long)w, r);
+                CHECK(8 * (ri < 0 ? -ri : ri) <= 7 * (i128)q, "mulk bound q=%lld a=%lld w=%lld r=%.0f", (long long)q,
+                      (long long)a, (long long)w, r);
+            }
             // muld: |a b| <= 2^50 q (operands up to the bound's square root and reduced ones)
             {
                 int64_t a, b;

@@ -606,12 +606,14 @@ def test_integer_arithmetic_gives_the_same_words(gpu, N):
     c.ring.set_arith(False)
     with pytest.raises(ValueError):
         c.hmult(a, b, rlk)
+    q_all = np.array(c.primes, dtype=np.uint64).reshape(1, 1, -1, 1)
     irlk = c.load_switch_key(rlk.coeff)
     igks = [c.load_switch_key(g.coeff, g.galois) for g in gks]
     # the same words in the integer form: (w, floor(w 2^63 / q))
-    words = to_numpy(rlk.prepared)[..., 0].view(np.float64).astype(np.uint64)
-    q = np.array(c.primes, dtype=np.uint64).reshape(1, 1, -1, 1)
-    assert np.array_equal(to_numpy(irlk.prepared), prepared_pairs(words, q, fp64=False))
+    words = to_numpy(rlk.prepared).view(np.float64)     # FP64 key rows: the words as doubles
+    assert np.array_equal(words, np.floor(words)) and np.all(words < q_all.astype(np.float64))
+    words = words.astype(np.uint64)
+    assert np.array_equal(to_numpy(irlk.prepared), prepared_pairs(words, q_all, fp64=False))
     logN = N.bit_length() - 1
     for front, tensor, ks in itertools.product((0, 1), BUILT_FUSED[logN], BUILT_FUSED[logN]):
         c.ring.set_front(bool(front))
