@@ -225,6 +225,19 @@ __device__ __forceinline__ void fused_out(const double (&x)[C::E], const uint64_
     }
 }
 
+// fused_out with the rows of limb l - 1 and of p already loaded (coalesced
+// layout, register e holding word e T + tid).
+template <class C>
+__device__ __forceinline__ void fused_out_rows(const double (&x)[C::E], const uint64_t (&rl)[C::E],
+                                               const uint64_t (&rp)[C::E], uint64_t* __restrict__ out_c, int t,
+                                               int l, const PrimeF* __restrict__ pfs, const LimbMap& map,
+                                               int pidx_p, const ModDownConst& mc, int tid) {
+    const PrimeF pc = pfs[map.idx[t]], pl = pfs[map.idx[l - 1]], pp = pfs[pidx_p];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e)
+        out_c[e * C::T + tid] = moddown_rescale_one(x[e], u2d(rl[e]), u2d(rp[e]), t, l - 1, pc, pl, pp, mc);
+}
+
 // Hybrid key switch for one (ciphertext, target limb t) pair, t = 0..l where
 // t = l is the special prime p.  Generalises the single-digit relinearisation
 // f0 = e0 + e2 b, f1 = e1 - e2 a (verify_keyswitch.py:62-68,
@@ -343,12 +356,25 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     const double2* itw = tw + C::N;
     if constexpr (ntt::pair_pays<C>()) {
         // both accumulators as one transform pair
-        nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid);
         if constexpr (FUSED) {
-            fused_out<C>(a0, acc_b, out_b, t, l, pfs, map, pidx_p, mc, tid);
-            fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p,
-                         mc, tid);
+            // The epilogue's rows (limb l - 1 and p) of poly 0 are loaded
+            // during the pair's last rounds, those of poly 1 before poly 0
+            // is finished: their latency overlaps work instead of following
+            // the transform.
+            uint64_t rl[C::E], rp[C::E];
+            auto prefetch = [&]() {
+                nttf::issue_row<C>(rl, acc_b + (size_t)(l - 1) * C::N, tid);
+                nttf::issue_row<C>(rp, acc_b + (size_t)l * C::N, tid);
+            };
+            nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid, prefetch);
+            const uint64_t* acc_b1 = acc_b + (size_t)(l + 1) * C::N;
+            uint64_t rl1[C::E], rp1[C::E];
+            nttf::issue_row<C>(rl1, acc_b1 + (size_t)(l - 1) * C::N, tid);
+            nttf::issue_row<C>(rp1, acc_b1 + (size_t)l * C::N, tid);
+            fused_out_rows<C>(a0, rl, rp, out_b, t, l, pfs, map, pidx_p, mc, tid);
+            fused_out_rows<C>(a1, rl1, rp1, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p, mc, tid);
         } else {
+            nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid);
             nttf::store_words<C>(a0, out, pf, tid);
             nttf::store_words<C>(a1, out + (size_t)(l + 1) * C::N, pf, tid);
         }

@@ -371,16 +371,64 @@ __device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const
     inv_rest_db<C, C::NP - 2>(x, lds, itw, pf, tid);
 }
 
+// The first inverse pass (counting down) from which every remaining pass
+// takes its twiddles through scalar loads (ntt::group_of: lowest index bit
+// of 6 or more); -1 if the last pass does not.  A vector load issued there
+// is waited for by no vector twiddle load of the rest of the transform.
+template <class C>
+constexpr int inv_scalar_from() {
+    int p = -1;
+    for (int q = 0; q < C::NP && C::lo(q) >= 6; ++q) p = q;
+    return p;
+}
+
+// inv_rest_db calling hook() once, at pass inv_scalar_from (after that
+// pass's twiddle loads are issued).
+template <class C, int P, class H>
+__device__ __forceinline__ void inv_rest_db_h(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+                                              const PrimeF& pf, int tid, H& hook) {
+    if constexpr (P >= 0) {
+        constexpr bool W = ntt::wave_local<C, P + 1>();
+        if constexpr (P == C::NP - 2) __syncthreads();
+        else if constexpr (P <= C::NP - 4 && ntt::wave_local<C, P + 2>()) __syncthreads();
+        double* buf = lds + ((P & 1) ? C::LDS_WORDS : 0);
+        ntt::to_lds<C, P + 1, ntt::xshift<C>(P + 1)>(x, buf, tid);
+        const InvTw<C, P> w = inv_tw<C, P>(itw, tid);
+        if constexpr (P == inv_scalar_from<C>()) hook();
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, ntt::xshift<C>(P + 1)>(x, buf, tid);
+        inv_pass<C, P>(x, w, itw, pf, tid);
+        inv_rest_db_h<C, P - 1>(x, lds, itw, pf, tid, hook);
+    }
+}
+
 // inv_ntt_db with the first pass's twiddles already loaded: a caller that
 // stores results just before the transform loads them first, since a wait
 // for a load also waits for every memory operation issued before it.
+// hook() runs once, where every later twiddle load is a scalar one (at the
+// start if all are, at the end if none is): the place for the caller's
+// loads of its next work item.
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+template <class C, class H>
+__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+                                           const PrimeF& pf, int tid, const InvTw<C, C::NP - 1>& first, H& hook) {
+    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
+    constexpr int PS = inv_scalar_from<C>();
+    opaque(itw, tid);
+    if constexpr (PS >= C::NP - 1) hook();
+    inv_pass<C, C::NP - 1>(x, first, itw, pf, tid);
+    inv_rest_db_h<C, C::NP - 2>(x, lds, itw, pf, tid, hook);
+    if constexpr (PS < 0) hook();
+}
+
 template <class C>
 __device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
                                            const PrimeF& pf, int tid, const InvTw<C, C::NP - 1>& first) {
-    static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
-    opaque(itw, tid);
-    inv_pass<C, C::NP - 1>(x, first, itw, pf, tid);
-    inv_rest_db<C, C::NP - 2>(x, lds, itw, pf, tid);
+    NoHook none;
+    inv_ntt_db<C>(x, lds, itw, pf, tid, first, none);
 }
 
 // ---- two transforms of one prime, software-pipelined -----------------------
@@ -479,6 +527,36 @@ __device__ __forceinline__ void inv_rest2(double (&x)[C::E], double (&y)[C::E], 
     }
 }
 
+// inv_rest2 calling hook() once, at round inv_scalar_from (after that
+// round's twiddle loads are issued).
+template <class C, int P, class H>
+__device__ __forceinline__ void inv_rest2_h(double (&x)[C::E], double (&y)[C::E], double* lds,
+                                            const double2* __restrict__ itw, const InvTw<C, P + 1>& prev,
+                                            const PrimeF& pf, int tid, H& hook) {
+    if constexpr (P >= 0) {
+        constexpr bool W = ntt::wave_local<C, P + 1>();
+        constexpr int K = ntt::xshift<C>(P + 1);
+        double* bx = lds;
+        double* by = lds + C::LDS_WORDS;
+        if constexpr (P == C::NP - 2) __syncthreads();
+        else if constexpr (ntt::relaid_after_wave_local<C, P + 2, P + 1>()) __syncthreads();
+        ntt::to_lds<C, P + 1, K>(x, bx, tid);
+        inv_pass<C, P + 1>(y, prev, itw, pf, tid);
+        const InvTw<C, P> w = inv_tw<C, P>(itw, tid);
+        if constexpr (P == inv_scalar_from<C>()) hook();
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, K>(x, bx, tid);
+        if constexpr (y_relaid_inv<C, P + 1>()) __syncthreads();
+        ntt::to_lds<C, P + 1, K>(y, by, tid);
+        inv_pass<C, P>(x, w, itw, pf, tid);
+        ntt::exchange_sync<W>();
+        ntt::from_lds<C, P, K>(y, by, tid);
+        inv_rest2_h<C, P - 1>(x, y, lds, itw, w, pf, tid, hook);
+    } else {
+        inv_pass<C, 0>(y, prev, itw, pf, tid);
+    }
+}
+
 // inv_ntt on x and y at once (same entry and exit conditions; the first
 // exchange starts with a workgroup barrier).  On exit other waves may still
 // be reading y's buffer: a workgroup barrier precedes any further LDS write.
@@ -490,6 +568,21 @@ __device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], d
     const InvTw<C, C::NP - 1> w = inv_tw<C, C::NP - 1>(itw, tid);
     inv_pass<C, C::NP - 1>(x, w, itw, pf, tid);
     inv_rest2<C, C::NP - 2>(x, y, lds, itw, w, pf, tid);
+}
+
+// inv_ntt2 with hook() run once where every later twiddle load is a scalar
+// one (as the single-transform inv_ntt_db above).
+template <class C, class H>
+__device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
+                                         const double2* __restrict__ itw, const PrimeF& pf, int tid, H& hook) {
+    static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
+    constexpr int PS = inv_scalar_from<C>();
+    opaque(itw, tid);
+    const InvTw<C, C::NP - 1> w = inv_tw<C, C::NP - 1>(itw, tid);
+    if constexpr (PS >= C::NP - 1) hook();
+    inv_pass<C, C::NP - 1>(x, w, itw, pf, tid);
+    inv_rest2_h<C, C::NP - 2>(x, y, lds, itw, w, pf, tid, hook);
+    if constexpr (PS < 0) hook();
 }
 
 // The built paired shapes: at N = 4096 with 512 threads exchanges 2 and 3
