@@ -97,7 +97,7 @@ struct ckks_ctx {
     PrimeConst* dpc = nullptr;
     ulonglong2* dtw = nullptr;
     PrimeF* dpf = nullptr;        // FP64 pipeline: per-prime constants
-    double2* dtwf = nullptr;      // FP64 pipeline: (w, RN(w / q)) twiddles, order of dtw
+    double* dtwf = nullptr;       // FP64 pipeline: the twiddle words as doubles, order of dtw
     int fp64 = 1;                 // transform arithmetic (ckks_ctx_set_arith)
     double2* fft_tw = nullptr;    // exp(2 pi i k / n), k < n/2, n = N/2
     double2* fft_zeta = nullptr;  // exp(i pi j / N), j < n
@@ -302,7 +302,7 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
     }
     const int logN = c->logN;
     std::vector<ulonglong2> tw((size_t)n_primes * 2 * N);
-    std::vector<double2> twf((size_t)n_primes * 2 * N);
+    std::vector<double> twf((size_t)n_primes * 2 * N);
     std::vector<PrimeF> pf(n_primes);
     for (int p = 0; p < n_primes; ++p) {
         const uint64_t q = primes[p];
@@ -337,12 +337,14 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
         pc.ninv_w1 = mulmod(pc.ninv, iv[1].x, q);
         pc.ninv_w1_s = shoup(pc.ninv_w1, q);
         // the FP64 pipeline's constants and twiddles: integers below 2^45 are
-        // exact doubles, and IEEE division rounds w / q once
+        // exact doubles, and IEEE division rounds 1 / q and the N^-1
+        // constants' quotients once (the twiddles' quotients are formed by
+        // the butterflies from 1 / q)
         const double dq = (double)q;
         pf[p] = PrimeF{dq, -dq, 1.0 / dq, (double)pc.ninv, (double)pc.ninv / dq, (double)pc.ninv_w1,
                        (double)pc.ninv_w1 / dq, 0.0};
-        double2* ff = &twf[(size_t)p * 2 * N];
-        for (uint64_t k = 0; k < 2 * N; ++k) ff[k] = make_double2((double)f[k].x, (double)f[k].x / dq);
+        double* ff = &twf[(size_t)p * 2 * N];
+        for (uint64_t k = 0; k < 2 * N; ++k) ff[k] = (double)f[k].x;
     }
     // canonical-embedding tables (negacyclic rings): the length-N/2 FFT's
     // twiddles, the twist zeta^j and the 5^j slot order
@@ -377,9 +379,9 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
     ok = ok && hipMemcpy(c->dpc, c->hpc, sizeof(PrimeConst) * n_primes, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(c->dtw, tw.data(), tw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMalloc(&c->dpf, pf.size() * sizeof(PrimeF)) == hipSuccess;
-    ok = ok && hipMalloc(&c->dtwf, twf.size() * sizeof(double2)) == hipSuccess;
+    ok = ok && hipMalloc(&c->dtwf, twf.size() * sizeof(double)) == hipSuccess;
     ok = ok && hipMemcpy(c->dpf, pf.data(), pf.size() * sizeof(PrimeF), hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpy(c->dtwf, twf.data(), twf.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->dtwf, twf.data(), twf.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && emb) {
         ok = ok && hipMalloc(&c->fft_tw, ft.size() * sizeof(double2)) == hipSuccess;
         ok = ok && hipMalloc(&c->fft_zeta, zt.size() * sizeof(double2)) == hipSuccess;

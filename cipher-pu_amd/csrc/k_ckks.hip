@@ -29,7 +29,7 @@ using ntt::Cfg;
 template <int LOGN, int T>
 __global__ void __launch_bounds__(T) k_tensor_intt(const uint64_t* __restrict__ ea, const uint64_t* __restrict__ eb,
                                                    uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
-                                                   uint64_t* __restrict__ d2c, const double2* __restrict__ twf_all,
+                                                   uint64_t* __restrict__ d2c, const double* __restrict__ twf_all,
                                                    const PrimeF* __restrict__ pfs, LimbMap map, int l,
                                                    ModDownConst mc) {
     using C = Cfg<LOGN, T>;
@@ -89,12 +89,12 @@ __device__ __forceinline__ double tensor_one(double x0, double x1, double y0, do
 // forward transforms of a0, a1, b0, b1, the tensor product, p*d0 and p*d1
 // and d2 written in evaluation form, d2 transformed back to coefficient form.
 // Limb-major grid: workgroups in flight share one prime's twiddles.  FP64
-// pipeline: the forward transforms leave values below (1 + 5 log2(N)/8) q,
+// pipeline: the forward transforms leave values below (1 + 7 log2(N)/8) q,
 // reduced before the data products.
 template <int LOGN, int T>
 __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
                                              uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
-                                             uint64_t* __restrict__ d2c, const double2* __restrict__ twf_all,
+                                             uint64_t* __restrict__ d2c, const double* __restrict__ twf_all,
                                              const PrimeF* __restrict__ pfs, LimbMap map, int l, int batch,
                                              ModDownConst mc) {
     using C = Cfg<LOGN, T>;
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
     const int i = wave_uniform(blockIdx.x / batch), b = wave_uniform(blockIdx.x % batch);
     const int p = map.idx[i];
     const PrimeF pf = pfs[p];
-    const double2* tw = twf_all + (size_t)p * 2 * C::N;
+    const double* tw = twf_all + (size_t)p * 2 * C::N;
     const size_t LN = (size_t)l * C::N;
     const size_t c0 = (size_t)b * 2 * LN + (size_t)i * C::N;   // poly 0 of pair b, limb i
     const double pm = mc.pmf[i], pmq = mc.pmf_q[i];
@@ -119,11 +119,11 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
         // b0 is loaded ahead, in flight while the first passes (scalar
         // twiddle loads only) transform a0 and a1
         nttf::issue_row<C>(nb0, B + c0, tid);
-        nttf::fwd_ntt2<C>(x0, x1, lds, tw, pf.nq, tid);
+        nttf::fwd_ntt2<C>(x0, x1, lds, tw, pf, tid);
 #pragma unroll
         for (int k = 0; k < C::E; ++k) y0[k] = u2d(nb0[k]);
         nttf::load_words<C>(y1, B + c0 + LN, tid);
-        nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf.nq, tid);
+        nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf, tid);
         // the inverse transform's first twiddles, loaded before the stores
         const auto pre = nttf::inv_tw<C, C::NP - 1>(tw + C::N, tid);
 #pragma unroll
@@ -136,11 +136,11 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
     double x0[C::E], x1[C::E], y[C::E], z[C::E];
     const double qi = pf.qinv, nq = pf.nq;
     nttf::load_words<C>(x0, A + c0, tid);
-    nttf::fwd_ntt<C>(x0, lds, tw, nq, tid);
+    nttf::fwd_ntt<C>(x0, lds, tw, pf, tid);
     nttf::load_words<C>(x1, A + c0 + LN, tid);
-    nttf::fwd_ntt<C>(x1, lds, tw, nq, tid);
+    nttf::fwd_ntt<C>(x1, lds, tw, pf, tid);
     nttf::load_words<C>(y, B + c0, tid);
-    nttf::fwd_ntt<C>(y, lds, tw, nq, tid);
+    nttf::fwd_ntt<C>(y, lds, tw, pf, tid);
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
         const double a0 = rmod(x0[k], qi, nq), a1 = rmod(x1[k], qi, nq), b0 = rmod(y[k], qi, nq);
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
         x1[k] = a1;
     }
     nttf::load_words<C>(y, B + c0 + LN, tid);
-    nttf::fwd_ntt<C>(y, lds, tw, nq, tid);
+    nttf::fwd_ntt<C>(y, lds, tw, pf, tid);
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
 #pragma clang fp contract(off)
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
                                                  const uint64_t* __restrict__ d2e,
                                                  const uint64_t* __restrict__ d01, const double* __restrict__ key,
                                                  int key_limbs, uint64_t* __restrict__ acc,
-                                                 const double2* __restrict__ twf_all, const PrimeF* __restrict__ pfs,
+                                                 const double* __restrict__ twf_all, const PrimeF* __restrict__ pfs,
                                                  LimbMap map, int l, int pidx_p, int batch, int reduce_digits,
                                                  const uint64_t* __restrict__ key_table, int t0,
                                                  uint64_t* __restrict__ out_ct, ModDownConst mc, int pair_major) {
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     const int kt = t < l ? t : key_limbs - 1;
     const PrimeF pf = pfs[pt];
     const double nq = pf.nq;
-    const double2* tw = twf_all + (size_t)pt * 2 * C::N;
+    const double* tw = twf_all + (size_t)pt * 2 * C::N;
     const size_t LN = (size_t)l * C::N;
     double a0[C::E], a1[C::E], x[C::E];
     if (t < l && d01) {
@@ -337,9 +337,9 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
             }
             jn = j + 1 + ((own && j + 1 == t) ? 1 : 0);
             if (PF && jn < l) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);
-            // left unreduced (below 11 q): the key product is exact for any such input
-            if constexpr (ntt::pair_pays<C>()) nttf::fwd_ntt_db<C>(x, lds, tw, nq, tid);
-            else nttf::fwd_ntt<C>(x, lds, tw, nq, tid);
+            // left unreduced (below 15 q): the key product is exact for any such input
+            if constexpr (ntt::pair_pays<C>()) nttf::fwd_ntt_db<C>(x, lds, tw, pf, tid);
+            else nttf::fwd_ntt<C>(x, lds, tw, pf, tid);
         }
         key_mac<C>(a0, a1, x, key, j, key_limbs, kt, pf.qinv, nq, tid);
     }
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
     uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
     const uint64_t* acc_b = acc + (size_t)b * 2 * (l + 1) * C::N;
     uint64_t* out_b = FUSED ? out_ct + ((size_t)b * 2 * (l - 1) + t) * C::N : nullptr;
-    const double2* itw = tw + C::N;
+    const double* itw = tw + C::N;
     if constexpr (ntt::pair_pays<C>()) {
         // both accumulators as one transform pair
         if constexpr (FUSED) {

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(T) k_modup(const uint64_t* __restrict__ ct, ui
 // apply pi_g through LDS, inverse-transform and store it in coefficient form.
 template <class C>
 __device__ __forceinline__ void permute_inverse(double (&a)[C::E], uint32_t g, double* lds,
-                                                const double2* __restrict__ itw, const PrimeF& pf,
+                                                const double* __restrict__ itw, const PrimeF& pf,
                                                 uint64_t* __restrict__ out, int tid) {
 #pragma unroll
     for (int k = 0; k < C::E; ++k) lds[ntt::pad<C>(k * C::T + tid)] = rmod(a[k], pf.qinv, pf.nq);
@@ -100,7 +100,7 @@ __device__ __forceinline__ void permute_store_eval(const double (&a)[C::E], uint
 // gather each, one transform pair.
 template <class C>
 __device__ __forceinline__ void permute_inverse2(double (&a0)[C::E], double (&a1)[C::E], uint32_t g, double* lds,
-                                                 const double2* __restrict__ itw, const PrimeF& pf,
+                                                 const double* __restrict__ itw, const PrimeF& pf,
                                                  uint64_t* __restrict__ out0, uint64_t* __restrict__ out1, int tid) {
     double* lds1 = lds + C::LDS_WORDS;
 #pragma unroll
@@ -129,7 +129,7 @@ __device__ __forceinline__ void permute_inverse2(double (&a0)[C::E], double (&a1
 template <int LOGN, int T, bool EVAL>
 __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D, const double* __restrict__ keys,
                                                   size_t key_stride, int key_limbs, GaloisList galois,
-                                                  uint64_t* __restrict__ acc, const double2* __restrict__ twf_all,
+                                                  uint64_t* __restrict__ acc, const double* __restrict__ twf_all,
                                                   const PrimeF* __restrict__ pfs, LimbMap map, int l, int pidx_p,
                                                   int batch, int R, HoistOrder order) {
     using C = Cfg<LOGN, T>;
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
         }
     }
     const uint32_t g = galois.g[r];
-    const double2* itw = twf_all + ((size_t)pt * 2 + 1) * C::N;
+    const double* itw = twf_all + ((size_t)pt * 2 + 1) * C::N;
     uint64_t* out = acc + (((size_t)r * batch + b) * 2 * (l + 1) + t) * C::N;
     if (EVAL && t < l) {
         // chain limbs stay in evaluation form; ModDown finishes there

@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(T) k_polymul(const uint64_t* __restrict__ A, c
 // transforms reduced to (q + 1) / 2, so every product is exact.
 template <int LOGN, int T>
 __global__ void __launch_bounds__(T) k_polymul_f(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
-                                                 uint64_t* __restrict__ out, const double2* __restrict__ twf_all,
+                                                 uint64_t* __restrict__ out, const double* __restrict__ twf_all,
                                                  const PrimeF* __restrict__ pfs, LimbMap map, int n_limbs) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds[];
@@ -113,15 +113,15 @@ __global__ void __launch_bounds__(T) k_polymul_f(const uint64_t* __restrict__ A,
     const int p = map.idx[limb];
     const PrimeF pf = pfs[p];
     const size_t off = (size_t)blockIdx.x * C::N;
-    const double2* tw = twf_all + (size_t)p * 2 * C::N;
+    const double* tw = twf_all + (size_t)p * 2 * C::N;
     double x[C::E], y[C::E];
     nttf::load_words<C>(x, A + off, tid);
     nttf::load_words<C>(y, B + off, tid);
     if constexpr (ntt::pair_pays<C>()) {
-        nttf::fwd_ntt2<C>(x, y, ldsf, tw, pf.nq, tid);
+        nttf::fwd_ntt2<C>(x, y, ldsf, tw, pf, tid);
     } else {
-        nttf::fwd_ntt<C>(x, ldsf, tw, pf.nq, tid);
-        nttf::fwd_ntt<C>(y, ldsf, tw, pf.nq, tid);
+        nttf::fwd_ntt<C>(x, ldsf, tw, pf, tid);
+        nttf::fwd_ntt<C>(y, ldsf, tw, pf, tid);
     }
 #pragma unroll
     for (int e = 0; e < C::E; ++e)
@@ -289,7 +289,7 @@ __global__ void k_fp64_words(const uint64_t* __restrict__ in, double* __restrict
 // the integer reduce64 first; otherwise below 2q, the integer kernels'
 // contract (the inverse reduces its inputs on load either way).
 template <int LOGN, int T>
-__global__ void __launch_bounds__(T) k_ntt_fwd_f(uint64_t* dst, const uint64_t* src, const double2* __restrict__ twf_all,
+__global__ void __launch_bounds__(T) k_ntt_fwd_f(uint64_t* dst, const uint64_t* src, const double* __restrict__ twf_all,
                                                  const PrimeF* __restrict__ pfs, const PrimeConst* __restrict__ pcs,
                                                  LimbMap map, int n_limbs, int reduce_in) {
     using C = Cfg<LOGN, T>;
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(T) k_ntt_fwd_f(uint64_t* dst, const uint64_t* 
     double x[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) x[e] = u2d(u[e]);
-    nttf::fwd_ntt<C>(x, ldsf, twf_all + (size_t)p * 2 * C::N, pf.nq, tid);
+    nttf::fwd_ntt<C>(x, ldsf, twf_all + (size_t)p * 2 * C::N, pf, tid);
 #pragma unroll
     for (int e = 0; e < C::E; ++e) u[e] = d2u(canon(x[e], pf.qinv, pf.nq));
     __syncthreads();
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(T) k_ntt_fwd_f(uint64_t* dst, const uint64_t* 
 }
 
 template <int LOGN, int T>
-__global__ void __launch_bounds__(T) k_ntt_inv_f(uint64_t* dst, const uint64_t* src, const double2* __restrict__ twf_all,
+__global__ void __launch_bounds__(T) k_ntt_inv_f(uint64_t* dst, const uint64_t* src, const double* __restrict__ twf_all,
                                                  const PrimeF* __restrict__ pfs, const PrimeConst* __restrict__ pcs,
                                                  LimbMap map, int n_limbs, int reduce_in) {
     using C = Cfg<LOGN, T>;

@@ -22,10 +22,10 @@ struct DevTables {
     // prime is then below 2q for any other q, which the lazy forward
     // transform accepts as input without a reduction.
     int narrow = 0;
-    // The FP64 pipeline's tables (modf64.h): per-prime constants and
-    // (w, RN(w / q)) twiddle pairs in the order of tw.
+    // The FP64 pipeline's tables (modf64.h): per-prime constants and the
+    // twiddle words as doubles, in the order of tw.
     const PrimeF* pf = nullptr;   // [K]
-    const double2* twf = nullptr; // [K][2][N]
+    const double* twf = nullptr;  // [K][2][N]
     // Arithmetic of the transform kernels that have both forms: 0 = 64-bit
     // integer words, 1 = FP64 (results identical).
     int fp64 = 0;

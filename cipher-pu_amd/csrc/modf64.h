@@ -105,7 +105,8 @@ MODF_FN uint64_t d2u(double x) {
 }
 
 // Forward (Cooley-Tukey) butterfly on signed residues: X' = X + W Y,
-// Y' = X - W Y.  Inputs bounded by B, outputs by B + 5/8.
+// Y' = X - W Y.  Inputs bounded by B, outputs by B + 7/8 with wq = RN(w
+// qinv) (the mulk bound; B + 5/8 with wq = RN(w / q)).
 MODF_FN void ct_bfly_f(double& X, double& Y, double w, double wq, double nq) {
 #pragma clang fp contract(off)
     const double t = mulw(Y, w, wq, nq);
@@ -115,7 +116,7 @@ MODF_FN void ct_bfly_f(double& X, double& Y, double w, double wq, double nq) {
 }
 
 // Inverse (Gentleman-Sande) butterfly: X' = X + Y, Y' = (X - Y) W.  Inputs
-// bounded by B (2 B q < 2^51), X' by 2 B, Y' by 5/8.
+// bounded by B (2 B q < 2^51), X' by 2 B, Y' by 7/8 (wq as above).
 MODF_FN void gs_bfly_f(double& X, double& Y, double w, double wq, double nq) {
 #pragma clang fp contract(off)
     const double x = X, y = Y;

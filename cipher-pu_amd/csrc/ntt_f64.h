@@ -8,17 +8,19 @@
 // residue; only the representative of each residue differs (signed, lazy).
 //
 //  * Forward (Cooley-Tukey, natural order in, bit-reversed out): each stage
-//    adds at most 5q/8 to the bound, so from inputs bounded by B q the
-//    outputs stay below (B + 5 log2(N) / 8) q; every value entering a product
-//    must be below 2^51, so B + 5 log2(N) / 8 < 2^51 / q (at least 64).
+//    adds at most 7q/8 to the bound, so from inputs bounded by B q the
+//    outputs stay below (B + 7 log2(N) / 8) q; every value entering a product
+//    must be below 2^51, so B + 7 log2(N) / 8 < 2^51 / q (at least 64).
 //  * Inverse (Gentleman-Sande, N^-1 folded into the last stage): the sums
 //    double the bound at each stage.  From inputs bounded by q, and again
 //    after each reduction to (q + 1) / 2, at most six stages keep every
 //    product input below 2^51; the plan below reduces all registers at the
 //    start of the passes where the next pass would exceed that.  Outputs
-//    are bounded by 5q/8.
-//  * Twiddles are (w, RN(w / q)) double pairs, one 16-byte load each, in the
-//    same [prime][forward | inverse][N] order as the integer table.
+//    are bounded by 5q/8 (the last stage's N^-1 constants carry their
+//    quotients).
+//  * Twiddles are the words w as doubles, one 8-byte load each, in the
+//    same [prime][forward | inverse][N] order as the integer table; each
+//    butterfly forms RN(w qinv) itself (mulk).
 #pragma once
 #include "modf64.h"
 #include "ntt_core.h"
@@ -27,7 +29,7 @@ namespace nttf {
 
 using ntt::Cfg;
 
-__device__ __forceinline__ void opaque(const double2* __restrict__& tw, int& tid) {
+__device__ __forceinline__ void opaque(const double* __restrict__& tw, int& tid) {
     int zero = 0;
     asm volatile("" : "+v"(tid), "+s"(zero));
     tw += zero;
@@ -39,9 +41,11 @@ __device__ __forceinline__ void opaque(const double2* __restrict__& tw, int& tid
 // between an exchange's LDS writes and its barrier -- the register data is
 // dead there -- so their latency overlaps the barrier instead of stalling
 // each stage (a wait for one vector load waits for all earlier ones).  A
-// pass that needs more than TW_SET pairs per thread (16 coefficients per
+// pass that needs more than TW_SET twiddles per thread (16 coefficients per
 // thread) keeps only its first stage's here and loads the rest as it goes,
-// so that the set does not crowd out the kernel's other registers.
+// so that the set does not crowd out the kernel's other registers.  The
+// tables hold the twiddle words alone (8 bytes each); each butterfly forms
+// the quotient RN(w qinv) with one multiply (modf64.h:mulk).
 constexpr int TW_SET = 8;
 template <class C, int P>
 struct FwdTw {
@@ -49,11 +53,11 @@ struct FwdTw {
     static constexpr bool FULL = G * (M - 1) <= TW_SET;
     // slot of stage b's k-th twiddle of group g (first stage only: g)
     static constexpr int slot(int g, int b, int k) { return FULL ? g * (M - 1) + (1 << (C::hi(P) - b)) - 1 + k : g; }
-    double2 w[FULL ? G * (M - 1) : G];
+    double w[FULL ? G * (M - 1) : G];
 };
 
 template <class C, int P>
-__device__ __forceinline__ FwdTw<C, P> fwd_tw(const double2* __restrict__ tw, int tid) {
+__device__ __forceinline__ FwdTw<C, P> fwd_tw(const double* __restrict__ tw, int tid) {
     constexpr int LO = C::lo(P), HI = C::hi(P);
     using S = FwdTw<C, P>;
     S f;
@@ -72,7 +76,7 @@ __device__ __forceinline__ FwdTw<C, P> fwd_tw(const double2* __restrict__ tw, in
 // Twiddle of stage b for butterfly k = e >> (b - LO + 1) of group g: from the
 // pass's set, or loaded here when the set holds the first stage only.
 template <class C, int P>
-__device__ __forceinline__ double2 fwd_w(const FwdTw<C, P>& set, const double2* __restrict__ tw, int g, int b, int k,
+__device__ __forceinline__ double fwd_w(const FwdTw<C, P>& set, const double* __restrict__ tw, int g, int b, int k,
                                          int tid) {
     constexpr int HI = C::hi(P);
     if (FwdTw<C, P>::FULL || b == HI) return set.w[FwdTw<C, P>::slot(g, b, k)];
@@ -81,8 +85,8 @@ __device__ __forceinline__ double2 fwd_w(const FwdTw<C, P>& set, const double2* 
 }
 
 template <class C, int P>
-__device__ __forceinline__ void fwd_pass(double (&x)[C::E], const FwdTw<C, P>& set, const double2* __restrict__ tw,
-                                         double nq, int tid) {
+__device__ __forceinline__ void fwd_pass(double (&x)[C::E], const FwdTw<C, P>& set, const double* __restrict__ tw,
+                                         const PrimeF& pf, int tid) {
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
@@ -93,8 +97,8 @@ __device__ __forceinline__ void fwd_pass(double (&x)[C::E], const FwdTw<C, P>& s
 #pragma unroll
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
-                const double2 w = fwd_w<C, P>(set, tw, g, b, e >> (b - LO + 1), tid);
-                ct_bfly_f(x[g * M + e], x[g * M + e + h], w.x, w.y, nq);
+                const double w = fwd_w<C, P>(set, tw, g, b, e >> (b - LO + 1), tid);
+                ct_bfly_f(x[g * M + e], x[g * M + e + h], w, w * pf.qinv, pf.nq);
             }
         }
     }
@@ -102,7 +106,7 @@ __device__ __forceinline__ void fwd_pass(double (&x)[C::E], const FwdTw<C, P>& s
 
 template <class C, int P>
 __device__ __forceinline__ void fwd_pass2(double (&x)[C::E], double (&y)[C::E], const FwdTw<C, P>& set,
-                                          const double2* __restrict__ tw, double nq, int tid) {
+                                          const double* __restrict__ tw, const PrimeF& pf, int tid) {
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
 #pragma unroll
@@ -113,9 +117,10 @@ __device__ __forceinline__ void fwd_pass2(double (&x)[C::E], double (&y)[C::E], 
 #pragma unroll
             for (int e = 0; e < M; ++e) {
                 if (e & h) continue;
-                const double2 w = fwd_w<C, P>(set, tw, g, b, e >> (b - LO + 1), tid);
-                ct_bfly_f(x[g * M + e], x[g * M + e + h], w.x, w.y, nq);
-                ct_bfly_f(y[g * M + e], y[g * M + e + h], w.x, w.y, nq);
+                const double w = fwd_w<C, P>(set, tw, g, b, e >> (b - LO + 1), tid);
+                const double wq = w * pf.qinv;
+                ct_bfly_f(x[g * M + e], x[g * M + e + h], w, wq, pf.nq);
+                ct_bfly_f(y[g * M + e], y[g * M + e + h], w, wq, pf.nq);
             }
         }
     }
@@ -148,11 +153,11 @@ struct InvTw {
     static constexpr int slot(int g, int b, int k) {
         return FULL ? g * (M - 1) + (1 << (C::hi(P) - b)) - 1 + k : g * (M / 2) + k;
     }
-    double2 w[FULL ? G * (M - 1) : G * (M / 2)];
+    double w[FULL ? G * (M - 1) : G * (M / 2)];
 };
 
 template <class C, int P>
-__device__ __forceinline__ InvTw<C, P> inv_tw(const double2* __restrict__ itw, int tid) {
+__device__ __forceinline__ InvTw<C, P> inv_tw(const double* __restrict__ itw, int tid) {
     constexpr int LO = C::lo(P), HI = C::hi(P);
     using S = InvTw<C, P>;
     S f;
@@ -171,7 +176,7 @@ __device__ __forceinline__ InvTw<C, P> inv_tw(const double2* __restrict__ itw, i
 }
 
 template <class C, int P>
-__device__ __forceinline__ double2 inv_w(const InvTw<C, P>& set, const double2* __restrict__ itw, int g, int b, int k,
+__device__ __forceinline__ double inv_w(const InvTw<C, P>& set, const double* __restrict__ itw, int g, int b, int k,
                                          int tid) {
     constexpr int LO = C::lo(P), HI = C::hi(P);
     if (InvTw<C, P>::FULL || b == LO) return set.w[InvTw<C, P>::slot(g, b, k)];
@@ -180,7 +185,7 @@ __device__ __forceinline__ double2 inv_w(const InvTw<C, P>& set, const double2* 
 }
 
 template <class C, int P>
-__device__ __forceinline__ void inv_pass(double (&x)[C::E], const InvTw<C, P>& set, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_pass(double (&x)[C::E], const InvTw<C, P>& set, const double* __restrict__ itw,
                                          const PrimeF& pf, int tid) {
     static_assert(C::rsize(P) <= 6, "an inverse pass longer than one reduction segment");
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
@@ -206,8 +211,8 @@ __device__ __forceinline__ void inv_pass(double (&x)[C::E], const InvTw<C, P>& s
                     X = mulw(u + v, pf.ninv, pf.ninv_q, nq);
                     Y = mulw(u - v, pf.ninv_w1, pf.ninv_w1_q, nq);
                 } else {
-                    const double2 w = inv_w<C, P>(set, itw, g, b, e >> (b - LO + 1), tid);
-                    gs_bfly_f(X, Y, w.x, w.y, nq);
+                    const double w = inv_w<C, P>(set, itw, g, b, e >> (b - LO + 1), tid);
+                    gs_bfly_f(X, Y, w, w * pf.qinv, nq);
                 }
             }
         }
@@ -216,7 +221,7 @@ __device__ __forceinline__ void inv_pass(double (&x)[C::E], const InvTw<C, P>& s
 
 template <class C, int P>
 __device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], const InvTw<C, P>& set,
-                                          const double2* __restrict__ itw, const PrimeF& pf, int tid) {
+                                          const double* __restrict__ itw, const PrimeF& pf, int tid) {
     static_assert(C::rsize(P) <= 6, "an inverse pass longer than one reduction segment");
     constexpr int RP = C::rsize(P), LO = C::lo(P), HI = C::hi(P);
     constexpr int M = 1 << RP, G = C::E >> RP;
@@ -248,9 +253,10 @@ __device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], 
                     U = mulw(u0 + u1, pf.ninv, pf.ninv_q, nq);
                     V = mulw(u0 - u1, pf.ninv_w1, pf.ninv_w1_q, nq);
                 } else {
-                    const double2 w = inv_w<C, P>(set, itw, g, b, e >> (b - LO + 1), tid);
-                    gs_bfly_f(X, Y, w.x, w.y, nq);
-                    gs_bfly_f(U, V, w.x, w.y, nq);
+                    const double w = inv_w<C, P>(set, itw, g, b, e >> (b - LO + 1), tid);
+                    const double wq = w * pf.qinv;
+                    gs_bfly_f(X, Y, w, wq, nq);
+                    gs_bfly_f(U, V, w, wq, nq);
                 }
             }
         }
@@ -260,7 +266,7 @@ __device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], 
 // ---- single transforms, one exchange buffer --------------------------------
 
 template <class C, int P>
-__device__ __forceinline__ void fwd_rest(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+__device__ __forceinline__ void fwd_rest(double (&x)[C::E], double* lds, const double* __restrict__ tw, const PrimeF& pf,
                                          int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = ntt::wave_local<C, P>();
@@ -271,23 +277,23 @@ __device__ __forceinline__ void fwd_rest(double (&x)[C::E], double* lds, const d
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, ntt::xshift<C>(P)>(x, lds, tid);
         ntt::exchange_sync<W>();
-        fwd_pass<C, P>(x, w, tw, nq, tid);
-        fwd_rest<C, P + 1>(x, lds, tw, nq, tid);
+        fwd_pass<C, P>(x, w, tw, pf, tid);
+        fwd_rest<C, P + 1>(x, lds, tw, pf, tid);
     }
 }
 
 // x holds pass-0 layout on entry (x[e] = a[e T + tid]), the last pass's
 // layout on exit (bit-reversed evaluation order, see ntt::eval_index).
 template <class C>
-__device__ __forceinline__ void fwd_ntt(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+__device__ __forceinline__ void fwd_ntt(double (&x)[C::E], double* lds, const double* __restrict__ tw, const PrimeF& pf,
                                         int tid) {
     opaque(tw, tid);
-    fwd_pass<C, 0>(x, fwd_tw<C, 0>(tw, tid), tw, nq, tid);
-    fwd_rest<C, 1>(x, lds, tw, nq, tid);
+    fwd_pass<C, 0>(x, fwd_tw<C, 0>(tw, tid), tw, pf, tid);
+    fwd_rest<C, 1>(x, lds, tw, pf, tid);
 }
 
 template <class C, int P>
-__device__ __forceinline__ void inv_rest(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_rest(double (&x)[C::E], double* lds, const double* __restrict__ itw,
                                          const PrimeF& pf, int tid) {
     if constexpr (P >= 0) {
         constexpr bool W = ntt::wave_local<C, P + 1>();
@@ -305,7 +311,7 @@ __device__ __forceinline__ void inv_rest(double (&x)[C::E], double* lds, const d
 // x holds the last forward pass's layout on entry (bounded by q), pass-0
 // (coalesced) layout on exit, bounded by 5q/8.
 template <class C>
-__device__ __forceinline__ void inv_ntt(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_ntt(double (&x)[C::E], double* lds, const double* __restrict__ itw,
                                         const PrimeF& pf, int tid) {
     opaque(itw, tid);
     inv_pass<C, C::NP - 1>(x, inv_tw<C, C::NP - 1>(itw, tid), itw, pf, tid);
@@ -320,7 +326,7 @@ __device__ __forceinline__ void inv_ntt(double (&x)[C::E], double* lds, const do
 // other LDS write.
 
 template <class C, int P>
-__device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+__device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, const double* __restrict__ tw, const PrimeF& pf,
                                             int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = ntt::wave_local<C, P>();
@@ -331,22 +337,22 @@ __device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, cons
         const FwdTw<C, P> w = fwd_tw<C, P>(tw, tid);
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, ntt::xshift<C>(P)>(x, buf, tid);
-        fwd_pass<C, P>(x, w, tw, nq, tid);
-        fwd_rest_db<C, P + 1>(x, lds, tw, nq, tid);
+        fwd_pass<C, P>(x, w, tw, pf, tid);
+        fwd_rest_db<C, P + 1>(x, lds, tw, pf, tid);
     }
 }
 
 template <class C>
-__device__ __forceinline__ void fwd_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ tw, double nq,
+__device__ __forceinline__ void fwd_ntt_db(double (&x)[C::E], double* lds, const double* __restrict__ tw, const PrimeF& pf,
                                            int tid) {
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     opaque(tw, tid);
-    fwd_pass<C, 0>(x, fwd_tw<C, 0>(tw, tid), tw, nq, tid);
-    fwd_rest_db<C, 1>(x, lds, tw, nq, tid);
+    fwd_pass<C, 0>(x, fwd_tw<C, 0>(tw, tid), tw, pf, tid);
+    fwd_rest_db<C, 1>(x, lds, tw, pf, tid);
 }
 
 template <class C, int P>
-__device__ __forceinline__ void inv_rest_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_rest_db(double (&x)[C::E], double* lds, const double* __restrict__ itw,
                                             const PrimeF& pf, int tid) {
     if constexpr (P >= 0) {
         constexpr bool W = ntt::wave_local<C, P + 1>();
@@ -363,7 +369,7 @@ __device__ __forceinline__ void inv_rest_db(double (&x)[C::E], double* lds, cons
 }
 
 template <class C>
-__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double* __restrict__ itw,
                                            const PrimeF& pf, int tid) {
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     opaque(itw, tid);
@@ -385,7 +391,7 @@ constexpr int inv_scalar_from() {
 // inv_rest_db calling hook() once, at pass inv_scalar_from (after that
 // pass's twiddle loads are issued).
 template <class C, int P, class H>
-__device__ __forceinline__ void inv_rest_db_h(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_rest_db_h(double (&x)[C::E], double* lds, const double* __restrict__ itw,
                                               const PrimeF& pf, int tid, H& hook) {
     if constexpr (P >= 0) {
         constexpr bool W = ntt::wave_local<C, P + 1>();
@@ -413,7 +419,7 @@ struct NoHook {
 };
 
 template <class C, class H>
-__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double* __restrict__ itw,
                                            const PrimeF& pf, int tid, const InvTw<C, C::NP - 1>& first, H& hook) {
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     constexpr int PS = inv_scalar_from<C>();
@@ -425,7 +431,7 @@ __device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const
 }
 
 template <class C>
-__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double2* __restrict__ itw,
+__device__ __forceinline__ void inv_ntt_db(double (&x)[C::E], double* lds, const double* __restrict__ itw,
                                            const PrimeF& pf, int tid, const InvTw<C, C::NP - 1>& first) {
     NoHook none;
     inv_ntt_db<C>(x, lds, itw, pf, tid, first, none);
@@ -462,7 +468,7 @@ constexpr bool y_relaid_inv() {   // inverse round through exchange X
 
 template <class C, int P>
 __device__ __forceinline__ void fwd_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                          const double2* __restrict__ tw, const FwdTw<C, P - 1>& prev, double nq,
+                                          const double* __restrict__ tw, const FwdTw<C, P - 1>& prev, const PrimeF& pf,
                                           int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = ntt::wave_local<C, P>();
@@ -472,18 +478,18 @@ __device__ __forceinline__ void fwd_rest2(double (&x)[C::E], double (&y)[C::E], 
         if constexpr (P == 1) __syncthreads();
         else if constexpr (ntt::relaid_after_wave_local<C, P - 1, P>()) __syncthreads();
         ntt::to_lds<C, P - 1, K>(x, bx, tid);
-        fwd_pass<C, P - 1>(y, prev, tw, nq, tid);
+        fwd_pass<C, P - 1>(y, prev, tw, pf, tid);
         const FwdTw<C, P> w = fwd_tw<C, P>(tw, tid);
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, K>(x, bx, tid);
         if constexpr (y_relaid<C, P>()) __syncthreads();
         ntt::to_lds<C, P - 1, K>(y, by, tid);
-        fwd_pass<C, P>(x, w, tw, nq, tid);
+        fwd_pass<C, P>(x, w, tw, pf, tid);
         ntt::exchange_sync<W>();
         ntt::from_lds<C, P, K>(y, by, tid);
-        fwd_rest2<C, P + 1>(x, y, lds, tw, w, nq, tid);
+        fwd_rest2<C, P + 1>(x, y, lds, tw, w, pf, tid);
     } else {
-        fwd_pass<C, C::NP - 1>(y, prev, tw, nq, tid);
+        fwd_pass<C, C::NP - 1>(y, prev, tw, pf, tid);
     }
 }
 
@@ -492,17 +498,17 @@ __device__ __forceinline__ void fwd_rest2(double (&x)[C::E], double (&y)[C::E], 
 // any further LDS write (the transforms here start with one).
 template <class C>
 __device__ __forceinline__ void fwd_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                         const double2* __restrict__ tw, double nq, int tid) {
+                                         const double* __restrict__ tw, const PrimeF& pf, int tid) {
     static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
     opaque(tw, tid);
     const FwdTw<C, 0> w0 = fwd_tw<C, 0>(tw, tid);
-    fwd_pass<C, 0>(x, w0, tw, nq, tid);
-    fwd_rest2<C, 1>(x, y, lds, tw, w0, nq, tid);
+    fwd_pass<C, 0>(x, w0, tw, pf, tid);
+    fwd_rest2<C, 1>(x, y, lds, tw, w0, pf, tid);
 }
 
 template <class C, int P>
 __device__ __forceinline__ void inv_rest2(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                          const double2* __restrict__ itw, const InvTw<C, P + 1>& prev,
+                                          const double* __restrict__ itw, const InvTw<C, P + 1>& prev,
                                           const PrimeF& pf, int tid) {
     if constexpr (P >= 0) {
         constexpr bool W = ntt::wave_local<C, P + 1>();
@@ -531,7 +537,7 @@ __device__ __forceinline__ void inv_rest2(double (&x)[C::E], double (&y)[C::E], 
 // round's twiddle loads are issued).
 template <class C, int P, class H>
 __device__ __forceinline__ void inv_rest2_h(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                            const double2* __restrict__ itw, const InvTw<C, P + 1>& prev,
+                                            const double* __restrict__ itw, const InvTw<C, P + 1>& prev,
                                             const PrimeF& pf, int tid, H& hook) {
     if constexpr (P >= 0) {
         constexpr bool W = ntt::wave_local<C, P + 1>();
@@ -562,7 +568,7 @@ __device__ __forceinline__ void inv_rest2_h(double (&x)[C::E], double (&y)[C::E]
 // be reading y's buffer: a workgroup barrier precedes any further LDS write.
 template <class C>
 __device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                         const double2* __restrict__ itw, const PrimeF& pf, int tid) {
+                                         const double* __restrict__ itw, const PrimeF& pf, int tid) {
     static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
     opaque(itw, tid);
     const InvTw<C, C::NP - 1> w = inv_tw<C, C::NP - 1>(itw, tid);
@@ -574,7 +580,7 @@ __device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], d
 // one (as the single-transform inv_ntt_db above).
 template <class C, class H>
 __device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], double* lds,
-                                         const double2* __restrict__ itw, const PrimeF& pf, int tid, H& hook) {
+                                         const double* __restrict__ itw, const PrimeF& pf, int tid, H& hook) {
     static_assert(ntt::pair_fits<C>(), "two exchange buffers exceed the LDS");
     constexpr int PS = inv_scalar_from<C>();
     opaque(itw, tid);
