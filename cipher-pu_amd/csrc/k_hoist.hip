@@ -61,6 +61,39 @@ __global__ void __launch_bounds__(T) k_modup(const uint64_t* __restrict__ ct, ui
     ntt::store_eval<C>(x, lds, D + (((size_t)b * l + j) * (l + 1) + t) * C::N, tid);
 }
 
+// k_modup on the FP64 pipeline: the same canonical words.  The digit's
+// residues (below 2 q_t on a narrow chain, reduced first otherwise) enter the
+// transform as doubles; the transform's outputs (below 16 q) go through LDS
+// to the coalesced evaluation order and are stored canonical.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_modup_f(const uint64_t* __restrict__ ct, uint64_t* __restrict__ D,
+                                               const double* __restrict__ twf_all, const PrimeF* __restrict__ pfs,
+                                               LimbMap map, int l, int pidx_p, int reduce_digits) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds_u[];
+    double* lds = reinterpret_cast<double*>(lds_u);
+    const int tid = threadIdx.x;
+    const int per_b = l * (l + 1);
+    const int b = wave_uniform(blockIdx.x / per_b), rem = wave_uniform(blockIdx.x % per_b);
+    const int j = rem / (l + 1), t = rem % (l + 1);
+    const int pt = t < l ? map.idx[t] : pidx_p;
+    const PrimeF pf = pfs[pt];
+    double x[C::E];
+    nttf::load_words<C>(x, ct + ((size_t)b * 2 + 1) * l * C::N + (size_t)j * C::N, tid);
+    if (reduce_digits) {
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) x[k] = rmod(x[k], pf.qinv, pf.nq);
+    }
+    nttf::fwd_ntt<C>(x, lds, twf_all + (size_t)pt * 2 * C::N, pf, tid);
+    __syncthreads();
+    ntt::to_lds<C, C::NP - 1>(x, lds, tid);
+    __syncthreads();
+    uint64_t* dst = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e)
+        dst[e * C::T + tid] = d2u(canon(lds[ntt::pad<C>(e * C::T + tid)], pf.qinv, pf.nq));
+}
+
 // Reduce an accumulator whose register k holds evaluation slot k T + tid
 // (the coalesced order the key products are summed in; FP64, below 14 q),
 // apply pi_g through LDS, inverse-transform and store it in coefficient form.
@@ -358,6 +391,70 @@ __global__ void __launch_bounds__(T) k_moddown_eval(const uint64_t* __restrict__
     }
 }
 
+// k_moddown_eval on the FP64 pipeline: the same words.  w enters the
+// transform reduced to (q + 1) / 2; NTT_i(w) (below 16 q) is read back from
+// LDS into registers, and for poly 0 the LDS then stages the row of c0 so
+// that its permuted gather reads the LDS instead of scattered words of
+// memory.  (X - NTT_i(w)) is below 15 q, its product by p^-1 below 5q/8, and
+// the sum with c0 below 2 q: one canonical reduction at the store.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_moddown_eval_f(const uint64_t* __restrict__ acc,
+                                                      const uint64_t* __restrict__ Cn, uint64_t* __restrict__ out,
+                                                      const double* __restrict__ twf_all,
+                                                      const PrimeF* __restrict__ pfs, LimbMap map, int l, int pidx_p,
+                                                      ModDownConst mc, GaloisList galois, int batch) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds_u[];
+    double* lds = reinterpret_cast<double*>(lds_u);
+    const int tid = threadIdx.x;
+    const int i = wave_uniform(blockIdx.x % l);
+    const int row = wave_uniform(blockIdx.x / l);                 // ((r batch + b) 2 + poly)
+    const int poly = row & 1, rb = row >> 1;
+    const int b = rb % batch, r = rb / batch;
+    const int p = map.idx[i];
+    const PrimeF pf = pfs[p];
+    const double qp = pfs[pidx_p].q, hp = __builtin_floor(0.5 * qp);
+    const uint64_t* arow = acc + (size_t)row * (l + 1) * C::N;
+    double x[C::E];
+    nttf::load_words<C>(x, arow + (size_t)l * C::N, tid);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+#pragma clang fp contract(off)
+        double v = x[e] + hp;                                      // (x_p + h) mod p, in [0, p)
+        v = v >= qp ? v - qp : v;
+        x[e] = rmod(v - mc.hpf[i], pf.qinv, pf.nq);               // w mod q_i
+    }
+    nttf::fwd_ntt<C>(x, lds, twf_all + (size_t)p * 2 * C::N, pf, tid);
+    __syncthreads();
+    ntt::to_lds<C, C::NP - 1>(x, lds, tid);        // NTT_i(w) at its evaluation slots
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = lds[ntt::pad<C>(e * C::T + tid)];
+    const uint64_t* X = arow + (size_t)i * C::N;
+    uint64_t* o = out + ((size_t)row * l + i) * C::N;
+    double y[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+#pragma clang fp contract(off)
+        y[e] = mulw(u2d(X[e * C::T + tid]) - x[e], mc.pinvf[i], mc.pinvf_q[i], pf.nq);
+    }
+    if (poly == 0) {
+        const uint64_t* c0 = Cn + ((size_t)b * 2 * l + i) * C::N;
+        const uint32_t g = galois.g[r];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) lds[ntt::pad<C>(e * C::T + tid)] = u2d(c0[e * C::T + tid]);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) {
+#pragma clang fp contract(off)
+            y[e] = y[e] + lds[ntt::pad<C>(eval_perm(e * C::T + tid, g, C::LOGN))];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) o[e * C::T + tid] = d2u(canon(y[e], pf.qinv, pf.nq));
+}
+
 // out[row][j(i)] = in[row][i]: the key rows permuted by pi_g^-1.
 template <typename W>
 __global__ void k_key_permute(const W* __restrict__ in, W* __restrict__ out, uint32_t g, int logN, size_t total) {
@@ -377,6 +474,13 @@ template <int LOGN, int T>
 hipError_t run_modup(const DevTables& t, const uint64_t* ct, uint64_t* D, int batch, int l, const LimbMap& map,
                      int pidx_p, hipStream_t s) {
     using C = Cfg<LOGN, T>;
+    if (t.fp64) {
+        hipError_t e = set_lds(k_modup_f<LOGN, T>, C::LDS_BYTES);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_modup_f<LOGN, T>), dim3(batch * l * (l + 1)), dim3(T), C::LDS_BYTES, s, ct, D, t.twf,
+                           t.pf, map, l, pidx_p, t.narrow ? 0 : 1);
+        return hipGetLastError();
+    }
     hipError_t e = set_lds(k_modup<LOGN, T>, C::LDS_BYTES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_modup<LOGN, T>), dim3(batch * l * (l + 1)), dim3(T), C::LDS_BYTES, s, ct, D, t.tw, t.pc,
@@ -411,6 +515,13 @@ hipError_t run_moddown_eval(const DevTables& t, const uint64_t* acc, const uint6
                             int l, const LimbMap& map, int pidx_p, const ModDownConst& mc, const GaloisList& galois,
                             int R, hipStream_t s) {
     using C = Cfg<LOGN, T>;
+    if (t.fp64) {
+        hipError_t e = set_lds(k_moddown_eval_f<LOGN, T>, C::LDS_BYTES);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_moddown_eval_f<LOGN, T>), dim3(R * batch * 2 * l), dim3(T), C::LDS_BYTES, s, acc, Cn,
+                           out, t.twf, t.pf, map, l, pidx_p, mc, galois, batch);
+        return hipGetLastError();
+    }
     hipError_t e = set_lds(k_moddown_eval<LOGN, T>, C::LDS_BYTES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_moddown_eval<LOGN, T>), dim3(R * batch * 2 * l), dim3(T), C::LDS_BYTES, s, acc, Cn, out,
