@@ -117,7 +117,8 @@ struct ckks_ctx {
     mutable Timer timer;
     int narrow = 0;        // every prime below twice every other one
     void* scratch = nullptr;   // this device's copy of the no-alloc scratch array
-    DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN, narrow, dpf, dtwf, fp64}; }
+    int n_cu = 256;        // compute units of the device
+    DevTables tables() const { return DevTables{dpc, dtw, (int)N, logN, narrow, dpf, dtwf, fp64, n_cu}; }
 };
 
 namespace {
@@ -397,6 +398,9 @@ int ckks_ctx_create(ckks_ctx** out, uint64_t N, int n_primes, const uint64_t* pr
     // set-up of the no-alloc entry point, with the device current; a failed
     // lookup leaves scratch NULL and only ckks_hmult_noalloc reports it
     c->scratch = scratch_on_device(device);
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n_cu > 0)
+        c->n_cu = n_cu;
     *out = c;
     return CKKS_OK;
 }

@@ -88,9 +88,15 @@ __device__ __forceinline__ double tensor_one(double x0, double x1, double y0, do
 // The whole front of HMult for one (pair, limb i), coefficient form in:
 // forward transforms of a0, a1, b0, b1, the tensor product, p*d0 and p*d1
 // and d2 written in evaluation form, d2 transformed back to coefficient form.
-// Limb-major grid: workgroups in flight share one prime's twiddles.  FP64
-// pipeline: the forward transforms leave values below (1 + 7 log2(N)/8) q,
-// reduced before the data products.
+// FP64 pipeline: the forward transforms leave values below
+// (1 + 7 log2(N)/8) q, reduced before the data products.
+// Items it = limb * batch + pair, limb-major (items in flight together share
+// one prime's twiddles).  With paired transforms each workgroup loops over
+// items with a stride of the grid, which run_front sizes to the workgroups
+// resident at once: the next item's a0 is then loaded during the current
+// item's last transform instead of stalling its start.  (All workgroups
+// start their items together, so loads issued at an item's start arrive as
+// one burst over the whole device.)
 template <int LOGN, int T>
 __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
                                              uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
@@ -101,38 +107,67 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
     extern __shared__ uint64_t lds_u[];
     double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
+    const size_t LN = (size_t)l * C::N;
+    if constexpr (ntt::pair_pays<C>()) {
+        // a0 with a1 and b0 with b1 as transform pairs
+        const int items = l * batch;
+        double x0[C::E], x1[C::E], y0[C::E], y1[C::E];
+        uint64_t na0[C::E], nb0[C::E];
+        int it = (int)blockIdx.x;
+        if (it < items) {
+            const int i = wave_uniform(it / batch), b = wave_uniform(it % batch);
+            nttf::issue_row<C>(na0, A + (size_t)b * 2 * LN + (size_t)i * C::N, tid);
+        }
+        for (; it < items; it += (int)gridDim.x) {
+            const int i = wave_uniform(it / batch), b = wave_uniform(it % batch);
+            const int p = map.idx[i];
+            const PrimeF pf = pfs[p];
+            const double* tw = twf_all + (size_t)p * 2 * C::N;
+            const size_t c0 = (size_t)b * 2 * LN + (size_t)i * C::N;   // poly 0 of pair b, limb i
+            const double pm = mc.pmf[i], pmq = mc.pmf_q[i];
+            nttf::load_words<C>(x1, A + c0 + LN, tid);
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) x0[k] = u2d(na0[k]);
+            // b0 is loaded ahead, in flight while the first passes (scalar
+            // twiddle loads only) transform a0 and a1
+            nttf::issue_row<C>(nb0, B + c0, tid);
+            nttf::fwd_ntt2<C>(x0, x1, lds, tw, pf, tid);
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) y0[k] = u2d(nb0[k]);
+            nttf::load_words<C>(y1, B + c0 + LN, tid);
+            nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf, tid);
+            // the inverse transform's first twiddles, loaded before the stores
+            const auto pre = nttf::inv_tw<C, C::NP - 1>(tw + C::N, tid);
+            uint64_t* o0 = d01 + c0;
+            uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
+#pragma unroll
+            for (int k = 0; k < C::E; ++k)
+                x0[k] = tensor_one<C>(x0[k], x1[k], y0[k], y1[k], o0, o2, LN, ntt::eval_index<C>(k, tid), pf, pm,
+                                      pmq);
+            // the next item's a0, issued where the inverse transform has no
+            // vector twiddle load left to wait behind it
+            const int next = it + (int)gridDim.x;
+            auto prefetch = [&]() {
+                if (next < items) {
+                    const int ni = wave_uniform(next / batch), nb = wave_uniform(next % batch);
+                    nttf::issue_row<C>(na0, A + (size_t)nb * 2 * LN + (size_t)ni * C::N, tid);
+                }
+            };
+            nttf::inv_ntt_db<C>(x0, lds, tw + C::N, pf, tid, pre, prefetch);
+            nttf::store_words<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, pf, tid);
+            // other waves may still read the LDS: the next item's transform
+            // pair starts with a workgroup barrier before its first write
+        }
+        return;
+    }
     const int i = wave_uniform(blockIdx.x / batch), b = wave_uniform(blockIdx.x % batch);
     const int p = map.idx[i];
     const PrimeF pf = pfs[p];
     const double* tw = twf_all + (size_t)p * 2 * C::N;
-    const size_t LN = (size_t)l * C::N;
     const size_t c0 = (size_t)b * 2 * LN + (size_t)i * C::N;   // poly 0 of pair b, limb i
     const double pm = mc.pmf[i], pmq = mc.pmf_q[i];
     uint64_t* o0 = d01 + c0;
     uint64_t* o2 = d2e + (size_t)b * LN + (size_t)i * C::N;
-    if constexpr (ntt::pair_pays<C>()) {
-        // a0 with a1 and b0 with b1 as transform pairs
-        double x0[C::E], x1[C::E], y0[C::E], y1[C::E];
-        uint64_t nb0[C::E];
-        nttf::load_words<C>(x0, A + c0, tid);
-        nttf::load_words<C>(x1, A + c0 + LN, tid);
-        // b0 is loaded ahead, in flight while the first passes (scalar
-        // twiddle loads only) transform a0 and a1
-        nttf::issue_row<C>(nb0, B + c0, tid);
-        nttf::fwd_ntt2<C>(x0, x1, lds, tw, pf, tid);
-#pragma unroll
-        for (int k = 0; k < C::E; ++k) y0[k] = u2d(nb0[k]);
-        nttf::load_words<C>(y1, B + c0 + LN, tid);
-        nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf, tid);
-        // the inverse transform's first twiddles, loaded before the stores
-        const auto pre = nttf::inv_tw<C, C::NP - 1>(tw + C::N, tid);
-#pragma unroll
-        for (int k = 0; k < C::E; ++k)
-            x0[k] = tensor_one<C>(x0[k], x1[k], y0[k], y1[k], o0, o2, LN, ntt::eval_index<C>(k, tid), pf, pm, pmq);
-        nttf::inv_ntt_db<C>(x0, lds, tw + C::N, pf, tid, pre);
-        nttf::store_words<C>(x0, d2c + (size_t)b * LN + (size_t)i * C::N, pf, tid);
-        return;
-    }
     double x0[C::E], x1[C::E], y[C::E], z[C::E];
     const double qi = pf.qinv, nq = pf.nq;
     nttf::load_words<C>(x0, A + c0, tid);
@@ -853,8 +888,20 @@ hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, u
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
     hipError_t e = set_lds(k_front<LOGN, T>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(batch * l), dim3(T), lds, s, a, b, d01, d2e, d2c, t.twf,
-                       t.pf, map, l, batch, mc);
+    // paired shapes loop over items: one workgroup per resident slot
+    int grid = batch * l;
+    if (ntt::pair_pays<C>()) {
+        static int per_cu = 0;   // workgroups a CU holds at once: a property of the compiled kernel
+        if (!per_cu) {
+            int n = 0;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_front<LOGN, T>, T, lds);
+            if (e != hipSuccess) return e;
+            per_cu = n > 0 ? n : 1;
+        }
+        grid = grid < per_cu * t.n_cu ? grid : per_cu * t.n_cu;
+    }
+    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(grid), dim3(T), lds, s, a, b, d01, d2e, d2c, t.twf, t.pf, map, l,
+                       batch, mc);
     return hipGetLastError();
 }
 

@@ -29,6 +29,9 @@ struct DevTables {
     // Arithmetic of the transform kernels that have both forms: 0 = 64-bit
     // integer words, 1 = FP64 (results identical).
     int fp64 = 0;
+    // Compute units of the device: the grid of kernels that loop over their
+    // work items is sized from it.
+    int n_cu = 256;
 };
 
 // Cumulative table of |e| for the rounded Gaussian: cdf[k] = floor(2^64
