@@ -296,134 +296,160 @@ __device__ __forceinline__ void fused_out_rows(const double (&x)[C::E], const ui
 template <int LOGN, int T, bool KEYTAB = false, bool FUSED = false>
 __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2c, size_t d2c_bstride,
                                                  const uint64_t* __restrict__ d2e,
-                                                 const uint64_t* __restrict__ d01, const double* __restrict__ key,
+                                                 const uint64_t* __restrict__ d01, const double* __restrict__ key0,
                                                  int key_limbs, uint64_t* __restrict__ acc,
                                                  const double* __restrict__ twf_all, const PrimeF* __restrict__ pfs,
                                                  LimbMap map, int l, int pidx_p, int batch, int reduce_digits,
                                                  const uint64_t* __restrict__ key_table, int t0,
-                                                 uint64_t* __restrict__ out_ct, ModDownConst mc, int pair_major) {
+                                                 uint64_t* __restrict__ out_ct, ModDownConst mc, int pair_major,
+                                                 int items) {
     using C = Cfg<LOGN, T>;
     extern __shared__ uint64_t lds_u[];
     double* lds = reinterpret_cast<double*>(lds_u);
     const int tid = threadIdx.x;
-    // Target-major order: workgroups in flight together share one target
-    // limb, so each XCD's L2 holds one limb's key slice (2 dnum N 16 B) and
-    // one prime's twiddles rather than the whole key.  Ciphertext-major
-    // (pair_major): workgroup i runs on XCD i mod 8, which takes logical
-    // index (i mod 8) G/8 + i/8, so each XCD walks one contiguous range of
-    // ciphertexts, all targets of one ciphertext back to back.
-    const int G = (int)gridDim.x, nt = G / batch;
-    int t, b;
-    if (pair_major) {
-        const int i = (int)blockIdx.x;
-        const int L = (G & 7) == 0 ? (i & 7) * (G >> 3) + (i >> 3) : i;
-        t = wave_uniform(t0 + L % nt);
-        b = wave_uniform(L / nt);
-    } else {
-        t = wave_uniform(t0 + (int)(blockIdx.x / batch));
-        b = wave_uniform(blockIdx.x % batch);
-    }
-    if constexpr (KEYTAB) key = reinterpret_cast<const double*>(key_table[b]);
-    const int pt = t < l ? map.idx[t] : pidx_p;
-    const int kt = t < l ? t : key_limbs - 1;
-    const PrimeF pf = pfs[pt];
-    const double nq = pf.nq;
-    const double* tw = twf_all + (size_t)pt * 2 * C::N;
+    const int nt = items / batch;
+    const bool own = d2e != nullptr;   // digit t comes in evaluation form
     const size_t LN = (size_t)l * C::N;
-    double a0[C::E], a1[C::E], x[C::E];
-    if (t < l && d01) {
-        const uint64_t* s0 = d01 + (size_t)b * 2 * LN + (size_t)t * C::N;
-#pragma unroll
-        for (int k = 0; k < C::E; ++k) {
-            const int j = ntt::eval_index<C>(k, tid);
-            a0[k] = u2d(s0[j]);
-            a1[k] = u2d(s0[LN + j]);   // acc1 = p d1 - sum d a
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
-    }
     // The coefficient-form digits are loaded one ahead: the next digit's
     // loads are issued as the current digit's transform starts, whose first
     // passes take their twiddles through scalar loads, so the loads are in
     // flight for those passes before any vector load must wait for them.
-    const bool own = d2e != nullptr;   // digit t comes in evaluation form
-    const uint64_t* dig = d2c + (size_t)b * d2c_bstride;
-    uint64_t nx[C::E];
-    int jn = (own && t == 0) ? 1 : 0;
-    // (16 coefficients per thread: the extra row would spill)
+    // (16 coefficients per thread: the extra row would spill.)
     constexpr bool PF = C::E <= 8;
-    if (PF && jn < l) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);
-    for (int j = 0; j < l; ++j) {
-        if (j == t && own) {
-            // the digit of the target's own prime is already in evaluation form
-            const uint64_t* s = d2e + (size_t)b * LN + (size_t)j * C::N;
-#pragma unroll
-            for (int k = 0; k < C::E; ++k) x[k] = u2d(s[ntt::eval_index<C>(k, tid)]);
+    // Items v = 0 .. items - 1, one (target, ciphertext) each; with paired
+    // transforms a workgroup loops over them with a stride of the grid
+    // (run_ks sizes it to the workgroups resident at once), and the next
+    // item's first digit is loaded during the current item's scalar-twiddle
+    // inverse rounds.  Target-major order: items in flight together share
+    // one target limb, so each XCD's L2 holds one limb's key slice and one
+    // prime's twiddles.  Ciphertext-major (pair_major): item v runs on XCD
+    // v mod 8 (the grid is a multiple of 8), which takes logical index
+    // (v mod 8) items/8 + v/8, so each XCD walks one contiguous range of
+    // ciphertexts, all targets of one ciphertext back to back.
+    auto slot = [&](int v, int& t, int& b) {
+        if (pair_major) {
+            const int L = (items & 7) == 0 && (gridDim.x & 7) == 0 ? (v & 7) * (items >> 3) + (v >> 3) : v;
+            t = wave_uniform(t0 + L % nt);
+            b = wave_uniform(L / nt);
         } else {
-            if (!PF) nttf::issue_row<C>(nx, dig + (size_t)j * C::N, tid);
-#pragma unroll
-            for (int k = 0; k < C::E; ++k) x[k] = u2d(nx[k]);
-            // the digit's residues mod q_j enter the transform mod q_t as they
-            // are when q_j < 2 q_t (reduce_digits = 0); else reduce them first
-            if (reduce_digits) {
-#pragma unroll
-                for (int k = 0; k < C::E; ++k) x[k] = rmod(x[k], pf.qinv, nq);
+            t = wave_uniform(t0 + v / batch);
+            b = wave_uniform(v % batch);
+        }
+    };
+    uint64_t nx[C::E];
+    bool have = false;   // nx holds this item's first digit
+    for (int v = (int)blockIdx.x; v < items; v += (int)gridDim.x) {
+        int t, b;
+        slot(v, t, b);
+        const double* key = key0;
+        if constexpr (KEYTAB) key = reinterpret_cast<const double*>(key_table[b]);
+        const int pt = t < l ? map.idx[t] : pidx_p;
+        const int kt = t < l ? t : key_limbs - 1;
+        const PrimeF pf = pfs[pt];
+        const double nq = pf.nq;
+        const double* tw = twf_all + (size_t)pt * 2 * C::N;
+        double a0[C::E], a1[C::E], x[C::E];
+        const uint64_t* dig = d2c + (size_t)b * d2c_bstride;
+        int jn = (own && t == 0) ? 1 : 0;
+        if (PF && jn < l && !have) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);   // else loaded ahead
+        if (t < l && d01) {
+            const uint64_t* s0 = d01 + (size_t)b * 2 * LN + (size_t)t * C::N;
+    #pragma unroll
+            for (int k = 0; k < C::E; ++k) {
+                const int j = ntt::eval_index<C>(k, tid);
+                a0[k] = u2d(s0[j]);
+                a1[k] = u2d(s0[LN + j]);   // acc1 = p d1 - sum d a
             }
-            jn = j + 1 + ((own && j + 1 == t) ? 1 : 0);
-            if (PF && jn < l) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);
-            // left unreduced (below 15 q): the key product is exact for any such input
-            if constexpr (ntt::pair_pays<C>()) nttf::fwd_ntt_db<C>(x, lds, tw, pf, tid);
-            else nttf::fwd_ntt<C>(x, lds, tw, pf, tid);
-        }
-        key_mac<C>(a0, a1, x, key, j, key_limbs, kt, pf.qinv, nq, tid);
-    }
-    // the double-buffered digit transforms leave reads of the LDS in flight
-    if constexpr (ntt::pair_pays<C>()) __syncthreads();
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-        a0[k] = rmod(a0[k], pf.qinv, nq);
-        a1[k] = rmod(a1[k], pf.qinv, nq);
-    }
-    uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
-    const uint64_t* acc_b = acc + (size_t)b * 2 * (l + 1) * C::N;
-    uint64_t* out_b = FUSED ? out_ct + ((size_t)b * 2 * (l - 1) + t) * C::N : nullptr;
-    const double* itw = tw + C::N;
-    if constexpr (ntt::pair_pays<C>()) {
-        // both accumulators as one transform pair
-        if constexpr (FUSED) {
-            // The epilogue's rows (limb l - 1 and p) of poly 0 are loaded
-            // during the pair's last rounds, those of poly 1 before poly 0
-            // is finished: their latency overlaps work instead of following
-            // the transform.
-            uint64_t rl[C::E], rp[C::E];
-            auto prefetch = [&]() {
-                nttf::issue_row<C>(rl, acc_b + (size_t)(l - 1) * C::N, tid);
-                nttf::issue_row<C>(rp, acc_b + (size_t)l * C::N, tid);
-            };
-            nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid, prefetch);
-            const uint64_t* acc_b1 = acc_b + (size_t)(l + 1) * C::N;
-            uint64_t rl1[C::E], rp1[C::E];
-            nttf::issue_row<C>(rl1, acc_b1 + (size_t)(l - 1) * C::N, tid);
-            nttf::issue_row<C>(rp1, acc_b1 + (size_t)l * C::N, tid);
-            fused_out_rows<C>(a0, rl, rp, out_b, t, l, pfs, map, pidx_p, mc, tid);
-            fused_out_rows<C>(a1, rl1, rp1, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p, mc, tid);
         } else {
-            nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid);
-            nttf::store_words<C>(a0, out, pf, tid);
-            nttf::store_words<C>(a1, out + (size_t)(l + 1) * C::N, pf, tid);
+    #pragma unroll
+            for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
         }
-        return;
+        for (int j = 0; j < l; ++j) {
+            if (j == t && own) {
+                // the digit of the target's own prime is already in evaluation form
+                const uint64_t* s = d2e + (size_t)b * LN + (size_t)j * C::N;
+    #pragma unroll
+                for (int k = 0; k < C::E; ++k) x[k] = u2d(s[ntt::eval_index<C>(k, tid)]);
+            } else {
+                if (!PF) nttf::issue_row<C>(nx, dig + (size_t)j * C::N, tid);
+    #pragma unroll
+                for (int k = 0; k < C::E; ++k) x[k] = u2d(nx[k]);
+                // the digit's residues mod q_j enter the transform mod q_t as they
+                // are when q_j < 2 q_t (reduce_digits = 0); else reduce them first
+                if (reduce_digits) {
+    #pragma unroll
+                    for (int k = 0; k < C::E; ++k) x[k] = rmod(x[k], pf.qinv, nq);
+                }
+                jn = j + 1 + ((own && j + 1 == t) ? 1 : 0);
+                if (PF && jn < l) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);
+                // left unreduced (below 15 q): the key product is exact for any such input
+                if constexpr (ntt::pair_pays<C>()) nttf::fwd_ntt_db<C>(x, lds, tw, pf, tid);
+                else nttf::fwd_ntt<C>(x, lds, tw, pf, tid);
+            }
+            key_mac<C>(a0, a1, x, key, j, key_limbs, kt, pf.qinv, nq, tid);
+        }
+        // the double-buffered digit transforms leave reads of the LDS in flight
+        if constexpr (ntt::pair_pays<C>()) __syncthreads();
+    #pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            a0[k] = rmod(a0[k], pf.qinv, nq);
+            a1[k] = rmod(a1[k], pf.qinv, nq);
+        }
+        uint64_t* out = acc + ((size_t)b * 2 * (l + 1) + t) * C::N;
+        const uint64_t* acc_b = acc + (size_t)b * 2 * (l + 1) * C::N;
+        uint64_t* out_b = FUSED ? out_ct + ((size_t)b * 2 * (l - 1) + t) * C::N : nullptr;
+        const double* itw = tw + C::N;
+        const int vn = v + (int)gridDim.x;
+        have = PF && vn < items;
+        auto next_digit = [&]() {
+            if (have) {
+                int tn, bn;
+                slot(vn, tn, bn);
+                const int j0 = (own && tn == 0) ? 1 : 0;
+                nttf::issue_row<C>(nx, d2c + (size_t)bn * d2c_bstride + (size_t)j0 * C::N, tid);
+            }
+        };
+        if constexpr (ntt::pair_pays<C>()) {
+            // both accumulators as one transform pair
+            if constexpr (FUSED) {
+                // The epilogue's rows (limb l - 1 and p) of poly 0 are loaded
+                // during the pair's last rounds, with the next item's first
+                // digit; those of poly 1 before poly 0 is finished: their latency
+                // overlaps work instead of following the transform.
+                uint64_t rl[C::E], rp[C::E];
+                auto prefetch = [&]() {
+                    nttf::issue_row<C>(rl, acc_b + (size_t)(l - 1) * C::N, tid);
+                    nttf::issue_row<C>(rp, acc_b + (size_t)l * C::N, tid);
+                    next_digit();
+                };
+                nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid, prefetch);
+                const uint64_t* acc_b1 = acc_b + (size_t)(l + 1) * C::N;
+                uint64_t rl1[C::E], rp1[C::E];
+                nttf::issue_row<C>(rl1, acc_b1 + (size_t)(l - 1) * C::N, tid);
+                nttf::issue_row<C>(rp1, acc_b1 + (size_t)l * C::N, tid);
+                fused_out_rows<C>(a0, rl, rp, out_b, t, l, pfs, map, pidx_p, mc, tid);
+                fused_out_rows<C>(a1, rl1, rp1, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p, mc, tid);
+            } else {
+                nttf::inv_ntt2<C>(a0, a1, lds, itw, pf, tid, next_digit);
+                nttf::store_words<C>(a0, out, pf, tid);
+                nttf::store_words<C>(a1, out + (size_t)(l + 1) * C::N, pf, tid);
+            }
+            // the next item's first transform starts with a workgroup barrier
+            // before its first LDS write
+            continue;
+        }
+        have = false;
+        nttf::inv_ntt<C>(a0, lds, itw, pf, tid);
+        if constexpr (FUSED) fused_out<C>(a0, acc_b, out_b, t, l, pfs, map, pidx_p, mc, tid);
+        else nttf::store_words<C>(a0, out, pf, tid);
+        __syncthreads();
+        nttf::inv_ntt<C>(a1, lds, itw, pf, tid);
+        if constexpr (FUSED)
+            fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p, mc,
+                         tid);
+        else nttf::store_words<C>(a1, out + (size_t)(l + 1) * C::N, pf, tid);
+        __syncthreads();
     }
-    nttf::inv_ntt<C>(a0, lds, itw, pf, tid);
-    if constexpr (FUSED) fused_out<C>(a0, acc_b, out_b, t, l, pfs, map, pidx_p, mc, tid);
-    else nttf::store_words<C>(a0, out, pf, tid);
-    __syncthreads();
-    nttf::inv_ntt<C>(a1, lds, itw, pf, tid);
-    if constexpr (FUSED)
-        fused_out<C>(a1, acc_b + (size_t)(l + 1) * C::N, out_b + (size_t)(l - 1) * C::N, t, l, pfs, map, pidx_p, mc,
-                     tid);
-    else nttf::store_words<C>(a1, out + (size_t)(l + 1) * C::N, pf, tid);
 }
 
 // ModDown by p (round-half-up of X/p), optional addition of coefficient-form
@@ -919,9 +945,23 @@ hipError_t run_ks(const DevTables& t, const uint64_t* d2c, size_t d2s, const uin
     ModDownConst mc;
     if (tg.mc) mc = *tg.mc;
     else memset(&mc, 0, sizeof(mc));
-    hipLaunchKernelGGL(k, dim3(batch * nt), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.twf,
+    // paired shapes loop over items: one workgroup per resident slot
+    const int items = batch * nt;
+    int grid = items;
+    if (ntt::pair_pays<C>()) {
+        static int per_cu[3];   // per kernel variant: workgroups a CU holds at once
+        const int kv = tg.out ? 2 : (key_table ? 1 : 0);
+        if (!per_cu[kv]) {
+            int n = 0;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, T, lds);
+            if (e != hipSuccess) return e;
+            per_cu[kv] = n > 0 ? n : 1;
+        }
+        grid = grid < per_cu[kv] * t.n_cu ? grid : per_cu[kv] * t.n_cu;
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), lds, s, d2c, d2s, d2e, d01, key, key_limbs, acc, t.twf,
                        t.pf, map, l, pidx_p, batch, t.narrow ? 0 : 1, key_table, tg.first, tg.out, mc,
-                       tg.pair_major);
+                       tg.pair_major, items);
     return hipGetLastError();
 }
 
