@@ -98,10 +98,10 @@ def kernel_model_bytes(kind: str, batch: int, l: int = L, fp64: bool = True):
 # copy, and wave-level VALU instructions (SQ_INSTS_VALU).  Keyed by (batch,
 # key-switch threads); the bench only runs shapes listed here.
 PROFILE_SOURCE = "profiles/hmult_kernels_n8192_l4_b256.md"
-MEASURED_TRAFFIC = {("front", 256, 1024): 545943248, ("keyswitch_fused", 256, 1024): 441354033,
-                    ("keyswitch", 256, 1024): 205835015}
-MEASURED_VALU_INSTR = {("front", 256, 1024): 53919744, ("keyswitch_fused", 256, 1024): 54042624,
-                       ("keyswitch", 256, 1024): 32890880}
+MEASURED_TRAFFIC = {("front", 256, 1024): 543272998, ("keyswitch_fused", 256, 1024): 475038519,
+                    ("keyswitch", 256, 1024): 205836936}
+MEASURED_VALU_INSTR = {("front", 256, 1024): 53895168, ("keyswitch_fused", 256, 1024): 54665216,
+                       ("keyswitch", 256, 1024): 33030144}
 PROFILED_THREADS = (1024,)   # key-switch and front/tensor threads of the profiled shapes
 # The issue peak counts one wave64 VALU instruction per SIMD every 4 cycles:
 # 1024 SIMDs x 2.4 GHz / 4.

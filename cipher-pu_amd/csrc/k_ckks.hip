@@ -354,30 +354,30 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
         if (PF && jn < l && !have) nttf::issue_row<C>(nx, dig + (size_t)jn * C::N, tid);   // else loaded ahead
         if (t < l && d01) {
             const uint64_t* s0 = d01 + (size_t)b * 2 * LN + (size_t)t * C::N;
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < C::E; ++k) {
                 const int j = ntt::eval_index<C>(k, tid);
                 a0[k] = u2d(s0[j]);
                 a1[k] = u2d(s0[LN + j]);   // acc1 = p d1 - sum d a
             }
         } else {
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
         }
         for (int j = 0; j < l; ++j) {
             if (j == t && own) {
                 // the digit of the target's own prime is already in evaluation form
                 const uint64_t* s = d2e + (size_t)b * LN + (size_t)j * C::N;
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < C::E; ++k) x[k] = u2d(s[ntt::eval_index<C>(k, tid)]);
             } else {
                 if (!PF) nttf::issue_row<C>(nx, dig + (size_t)j * C::N, tid);
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < C::E; ++k) x[k] = u2d(nx[k]);
                 // the digit's residues mod q_j enter the transform mod q_t as they
                 // are when q_j < 2 q_t (reduce_digits = 0); else reduce them first
                 if (reduce_digits) {
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < C::E; ++k) x[k] = rmod(x[k], pf.qinv, nq);
                 }
                 jn = j + 1 + ((own && j + 1 == t) ? 1 : 0);
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
         }
         // the double-buffered digit transforms leave reads of the LDS in flight
         if constexpr (ntt::pair_pays<C>()) __syncthreads();
-    #pragma unroll
+#pragma unroll
         for (int k = 0; k < C::E; ++k) {
             a0[k] = rmod(a0[k], pf.qinv, nq);
             a1[k] = rmod(a1[k], pf.qinv, nq);
