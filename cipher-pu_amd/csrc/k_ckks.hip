@@ -817,17 +817,26 @@ __global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restri
 // loaded once per call and each plaintext value once per R rows (rows are
 // the (ciphertext, poly) pairs, 2 B of them).  ng <= NG; rows past the end
 // of the last block are skipped.
+// Workgroup order: a workgroup covers blockDim (256, or N when smaller)
+// coefficients of one limb and one row block; the row blocks of one (limb, coefficient chunk) are adjacent
+// logical indices, and logical indices are dealt to the XCDs in contiguous
+// ranges (workgroup w runs on XCD w mod 8), so the plaintext rows of a chunk
+// (ng nb blockDim pairs) are read from HBM once and then from that XCD's L2 by
+// every row block, instead of once per row block.
 template <int NG, int R>
 __global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict__ E,
                                                      const double2* __restrict__ D, uint64_t* __restrict__ out,
                                                      const PrimeF* __restrict__ pfs, LimbMap map, int nb, int ng,
-                                                     int rows, int l, int logN, size_t total, int scalar) {
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total) return;
+                                                     int rows, int l, int logN, int scalar) {
+    const int G = (int)gridDim.x, w = (int)blockIdx.x;
+    const int L = (G & 7) == 0 ? (w & 7) * (G >> 3) + (w >> 3) : w;
+    const int nrb = (rows + R - 1) / R;
+    const int chunks = (1 << logN) / (int)blockDim.x;
+    const int rb = L % nrb, rest = L / nrb;
     const size_t N = (size_t)1 << logN, LN = (size_t)l * N;
-    const size_t n = gid & (N - 1), rest = gid >> logN;
-    const int i = (int)(rest % l);
-    const int r0 = (int)(rest / l) * R;
+    const size_t n = (size_t)(rest % chunks) * blockDim.x + threadIdx.x;
+    const int i = wave_uniform(rest / chunks);
+    const int r0 = rb * R;
     const PrimeF& pf = pfs[map.idx[i]];
     const double nq = pf.nq;
     const size_t dn = scalar ? 1 : N;
@@ -1167,9 +1176,10 @@ namespace {
 template <int NG, int R>
 hipError_t run_lincomb_blk(const DevTables& t, const uint64_t* E, const double2* D, uint64_t* out, int nb, int ng,
                            int rows, int l, const LimbMap& map, int scalar, hipStream_t s) {
-    const size_t total = ((size_t)((rows + R - 1) / R) * l) << t.logN;
-    hipLaunchKernelGGL((k_lincomb_blk<NG, R>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, E, D, out,
-                       t.pf, map, nb, ng, rows, l, t.logN, total, scalar);
+    const int width = t.logN < 8 ? 1 << t.logN : 256;   // coefficients of one row per workgroup
+    const size_t blocks = (size_t)((rows + R - 1) / R) * l * (((size_t)1 << t.logN) / width);
+    hipLaunchKernelGGL((k_lincomb_blk<NG, R>), dim3((unsigned)blocks), dim3(width), 0, s, E, D, out, t.pf, map, nb,
+                       ng, rows, l, t.logN, scalar);
     return hipGetLastError();
 }
 
