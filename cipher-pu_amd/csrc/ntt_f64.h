@@ -266,8 +266,8 @@ __device__ __forceinline__ void inv_pass2(double (&x)[C::E], double (&y)[C::E], 
 // ---- single transforms, one exchange buffer --------------------------------
 
 template <class C, int P>
-__device__ __forceinline__ void fwd_rest(double (&x)[C::E], double* lds, const double* __restrict__ tw, const PrimeF& pf,
-                                         int tid) {
+__device__ __forceinline__ void fwd_rest(double (&x)[C::E], double* lds, const double* __restrict__ tw,
+                                         const PrimeF& pf, int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = ntt::wave_local<C, P>();
         if constexpr (P == 1) __syncthreads();
@@ -326,8 +326,8 @@ __device__ __forceinline__ void inv_ntt(double (&x)[C::E], double* lds, const do
 // other LDS write.
 
 template <class C, int P>
-__device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, const double* __restrict__ tw, const PrimeF& pf,
-                                            int tid) {
+__device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, const double* __restrict__ tw,
+                                            const PrimeF& pf, int tid) {
     if constexpr (P < C::NP) {
         constexpr bool W = ntt::wave_local<C, P>();
         if constexpr (P == 1) __syncthreads();
@@ -343,8 +343,8 @@ __device__ __forceinline__ void fwd_rest_db(double (&x)[C::E], double* lds, cons
 }
 
 template <class C>
-__device__ __forceinline__ void fwd_ntt_db(double (&x)[C::E], double* lds, const double* __restrict__ tw, const PrimeF& pf,
-                                           int tid) {
+__device__ __forceinline__ void fwd_ntt_db(double (&x)[C::E], double* lds, const double* __restrict__ tw,
+                                           const PrimeF& pf, int tid) {
     static_assert(2 * C::LDS_BYTES <= 160 * 1024, "two exchange buffers exceed the LDS");
     opaque(tw, tid);
     fwd_pass<C, 0>(x, fwd_tw<C, 0>(tw, tid), tw, pf, tid);
