@@ -317,6 +317,37 @@ def test_hmult_uniform_inputs_and_small_n(gpu):
         assert np.array_equal(out[i], OC.hmult(a[i], b[i], key, pr, p, ps, pp))
 
 
+@pytest.mark.parametrize("N,batch,threads", [(4096, 523, 512), (8192, 261, 1024)])
+def test_looping_kernels_over_many_items(gpu, N, batch, threads):
+    """Batches that give the looping front and key-switch kernels several
+    items per workgroup, with item counts that are not multiples of 8 (the
+    XCD deal falls back to the plain order): HMult under both key-switch
+    placements and both fronts, and the rotation's key switch, give the same
+    words, and the first, a middle and the last ciphertext equal the oracle."""
+    from cipherpu.ckks import CKKS, CKKSParams, Ciphertext
+    from cipherpu.ring import to_numpy
+    c = CKKS(CKKSParams(N=N), seed=29)
+    c.ring.set_threads(0, threads, threads)   # paired shapes: the looping kernels
+    sk = c.keygen()
+    rlk = c.relin_key(sk)
+    gk = c.galois_key(sk, 3)
+    rng = np.random.default_rng(batch)
+    mk = lambda: np.stack([rng.integers(0, q, (batch, 2, N), dtype=np.uint64) for q in c.primes[:4]], axis=2)
+    a, b = mk(), mk()
+    ca, cb = Ciphertext(c.ring.tensor(a), 1.0), Ciphertext(c.ring.tensor(b), 1.0)
+    ref = to_numpy(c.hmult(ca, cb, rlk).data)
+    for front, order in ((1, 0), (0, 1), (0, 0)):
+        c.ring.set_front(bool(front))
+        c.ring.set_ks_order(bool(order))
+        assert np.array_equal(to_numpy(c.hmult(ca, cb, rlk).data), ref), (front, order)
+    rot = to_numpy(c.rotate(ca, gk).data)
+    key, gkey = to_numpy(rlk.coeff), to_numpy(gk.coeff)
+    pr, p, ps, pp = _oracle_args(c)
+    for i in (0, batch // 2, batch - 1):
+        assert np.array_equal(ref[i], OC.hmult(a[i], b[i], key, pr, p, ps, pp)), i
+        assert np.array_equal(rot[i], OC.rotate(a[i], gkey, pr, p, ps, pp, gk.galois)), i
+
+
 def test_rotate_bit_exact_and_slots_move(ck):
     from cipherpu.ring import to_numpy
     c, sk = ck
