@@ -15,14 +15,19 @@ coefficient domain: a constant multiplies every coefficient alike).  The
 reference computes its convolution as coefficient-encoded dot products, one
 decryption per output (scripts/exp_e14_cnn_e2e.py:90-127).
 
-After DARE and x^2 (one batched HMult over the 8 channel ciphertexts), the
-pool is (1 + rot_1)(1 + rot_16): the sum of a 2x2 window lands on slot
-32 a + 2 b.  FC output k is sum_c W_(k,c) * S_c with W_(k,c)[32 a + 2 b] =
-F[k][36 c + 6 a + b] / 4, one evaluation-domain ckks_lincomb call for all
-10 outputs; outputs 4p..4p+3 are shifted into the four 256-slot windows of
-one ciphertext (all shifts, by three different steps, in one call) and
-summed over each window by rotate-and-sum (128 ... 1), every rotation
-batched over all the packed ciphertexts.
+After DARE and x^2 (one batched HMult over the 8 channel ciphertexts) come
+the pool and the FC layer.  FC output k sums F[k][36 c + 6 a + b] / 4 times
+the 2x2 window of channel c at rows 2a, 2a + 1 and columns 2b, 2b + 1 of
+the output grid, slots 32 a + 2 b + {0, 1, 16, 17}.  The FC result is summed
+over all slots of a 256-slot window in the end, so the pool needs no
+rotation: W_(k,c) holds the weight on all four slots of each window, and
+output k is sum_c W_(k,c) * A_c, one evaluation-domain ckks_lincomb call for
+all 10 outputs.  (Pooling by (1 + rot_1)(1 + rot_16) first, with the weight
+on slot 32 a + 2 b alone, gives the same slot sums after two more rotations
+of all 8 channel ciphertexts.)  Outputs 4p..4p+3 are shifted into the four
+256-slot windows of one ciphertext (all shifts, by three different steps, in
+one call) and summed over each window by rotate-and-sum (128 ... 1), every
+rotation batched over all the packed ciphertexts.
 Output k ends on slot 256 (k mod 4) of packed ciphertext k // 4.
 """
 from __future__ import annotations
@@ -86,6 +91,21 @@ def tap_step(u: int, v: int) -> int:
     return PHASE * (2 * (u % 2) + (v % 2)) + ROW * (u // 2) + (v // 2)
 
 
+def fc_weight_blocks(F: np.ndarray) -> np.ndarray:
+    """[10 * 8, 1024] per-sample slot blocks W_(k,c), row 8 k + c: F[k][36 c +
+    6 a + b] / 4 on slots 32 a + 2 b + {0, 1, 16, 17}, the 2x2 pooling window
+    (a, b) of the 12 x 12 output grid (the pool and the FC weight in one)."""
+    W = np.zeros((N_OUT * N_CH, SLOTS_PER_SAMPLE))
+    for k in range(N_OUT):
+        for c in range(N_CH):
+            for a in range(6):
+                for b in range(6):
+                    w = F[k, 36 * c + 6 * a + b] / 4.0
+                    for d in (0, 1, ROW, ROW + 1):
+                        W[N_CH * k + c, 2 * ROW * a + 2 * b + d] = w
+    return W
+
+
 def pack_images(X: np.ndarray, S: int) -> np.ndarray:
     """[n, 784] images -> [ceil(n / S), S * 1024] polyphase slot vectors."""
     n = X.shape[0]
@@ -139,7 +159,6 @@ class EncryptedCNN(EncryptedNet):
 
     def rotation_steps(self) -> List[int]:
         steps = {tap_step(u, v) for u, v in self.taps} - {0}
-        steps |= {1, ROW}
         steps |= {-WINDOW * m for m in range(1, 4)}
         steps |= {WINDOW >> k for k in range(1, 9)}
         return sorted(steps)
@@ -163,16 +182,10 @@ class EncryptedCNN(EncryptedNet):
         return self._conv[key]
 
     def _fc_weights(self, level: int, scale: float) -> torch.Tensor:
+        """[10][8] evaluation-form prepared plaintexts of fc_weight_blocks."""
         key = (self.gears[1], level, scale)
         if key not in self._fc:
-            rows = []
-            for k in range(N_OUT):
-                for c in range(N_CH):
-                    blk = np.zeros(SLOTS_PER_SAMPLE)
-                    for a in range(6):
-                        for b in range(6):
-                            blk[2 * ROW * a + 2 * b] = self.PF[k, 36 * c + 6 * a + b] / 4.0
-                    rows.append(self._slots(blk))
+            rows = [self._slots(blk) for blk in fc_weight_blocks(self.PF)]
             prep = self.prepared_pt(np.asarray(rows), level, scale)
             self._fc[key] = prep.view(N_OUT, N_CH, level, self.ck.N, 2)
         return self._fc[key]
@@ -200,17 +213,13 @@ class EncryptedCNN(EncryptedNet):
         self.ops["hmult"] += ct.batch
         return self.ck.hmult(ct, ct, self.rlk)
 
-    def pool(self, ct: Ciphertext) -> Ciphertext:
-        ck = self.ck
-        t = ck.add(ct, self.rot(ct, 1))
-        return ck.add(t, self.rot(t, ROW))
-
-    def fc(self, ct: Ciphertext, B: int) -> List[Ciphertext]:
-        """[8 B] pooled channel ciphertexts -> ceil(10 / 4) packed output
-        ciphertexts [B] each."""
+    def fc(self, ct: Ciphertext, B: int, in_place: bool = False) -> List[Ciphertext]:
+        """[8 B] activated channel ciphertexts -> ceil(10 / 4) packed output
+        ciphertexts [B] each (the 2x2 pool is in the FC weights).  in_place:
+        ct's words are transformed in place (the caller drops ct)."""
         ck = self.ck
         l = ct.level
-        E = ct.data.clone()
+        E = ct.data if in_place else ct.data.clone()
         ck.ring.ntt_(E.view(-1, l, ck.N))
         scale = float(ck.primes[l - 1]) * ck.p.scale / ct.scale
         D = self._fc_weights(l, scale)
@@ -241,7 +250,7 @@ class EncryptedCNN(EncryptedNet):
         B = ct.batch
         y = self.conv(ct)
         y = self.dare(y)
-        return self.fc(self.pool(self.activation(y)), B)
+        return self.fc(self.activation(y), B, in_place=True)
 
     def decrypt_outputs(self, cts: List[Ciphertext], n: int) -> np.ndarray:
         ck = self.ck

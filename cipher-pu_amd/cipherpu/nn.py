@@ -459,13 +459,14 @@ class EncryptedMLP(EncryptedNet):
             z = ck.add(z, self.rot(z, s))
         return z
 
-    def _mask(self, ct: Ciphertext, keep: int) -> Ciphertext:
+    def _mask(self, ct: Ciphertext, keep: int, in_place: bool = False) -> Ciphertext:
         """Multiply by the 0/1 slot mask [0, keep) of each block, encoded at
         scale q_(l-1) * delta / scale(ct) so that the rescale brings the
         ciphertext back to the base scale delta (the DARE refresh keeps the
         scale, and the activation and layer 2 need its headroom: at a lower
         scale each later rotation's key-switch noise, about 700 per
-        coefficient, would reach 1e-2 in the decoded slots)."""
+        coefficient, would reach 1e-2 in the decoded slots).  in_place: ct's
+        words are transformed in place (the caller drops ct)."""
         ck = self.ck
         l = ct.level
         m_scale = float(ck.primes[l - 1]) * ck.p.scale / ct.scale
@@ -477,7 +478,7 @@ class EncryptedMLP(EncryptedNet):
             prep = torch.empty(*pt.shape, 2, dtype=torch.uint64, device=ck.device)
             call("ckks_pt_prepare", ck.ring.handle, _ptr(pt), _ptr(prep), 1, l, 0, ck.ring.stream())
             self._masks[key] = prep
-        E = ct.data.clone()
+        E = ct.data if in_place else ct.data.clone()
         ck.ring.ntt_(E.view(-1, l, ck.N))
         out = torch.empty_like(E)
         call("ckks_lincomb", ck.ring.handle, _ptr(E), _ptr(self._masks[key]), _ptr(out), 1, 1, ct.batch, l,
@@ -487,7 +488,7 @@ class EncryptedMLP(EncryptedNet):
 
     def layer1(self, ct: Ciphertext) -> Ciphertext:
         y = self._linear(ct, self.p1)                 # outputs in slots [0, 128) of each block
-        y = self._mask(y, 128)
+        y = self._mask(y, 128, in_place=True)
         return self.ck.add(y, self.rot(y, -128))      # copy to [128, 256) for layer 2's wrap
 
     def activation(self, ct: Ciphertext) -> Ciphertext:

@@ -206,7 +206,7 @@ def test_mixed_step_rotations_keep_every_word(mlp_env):
     h = mlp.activation(mlp.dare(mlp.layer1(ct)))
     K, F = cnn.make_cnn(0)
     net = cnn.EncryptedCNN(ck, sk, K, F)
-    y = net.pool(net.activation(net.dare(net.conv(net.encrypt_inputs(X)))))
+    y = net.activation(net.dare(net.conv(net.encrypt_inputs(X))))
     stages = [(mlp, lambda: [mlp.layer1(ct)]), (mlp, lambda: [mlp.layer2(h)]), (net, lambda: net.fc(y, ct.batch))]
     for model, stage in stages:
         got = []

@@ -147,37 +147,39 @@ def test_diagonal_plan_in_plaintext():
 
 def test_cnn_packing_in_plaintext():
     """The encrypted CNN's slot plan, run on plaintext slot vectors with
-    rotations as cyclic shifts, equals the FP64 network."""
+    rotations as cyclic shifts, equals the FP64 network: the pool folded
+    into the FC weights (EncryptedCNN's fc_weight_blocks, the weight on all
+    four slots of each 2x2 window), and the pool by (1 + rot_1)(1 + rot_16)
+    with the weight on the window's first slot."""
     from cipherpu import cnn, nn
     K, F = cnn.make_cnn(1)
     X = nn.synthetic_images(4, seed=2)
     v = cnn.pack_images(X, 4)[0]
     rot = lambda z, k: np.roll(z, -k)
-    chans = []
+    acts = []
     for c in range(cnn.N_CH):
         y = sum(K[c, u, w] * rot(v, cnn.tap_step(u, w)) for u in range(5) for w in range(5))
-        a = y ** 2
-        t = a + rot(a, 1)
-        chans.append(t + rot(t, cnn.ROW))
-    outs = []
+        acts.append(y ** 2)
+    pooled = [(a + rot(a, 1)) + rot(a + rot(a, 1), cnn.ROW) for a in acts]
+    first = np.zeros((cnn.N_OUT * cnn.N_CH, 1024))
     for k in range(cnn.N_OUT):
-        z = np.zeros_like(v)
         for c in range(cnn.N_CH):
-            blk = np.zeros(1024)
             for i in range(6):
                 for j in range(6):
-                    blk[32 * i + 2 * j] = F[k, 36 * c + 6 * i + j] / 4
-            z += np.tile(blk, 4) * chans[c]
-        outs.append(z)
-    got = np.zeros((4, 10))
-    for p in range(0, 10, 4):
-        z = sum(rot(outs[p + m], -256 * m) for m in range(min(4, 10 - p)))
-        for k in range(1, 9):
-            z = z + rot(z, 256 >> k)
-        for s in range(4):
-            for m in range(min(4, 10 - p)):
-                got[s, p + m] = z[1024 * s + 256 * m]
-    assert np.allclose(got, cnn.cnn_forward(K, F, X), atol=1e-10)
+                    first[cnn.N_CH * k + c, 32 * i + 2 * j] = F[k, 36 * c + 6 * i + j] / 4
+    folded = cnn.fc_weight_blocks(F)
+    assert np.count_nonzero(folded) == 4 * np.count_nonzero(first)
+    for chans, W in ((acts, folded), (pooled, first)):
+        outs = [sum(np.tile(W[cnn.N_CH * k + c], 4) * chans[c] for c in range(cnn.N_CH)) for k in range(cnn.N_OUT)]
+        got = np.zeros((4, 10))
+        for p in range(0, 10, 4):
+            z = sum(rot(outs[p + m], -256 * m) for m in range(min(4, 10 - p)))
+            for k in range(1, 9):
+                z = z + rot(z, 256 >> k)
+            for s in range(4):
+                for m in range(min(4, 10 - p)):
+                    got[s, p + m] = z[1024 * s + 256 * m]
+        assert np.allclose(got, cnn.cnn_forward(K, F, X), atol=1e-10)
 
 
 def test_cnn_forward_against_direct_convolution():
