@@ -417,6 +417,7 @@ def mlp_leg(samples: int, targets, chunk_cts: int, kind: str = "mlp"):
     out = {"metric": f"enc-{kind.upper()} samples/s at fixed output SNR", "network": desc,
            "samples": samples, "target_snr_db": list(targets), "chunk_ciphertexts": chunk_cts,
            "samples_per_ciphertext": mlp.S, "controller_weights": trained,
+           "levels": {"encrypted": ck.L, "stage_entry": list(mlp.entry_levels())},
            "predicted_snr_db_per_gear": None}
 
     def summary(r):
@@ -433,6 +434,11 @@ def mlp_leg(samples: int, targets, chunk_cts: int, kind: str = "mlp"):
     chunks = gear._setup(mlp, X[:chunk_cts * mlp.S], chunk_cts)
     out["predicted_snr_db_per_gear"] = [round(float(v), 2) for v in gear._gear_table(mlp, net, chunks)[0, 0]]
     out["static_gear0"] = summary(gear.static_loop(mlp, X, 0, chunk_cts))
+    # the same with every stage from the top level (inputs and refresh at L)
+    mlp.drop_levels = False
+    gear.static_loop(mlp, Xw, 0, chunk_cts)
+    out["static_gear0_top_levels"] = summary(gear.static_loop(mlp, X, 0, chunk_cts))
+    mlp.drop_levels = True
     by = {}
     for t in targets:
         plain = summary(gear.plain_monitor_loop(mlp, X, t, chunk_cts))

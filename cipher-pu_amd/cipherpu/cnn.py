@@ -125,6 +125,8 @@ def pack_images(X: np.ndarray, S: int) -> np.ndarray:
 
 
 class EncryptedCNN(EncryptedNet):
+    stage_levels = (2, 3)   # conv, then x^2 + FC
+
     def __init__(self, ck: CKKS, sk: SecretKey, K: np.ndarray, F: np.ndarray, gear1: int = 0, gear2: int = 0,
                  w_scale: float = 2.0 ** 40):
         super().__init__(ck, sk, w_scale)
@@ -248,8 +250,9 @@ class EncryptedCNN(EncryptedNet):
     def forward(self, ct: Ciphertext) -> List[Ciphertext]:
         self.check_arith()
         B = ct.batch
-        y = self.conv(ct)
-        y = self.dare(y)
+        l1, l2 = self.entry_levels()
+        y = self.conv(self.enter(ct, l1))
+        y = self.dare(y, level=l2)
         return self.fc(self.activation(y), B, in_place=True)
 
     def decrypt_outputs(self, cts: List[Ciphertext], n: int) -> np.ndarray:

@@ -202,13 +202,37 @@ class LayerPlan:
     used_b: List[int] = field(default_factory=list)
     used_g: List[int] = field(default_factory=list)
     nnz_diagonals: int = 0
+    _lower: Dict[int, torch.Tensor] = field(default_factory=dict)
+
+    def rows_at(self, l: int) -> torch.Tensor:
+        """D for a ciphertext at level l <= D's: its first l limbs (an
+        encoding's residues do not depend on the primes after them)."""
+        if l == self.D.shape[2]:
+            return self.D
+        if l > self.D.shape[2]:
+            raise ValueError(f"plan prepared at level {self.D.shape[2]}, ciphertext at {l}")
+        if l not in self._lower:
+            self._lower[l] = self.D[:, :, :l].contiguous()
+        return self._lower[l]
 
 
 class EncryptedNet:
     """What the encrypted networks share: keys, rotations, the DARE refresh,
     the slot pattern of the S samples a ciphertext carries, and the
     interface the gear loops drive (set_gears, prepare, encrypt_inputs,
-    forward, decrypt_outputs, reference_outputs)."""
+    forward, decrypt_outputs, reference_outputs).
+
+    Levels.  Inputs are encrypted at the top level L and the DARE refresh
+    lifts to L.  With drop_levels, each of forward's two stages (before and
+    after the refresh) starts at the lowest level it can run at, its
+    rescales + 1 (stage_levels): the refresh and the final decryption read
+    limb 0 alone, so a stage needs to end at level 1, no higher.  The input's
+    unused limbs are dropped (modulus reduction, no division) and the
+    refresh encrypts at the second stage's level; each is the top-level
+    ciphertext's residues on fewer primes.  The gear features are taken on
+    the input ciphertext before the drop."""
+
+    stage_levels = (3, 3)   # the MLP: linear + mask, then x^2 + linear
 
     def __init__(self, ck: CKKS, sk: SecretKey, w_scale: float = 2.0 ** 40):
         self.ck, self.sk = ck, sk
@@ -224,6 +248,16 @@ class EncryptedNet:
         self.ops = {"rot": 0, "ptmul_rows": 0, "hmult": 0, "dare": 0}
         self.gears = (0, 0)
         self.fp64 = ck.ring.fp64    # prepared plaintexts and keys are in this arithmetic
+        self.drop_levels = True     # each stage at the lowest level it needs
+
+    def entry_levels(self) -> Tuple[int, int]:
+        """Levels at which forward's two stages start (L, L without drop_levels)."""
+        L = self.ck.L
+        return tuple(min(L, l) for l in self.stage_levels) if self.drop_levels else (L, L)
+
+    def enter(self, ct: Ciphertext, l: int) -> Ciphertext:
+        """ct at level l: the limbs above l dropped (ct itself when at or below)."""
+        return self.ck.drop_levels(ct, l) if ct.level > l else ct
 
     def check_arith(self):
         if self.ck.ring.fp64 != self.fp64:
@@ -326,14 +360,16 @@ class EncryptedNet:
                 self.gk(s)
         self.set_gears(*cur)
 
-    def dare(self, ct: Ciphertext) -> Ciphertext:
-        """Decrypt, keep the scale-carrying integer message, encrypt afresh
-        at the top level (exp_e24_rotation_bootstrap.py:287-304)."""
+    def dare(self, ct: Ciphertext, level: Optional[int] = None) -> Ciphertext:
+        """Decrypt, keep the scale-carrying integer message (centred limb 0),
+        encrypt afresh at the top level, or at `level`: the top-level
+        ciphertext's first `level` limbs (exp_e24_rotation_bootstrap.py:287-304)."""
         ck = self.ck
+        lo = ck.L if level is None else level
         pt = ck.decrypt(ct, self.sk)                                            # [B, l, N]
         B, l = pt.shape[0], pt.shape[1]
-        top = torch.empty(B, ck.L, ck.N, dtype=torch.uint64, device=ck.device)
-        call("ckks_lift_small", ck.ring.handle, _ptr(pt), _ptr(top), B, l, ck.L, ck.ring.stream())
+        top = torch.empty(B, lo, ck.N, dtype=torch.uint64, device=ck.device)
+        call("ckks_lift_small", ck.ring.handle, _ptr(pt), _ptr(top), B, l, lo, ck.ring.stream())
         self.ops["dare"] += B
         return ck.encrypt(top, self.sk, scale=ct.scale)
 
@@ -446,7 +482,7 @@ class EncryptedMLP(EncryptedNet):
             ck.ring.ntt_(E.view(-1, l, ck.N))
         nb, ng = len(p.used_b), len(p.used_g)
         acc = torch.empty(ng, B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)
-        call("ckks_lincomb", ck.ring.handle, _ptr(E), _ptr(p.D), _ptr(acc), nb, ng, B, l, ck.ring.stream())
+        call("ckks_lincomb", ck.ring.handle, _ptr(E), _ptr(p.rows_at(l)), _ptr(acc), nb, ng, B, l, ck.ring.stream())
         self.ops["ptmul_rows"] += nb * ng * B
         ck.ring.intt_(acc.view(-1, l, ck.N))
         # every giant step rescaled and rotated at once, each by its own step
@@ -500,7 +536,8 @@ class EncryptedMLP(EncryptedNet):
 
     def forward(self, ct: Ciphertext) -> Ciphertext:
         self.check_arith()
-        h = self.dare(self.layer1(ct))
+        l1, l2 = self.entry_levels()
+        h = self.dare(self.layer1(self.enter(ct, l1)), level=l2)
         return self.layer2(self.activation(h))
 
     def decrypt_outputs(self, ct: Ciphertext, n: int) -> np.ndarray:

@@ -150,6 +150,35 @@ def test_cnn_matches_plaintext(mlp_env):
     assert np.mean(PF == 0) > 0.45
 
 
+def test_networks_from_their_lowest_levels(mlp_env):
+    """forward with drop_levels (each stage from the lowest level it needs:
+    the input's upper limbs dropped, the DARE refresh at the second stage's
+    level) and without (every stage from the top level) both match the FP64
+    networks, and the outputs end at level 1 with the drop.  A layer plan's
+    rows for a lower level equal the rows prepared at that level."""
+    import torch
+    from cipherpu import cnn, nn
+    ck, sk, W1, W2 = mlp_env
+    X = nn.synthetic_images(8, seed=21)
+    K, F = cnn.make_cnn(0)
+    for net in (nn.EncryptedMLP(ck, sk, W1, W2), cnn.EncryptedCNN(ck, sk, K, F)):
+        ref = net.reference_outputs(X)
+        for drop in (True, False):
+            net.drop_levels = drop
+            ct = net.encrypt_inputs(X)
+            assert ct.level == ck.L
+            y = net.forward(ct)
+            ys = y if isinstance(y, list) else [y]
+            assert all(o.level == (1 if drop else ck.L - 2) for o in ys), (type(net).__name__, drop)
+            out = net.decrypt_outputs(y, 8)
+            assert np.max(np.abs(out - ref)) < 1e-3 * np.max(np.abs(ref)), (type(net).__name__, drop)
+            assert nn.output_snr_db(ref, out) > 60
+    mlp = nn.EncryptedMLP(ck, sk, W1, W2)
+    direct = mlp._plan(nn.extended_diagonals(mlp.P1, 128, 896), 128, 896, 16, 8, (512, 256, 128), 3)
+    assert torch.equal(mlp.p1.rows_at(3), direct.D)
+    assert mlp.p1.rows_at(ck.L) is mlp.p1.D
+
+
 def test_device_samplers(mlp_env):
     """Uniform residues in range with the right mean; rounded-Gaussian errors
     the same integer in every limb, with the table's distribution; both

@@ -548,7 +548,7 @@ def test_rotate_hoisted_bit_exact_and_slots_move(ck):
     rng = np.random.default_rng(14)
     z = rng.uniform(-1, 1, (2, c.N // 2))
     pr, p, ps, pp = _oracle_args(c)
-    for l in (4, 3):
+    for l in (4, 3, 2):
         ct = c.encrypt(c.encode(z, level=l), sk)
         steps = [1, 5, 64, -3]
         gks = [c.galois_key(sk, st) for st in steps]
@@ -781,7 +781,7 @@ def test_rotate_hoisted_eval_form_is_ntt_of_coefficient_form(gpu, N):
     sk = c.keygen()
     gks = [c.galois_key(sk, s) for s in (1, 5, N // 4 - 1)]
     rng = np.random.default_rng(N + 7)
-    for l in (4, 3, 1):
+    for l in (4, 3, 2, 1):
         ct = c.encrypt(c.encode(rng.uniform(-1, 1, (3, N // 2)), level=l), sk)
         coef = c.rotate_hoisted(ct, gks)
         ev = c.rotate_hoisted(ct, gks, eval_form=True)
