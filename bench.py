@@ -400,6 +400,7 @@ def mlp_leg(samples: int, targets, chunk_cts: int, kind: str = "mlp"):
     from cipherpu import gear, nn
     from cipherpu.ckks import CKKS, CKKSParams
     ck = CKKS(CKKSParams(), seed=99)
+    ck.ring.set_front(True)   # HMult's fused front, as the HMult leg's tuner picks at N = 8192
     sk = ck.keygen()
     if kind == "mlp":
         mlp = nn.EncryptedMLP(ck, sk, *nn.make_mlp(0))
@@ -417,7 +418,7 @@ def mlp_leg(samples: int, targets, chunk_cts: int, kind: str = "mlp"):
     out = {"metric": f"enc-{kind.upper()} samples/s at fixed output SNR", "network": desc,
            "samples": samples, "target_snr_db": list(targets), "chunk_ciphertexts": chunk_cts,
            "samples_per_ciphertext": mlp.S, "controller_weights": trained,
-           "levels": {"encrypted": ck.L, "stage_entry": list(mlp.entry_levels())},
+           "levels": {"encrypted": ck.L, "stage_entry": list(mlp.entry_levels())}, "hmult_front": "fused",
            "predicted_snr_db_per_gear": None}
 
     def summary(r):

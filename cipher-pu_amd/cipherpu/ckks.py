@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -73,6 +73,7 @@ class Ciphertext:
 class SecretKey:
     s: np.ndarray               # [N] int64 in {-1, 0, 1}
     rns: torch.Tensor           # [K, N] uint64 over all primes (incl. special)
+    rows: Dict[int, torch.Tensor] = field(default_factory=dict, repr=False)   # CKKS.key_rows, per level
 
 
 @dataclass
@@ -230,7 +231,7 @@ class CKKS:
             E = self.gaussian_device(B, l) if dev else self.small_to_rns(self.gaussian(B), range(l))
         else:
             E = self.small_to_rns(e, range(l))
-        S = sk.rns[:l].unsqueeze(0).expand(B, l, self.N).contiguous()
+        S = self.key_rows(sk, B, l)
         c0 = self.ring.add(self.ring.add(self.ring.poly_mul(a, S), E), pt)
         c1 = self.ring.neg(a)
         return Ciphertext(torch.stack([c0, c1], dim=1).contiguous(),
@@ -239,8 +240,23 @@ class CKKS:
     def decrypt(self, ct: Ciphertext, sk: SecretKey) -> torch.Tensor:
         d = ct.data
         B, l = d.shape[0], d.shape[2]
-        S = sk.rns[:l].unsqueeze(0).expand(B, l, self.N).contiguous()
+        S = self.key_rows(sk, B, l)
         return self.ring.add(d[:, 0].contiguous(), self.ring.poly_mul(d[:, 1].contiguous(), S))
+
+    def key_rows(self, sk: SecretKey, B: int, l: int) -> torch.Tensor:
+        """[B, l, N] contiguous copies of s on primes 0..l-1, the second
+        operand of encryption's and decryption's products.  Kept on the key
+        per level and grown to the largest batch asked for, so that a call
+        takes a prefix instead of a fresh copy.  The rows are marked in use
+        on the calling stream, so a copy replaced by a larger one is not
+        reused while another stream's kernels may still read it."""
+        t = sk.rows.get(l)
+        if t is None or t.shape[0] < B or t.device != self.device:
+            t = sk.rns[:l].to(self.device).unsqueeze(0).expand(B, l, self.N).contiguous()
+            sk.rows[l] = t
+        if t.is_cuda:
+            t.record_stream(torch.cuda.current_stream(t.device))
+        return t[:B]
 
     # -- evaluation -----------------------------------------------------------------
     def workspace(self, batch: int, l: int, nbytes: Optional[int] = None) -> torch.Tensor:

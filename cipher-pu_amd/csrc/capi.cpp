@@ -688,10 +688,11 @@ int ckks_hmult(const ckks_ctx* c, const uint64_t* a, const uint64_t* b, const ui
     if (c->front_fused) {
         TIMED(CKKS_KIND_FRONT, launch_front(t, a, b, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
     } else {
+        // a squaring (b == a) transforms its operand once
         TIMED(CKKS_KIND_NTT, launch_ntt(t, ea, a, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
-        TIMED(CKKS_KIND_NTT, launch_ntt(t, eb, b, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
-        TIMED(CKKS_KIND_TENSOR,
-              launch_tensor_intt(t, ea, eb, d01, d2e, d2c, batch, l, m, mc, c->tensor_threads, S(stream)));
+        if (b != a) TIMED(CKKS_KIND_NTT, launch_ntt(t, eb, b, batch * 2, l, m, false, false, c->ntt_threads, S(stream)));
+        TIMED(CKKS_KIND_TENSOR, launch_tensor_intt(t, ea, b != a ? eb : ea, d01, d2e, d2c, batch, l, m, mc,
+                                                   c->tensor_threads, S(stream)));
     }
     // the key switch of limb l-1 and of the special prime first, then the
     // other chain limbs with ModDown and the rescale fused in

@@ -96,8 +96,11 @@ __device__ __forceinline__ double tensor_one(double x0, double x1, double y0, do
 // resident at once: the next item's a0 is then loaded during the current
 // item's last transform instead of stalling its start.  (All workgroups
 // start their items together, so loads issued at an item's start arrive as
-// one burst over the whole device.)
-template <int LOGN, int T>
+// one burst over the whole device.)  SQ (squaring, B == A; paired shapes):
+// the transforms of b0, b1 are those of a0, a1, so they are not repeated and
+// the tensor product reads the same values twice: the same words as the
+// general kernel with B = A.
+template <int LOGN, int T, bool SQ = false>
 __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
                                              uint64_t* __restrict__ d01, uint64_t* __restrict__ d2e,
                                              uint64_t* __restrict__ d2c, const double* __restrict__ twf_all,
@@ -130,12 +133,20 @@ __global__ void __launch_bounds__(T) k_front(const uint64_t* __restrict__ A, con
             for (int k = 0; k < C::E; ++k) x0[k] = u2d(na0[k]);
             // b0 is loaded ahead, in flight while the first passes (scalar
             // twiddle loads only) transform a0 and a1
-            nttf::issue_row<C>(nb0, B + c0, tid);
+            if constexpr (!SQ) nttf::issue_row<C>(nb0, B + c0, tid);
             nttf::fwd_ntt2<C>(x0, x1, lds, tw, pf, tid);
+            if constexpr (SQ) {
 #pragma unroll
-            for (int k = 0; k < C::E; ++k) y0[k] = u2d(nb0[k]);
-            nttf::load_words<C>(y1, B + c0 + LN, tid);
-            nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf, tid);
+                for (int k = 0; k < C::E; ++k) {
+                    y0[k] = x0[k];
+                    y1[k] = x1[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < C::E; ++k) y0[k] = u2d(nb0[k]);
+                nttf::load_words<C>(y1, B + c0 + LN, tid);
+                nttf::fwd_ntt2<C>(y0, y1, lds, tw, pf, tid);
+            }
             // the inverse transform's first twiddles, loaded before the stores
             const auto pre = nttf::inv_tw<C, C::NP - 1>(tw + C::N, tid);
             uint64_t* o0 = d01 + c0;
@@ -921,22 +932,24 @@ hipError_t run_front(const DevTables& t, const uint64_t* a, const uint64_t* b, u
                      uint64_t* d2c, int batch, int l, const LimbMap& map, const ModDownConst& mc, hipStream_t s) {
     using C = Cfg<LOGN, T>;
     constexpr int lds = ntt::pair_pays<C>() ? 2 * C::LDS_BYTES : C::LDS_BYTES;
-    hipError_t e = set_lds(k_front<LOGN, T>, lds);
+    // the squaring kernel where both operands are one ciphertext batch
+    const bool sq = ntt::pair_pays<C>() && a == b;
+    auto k = sq ? k_front<LOGN, T, true> : k_front<LOGN, T>;
+    hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
     // paired shapes loop over items: one workgroup per resident slot
     int grid = batch * l;
     if (ntt::pair_pays<C>()) {
-        static int per_cu = 0;   // workgroups a CU holds at once: a property of the compiled kernel
-        if (!per_cu) {
+        static int per_cu[2];   // per variant, workgroups a CU holds at once: a property of the compiled kernel
+        if (!per_cu[sq]) {
             int n = 0;
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_front<LOGN, T>, T, lds);
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, T, lds);
             if (e != hipSuccess) return e;
-            per_cu = n > 0 ? n : 1;
+            per_cu[sq] = n > 0 ? n : 1;
         }
-        grid = grid < per_cu * t.n_cu ? grid : per_cu * t.n_cu;
+        grid = grid < per_cu[sq] * t.n_cu ? grid : per_cu[sq] * t.n_cu;
     }
-    hipLaunchKernelGGL((k_front<LOGN, T>), dim3(grid), dim3(T), lds, s, a, b, d01, d2e, d2c, t.twf, t.pf, map, l,
-                       batch, mc);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), lds, s, a, b, d01, d2e, d2c, t.twf, t.pf, map, l, batch, mc);
     return hipGetLastError();
 }
 

@@ -348,6 +348,35 @@ def test_looping_kernels_over_many_items(gpu, N, batch, threads):
         assert np.array_equal(rot[i], OC.rotate(a[i], gkey, pr, p, ps, pp, gk.galois)), i
 
 
+@pytest.mark.parametrize("N,threads,l", [(8192, 1024, 4), (8192, 512, 3), (4096, 512, 2), (4096, 256, 4)])
+def test_squaring_gives_the_words_of_the_product(gpu, N, threads, l):
+    """HMult of a ciphertext batch with itself (one pointer: the squaring
+    front, or one forward transform on the unfused path) gives the words of
+    HMult with a copy of it, on both fronts and in both arithmetics, and
+    the first and last ciphertexts equal the oracle's product."""
+    from cipherpu.ckks import CKKS, CKKSParams, Ciphertext
+    from cipherpu.ring import to_numpy
+    c = CKKS(CKKSParams(N=N), seed=37)
+    c.ring.set_threads(threads, threads, threads)
+    sk = c.keygen()
+    rng = np.random.default_rng(N + l)
+    batch = 37
+    a = np.stack([rng.integers(0, q, (batch, 2, N), dtype=np.uint64) for q in c.primes[:l]], axis=2)
+    ca = Ciphertext(c.ring.tensor(a), 1.0)
+    cb = Ciphertext(ca.data.clone(), 1.0)
+    for fp64 in (True, False):
+        c.ring.set_arith(fp64)
+        rlk = c.relin_key(sk)
+        for front in (True, False):
+            c.ring.set_front(front)
+            want = to_numpy(c.hmult(ca, cb, rlk).data)
+            assert np.array_equal(to_numpy(c.hmult(ca, ca, rlk).data), want), (fp64, front)
+    key = to_numpy(rlk.coeff)
+    pr, p, ps, pp = _oracle_args(c)
+    for i in (0, batch - 1):
+        assert np.array_equal(want[i], OC.hmult(a[i], a[i], key, pr[:l], p, ps[:l], pp)), i
+
+
 def test_rotate_bit_exact_and_slots_move(ck):
     from cipherpu.ring import to_numpy
     c, sk = ck
