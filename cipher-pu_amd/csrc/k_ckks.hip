@@ -827,13 +827,16 @@ __global__ void k_mod_switch(const uint64_t* __restrict__ in, uint64_t* __restri
 // keeps NG x R accumulators, so each input value is
 // loaded once per call and each plaintext value once per R rows (rows are
 // the (ciphertext, poly) pairs, 2 B of them).  ng <= NG; rows past the end
-// of the last block are skipped.
-// Workgroup order: a workgroup covers blockDim (256, or N when smaller)
-// coefficients of one limb and one row block; the row blocks of one (limb, coefficient chunk) are adjacent
-// logical indices, and logical indices are dealt to the XCDs in contiguous
-// ranges (workgroup w runs on XCD w mod 8), so the plaintext rows of a chunk
-// (ng nb blockDim pairs) are read from HBM once and then from that XCD's L2 by
-// every row block, instead of once per row block.
+// of the last block are not stored.
+// Workgroup order: a workgroup covers a chunk of 64 coefficients (N when
+// smaller) of one limb, each of its waves one row block, so the waves read
+// the same plaintext values (the second and later from the CU's L1).  The
+// row-block groups of one (limb, chunk) are adjacent logical indices, and
+// logical indices are dealt to the XCDs in contiguous ranges (workgroup w
+// runs on XCD w mod 8), so a chunk's plaintext rows are read from HBM once
+// and then from that XCD's L2.  Each baby step's input rows are loaded one
+// step ahead, and no load sits behind a branch: the kernel is bound by the
+// latency of its loads, which a wave would otherwise wait out one by one.
 template <int NG, int R>
 __global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict__ E,
                                                      const double2* __restrict__ D, uint64_t* __restrict__ out,
@@ -841,13 +844,18 @@ __global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict_
                                                      int rows, int l, int logN, int scalar) {
     const int G = (int)gridDim.x, w = (int)blockIdx.x;
     const int L = (G & 7) == 0 ? (w & 7) * (G >> 3) + (w >> 3) : w;
+    const int lanes = blockDim.x < 64 ? (int)blockDim.x : 64;   // coefficients of a chunk
+    const int W = (int)blockDim.x / lanes;                      // row blocks (waves) per workgroup
+    const int lane = (int)threadIdx.x % lanes, wv = (int)threadIdx.x / lanes;
     const int nrb = (rows + R - 1) / R;
-    const int chunks = (1 << logN) / (int)blockDim.x;
-    const int rb = L % nrb, rest = L / nrb;
+    const int nrbg = (nrb + W - 1) / W;
+    const int chunks = (1 << logN) / lanes;
+    const int rbg = L % nrbg, rest = L / nrbg;
     const size_t N = (size_t)1 << logN, LN = (size_t)l * N;
-    const size_t n = (size_t)(rest % chunks) * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)(rest % chunks) * lanes + lane;
     const int i = wave_uniform(rest / chunks);
-    const int r0 = rb * R;
+    const int r0 = (rbg * W + wv) * R;
+    if (r0 >= rows) return;   // a wave past the last row block (no barrier follows)
     const PrimeF& pf = pfs[map.idx[i]];
     const double nq = pf.nq;
     const size_t dn = scalar ? 1 : N;
@@ -860,19 +868,30 @@ __global__ void __launch_bounds__(256) k_lincomb_blk(const uint64_t* __restrict_
     for (int g = 0; g < NG; ++g)
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[g][r] = 0.0;
+    // Every load of a baby step is issued before the first product: rows
+    // past the end and giant steps past ng read clamped (valid) addresses
+    // whose products are never stored, so no load sits behind a branch that
+    // would make it wait alone.
+    const int rlast = rows - 1 - r0;
+    uint64_t en[R];   // the next baby step's rows, loaded one step ahead
+#pragma unroll
+    for (int r = 0; r < R; ++r) en[r] = ebase[(size_t)(r < rlast ? r : rlast) * LN];
     for (int b = 0; b < nb; ++b) {
         double e[R];
+        double2 w[NG];
 #pragma unroll
-        for (int r = 0; r < R; ++r) e[r] = r0 + r < rows ? u2d(ebase[(size_t)b * brow + (size_t)r * LN]) : 0.0;
+        for (int r = 0; r < R; ++r) e[r] = u2d(en[r]);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) w[g] = dbase[((size_t)(g < ng ? g : ng - 1) * nb + b) * dstride];
+        const int bn = b + 1 < nb ? b + 1 : b;
+#pragma unroll
+        for (int r = 0; r < R; ++r) en[r] = ebase[(size_t)bn * brow + (size_t)(r < rlast ? r : rlast) * LN];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            if (g < ng) {
-                const double2 w = dbase[((size_t)g * nb + b) * dstride];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < R; ++r) {
 #pragma clang fp contract(off)
-                    acc[g][r] = acc[g][r] + mulw(e[r], w.x, w.y, nq);   // nb <= 32: below 20 q
-                }
+                acc[g][r] = acc[g][r] + mulw(e[r], w[g].x, w[g].y, nq);   // nb <= 32: below 20 q
             }
         }
     }
@@ -1189,8 +1208,10 @@ namespace {
 template <int NG, int R>
 hipError_t run_lincomb_blk(const DevTables& t, const uint64_t* E, const double2* D, uint64_t* out, int nb, int ng,
                            int rows, int l, const LimbMap& map, int scalar, hipStream_t s) {
-    const int width = t.logN < 8 ? 1 << t.logN : 256;   // coefficients of one row per workgroup
-    const size_t blocks = (size_t)((rows + R - 1) / R) * l * (((size_t)1 << t.logN) / width);
+    const int width = t.logN < 8 ? 1 << t.logN : 256;   // threads per workgroup
+    const int lanes = width < 64 ? width : 64, W = width / lanes;
+    const int nrbg = ((rows + R - 1) / R + W - 1) / W;
+    const size_t blocks = (size_t)nrbg * l * (((size_t)1 << t.logN) / lanes);
     hipLaunchKernelGGL((k_lincomb_blk<NG, R>), dim3((unsigned)blocks), dim3(width), 0, s, E, D, out, t.pf, map, nb,
                        ng, rows, l, t.logN, scalar);
     return hipGetLastError();
@@ -1206,15 +1227,15 @@ hipError_t launch_lincomb(const DevTables& t, const uint64_t* E, const void* D_p
     const double2* D = (const double2*)D_pairs;
     const size_t dn = scalar ? 1 : ((size_t)1 << t.logN);
     const size_t LN = (size_t)l << t.logN;
-    // giant steps in chunks of at most 16
-    for (int g0 = 0; g0 < ng; g0 += 16) {
-        const int gc = ng - g0 < 16 ? ng - g0 : 16;
+    // giant steps in chunks of at most 8 (ng = 10 as 8 + 2: an input read
+    // twice costs less than 16 accumulators a thread)
+    for (int g0 = 0; g0 < ng; g0 += 8) {
+        const int gc = ng - g0 < 8 ? ng - g0 : 8;
         const double2* Dg = D + (size_t)g0 * nb * l * dn;
         uint64_t* og = out + (size_t)g0 * rows * LN;
         hipError_t e;
         if (gc <= 4) e = run_lincomb_blk<4, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
-        else if (gc <= 8) e = run_lincomb_blk<8, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
-        else e = run_lincomb_blk<16, 2>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
+        else e = run_lincomb_blk<8, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -345,7 +345,9 @@ __global__ void __launch_bounds__(T) k_keyswitch_i(const uint64_t* __restrict__ 
 // keeps NG x R accumulators, so each input value is
 // loaded once per call and each plaintext value once per R rows (rows are
 // the (ciphertext, poly) pairs, 2 B of them).  ng <= NG; rows past the end
-// of the last block are skipped.
+// of the last block are not stored.  Each baby step's input rows are loaded
+// one step ahead, and no load sits behind a branch (rows past the end and
+// giant steps past ng read clamped, valid addresses).
 template <int NG, int R>
 __global__ void __launch_bounds__(256) k_lincomb_blk_i(const uint64_t* __restrict__ E,
                                                      const ulonglong2* __restrict__ D, uint64_t* __restrict__ out,
@@ -369,17 +371,24 @@ __global__ void __launch_bounds__(256) k_lincomb_blk_i(const uint64_t* __restric
     for (int g = 0; g < NG; ++g)
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[g][r] = 0;
+    const int rlast = rows - 1 - r0;
+    uint64_t en[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) en[r] = ebase[(size_t)(r < rlast ? r : rlast) * LN];
     for (int b = 0; b < nb; ++b) {
         uint64_t e[R];
+        ulonglong2 w[NG];
 #pragma unroll
-        for (int r = 0; r < R; ++r) e[r] = r0 + r < rows ? ebase[(size_t)b * brow + (size_t)r * LN] : 0;
+        for (int r = 0; r < R; ++r) e[r] = en[r];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) w[g] = dbase[((size_t)(g < ng ? g : ng - 1) * nb + b) * dstride];
+        const int bn = b + 1 < nb ? b + 1 : b;
+#pragma unroll
+        for (int r = 0; r < R; ++r) en[r] = ebase[(size_t)bn * brow + (size_t)(r < rlast ? r : rlast) * LN];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            if (g < ng) {
-                const ulonglong2 w = dbase[((size_t)g * nb + b) * dstride];
 #pragma unroll
-                for (int r = 0; r < R; ++r) acc[g][r] += mul_shoup4_nq(e[r], w.x, w.y, nq);   // nb <= 32: < 2^52
-            }
+            for (int r = 0; r < R; ++r) acc[g][r] += mul_shoup4_nq(e[r], w[g].x, w[g].y, nq);   // nb <= 32: < 2^52
         }
     }
     uint64_t* obase = out + (size_t)r0 * LN + (size_t)i * N + n;
@@ -503,14 +512,14 @@ hipError_t launch_lincomb_int(const DevTables& t, const uint64_t* E, const ulong
     const int rows = batch * 2;
     const size_t dn = scalar ? 1 : ((size_t)1 << t.logN);
     const size_t LN = (size_t)l << t.logN;
-    for (int g0 = 0; g0 < ng; g0 += 16) {
-        const int gc = ng - g0 < 16 ? ng - g0 : 16;
+    // giant steps in chunks of at most 8, as on the FP64 path
+    for (int g0 = 0; g0 < ng; g0 += 8) {
+        const int gc = ng - g0 < 8 ? ng - g0 : 8;
         const ulonglong2* Dg = D + (size_t)g0 * nb * l * dn;
         uint64_t* og = out + (size_t)g0 * rows * LN;
         hipError_t e;
         if (gc <= 4) e = run_lincomb_blk<4, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
-        else if (gc <= 8) e = run_lincomb_blk<8, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
-        else e = run_lincomb_blk<16, 2>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
+        else e = run_lincomb_blk<8, 4>(t, E, Dg, og, nb, gc, rows, l, map, scalar, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
