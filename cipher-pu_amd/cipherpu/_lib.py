@@ -60,6 +60,7 @@ _SIGS = {
     "ckks_hmult_noalloc": (_I, [_P, _P, _P, _P, _I, _P, _I, _I, _I, _P]),
     "ckks_automorphism": (_I, [_P, _P, _P, _I, _I, _I, _U, _P]),
     "ckks_rotate": (_I, [_P, _P, _P, _I, _P, _I, _I, _I, _U, _P, _P]),
+    "ckks_rotate_add": (_I, [_P, _P, _P, _I, _P, _I, _I, _I, _U, _P, _P]),
     "ckks_rotate_many": (_I, [_P, _P, _P, _I, _P, _P, _I, _I, _I, _P, _P]),
     "ckks_encode": (_I, [_P, _P, _P, _I, _I, _I, _D, _P, _P]),
     "ckks_decode": (_I, [_P, _P, _P, _I, _I, _I, _D, _P, _P]),

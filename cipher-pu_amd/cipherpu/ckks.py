@@ -336,6 +336,17 @@ class CKKS:
              self.sp, gk.galois, _ptr(ws), self.ring.stream())
         return Ciphertext(out, ct.scale)
 
+    def rotate_add(self, ct: Ciphertext, gk: SwitchKey) -> Ciphertext:
+        """ct + rotate(ct, gk) in one call (ckks_rotate_add): the same words
+        as add(ct, rotate(ct, gk))."""
+        self.check_keys(gk)
+        B, l = ct.batch, ct.level
+        out = torch.empty_like(ct.data)
+        ws = self.workspace(B, l)
+        call("ckks_rotate_add", self.ring.handle, _ptr(ct.data), _ptr(gk.prepared), self.L + 1, _ptr(out), B, l,
+             self.sp, gk.galois, _ptr(ws), self.ring.stream())
+        return Ciphertext(out, ct.scale)
+
     @staticmethod
     def rotation_tables(gks: Sequence[SwitchKey], device) -> Tuple[torch.Tensor, torch.Tensor]:
         """Device tables for rotate_many: the prepared key address and the

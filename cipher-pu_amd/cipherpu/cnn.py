@@ -244,7 +244,7 @@ class EncryptedCNN(EncryptedNet):
         # rotate-and-sum over each 256-slot window, all packed ciphertexts at once
         zc = Ciphertext(z, outs.scale)
         for k in range(1, 9):
-            zc = ck.add(zc, self.rot(zc, WINDOW >> k))
+            zc = self.rot_add(zc, WINDOW >> k)
         return [Ciphertext(zc.data[i * B:(i + 1) * B], zc.scale) for i in range(len(groups))]
 
     def forward(self, ct: Ciphertext) -> List[Ciphertext]:

@@ -277,6 +277,13 @@ class EncryptedNet:
         self.ops["rot"] += ct.batch
         return self.ck.rotate(ct, self.gk(step))
 
+    def rot_add(self, ct: Ciphertext, step: int) -> Ciphertext:
+        """ct + rot(ct, step), one call (ckks_rotate_add)."""
+        if step % (self.ck.N // 2) == 0:
+            return self.ck.add(ct, ct)
+        self.ops["rot"] += ct.batch
+        return self.ck.rotate_add(ct, self.gk(step))
+
     def rot_blocks(self, ct: Ciphertext, B: int, steps: List[int]) -> List[Ciphertext]:
         """ct holds len(steps) blocks of B rows; block i rotated by steps[i]
         (a multiple of N/2 leaves it as it is).  The blocks that move go
@@ -492,7 +499,7 @@ class EncryptedMLP(EncryptedNet):
         for part in blocks[1:]:
             z = ck.add(z, part)
         for s in p.folds:
-            z = ck.add(z, self.rot(z, s))
+            z = self.rot_add(z, s)
         return z
 
     def _mask(self, ct: Ciphertext, keep: int, in_place: bool = False) -> Ciphertext:
@@ -525,7 +532,7 @@ class EncryptedMLP(EncryptedNet):
     def layer1(self, ct: Ciphertext) -> Ciphertext:
         y = self._linear(ct, self.p1)                 # outputs in slots [0, 128) of each block
         y = self._mask(y, 128, in_place=True)
-        return self.ck.add(y, self.rot(y, -128))      # copy to [128, 256) for layer 2's wrap
+        return self.rot_add(y, -128)                  # copy to [128, 256) for layer 2's wrap
 
     def activation(self, ct: Ciphertext) -> Ciphertext:
         self.ops["hmult"] += ct.batch

@@ -738,8 +738,11 @@ int ckks_automorphism(const ckks_ctx* c, const uint64_t* in, uint64_t* out, int 
     return CKKS_OK;
 }
 
-int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int key_limbs, uint64_t* out, int batch,
-                int l, int special, uint64_t galois, void* workspace, void* stream) {
+namespace {
+// ckks_rotate, and with add = true ckks_rotate_add (the input added in the
+// ModDown pass: out = ct + rot(ct)).
+int rotate_impl(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int key_limbs, uint64_t* out,
+                int batch, int l, int special, uint64_t galois, void* workspace, void* stream, bool add) {
     int st = ks_common_check(c, gkey, key_limbs, out, batch, l, special, workspace);
     if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
     if (!ct) return CKKS_ERR_NULL;
@@ -756,8 +759,20 @@ int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int
     TIMED(CKKS_KIND_AUTOMORPHISM, launch_automorphism(t, ct, sct, batch * 2, l, m, galois, S(stream)));
     TIMED(CKKS_KIND_KEYSWITCH, launch_keyswitch(t, sct + LN, 2 * LN, nullptr, nullptr, gkey,
                                                 key_limbs, acc, batch, l, m, special, c->ks_threads, S(stream)));
-    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream)));
+    TIMED(CKKS_KIND_MODDOWN, launch_moddown_rescale(t, acc, sct, 1, out, batch, l, m, special, mc, false, S(stream),
+                                                    add ? ct : nullptr));
     return CKKS_OK;
+}
+}  // namespace
+
+int ckks_rotate(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int key_limbs, uint64_t* out, int batch,
+                int l, int special, uint64_t galois, void* workspace, void* stream) {
+    return rotate_impl(c, ct, gkey, key_limbs, out, batch, l, special, galois, workspace, stream, false);
+}
+
+int ckks_rotate_add(const ckks_ctx* c, const uint64_t* ct, const uint64_t* gkey, int key_limbs, uint64_t* out,
+                    int batch, int l, int special, uint64_t galois, void* workspace, void* stream) {
+    return rotate_impl(c, ct, gkey, key_limbs, out, batch, l, special, galois, workspace, stream, true);
 }
 
 int ckks_rotate_many(const ckks_ctx* c, const uint64_t* ct, const uint64_t* key_table, int key_limbs,

@@ -468,11 +468,13 @@ __global__ void __launch_bounds__(T) k_keyswitch(const uint64_t* __restrict__ d2
 // on the CRT value, in the per-limb form
 //   ((y_i + h) - [(y_L + h) mod q_L]) q_L^-1 mod q_i,  h = (q_L - 1)/2).
 // narrow = 1: every prime is below twice every other one, so a residue mod
-// one prime is reduced mod another by one conditional subtraction.
+// one prime is reduced mod another by one conditional subtraction.  dadd
+// (optional, [batch][2][l][N] like dcoef) is added to both polynomials: a
+// rotation's input, for out = ct + rot(ct) without rescale.
 __global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64_t* __restrict__ dcoef, int dmask,
-                                  uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map,
-                                  int l, int pidx_p, ModDownConst mc, int rescale, int narrow, int logN,
-                                  size_t total) {
+                                  const uint64_t* __restrict__ dadd, uint64_t* __restrict__ out,
+                                  const PrimeConst* __restrict__ pcs, LimbMap map, int l, int pidx_p,
+                                  ModDownConst mc, int rescale, int narrow, int logN, size_t total) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const size_t N = (size_t)1 << logN;
@@ -484,12 +486,14 @@ __global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64
     const int lo = rescale ? l - 1 : l;
     uint64_t* o = out + bp * lo * N + n;
     const uint64_t* dc = (dcoef && ((dmask >> (bp & 1)) & 1)) ? dcoef + bp * l * N + n : nullptr;
+    const uint64_t* da = dadd ? dadd + bp * l * N + n : nullptr;
     auto moddown = [&](int i) -> uint64_t {
         const PrimeConst& pc = pcs[map.idx[i]];
         const uint64_t tpi = narrow ? csub(tp, pc.q) : reduce64(tp, pc);
         uint64_t v = sub_mod(add_mod(in[(size_t)i * N], mc.hp[i], pc.q), tpi, pc.q);
         v = csub(mul_shoup(v, mc.pinv[i], mc.pinv_s[i], pc.q), pc.q);
         if (dc) v = add_mod(v, dc[(size_t)i * N], pc.q);
+        if (da) v = add_mod(v, da[(size_t)i * N], pc.q);
         return v;
     };
     if (!rescale) {
@@ -517,6 +521,7 @@ __global__ void k_moddown_rescale(const uint64_t* __restrict__ acc, const uint64
 template <int L, bool RESCALE>
 __global__ void __launch_bounds__(256) k_moddown_rescale_l(const uint64_t* __restrict__ acc,
                                                            const uint64_t* __restrict__ dcoef, int dmask,
+                                                           const uint64_t* __restrict__ dadd,
                                                            uint64_t* __restrict__ out,
                                                            const PrimeConst* __restrict__ pcs, LimbMap map,
                                                            int pidx_p, ModDownConst mc, int narrow, int logN,
@@ -535,6 +540,7 @@ __global__ void __launch_bounds__(256) k_moddown_rescale_l(const uint64_t* __res
     constexpr int LO = RESCALE ? L - 1 : L;
     uint64_t* o = out + bp * LO * N + n;
     const uint64_t* dc = (dcoef && ((dmask >> (bp & 1)) & 1)) ? dcoef + bp * L * N + n : nullptr;
+    const uint64_t* da = dadd ? dadd + bp * L * N + n : nullptr;
     uint64_t md[L][V];
 #pragma unroll
     for (int i = 0; i < L; ++i) {
@@ -546,6 +552,7 @@ __global__ void __launch_bounds__(256) k_moddown_rescale_l(const uint64_t* __res
             const uint64_t u = in[(size_t)i * N + v * H] + mc.hp[i] + q - tpi;   // < 3q
             uint64_t r = csub(mul_shoup(u, mc.pinv[i], mc.pinv_s[i], q), q);
             if (dc) r = add_mod(r, dc[(size_t)i * N + v * H], q);
+            if (da) r = add_mod(r, da[(size_t)i * N + v * H], q);
             md[i][v] = r;
         }
     }
@@ -1076,7 +1083,7 @@ hipError_t launch_keyswitch(const DevTables& t, const uint64_t* d2c, size_t d2c_
 hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
                                   uint64_t* out,
                                   int batch, int l, const LimbMap& map, int pidx_special, const ModDownConst& mc,
-                                  bool rescale, hipStream_t s) {
+                                  bool rescale, hipStream_t s, const uint64_t* dadd) {
     const size_t total = ((size_t)batch * 2) << t.logN;
     if (!total) return hipSuccess;
     if (l >= 1 && l <= 8 && t.logN >= 1) {
@@ -1085,11 +1092,11 @@ hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const
 #define CKKS_MD_CASE(LL)                                                                                          \
     case LL:                                                                                                      \
         if (rescale)                                                                                              \
-            hipLaunchKernelGGL((k_moddown_rescale_l<LL, true>), grid, dim3(256), 0, s, acc, dcoef, dmask, out,     \
-                               t.pc, map, pidx_special, mc, t.narrow, t.logN, half);                              \
+            hipLaunchKernelGGL((k_moddown_rescale_l<LL, true>), grid, dim3(256), 0, s, acc, dcoef, dmask, dadd,    \
+                               out, t.pc, map, pidx_special, mc, t.narrow, t.logN, half);                         \
         else                                                                                                      \
-            hipLaunchKernelGGL((k_moddown_rescale_l<LL, false>), grid, dim3(256), 0, s, acc, dcoef, dmask, out,    \
-                               t.pc, map, pidx_special, mc, t.narrow, t.logN, half);                              \
+            hipLaunchKernelGGL((k_moddown_rescale_l<LL, false>), grid, dim3(256), 0, s, acc, dcoef, dmask, dadd,   \
+                               out, t.pc, map, pidx_special, mc, t.narrow, t.logN, half);                         \
         return hipGetLastError();
         switch (l) {
             CKKS_MD_CASE(1)
@@ -1104,8 +1111,8 @@ hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const
         }
 #undef CKKS_MD_CASE
     }
-    hipLaunchKernelGGL(k_moddown_rescale, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, acc, dcoef, dmask, out,
-                       t.pc, map, l, pidx_special, mc, rescale ? 1 : 0, t.narrow, t.logN, total);
+    hipLaunchKernelGGL(k_moddown_rescale, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, acc, dcoef, dmask,
+                       dadd, out, t.pc, map, l, pidx_special, mc, rescale ? 1 : 0, t.narrow, t.logN, total);
     return hipGetLastError();
 }
 

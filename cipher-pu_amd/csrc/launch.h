@@ -129,9 +129,12 @@ hipError_t launch_keyswitch_int(const DevTables& t, const uint64_t* d2c, size_t 
                                 const uint64_t* key_table, const KsTargets& tg);
 hipError_t launch_lincomb_int(const DevTables& t, const uint64_t* E, const ulonglong2* D, uint64_t* out, int nb,
                               int ng, int batch, int l, const LimbMap& map, int scalar, hipStream_t s);
+// dcoef (rows selected by dmask bit poly) and dadd (both polys), optional, are
+// added to the result; both are [batch][2][l][N].
 hipError_t launch_moddown_rescale(const DevTables& t, const uint64_t* acc, const uint64_t* dcoef, int dmask,
                                   uint64_t* out, int batch, int l, const LimbMap& map, int pidx_special,
-                                  const ModDownConst& mc, bool rescale, hipStream_t s);
+                                  const ModDownConst& mc, bool rescale, hipStream_t s,
+                                  const uint64_t* dadd = nullptr);
 hipError_t launch_rescale(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys, int l,
                           const LimbMap& map, const ModDownConst& mc, hipStream_t s);
 hipError_t launch_automorphism(const DevTables& t, const uint64_t* in, uint64_t* out, int n_polys,

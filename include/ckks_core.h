@@ -171,6 +171,14 @@ int ckks_automorphism(const ckks_ctx *ctx, const uint64_t *in, uint64_t *out, in
 int ckks_rotate(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *gkey, int key_limbs, uint64_t *out,
                 int batch, int l, int special, uint64_t galois, void *workspace, void *stream);
 
+/* out = ct + rotate(ct): ckks_rotate with the input added in its ModDown
+ * pass, the same words as ckks_rotate followed by ckks_pointwise op 1.  One
+ * step of a rotate-and-sum (the slot sums of the encrypted layers, where
+ * the reference sums coefficient-encoded products after decryption,
+ * exp_e14_cnn_e2e.py:90-127).  Arguments as for ckks_rotate. */
+int ckks_rotate_add(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *gkey, int key_limbs, uint64_t *out,
+                    int batch, int l, int special, uint64_t galois, void *workspace, void *stream);
+
 /* ckks_rotate with a step per ciphertext, in one call: ciphertext b is
  * rotated by galois_table[b] with the prepared Galois key at device address
  * key_table[b].  Both tables are device arrays of batch uint64 words; the

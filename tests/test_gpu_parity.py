@@ -377,6 +377,28 @@ def test_squaring_gives_the_words_of_the_product(gpu, N, threads, l):
         assert np.array_equal(want[i], OC.hmult(a[i], a[i], key, pr[:l], p, ps[:l], pp)), i
 
 
+@pytest.mark.parametrize("l", [4, 2, 1])
+def test_rotate_add_gives_the_words_of_rotate_then_add(gpu, l):
+    """ckks_rotate_add (the input added in the rotation's ModDown pass)
+    against ckks_rotate followed by the point-wise sum, in both arithmetics,
+    and the decoded slots of ct + rot(ct)."""
+    from cipherpu.ckks import CKKS, CKKSParams
+    from cipherpu.ring import to_numpy
+    c = CKKS(CKKSParams(), seed=41)
+    sk = c.keygen()
+    z = np.random.default_rng(l).uniform(-1, 1, (5, c.N // 2))
+    ct = c.encrypt(c.encode(z, level=l), sk)
+    for fp64 in (True, False):
+        c.ring.set_arith(fp64)
+        for step in (1, -128, 512):
+            gk = c.galois_key(sk, step)
+            want = to_numpy(c.add(ct, c.rotate(ct, gk)).data)
+            got = c.rotate_add(ct, gk)
+            assert np.array_equal(to_numpy(got.data), want), (fp64, step)
+            dec = to_numpy(c.decode(c.decrypt(got, sk), got.scale)).real
+            assert np.max(np.abs(dec - (z + np.roll(z, -step, axis=1)))) < 1e-5, (fp64, step)
+
+
 def test_rotate_bit_exact_and_slots_move(ck):
     from cipherpu.ring import to_numpy
     c, sk = ck
