@@ -428,7 +428,8 @@ def mlp_leg(samples: int, targets, chunk_cts: int, kind: str = "mlp"):
     # one untimed chunk through every loop first, so that first-launch costs
     # (code-object loading, allocator growth) fall on none of the timed loops
     Xw = X[:chunk_cts * mlp.S]
-    gear.static_loop(mlp, Xw, 0, chunk_cts)
+    for g in range(4):   # every gear's first evaluation (plans at the stage levels) untimed
+        gear.static_loop(mlp, Xw, g, chunk_cts)
     gear.plain_monitor_loop(mlp, Xw, targets[0], chunk_cts)
     gear.learned_loop(mlp, net, Xw, targets[0], chunk_cts)
     torch.cuda.synchronize()

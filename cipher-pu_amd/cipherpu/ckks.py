@@ -357,16 +357,20 @@ class CKKS:
         return keys, gal
 
     def rotate_many(self, ct: Ciphertext, gks: Sequence[SwitchKey],
-                    tables: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> Ciphertext:
+                    tables: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
+                    out: Optional[torch.Tensor] = None) -> Ciphertext:
         """Ciphertext b rotated by the step gks[b] was made for, all in one
         call (ckks_rotate_many); tables may hold rotation_tables(gks) for
-        reuse."""
+        reuse; out, if given, is a contiguous tensor of ct's shape to write."""
         B, l = ct.batch, ct.level
         if len(gks) != B:
             raise ValueError("one Galois key per ciphertext")
         self.check_keys(*gks)
         keys, gal = tables if tables is not None else self.rotation_tables(gks, self.device)
-        out = torch.empty_like(ct.data)
+        if out is None:
+            out = torch.empty_like(ct.data)
+        elif out.shape != ct.data.shape or not out.is_contiguous():
+            raise ValueError("out must be a contiguous tensor of the input's shape")
         ws = self.workspace(B, l)
         call("ckks_rotate_many", self.ring.handle, _ptr(ct.data), _ptr(keys), self.L + 1, _ptr(gal), _ptr(out), B,
              l, self.sp, _ptr(ws), self.ring.stream())

@@ -25,7 +25,9 @@ arithmetic, and the plain monitor measures every reported SNR.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
+import gc
 import math
 import os
 import time
@@ -352,6 +354,21 @@ def _sync(mlp):
     torch.cuda.synchronize(mlp.ck.device)
 
 
+@contextlib.contextmanager
+def _timed_region():
+    """The loops' timed regions run with Python's cyclic collector paused
+    (as timeit does), after one collection: a collection pause would fall
+    on whichever loop happened to be running."""
+    gc.collect()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
+
+
 def _plain_select(mlp, ct, Xc, target_db, start_gear):
     """Monitor-driven gear search for one chunk: evaluate at the start gear,
     step to the next safer gear until the decrypt-based SNR meets the
@@ -377,18 +394,19 @@ def plain_monitor_loop(mlp, X: np.ndarray, target_db: float, chunk_cts: int = 64
     chunks = _setup(mlp, X, chunk_cts)
     outs, gears, evals, mon = [], [], 0, 0
     last = N_GEARS - 1
-    _sync(mlp)
-    t0 = time.perf_counter()
-    for i, (Xc, ct) in enumerate(chunks):
-        start = min(N_GEARS - 1, last + 1) if (i % probe_every == 0 and i) else last
-        out, g, e = _plain_select(mlp, ct, Xc, target_db, start)
-        evals += e
-        mon += e
-        last = g
-        gears.append(g)
-        outs.append(out)
-    _sync(mlp)
-    dt = time.perf_counter() - t0
+    with _timed_region():
+        _sync(mlp)
+        t0 = time.perf_counter()
+        for i, (Xc, ct) in enumerate(chunks):
+            start = min(N_GEARS - 1, last + 1) if (i % probe_every == 0 and i) else last
+            out, g, e = _plain_select(mlp, ct, Xc, target_db, start)
+            evals += e
+            mon += e
+            last = g
+            gears.append(g)
+            outs.append(out)
+        _sync(mlp)
+        dt = time.perf_counter() - t0
     res = np.concatenate([mlp.decrypt_outputs(o, Xc.shape[0]) for o, (Xc, _) in zip(outs, chunks)])
     return LoopResult(res, gears, evals, mon, 0, dt, X.shape[0])
 
@@ -417,26 +435,27 @@ def learned_loop(mlp, net: GearNet, X: np.ndarray, target_db: float, chunk_cts: 
     _gear_table(mlp, net, chunks[:2])            # warm the feature and network kernels
     outs, gears, evals, mon, fb = [], [], 0, 0, 0
     prev = 0
-    _sync(mlp)
-    t0 = time.perf_counter()
-    table = _gear_table(mlp, net, chunks)
-    for i, (Xc, ct) in enumerate(chunks):
-        pred = table[prev, i]
-        g = choose_gear(pred, target_db, margin_db)
-        if g is None:
-            fb += 1
-            out, g, e = _plain_select(mlp, ct, Xc, target_db, N_GEARS - 1)
-            evals += e
-            mon += e
-        else:
-            mlp.set_gears(g, g)
-            out = mlp.forward(ct)
-            evals += 1
-        prev = g
-        gears.append(g)
-        outs.append(out)
-    _sync(mlp)
-    dt = time.perf_counter() - t0
+    with _timed_region():
+        _sync(mlp)
+        t0 = time.perf_counter()
+        table = _gear_table(mlp, net, chunks)
+        for i, (Xc, ct) in enumerate(chunks):
+            pred = table[prev, i]
+            g = choose_gear(pred, target_db, margin_db)
+            if g is None:
+                fb += 1
+                out, g, e = _plain_select(mlp, ct, Xc, target_db, N_GEARS - 1)
+                evals += e
+                mon += e
+            else:
+                mlp.set_gears(g, g)
+                out = mlp.forward(ct)
+                evals += 1
+            prev = g
+            gears.append(g)
+            outs.append(out)
+        _sync(mlp)
+        dt = time.perf_counter() - t0
     res = np.concatenate([mlp.decrypt_outputs(o, Xc.shape[0]) for o, (Xc, _) in zip(outs, chunks)])
     return LoopResult(res, gears, evals, mon, fb, dt, X.shape[0])
 
@@ -446,11 +465,12 @@ def static_loop(mlp, X: np.ndarray, g: int, chunk_cts: int = 64) -> LoopResult:
     chunks = _setup(mlp, X, chunk_cts)
     mlp.set_gears(g, g)
     outs = []
-    _sync(mlp)
-    t0 = time.perf_counter()
-    for _, ct in chunks:
-        outs.append(mlp.forward(ct))
-    _sync(mlp)
-    dt = time.perf_counter() - t0
+    with _timed_region():
+        _sync(mlp)
+        t0 = time.perf_counter()
+        for _, ct in chunks:
+            outs.append(mlp.forward(ct))
+        _sync(mlp)
+        dt = time.perf_counter() - t0
     res = np.concatenate([mlp.decrypt_outputs(o, Xc.shape[0]) for o, (Xc, _) in zip(outs, chunks)])
     return LoopResult(res, [g] * len(chunks), len(chunks), 0, 0, dt, X.shape[0])

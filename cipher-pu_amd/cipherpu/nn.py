@@ -249,6 +249,7 @@ class EncryptedNet:
         self.gears = (0, 0)
         self.fp64 = ck.ring.fp64    # prepared plaintexts and keys are in this arithmetic
         self.drop_levels = True     # each stage at the lowest level it needs
+        self._ones: Dict[tuple, torch.Tensor] = {}   # rot_sum weights, per (blocks, level, arithmetic)
 
     def entry_levels(self) -> Tuple[int, int]:
         """Levels at which forward's two stages start (L, L without drop_levels)."""
@@ -283,6 +284,44 @@ class EncryptedNet:
             return self.ck.add(ct, ct)
         self.ops["rot"] += ct.batch
         return self.ck.rotate_add(ct, self.gk(step))
+
+    def rot_sum(self, ct: Ciphertext, B: int, steps: List[int]) -> Ciphertext:
+        """sum_i rot(block i, steps[i]) over the len(steps) blocks of B rows
+        of ct: the moving blocks rotated by one ckks_rotate_many call into a
+        stacked buffer, summed by one ckks_lincomb_scalar call with weights
+        1 (the words of rot_blocks followed by point-wise sums)."""
+        ck = self.ck
+        half = ck.N // 2
+        k = len(steps)
+        moving = [i for i, st in enumerate(steps) if st % half]
+        m0 = moving[0] if moving else 0
+        if k == 1 or not self.mixed_rotations or moving != list(range(m0, m0 + len(moving))):
+            blocks = self.rot_blocks(ct, B, steps)
+            z = blocks[0]
+            for part in blocks[1:]:
+                z = ck.add(z, part)
+            return z
+        l = ct.level
+        buf = torch.empty_like(ct.data)
+        for i in range(k):
+            if i not in moving:
+                buf[i * B:(i + 1) * B].copy_(ct.data[i * B:(i + 1) * B])
+        key = tuple(steps[i] % half for i in moving for _ in range(B))
+        gks = [self.gk(st) for st in key]
+        if key not in self._rtab:
+            self._rtab[key] = ck.rotation_tables(gks, ck.device)
+        self.ops["rot"] += len(key)
+        lo, hi = m0 * B, (m0 + len(moving)) * B
+        ck.rotate_many(Ciphertext(ct.data[lo:hi], ct.scale), gks, tables=self._rtab[key], out=buf[lo:hi])
+        ones = self._ones.get((k, l, ck.ring.fp64))
+        if ones is None:
+            W = np.zeros((1, k, l, 2), dtype=np.uint64)
+            for i in range(l):
+                W[0, :, i] = ck.ring.prepared_pairs(1, ck.primes[i])
+            ones = self._ones[(k, l, ck.ring.fp64)] = ck.ring.tensor(W)
+        out = torch.empty(B, 2, l, ck.N, dtype=torch.uint64, device=ck.device)
+        call("ckks_lincomb_scalar", ck.ring.handle, _ptr(buf), _ptr(ones), _ptr(out), k, 1, B, l, ck.ring.stream())
+        return Ciphertext(out, ct.scale)
 
     def rot_blocks(self, ct: Ciphertext, B: int, steps: List[int]) -> List[Ciphertext]:
         """ct holds len(steps) blocks of B rows; block i rotated by steps[i]
@@ -494,10 +533,7 @@ class EncryptedMLP(EncryptedNet):
         ck.ring.intt_(acc.view(-1, l, ck.N))
         # every giant step rescaled and rotated at once, each by its own step
         parts = ck.rescale(Ciphertext(acc.view(ng * B, 2, l, ck.N), ct.scale * self.w_scale))
-        blocks = self.rot_blocks(parts, B, [p.baby * g for g in p.used_g])
-        z = blocks[0]
-        for part in blocks[1:]:
-            z = ck.add(z, part)
+        z = self.rot_sum(parts, B, [p.baby * g for g in p.used_g])
         for s in p.folds:
             z = self.rot_add(z, s)
         return z
