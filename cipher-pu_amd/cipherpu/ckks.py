@@ -73,7 +73,7 @@ class Ciphertext:
 class SecretKey:
     s: np.ndarray               # [N] int64 in {-1, 0, 1}
     rns: torch.Tensor           # [K, N] uint64 over all primes (incl. special)
-    rows: Dict[int, torch.Tensor] = field(default_factory=dict, repr=False)   # CKKS.key_rows, per level
+    rows: Dict[object, torch.Tensor] = field(default_factory=dict, repr=False)   # CKKS.key_rows / key_eval caches
 
 
 @dataclass
@@ -106,6 +106,9 @@ class CKKS:
         self._seed = int(seed) & ((1 << 64) - 1)
         self._draws = 0
         self._ws = {}      # stream handle -> workspace
+        # device-sampled encryption in one kernel (ckks_encrypt_sk) where it
+        # applies: the words of the separate samplers, product and sums
+        self.fused_encrypt = True
 
     # -- sampling (host) ------------------------------------------------------
     def ternary(self, *shape) -> np.ndarray:
@@ -225,6 +228,14 @@ class CKKS:
         (exp_e33_real_ciphertext.py:171-175)."""
         B, l = pt.shape[0], pt.shape[1]
         dev = self.sampler == "device"
+        if (dev and a is None and e is None and self.fused_encrypt and self.ring.fp64
+                and self.N in (4096, 8192, 16384) and not self.ring.cyclic):
+            seed_a = self._next_seed()
+            seed_e = self._next_seed()
+            out = torch.empty(B, 2, l, self.N, dtype=torch.uint64, device=self.device)
+            call("ckks_encrypt_sk", self.ring.handle, _ptr(pt.contiguous()), _ptr(self.key_eval(sk, l)), _ptr(out),
+                 B, l, float(self.p.sigma), seed_a, seed_e, self.ring.stream())
+            return Ciphertext(out, self.p.scale if scale is None else scale)
         if a is None:
             a = self.uniform_device(B, l) if dev else self.uniform(B, limbs=range(l))
         if e is None:
@@ -242,6 +253,15 @@ class CKKS:
         B, l = d.shape[0], d.shape[2]
         S = self.key_rows(sk, B, l)
         return self.ring.add(d[:, 0].contiguous(), self.ring.poly_mul(d[:, 1].contiguous(), S))
+
+    def key_eval(self, sk: SecretKey, l: int) -> torch.Tensor:
+        """[l, N] s on primes 0..l-1 in evaluation order (ckks_ntt_fwd),
+        kept on the key."""
+        t = sk.rows.get(("eval", l))
+        if t is None or t.device != self.device:
+            t = self.ring.ntt(sk.rns[:l].to(self.device).contiguous())
+            sk.rows[("eval", l)] = t
+        return t
 
     def key_rows(self, sk: SecretKey, B: int, l: int) -> torch.Tensor:
         """[B, l, N] contiguous copies of s on primes 0..l-1, the second

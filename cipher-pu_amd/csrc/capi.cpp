@@ -498,6 +498,23 @@ int ckks_sample_gaussian(const ckks_ctx* c, uint64_t* out, int rows, int l, int 
     return CKKS_OK;
 }
 
+int ckks_encrypt_sk(const ckks_ctx* c, const uint64_t* pt, const uint64_t* s_eval, uint64_t* out, int batch, int l,
+                    double sigma, uint64_t seed_a, uint64_t seed_e, void* stream) {
+    int st = check_rows(c, batch, l, 0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!pt || !s_eval || !out) return CKKS_ERR_NULL;
+    if (pt == out || s_eval == out) return CKKS_ERR_BAD_ARG;
+    if (!(sigma > 0.0) || sigma > 10.0) return CKKS_ERR_BAD_ARG;
+    if (!c->fp64) return CKKS_ERR_BAD_ARG;
+    if (c->logN < 12 || c->logN > 14 || (c->flags & CKKS_CYCLIC)) return CKKS_ERR_BAD_SHAPE;
+    if (misaligned(pt) || misaligned(s_eval) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    GaussTable t;
+    gauss_table(sigma, t);
+    HIPCHK(launch_encrypt_sk(c->tables(), pt, s_eval, out, batch, l, limb_map(0, l), seed_a, seed_e, t, S(stream)));
+    return CKKS_OK;
+}
+
 int ckks_timing_start(ckks_ctx* c, int max_launches) {
     if (!c) return CKKS_ERR_NULL;
     if (max_launches < 1 || max_launches > (1 << 20)) return CKKS_ERR_BAD_SHAPE;

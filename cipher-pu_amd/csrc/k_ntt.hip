@@ -6,6 +6,7 @@
 #include "launch.h"
 #include "ntt_core.h"
 #include "ntt_f64.h"
+#include "sample.h"
 
 namespace {
 
@@ -129,6 +130,56 @@ __global__ void __launch_bounds__(T) k_polymul_f(const uint64_t* __restrict__ A,
     if constexpr (ntt::pair_pays<C>()) nttf::inv_ntt_db<C>(x, ldsf, tw + C::N, pf, tid);
     else nttf::inv_ntt<C>(x, ldsf, tw + C::N, pf, tid);
     nttf::store_words<C>(x, out + off, pf, tid);
+}
+
+// Symmetric RLWE encryption, one (ciphertext b, limb i) row a workgroup:
+// c1 = -a, c0 = a s + e + m (exp_e33_real_ciphertext.py:171-175).  The mask
+// a and the error e are drawn as k_sample_uniform and k_sample_gauss draw
+// them (element (b l + i) N + n of seed_a; element b N + n of seed_e, one
+// integer for every limb), so the words are those of the two samplers, the
+// product a s and the point-wise sums.  s_eval: s in evaluation order, [l][N].
+// The mask is drawn again for c1 rather than kept across the transforms.
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_encrypt_f(const uint64_t* __restrict__ pt, const uint64_t* __restrict__ s_eval,
+                                                 uint64_t* __restrict__ out, const double* __restrict__ twf_all,
+                                                 const PrimeF* __restrict__ pfs, const PrimeConst* __restrict__ pcs,
+                                                 LimbMap map, int l, uint64_t seed_a, uint64_t seed_e,
+                                                 GaussTable tab) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    double* ldsf = reinterpret_cast<double*>(lds);
+    const int tid = threadIdx.x;
+    const int row = (int)blockIdx.x;   // b l + i
+    const int i = wave_uniform(row % l), b = wave_uniform(row / l);
+    const int p = map.idx[i];
+    const PrimeF pf = pfs[p];
+    const uint64_t q = pcs[p].q;
+    const double* tw = twf_all + (size_t)p * 2 * C::N;
+    const size_t base = (size_t)row * C::N;
+    double x[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) x[e] = u2d(smp::uniform(seed_a, base + e * C::T + tid, q));
+    nttf::fwd_ntt<C>(x, ldsf, tw, pf, tid);
+    const uint64_t* sr = s_eval + (size_t)i * C::N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e)
+        x[e] = muld(rmod(x[e], pf.qinv, pf.nq), u2d(sr[ntt::eval_index<C>(e, tid)]), pf.qinv, pf.nq);
+    nttf::inv_ntt<C>(x, ldsf, tw + C::N, pf, tid);
+    const uint64_t* m = pt + base;
+    uint64_t* c0 = out + ((size_t)b * 2 * l + i) * C::N;
+    uint64_t* c1 = c0 + (size_t)l * C::N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+        const int n = e * C::T + tid;
+        bool neg;
+        const uint64_t k = (uint64_t)smp::gauss(seed_e, (size_t)b * C::N + n, tab, neg);
+        uint64_t v = d2u(canon(x[e], pf.qinv, pf.nq)) + (neg ? q - k : k);
+        v = v >= q ? v - q : v;
+        v += m[n];
+        c0[n] = v >= q ? v - q : v;
+        const uint64_t a = smp::uniform(seed_a, base + n, q);
+        c1[n] = a ? q - a : 0;
+    }
 }
 
 // Generic transform for small rings (N <= 2048): one stage per barrier.
@@ -389,6 +440,18 @@ hipError_t run_polymul_f(const DevTables& t, const uint64_t* a, const uint64_t* 
     return hipGetLastError();
 }
 
+template <int LOGN, int T>
+hipError_t run_encrypt_f(const DevTables& t, const uint64_t* pt, const uint64_t* s_eval, uint64_t* out, int batch,
+                         int l, const LimbMap& map, uint64_t seed_a, uint64_t seed_e, const GaussTable& tab,
+                         hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_encrypt_f<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_encrypt_f<LOGN, T>), dim3(batch * l), dim3(T), C::LDS_BYTES, s, pt, s_eval, out, t.twf,
+                       t.pf, t.pc, map, l, seed_a, seed_e, tab);
+    return hipGetLastError();
+}
+
 // Untuned launch shape of the batched transforms (fastest standalone
 // transform on MI355X at N = 8192: 512 threads).
 int default_threads(int logN) { return logN >= 13 ? 512 : 256; }
@@ -454,6 +517,19 @@ hipError_t launch_polymul(const DevTables& t, const uint64_t* a, const uint64_t*
     hipLaunchKernelGGL(k_polymul_small, dim3(rows), dim3(thr), 2 * N * 8, s, a, b, out, t.tw, t.pc, map, n_limbs,
                        t.logN);
     return hipGetLastError();
+}
+
+hipError_t launch_encrypt_sk(const DevTables& t, const uint64_t* pt, const uint64_t* s_eval, uint64_t* out, int batch,
+                             int l, const LimbMap& map, uint64_t seed_a, uint64_t seed_e, const GaussTable& tab,
+                             hipStream_t s) {
+    if (batch * l == 0) return hipSuccess;
+    if (!t.fp64) return hipErrorInvalidValue;
+    switch (t.logN) {
+        case 12: return run_encrypt_f<12, 256>(t, pt, s_eval, out, batch, l, map, seed_a, seed_e, tab, s);
+        case 13: return run_encrypt_f<13, 512>(t, pt, s_eval, out, batch, l, map, seed_a, seed_e, tab, s);
+        case 14: return run_encrypt_f<14, 512>(t, pt, s_eval, out, batch, l, map, seed_a, seed_e, tab, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_bitrev(uint64_t* data, int n_rows, int logN, hipStream_t s) {

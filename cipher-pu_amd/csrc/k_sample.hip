@@ -18,16 +18,10 @@
 
 #include "launch.h"
 #include "modarith.h"
+#include "sample.h"
 
 namespace {
 
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-__device__ __forceinline__ uint64_t draw(uint64_t seed, uint64_t k) { return mix64(seed + k * 0x9E3779B97F4A7C15ull); }
 
 // out[row][i][n] uniform in [0, q_i)
 __global__ void k_sample_uniform(uint64_t* __restrict__ out, const PrimeConst* __restrict__ pcs, LimbMap map, int l,
@@ -35,7 +29,7 @@ __global__ void k_sample_uniform(uint64_t* __restrict__ out, const PrimeConst* _
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const int i = (int)((g >> logN) % l);
-    out[g] = __umul64hi(draw(seed, g), pcs[map.idx[i]].q);
+    out[g] = smp::uniform(seed, g, pcs[map.idx[i]].q);
 }
 
 // out[row][i][n] = e[row][n] mod q_i, e rounded-Gaussian
@@ -45,10 +39,8 @@ __global__ void k_sample_gauss(uint64_t* __restrict__ out, const PrimeConst* __r
     if (g >= total) return;
     const size_t N = (size_t)1 << logN;
     const size_t row = g >> logN, n = g & (N - 1);
-    const uint64_t u = draw(seed, 2 * g), v = draw(seed, 2 * g + 1);
-    int k = 0;
-    while (k < GAUSS_TABLE - 1 && u >= tab.cdf[k]) ++k;       // |e| = first k with u < cdf[k]
-    const bool neg = (v & 1) && k;
+    bool neg;
+    const int k = smp::gauss(seed, g, tab, neg);
     uint64_t* o = out + row * l * N + n;
     for (int i = 0; i < l; ++i) {
         const uint64_t q = pcs[map.idx[i]].q;

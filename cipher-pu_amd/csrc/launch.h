@@ -194,6 +194,12 @@ hipError_t launch_decode_crt(const DevTables& t, const uint64_t* pt, double* coe
 
 hipError_t launch_sample_uniform(const DevTables& t, uint64_t* out, int rows, int l, const LimbMap& map,
                                  uint64_t seed, hipStream_t s);
+// Fused symmetric encryption on the FP64 pipeline (N = 2^12..2^14; other
+// rings and the integer arithmetic: hipErrorInvalidValue): pt [batch][l][N],
+// s_eval [l][N] evaluation order, out [batch][2][l][N].
+hipError_t launch_encrypt_sk(const DevTables& t, const uint64_t* pt, const uint64_t* s_eval, uint64_t* out, int batch,
+                             int l, const LimbMap& map, uint64_t seed_a, uint64_t seed_e, const GaussTable& tab,
+                             hipStream_t s);
 hipError_t launch_sample_gauss(const DevTables& t, uint64_t* out, int rows, int l, const LimbMap& map,
                                uint64_t seed, const GaussTable& tab, hipStream_t s);
 

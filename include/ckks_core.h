@@ -389,6 +389,17 @@ int ckks_sample_uniform(const ckks_ctx *ctx, uint64_t *out, int rows, int l, int
                         void *stream);
 int ckks_sample_gaussian(const ckks_ctx *ctx, uint64_t *out, int rows, int l, int prime0, double sigma,
                          uint64_t seed, void *stream);
+/* Symmetric encryption in one pass (encrypt of
+ * scripts/exp_e33_real_ciphertext.py:171-175: c1 = -a, c0 = a s + e + m):
+ * pt [batch][l][N] coefficient form, s_eval [l][N] the secret key's
+ * residues in evaluation order (ckks_ntt_fwd of s), out [batch][2][l][N].
+ * The mask is ckks_sample_uniform(rows = batch, l, prime 0, seed_a) and the
+ * error ckks_sample_gaussian(batch, l, 0, sigma, seed_e), so the words are
+ * those of the two samplers followed by ckks_poly_mul and ckks_pointwise.
+ * FP64 arithmetic, negacyclic rings N = 2^12 .. 2^14 (else
+ * CKKS_ERR_BAD_ARG / CKKS_ERR_BAD_SHAPE). */
+int ckks_encrypt_sk(const ckks_ctx *ctx, const uint64_t *pt, const uint64_t *s_eval, uint64_t *out, int batch,
+                    int l, double sigma, uint64_t seed_a, uint64_t seed_e, void *stream);
 /* Host only: the Gaussian sampler's table, cdf[k] = floor(2^64
  * P(|round(Z)| <= k)) for k < n <= 48. */
 int ckks_gaussian_cdf(double sigma, uint64_t *cdf, int n);
