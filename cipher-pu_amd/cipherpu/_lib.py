@@ -85,6 +85,7 @@ _SIGS = {
     "ckks_sample_uniform": (_I, [_P, _P, _I, _I, _I, _U, _P]),
     "ckks_sample_gaussian": (_I, [_P, _P, _I, _I, _I, ctypes.c_double, _U, _P]),
     "ckks_encrypt_sk": (_I, [_P, _P, _P, _P, _I, _I, ctypes.c_double, _U, _U, _P]),
+    "ckks_decrypt_sk": (_I, [_P, _P, _P, _P, _I, _I, _P]),
     "ckks_gaussian_cdf": (_I, [ctypes.c_double, _P, _I]),
     "ckks_workspace_hoisted_bytes": (_U, [_P, _I, _I, _I]),
     "ckks_rotate_hoisted": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, _I, _I, _P, _P]),

@@ -106,9 +106,11 @@ class CKKS:
         self._seed = int(seed) & ((1 << 64) - 1)
         self._draws = 0
         self._ws = {}      # stream handle -> workspace
-        # device-sampled encryption in one kernel (ckks_encrypt_sk) where it
-        # applies: the words of the separate samplers, product and sums
+        # device-sampled encryption and decryption in one kernel each
+        # (ckks_encrypt_sk / ckks_decrypt_sk) where they apply: the words of
+        # the separate samplers, product and sums
         self.fused_encrypt = True
+        self.fused_decrypt = True
 
     # -- sampling (host) ------------------------------------------------------
     def ternary(self, *shape) -> np.ndarray:
@@ -251,6 +253,11 @@ class CKKS:
     def decrypt(self, ct: Ciphertext, sk: SecretKey) -> torch.Tensor:
         d = ct.data
         B, l = d.shape[0], d.shape[2]
+        if self.fused_decrypt and self.ring.fp64 and self.N in (4096, 8192, 16384) and not self.ring.cyclic:
+            out = torch.empty(B, l, self.N, dtype=torch.uint64, device=self.device)
+            call("ckks_decrypt_sk", self.ring.handle, _ptr(d.contiguous()), _ptr(self.key_eval(sk, l)), _ptr(out),
+                 B, l, self.ring.stream())
+            return out
         S = self.key_rows(sk, B, l)
         return self.ring.add(d[:, 0].contiguous(), self.ring.poly_mul(d[:, 1].contiguous(), S))
 

@@ -515,6 +515,20 @@ int ckks_encrypt_sk(const ckks_ctx* c, const uint64_t* pt, const uint64_t* s_eva
     return CKKS_OK;
 }
 
+int ckks_decrypt_sk(const ckks_ctx* c, const uint64_t* ct, const uint64_t* s_eval, uint64_t* out, int batch, int l,
+                    void* stream) {
+    int st = check_rows(c, batch, l, 0);
+    if (st) return st == NOTHING_TO_DO ? CKKS_OK : st;
+    if (!ct || !s_eval || !out) return CKKS_ERR_NULL;
+    if (ct == out || s_eval == out) return CKKS_ERR_BAD_ARG;
+    if (!c->fp64) return CKKS_ERR_BAD_ARG;
+    if (c->logN < 12 || c->logN > 14 || (c->flags & CKKS_CYCLIC)) return CKKS_ERR_BAD_SHAPE;
+    if (misaligned(ct) || misaligned(s_eval) || misaligned(out)) return CKKS_ERR_BAD_ALIGNMENT;
+    if (set_device(c)) return CKKS_ERR_HIP;
+    HIPCHK(launch_decrypt_sk(c->tables(), ct, s_eval, out, batch, l, limb_map(0, l), S(stream)));
+    return CKKS_OK;
+}
+
 int ckks_timing_start(ckks_ctx* c, int max_launches) {
     if (!c) return CKKS_ERR_NULL;
     if (max_launches < 1 || max_launches > (1 << 20)) return CKKS_ERR_BAD_SHAPE;

@@ -182,6 +182,43 @@ __global__ void __launch_bounds__(T) k_encrypt_f(const uint64_t* __restrict__ pt
     }
 }
 
+// Decryption, one (ciphertext b, limb i) row a workgroup: m = c0 + c1 s
+// (exp_e33_real_ciphertext.py:178-180), the words of ckks_poly_mul of c1 and
+// s followed by the point-wise sum.  ct [batch][2][l][N], s_eval [l][N]
+// evaluation order, out [batch][l][N].
+template <int LOGN, int T>
+__global__ void __launch_bounds__(T) k_decrypt_f(const uint64_t* __restrict__ ct, const uint64_t* __restrict__ s_eval,
+                                                 uint64_t* __restrict__ out, const double* __restrict__ twf_all,
+                                                 const PrimeF* __restrict__ pfs, const PrimeConst* __restrict__ pcs,
+                                                 LimbMap map, int l) {
+    using C = Cfg<LOGN, T>;
+    extern __shared__ uint64_t lds[];
+    double* ldsf = reinterpret_cast<double*>(lds);
+    const int tid = threadIdx.x;
+    const int row = (int)blockIdx.x;   // b l + i
+    const int i = wave_uniform(row % l), b = wave_uniform(row / l);
+    const int p = map.idx[i];
+    const PrimeF pf = pfs[p];
+    const uint64_t q = pcs[p].q;
+    const double* tw = twf_all + (size_t)p * 2 * C::N;
+    const uint64_t* c0 = ct + ((size_t)b * 2 * l + i) * C::N;
+    double x[C::E];
+    nttf::load_words<C>(x, c0 + (size_t)l * C::N, tid);
+    nttf::fwd_ntt<C>(x, ldsf, tw, pf, tid);
+    const uint64_t* sr = s_eval + (size_t)i * C::N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e)
+        x[e] = muld(rmod(x[e], pf.qinv, pf.nq), u2d(sr[ntt::eval_index<C>(e, tid)]), pf.qinv, pf.nq);
+    nttf::inv_ntt<C>(x, ldsf, tw + C::N, pf, tid);
+    uint64_t* o = out + (size_t)row * C::N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+        const int n = e * C::T + tid;
+        const uint64_t v = d2u(canon(x[e], pf.qinv, pf.nq)) + c0[n];
+        o[n] = v >= q ? v - q : v;
+    }
+}
+
 // Generic transform for small rings (N <= 2048): one stage per barrier.
 __device__ void small_fwd(uint64_t* s, const ulonglong2* tw, const PrimeConst& pc, int logN) {
     const int H = 1 << (logN - 1);
@@ -452,6 +489,17 @@ hipError_t run_encrypt_f(const DevTables& t, const uint64_t* pt, const uint64_t*
     return hipGetLastError();
 }
 
+template <int LOGN, int T>
+hipError_t run_decrypt_f(const DevTables& t, const uint64_t* ct, const uint64_t* s_eval, uint64_t* out, int batch,
+                         int l, const LimbMap& map, hipStream_t s) {
+    using C = Cfg<LOGN, T>;
+    hipError_t e = set_lds(k_decrypt_f<LOGN, T>, C::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_decrypt_f<LOGN, T>), dim3(batch * l), dim3(T), C::LDS_BYTES, s, ct, s_eval, out, t.twf,
+                       t.pf, t.pc, map, l);
+    return hipGetLastError();
+}
+
 // Untuned launch shape of the batched transforms (fastest standalone
 // transform on MI355X at N = 8192: 512 threads).
 int default_threads(int logN) { return logN >= 13 ? 512 : 256; }
@@ -528,6 +576,18 @@ hipError_t launch_encrypt_sk(const DevTables& t, const uint64_t* pt, const uint6
         case 12: return run_encrypt_f<12, 256>(t, pt, s_eval, out, batch, l, map, seed_a, seed_e, tab, s);
         case 13: return run_encrypt_f<13, 512>(t, pt, s_eval, out, batch, l, map, seed_a, seed_e, tab, s);
         case 14: return run_encrypt_f<14, 512>(t, pt, s_eval, out, batch, l, map, seed_a, seed_e, tab, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_decrypt_sk(const DevTables& t, const uint64_t* ct, const uint64_t* s_eval, uint64_t* out, int batch,
+                             int l, const LimbMap& map, hipStream_t s) {
+    if (batch * l == 0) return hipSuccess;
+    if (!t.fp64) return hipErrorInvalidValue;
+    switch (t.logN) {
+        case 12: return run_decrypt_f<12, 256>(t, ct, s_eval, out, batch, l, map, s);
+        case 13: return run_decrypt_f<13, 512>(t, ct, s_eval, out, batch, l, map, s);
+        case 14: return run_decrypt_f<14, 512>(t, ct, s_eval, out, batch, l, map, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -200,6 +200,10 @@ hipError_t launch_sample_uniform(const DevTables& t, uint64_t* out, int rows, in
 hipError_t launch_encrypt_sk(const DevTables& t, const uint64_t* pt, const uint64_t* s_eval, uint64_t* out, int batch,
                              int l, const LimbMap& map, uint64_t seed_a, uint64_t seed_e, const GaussTable& tab,
                              hipStream_t s);
+// Fused decryption m = c0 + c1 s, same rings and arithmetic: ct
+// [batch][2][l][N], s_eval [l][N], out [batch][l][N].
+hipError_t launch_decrypt_sk(const DevTables& t, const uint64_t* ct, const uint64_t* s_eval, uint64_t* out, int batch,
+                             int l, const LimbMap& map, hipStream_t s);
 hipError_t launch_sample_gauss(const DevTables& t, uint64_t* out, int rows, int l, const LimbMap& map,
                                uint64_t seed, const GaussTable& tab, hipStream_t s);
 

@@ -400,6 +400,12 @@ int ckks_sample_gaussian(const ckks_ctx *ctx, uint64_t *out, int rows, int l, in
  * CKKS_ERR_BAD_ARG / CKKS_ERR_BAD_SHAPE). */
 int ckks_encrypt_sk(const ckks_ctx *ctx, const uint64_t *pt, const uint64_t *s_eval, uint64_t *out, int batch,
                     int l, double sigma, uint64_t seed_a, uint64_t seed_e, void *stream);
+/* Decryption in one pass (decrypt of scripts/exp_e33_real_ciphertext.py:
+ * 178-180: m = c0 + c1 s): ct [batch][2][l][N], s_eval as for
+ * ckks_encrypt_sk, out [batch][l][N]; the words of ckks_poly_mul of c1 and
+ * s followed by ckks_pointwise op 1.  Same arithmetic and rings. */
+int ckks_decrypt_sk(const ckks_ctx *ctx, const uint64_t *ct, const uint64_t *s_eval, uint64_t *out, int batch,
+                    int l, void *stream);
 /* Host only: the Gaussian sampler's table, cdf[k] = floor(2^64
  * P(|round(Z)| <= k)) for k < n <= 48. */
 int ckks_gaussian_cdf(double sigma, uint64_t *cdf, int n);

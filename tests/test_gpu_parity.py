@@ -401,7 +401,8 @@ def test_rotate_add_gives_the_words_of_rotate_then_add(gpu, l):
 
 @pytest.mark.parametrize("N", [4096, 8192, 16384])
 def test_fused_encryption_gives_the_words_of_the_separate_kernels(gpu, N):
-    """ckks_encrypt_sk (mask, product, error and sums in one kernel) against
+    """ckks_encrypt_sk (mask, product, error and sums in one kernel) and
+    ckks_decrypt_sk against
     the separate device samplers, ckks_poly_mul and the point-wise sums under
     the same seeds, call after call and at several levels, in words; and the
     fused ciphertexts decrypt to the message."""
@@ -409,7 +410,7 @@ def test_fused_encryption_gives_the_words_of_the_separate_kernels(gpu, N):
     from cipherpu.ckks import CKKS, CKKSParams
     from cipherpu.ring import to_numpy
     fused, split = CKKS(CKKSParams(N=N), seed=51), CKKS(CKKSParams(N=N), seed=51)
-    split.fused_encrypt = False
+    split.fused_encrypt = split.fused_decrypt = False
     skf, sks = fused.keygen(), split.keygen()
     rng = np.random.default_rng(N)
     for l in (4, 3, 1):
@@ -418,6 +419,7 @@ def test_fused_encryption_gives_the_words_of_the_separate_kernels(gpu, N):
             a = fused.encrypt(fused.encode(z, level=l), skf)
             b = split.encrypt(split.encode(z, level=l), sks)
             assert torch.equal(a.data, b.data), (N, l)
+            assert torch.equal(fused.decrypt(a, skf), split.decrypt(b, sks)), (N, l)   # ckks_decrypt_sk
         dec = to_numpy(fused.decode(fused.decrypt(a, skf), a.scale)).real
         assert np.max(np.abs(dec - z)) < 1e-7, (N, l)
 
