@@ -48,6 +48,9 @@ ROW = 16          # row stride of the polyphase grids and of the output grid
 PHASE = 256       # slots per polyphase component
 WINDOW = 256      # FC summation window
 N_CH, KS, N_OUT = 8, 5, 10
+# Row order of the FC outputs: those shifted into windows 1..3 first, then
+# 0, 4, 8 (window 0), so each set is one contiguous run of rows.
+FC_ORDER = [k for k in range(N_OUT) if k % 4] + [k for k in range(N_OUT) if k % 4 == 0]
 
 
 def _ptr(t):
@@ -184,10 +187,12 @@ class EncryptedCNN(EncryptedNet):
         return self._conv[key]
 
     def _fc_weights(self, level: int, scale: float) -> torch.Tensor:
-        """[10][8] evaluation-form prepared plaintexts of fc_weight_blocks."""
+        """[10][8] evaluation-form prepared plaintexts of fc_weight_blocks,
+        outputs in FC_ORDER."""
         key = (self.gears[1], level, scale)
         if key not in self._fc:
-            rows = [self._slots(blk) for blk in fc_weight_blocks(self.PF)]
+            W = fc_weight_blocks(self.PF)
+            rows = [self._slots(W[N_CH * k + c]) for k in FC_ORDER for c in range(N_CH)]
             prep = self.prepared_pt(np.asarray(rows), level, scale)
             self._fc[key] = prep.view(N_OUT, N_CH, level, self.ck.N, 2)
         return self._fc[key]
@@ -231,11 +236,17 @@ class EncryptedCNN(EncryptedNet):
         ck.ring.intt_(acc.view(-1, l, ck.N))
         outs = ck.rescale(Ciphertext(acc.view(N_OUT * B, 2, l, ck.N), ct.scale * scale))
         # output k goes to window k mod 4 of packed ciphertext k // 4: every
-        # output shifted by -256 (k mod 4) in one call
-        sh = self.rot_blocks(outs, B, [-WINDOW * (k % 4) for k in range(N_OUT)])
-        out = lambda k: sh[k].data
+        # output shifted by -256 (k mod 4) in one call (rows in FC_ORDER: the
+        # shifted ones one contiguous run, outputs 0, 4, 8 the last rows)
+        sh = self.rot_blocks(outs, B, [-WINDOW * (k % 4) for k in FC_ORDER])
+        pos = {k: j for j, k in enumerate(FC_ORDER)}
+        out = lambda k: sh[pos[k]].data
         groups = list(range(0, N_OUT, 4))
-        z = torch.cat([out(p) for p in groups])
+        first = pos[groups[0]]
+        if [pos[p] for p in groups] == list(range(first, first + len(groups))):
+            z = outs.data[first * B:(first + len(groups)) * B]    # a view: summed into in place
+        else:
+            z = torch.cat([out(p) for p in groups])
         for m in range(1, 4):
             for i, p in enumerate(groups):
                 if p + m < N_OUT:
