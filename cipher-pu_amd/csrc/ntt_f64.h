@@ -593,7 +593,7 @@ __device__ __forceinline__ void inv_ntt2(double (&x)[C::E], double (&y)[C::E], d
 
 // The built paired shapes: at N = 4096 with 512 threads exchanges 2 and 3
 // are wave-local and exchange 2 pads differently from exchange 1 (forward
-// round 2 needs the barrier); at N = 8192 with 1024 threads the one
+// pass 2 needs the barrier); at N = 8192 with 1024 threads the one
 // wave-local exchange (4) keeps exchange 3's padding.
 static_assert(y_relaid<Cfg<12, 512>, 2>() && !y_relaid<Cfg<12, 512>, 3>());
 static_assert(!y_relaid_inv<Cfg<12, 512>, 2>());
