@@ -179,17 +179,57 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
     double a0[C::E], a1[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) a0[k] = a1[k] = 0.0;
-    for (int j = 0; j < l; ++j) {
+    // the digit row and its two key rows, coalesced (the sum is slot by slot)
+    auto load_digit = [&](int j, uint64_t (&dv)[C::E], double (&bv)[C::E], double (&av)[C::E]) {
         const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
         const double* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
         const double* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
+            dv[k] = d[k * C::T + tid];
+            bv[k] = kb[k * C::T + tid];
+            av[k] = ka[k * C::T + tid];
+        }
+    };
+    // Without EVAL's epilogue (and at most 8 words a thread) the registers
+    // allow the next digit's rows to be loaded during the current digit's
+    // products (the last digit loads its own rows again: a clamped, valid
+    // address).
+    if constexpr (!EVAL && C::E <= 8) {
+        uint64_t nd[C::E];
+        double nb[C::E], na[C::E];
+        load_digit(0, nd, nb, na);
+        for (int j = 0; j < l; ++j) {
+            uint64_t dv[C::E];
+            double bv[C::E], av[C::E];
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) {
+                dv[k] = nd[k];
+                bv[k] = nb[k];
+                av[k] = na[k];
+            }
+            load_digit(j + 1 < l ? j + 1 : j, nd, nb, na);
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) {
 #pragma clang fp contract(off)
-            const int idx = k * C::T + tid;          // coalesced: the sum is slot by slot
-            const double x = u2d(d[idx]);
-            a0[k] = a0[k] + mulk(x, kb[idx], pf.qinv, nq);   // 7q/8 each, l <= 15 digits
-            a1[k] = a1[k] - mulk(x, ka[idx], pf.qinv, nq);
+                const double x = u2d(dv[k]);
+                a0[k] = a0[k] + mulk(x, bv[k], pf.qinv, nq);   // 7q/8 each, l <= 15 digits
+                a1[k] = a1[k] - mulk(x, av[k], pf.qinv, nq);
+            }
+        }
+    } else {
+        for (int j = 0; j < l; ++j) {
+            const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
+            const double* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+            const double* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) {
+#pragma clang fp contract(off)
+                const int idx = k * C::T + tid;
+                const double x = u2d(d[idx]);
+                a0[k] = a0[k] + mulk(x, kb[idx], pf.qinv, nq);   // 7q/8 each, l <= 15 digits
+                a1[k] = a1[k] - mulk(x, ka[idx], pf.qinv, nq);
+            }
         }
     }
     const uint32_t g = galois.g[r];
