@@ -287,13 +287,15 @@ def snr_db(ref: np.ndarray, got: np.ndarray) -> float:
     return math.inf if noise == 0 else 10.0 * math.log10(float(np.sum(ref ** 2)) / noise)
 
 
-def monitor(mlp, ct_out, X: np.ndarray) -> Tuple[float, np.ndarray]:
+def monitor(mlp, ct_out, X: np.ndarray, ref: Optional[np.ndarray] = None) -> Tuple[float, np.ndarray]:
     """The plain decrypt-based monitor: decrypt and decode the network output,
-    compare it with the dense FP64 plaintext network on the same inputs.
-    Returns (SNR over all samples, per-ciphertext SNR) in dB."""
+    compare it with the dense FP64 plaintext network on the same inputs (ref,
+    when the caller has it already).  Returns (SNR over all samples,
+    per-ciphertext SNR) in dB."""
     n = X.shape[0]
     got = mlp.decrypt_outputs(ct_out, n)
-    ref = mlp.reference_outputs(X)
+    if ref is None:
+        ref = mlp.reference_outputs(X)
     S = mlp.S
     per = np.array([snr_db(ref[i:i + S], got[i:i + S]) for i in range(0, n, S)])
     return snr_db(ref, got), per
@@ -374,11 +376,12 @@ def _plain_select(mlp, ct, Xc, target_db, start_gear):
     step to the next safer gear until the decrypt-based SNR meets the
     target (gear 0 is always accepted).  Returns (output ct, gear, evals)."""
     g, evals = start_gear, 0
+    ref = mlp.reference_outputs(Xc)   # once per chunk, whatever the number of trials
     while True:
         mlp.set_gears(g, g)
         out = mlp.forward(ct)
         evals += 1
-        s, _ = monitor(mlp, out, Xc)
+        s, _ = monitor(mlp, out, Xc, ref)
         if s >= target_db or g == 0:
             return out, g, evals
         g -= 1
