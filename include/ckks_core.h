@@ -141,7 +141,9 @@ int ckks_relin(const ckks_ctx *ctx, const uint64_t *d01, const uint64_t *d2, con
 /* Fused HMult + relinearise + rescale of ciphertext pairs, coefficient form:
  * a, b [batch][2][l][N] -> out [batch][2][l-1][N].  Replaces e2e_mult
  * (scripts/verify_e2e_mult.py:78-102; RTL rtl/m4/e2e_ckks_mult_ctrl_m4.sv
- * S_MUL_0..S_RESCALE_1) with the exact RNS rescale. */
+ * S_MUL_0..S_RESCALE_1) with the exact RNS rescale.  With b == a (one
+ * pointer: a squaring) the operand is transformed once; the words are those
+ * of a product with a copy. */
 int ckks_hmult(const ckks_ctx *ctx, const uint64_t *a, const uint64_t *b, const uint64_t *key, int key_limbs,
                uint64_t *out, int batch, int l, int special, void *workspace, void *stream);
 
