@@ -217,6 +217,48 @@ __global__ void __launch_bounds__(T) k_ks_hoisted(const uint64_t* __restrict__ D
                 a1[k] = a1[k] - mulk(x, av[k], pf.qinv, nq);
             }
         }
+    } else if constexpr (EVAL && C::E == 8) {
+        // EVAL's epilogue leaves registers for half a digit ahead: the words
+        // in two halves of 4, each summed over the digits in order, the next
+        // (digit, half)'s rows loaded during the current one's products
+        constexpr int H = C::E / 2;
+        auto load_half = [&](int j, int h, uint64_t (&dv)[H], double (&bv)[H], double (&av)[H]) {
+            const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
+            const double* kb = key + ((size_t)(j * 2 + 0) * key_limbs + kt) * C::N;
+            const double* ka = key + ((size_t)(j * 2 + 1) * key_limbs + kt) * C::N;
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                const int idx = (h * H + k) * C::T + tid;
+                dv[k] = d[idx];
+                bv[k] = kb[idx];
+                av[k] = ka[idx];
+            }
+        };
+        uint64_t nd[H];
+        double nb[H], na[H];
+        load_half(0, 0, nd, nb, na);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            for (int j = 0; j < l; ++j) {
+                uint64_t dv[H];
+                double bv[H], av[H];
+#pragma unroll
+                for (int k = 0; k < H; ++k) {
+                    dv[k] = nd[k];
+                    bv[k] = nb[k];
+                    av[k] = na[k];
+                }
+                const bool last = j + 1 == l;
+                load_half(last ? (h == 0 ? 0 : j) : j + 1, last && h == 0 ? 1 : h, nd, nb, na);
+#pragma unroll
+                for (int k = 0; k < H; ++k) {
+#pragma clang fp contract(off)
+                    const double x = u2d(dv[k]);
+                    a0[h * H + k] = a0[h * H + k] + mulk(x, bv[k], pf.qinv, nq);   // 7q/8 each, l <= 15 digits
+                    a1[h * H + k] = a1[h * H + k] - mulk(x, av[k], pf.qinv, nq);
+                }
+            }
+        }
     } else {
         for (int j = 0; j < l; ++j) {
             const uint64_t* d = D + (((size_t)b * l + j) * (l + 1) + t) * C::N;
